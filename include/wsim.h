@@ -1,0 +1,227 @@
+/*
+ * wsim.h -- C ABI of libwsim, the MI355X (gfx950) DFSPH neighbour-search and
+ * pressure-solve library.
+ *
+ * The reference (Qervas/water_simulation) exposes this path as a C++ class
+ * API with no C layer; every entry point below names the reference member it
+ * replaces (file:line under the reference tree).  Conventions:
+ *   - every call returns wsim_status (0 = ok) unless it is a plain getter;
+ *     wsim_last_error() returns a thread-local message for the last failure;
+ *   - no C++ exception ever crosses this boundary; the status codes map onto
+ *     the reference's exception types (see wsim_status);
+ *   - handles are opaque; device buffers are plain device pointers;
+ *   - streams are hipStream_t passed as void* (NULL = the null stream);
+ *   - positions and velocities are N x 3 float, AoS (x, y, z per particle),
+ *     in the caller's original particle order.
+ */
+#ifndef WSIM_H
+#define WSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WSIM_ABI_VERSION 1
+
+typedef enum {
+    WSIM_OK = 0,
+    WSIM_ERR_RUNTIME = 1,          /* reference: std::runtime_error */
+    WSIM_ERR_OUT_OF_RANGE = 2,     /* reference: std::out_of_range */
+    WSIM_ERR_INVALID_ARGUMENT = 3, /* reference: std::invalid_argument / bad input */
+    WSIM_ERR_DEVICE = 4            /* reference: CUDA error -> std::runtime_error
+                                      (core/include/water/core/cuda_check.h:10-36) */
+} wsim_status;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* wsim_last_error(void);
+int wsim_abi_version(void);
+
+/* ---- device memory (core/include/water/core/device_buffer.h:21-89) ---- */
+wsim_status wsim_device_count(int* count);
+wsim_status wsim_device_alloc(size_t bytes, void** out);
+wsim_status wsim_device_free(void* ptr);
+wsim_status wsim_memcpy_h2d(void* dst_device, const void* src_host, size_t bytes, void* stream);
+wsim_status wsim_memcpy_d2h(void* dst_host, const void* src_device, size_t bytes, void* stream);
+wsim_status wsim_memcpy_d2d(void* dst_device, const void* src_device, size_t bytes, void* stream);
+wsim_status wsim_memset(void* dst_device, int value, size_t bytes, void* stream);
+wsim_status wsim_stream_create(void** out_stream);
+wsim_status wsim_stream_destroy(void* stream);
+wsim_status wsim_stream_synchronize(void* stream);
+wsim_status wsim_device_synchronize(void);
+
+/* ---- ParticleStore (core/include/water/core/particle_store.h:48-107) ---- */
+typedef struct wsim_store_t* wsim_store;
+typedef enum { WSIM_ATTRIB_F32 = 0, WSIM_ATTRIB_F32X3 = 1, WSIM_ATTRIB_U32 = 2 } wsim_attrib_type;
+
+/* ParticleStore(capacity, stream): registers "position" and "velocity" (F32x3). */
+wsim_status wsim_store_create(size_t capacity, void* stream, wsim_store* out);
+void wsim_store_destroy(wsim_store store);
+/* resize(count): WSIM_ERR_OUT_OF_RANGE if count > capacity (particle_store.cu:12-16). */
+wsim_status wsim_store_resize(wsim_store store, size_t count);
+size_t wsim_store_count(wsim_store store);
+size_t wsim_store_capacity(wsim_store store);
+/* register_attribute<T>(name, type): zero-filled; WSIM_ERR_RUNTIME on a duplicate name
+ * (particle_store.h:109-127). */
+wsim_status wsim_store_register_attribute(wsim_store store, const char* name,
+                                          wsim_attrib_type type, size_t* handle);
+int wsim_store_has_attribute(wsim_store store, const char* name);
+/* Handle of a registered attribute by name; WSIM_ERR_OUT_OF_RANGE if absent.  The
+ * reference keeps its name map private; this lets callers reach the attributes a
+ * solver registers (dfsph_density, dfsph_alpha, ...). */
+wsim_status wsim_store_find_attribute(wsim_store store, const char* name, size_t* handle);
+/* attribute_data(handle): WSIM_ERR_OUT_OF_RANGE on an invalid handle (particle_store.h:85-96). */
+wsim_status wsim_store_attribute_data(wsim_store store, size_t handle, void** out);
+/* positions() / velocities() (particle_store.h:61-64): N x 3 float device arrays. */
+wsim_status wsim_store_positions(wsim_store store, float** out);
+wsim_status wsim_store_velocities(wsim_store store, float** out);
+size_t wsim_store_position_handle(wsim_store store);
+size_t wsim_store_velocity_handle(wsim_store store);
+
+/* ---- CellGrid (core/include/water/core/cell_grid.h:16-50) ---- */
+typedef struct wsim_grid_t* wsim_grid;
+/* CellGrid(origin, cells_per_axis, cell_length, stream); cell_start zero-filled. */
+wsim_status wsim_grid_create(const float origin[3], const int32_t cells_per_axis[3],
+                             float cell_length, void* stream, wsim_grid* out);
+void wsim_grid_destroy(wsim_grid grid);
+wsim_status wsim_grid_reserve(wsim_grid grid, size_t capacity);
+/* build(positions, count) (cell_grid.cu:73-138): device N x 3 positions. */
+wsim_status wsim_grid_build(wsim_grid grid, const float* positions_xyz, size_t count);
+/* cell_start(): total_cells()+1 entries; cell_start[c] = #particles with key < c. */
+const uint32_t* wsim_grid_cell_start(wsim_grid grid);
+/* sorted_indices(): sorted slot -> original particle index. */
+const uint32_t* wsim_grid_sorted_indices(wsim_grid grid);
+/* Sorted cell keys (not exposed by the reference; its cell_ids_ buffer). */
+const uint32_t* wsim_grid_sorted_keys(wsim_grid grid);
+void wsim_grid_origin(wsim_grid grid, float out[3]);
+void wsim_grid_cells_per_axis(wsim_grid grid, int32_t out[3]);
+float wsim_grid_cell_length(wsim_grid grid);
+uint32_t wsim_grid_total_cells(wsim_grid grid);
+size_t wsim_grid_count(wsim_grid grid);
+
+/* ---- SpatialAccel, Morton part (core/include/water/core/spatial_accel.h:19-53) ---- */
+typedef struct wsim_accel_t* wsim_accel;
+wsim_status wsim_accel_create(wsim_accel* out);
+void wsim_accel_destroy(wsim_accel accel);
+/* build(positions, count, scene_bounds, stream) (spatial_accel.cu:111-179):
+ * 30-bit Morton codes and their stable sort. */
+wsim_status wsim_accel_build(wsim_accel accel, const float* positions_xyz, size_t count,
+                             const float bounds_min[3], const float bounds_max[3], void* stream);
+size_t wsim_accel_leaf_count(wsim_accel accel);
+const uint32_t* wsim_accel_sorted_indices(wsim_accel accel);
+/* Sorted Morton codes (the reference keeps them in morton_codes_). */
+const uint32_t* wsim_accel_morton_codes(wsim_accel accel);
+
+/* ---- DFSPHSolver (solvers/dfsph/include/water/solvers/dfsph.h:17-84) ---- */
+typedef struct {
+    float rest_density;          /* dfsph.h:19-33 defaults via wsim_dfsph_config_default */
+    float particle_radius;
+    float smoothing_length;
+    float viscosity;             /* XSPH epsilon */
+    float damping;
+    float surface_tension;       /* accepted, unused (as in the reference) */
+    float gravity[3];
+    uint32_t max_density_iters;
+    uint32_t max_divergence_iters;
+    float eta_density;
+    float eta_divergence;
+    float domain_min[3];
+    float domain_max[3];
+    uint32_t enable_divergence_solve; /* extension: 0 = the reference's step() */
+    uint32_t pressure_form;           /* extension: 0 = reference kappa/rho^2, 1 = kappa/rho */
+} wsim_dfsph_config;
+
+typedef struct {
+    uint32_t density_iters;      /* last_density_iters() (dfsph.h:51) */
+    uint32_t divergence_iters;
+    double mean_density_error;   /* binary64 mean of max(0, rho* - rho0), last iteration */
+    double mean_divergence_error;
+    float max_velocity;          /* max_velocity() (dfsph.h:39) */
+    uint32_t nan_count;          /* particles with a non-finite position or velocity */
+    uint64_t substeps;
+} wsim_step_stats;
+
+typedef struct wsim_solver_t* wsim_solver;
+void wsim_dfsph_config_default(wsim_dfsph_config* cfg);
+/* DFSPHSolver(fluid, boundary (may be NULL), cfg, stream) (dfsph.cu:71-100).
+ * Registers dfsph_density/alpha/kappa/density_adv on the fluid store and
+ * dfsph_boundary_mass on the boundary store; both stores must outlive the
+ * solver. */
+wsim_status wsim_solver_create(wsim_store fluid, wsim_store boundary, const wsim_dfsph_config* cfg,
+                               void* stream, wsim_solver* out);
+void wsim_solver_destroy(wsim_solver solver);
+/* step(dt) (dfsph.cu:120-133). */
+wsim_status wsim_solver_step(wsim_solver solver, float dt);
+float wsim_solver_max_velocity(wsim_solver solver);
+float wsim_solver_particle_mass(wsim_solver solver);
+uint32_t wsim_solver_last_density_iters(wsim_solver solver);
+/* density_at / alpha_at / boundary_mass_at (dfsph.cu:330-353). */
+wsim_status wsim_solver_density_at(wsim_solver solver, size_t i, float* out);
+wsim_status wsim_solver_alpha_at(wsim_solver solver, size_t i, float* out);
+wsim_status wsim_solver_boundary_mass_at(wsim_solver solver, size_t i, float* out);
+wsim_status wsim_solver_stats(wsim_solver solver, wsim_step_stats* out);
+/* fluid_grid() / boundary_grid() (dfsph.h:41-42): borrowed, valid while the solver lives;
+ * boundary grid is NULL without a boundary store.  Do not destroy. */
+wsim_grid wsim_solver_fluid_grid(wsim_solver solver);
+wsim_grid wsim_solver_boundary_grid(wsim_solver solver);
+/* Per-kernel timing with HIP events on the solver's stream (profiling only). */
+wsim_status wsim_solver_set_timing(wsim_solver solver, int enable);
+wsim_status wsim_solver_reset_timing(wsim_solver solver);
+/* kind in [0, wsim_solver_kernel_kinds()); name of the kernel class, launches
+ * that did work and their summed device time in milliseconds. */
+int wsim_solver_kernel_kinds(void);
+const char* wsim_solver_kernel_name(int kind);
+wsim_status wsim_solver_kernel_time(wsim_solver solver, int kind, uint64_t* launches,
+                                    double* total_ms);
+
+/* ---- host-side surface ---- */
+/* sample_aabb_boundary(lo, hi, spacing, n_layers)
+ * (solvers/dfsph/include/water/solvers/boundary_sampler.h:24-25).  Writes up to
+ * capacity points (3 floats each) to out (may be NULL) and the total count. */
+wsim_status wsim_sample_aabb_boundary(const float lo[3], const float hi[3], float spacing,
+                                      int n_layers, float* out, size_t capacity, size_t* count);
+/* TimeStepper (core/include/water/core/timestep.h:8-34). */
+typedef struct {
+    float frame_dt;
+    float dt_max;
+    float cfl_lambda;
+    uint32_t max_substeps;
+} wsim_timestep_config;
+void wsim_timestep_config_default(wsim_timestep_config* cfg);
+float wsim_timestep_next_dt(const wsim_timestep_config* cfg, float max_velocity,
+                            float particle_radius, float remaining_in_frame);
+/* Lattice fluid block of sim_cli (apps/sim_cli/main.cpp:65-74). */
+wsim_status wsim_lattice_block(const float lo[3], const float hi[3], float spacing, float* out,
+                               size_t capacity, size_t* count);
+/* frame_%04d.bin writer (apps/sim_cli/main.cpp:76-83): u32 N, then N x 3 float LE. */
+wsim_status wsim_write_frame(const char* path, const float* positions_host, size_t count);
+
+/* ---- scene loader (scene/include/water/scene/scene.h:9-45) ---- */
+typedef struct {
+    char name[256];
+    int32_t width, height;
+    float fps;
+    int32_t frame_start, frame_end;
+    char format[32];
+    char video[32];
+    float cam_position[3], cam_look_at[3], cam_up[3];
+    float cam_fov_y_deg;
+    char solver[32];
+    float particle_radius;
+    float rest_density;
+    float viscosity;
+    float surface_tension;
+    float gravity[3];
+    float initial_block_min[3];
+    float initial_block_max[3];
+} wsim_scene;
+/* load_scene(path) (scene/src/loader.cpp:58-83): WSIM_ERR_RUNTIME on a missing
+ * file, malformed JSON, missing or unsupported schema_version. */
+wsim_status wsim_scene_load(const char* path, wsim_scene* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WSIM_H */

@@ -1,0 +1,808 @@
+// wsim_oracle.cpp -- CPU restatement of the reference DFSPH hot path.
+//
+// TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py.  The product library never links it.
+//
+// The code follows the reference's data flow literally: particle arrays stay
+// in original order, neighbour loops gather through the cell grid's sorted
+// index array, the 27-cell stencil is walked dz, dy, dx with ascending original
+// id inside a cell, and every arithmetic expression keeps the reference's
+// operand order.  Compiled with -ffp-contract=off, the binary32 instance is the
+// bit-level parity oracle for the HIP kernels.  The binary64 instance runs the
+// same algorithm with the same (binary32-rounded) constants.
+//
+// Deviations from the reference, all deliberate and documented:
+//  * the reference compiles device code with -use_fast_math
+//    (CMakeLists.txt:56); here every divide, sqrt and exp is IEEE, and
+//    exp(-damping*dt) is evaluated once on the host;
+//  * XSPH is a Jacobi sweep (all neighbours read pre-XSPH velocities); the
+//    reference updates velocities in place inside the same launch
+//    (dfsph_kernels.cu:317,338-340), which is schedule dependent;
+//  * a NaN or far out-of-range coordinate maps to the sentinel cell (the
+//    reference's float->int cast is undefined there, cell_grid.cu:11-13);
+//  * convergence errors are accumulated in binary64 (reference: CUB float sum,
+//    dfsph.cu:288-300).
+#include "wsim_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+using u32 = uint32_t;
+
+template <class R> struct V3 { R x, y, z; };
+
+// Constants are the reference's binary32 literals, widened for R = double.
+template <class R> inline R kpi() { return R(3.14159265358979323846f); }
+template <class R> inline R k1em9() { return R(1e-9f); }
+template <class R> inline R k1em12() { return R(1e-12f); }
+
+// ---------------------------------------------------------------------------
+// sph_kernels.cuh:12-22  cubic spline W(r, l), support [0, 2l)
+template <class R> inline R W(R r, R l) {
+    const R q = r / l;
+    if (q >= R(2)) return R(0);
+    const R sigma = R(1) / (kpi<R>() * l * l * l);
+    if (q < R(1)) return sigma * (R(1) - R(1.5) * q * q + R(0.75) * q * q * q);
+    const R t = R(2) - q;
+    return sigma * R(0.25) * t * t * t;
+}
+
+// sph_kernels.cuh:25-41  grad_i W(x_i - x_j, l)
+template <class R> inline V3<R> gradW(const V3<R>& rij, R l) {
+    const R r2 = rij.x * rij.x + rij.y * rij.y + rij.z * rij.z;
+    const R r = std::sqrt(r2);
+    if (r < k1em12<R>()) return {R(0), R(0), R(0)};
+    const R q = r / l;
+    if (q >= R(2)) return {R(0), R(0), R(0)};
+    const R sigma = R(1) / (kpi<R>() * l * l * l);
+    R dWdq;
+    if (q < R(1)) {
+        dWdq = sigma * (R(-3) * q + R(2.25) * q * q);
+    } else {
+        const R t = R(2) - q;
+        dWdq = sigma * R(-0.75) * t * t;
+    }
+    const R c = dWdq / (l * r);
+    return {rij.x * c, rij.y * c, rij.z * c};
+}
+
+// ---------------------------------------------------------------------------
+// Grid: cell_grid.cu:10-22 (key), :93-117 (stable sort by key, ties by id),
+// :119-137 (cell_start = lower_bound of each cell in the sorted keys).
+
+// (int)floorf((p - o) * inv) (cell_grid.cu:11-13, dfsph_kernels.cu:15-17),
+// made total: NaN and coordinates beyond the stencil's reach are clamped to
+// values whose whole stencil lies outside the grid.
+template <class R> inline int cell_coord(R p, R o, R inv, int D) {
+    const R f = std::floor((p - o) * inv);
+    if (!(f >= R(-2))) return -2;
+    if (f > R(D + 1)) return D + 1;
+    return static_cast<int>(f);
+}
+
+inline u32 cell_key(int cx, int cy, int cz, const int32_t D[3]) {
+    if (cx < 0 || cx >= D[0] || cy < 0 || cy >= D[1] || cz < 0 || cz >= D[2])
+        return u32(D[0]) * u32(D[1]) * u32(D[2]);
+    return (u32(cx) * u32(D[1]) + u32(cy)) * u32(D[2]) + u32(cz);
+}
+
+template <class R> struct Grid {
+    R ox = 0, oy = 0, oz = 0, inv = 0;
+    int32_t D[3] = {0, 0, 0};
+    u32 C = 0;
+    std::vector<u32> keys, sorted_keys, perm, cs;
+
+    void init(const float origin[3], const int32_t dims[3], float cell) {
+        ox = R(origin[0]); oy = R(origin[1]); oz = R(origin[2]);
+        // cell_grid.cu:81: inverse cell length computed once on the host
+        inv = R(1) / R(cell);
+        if (sizeof(R) == sizeof(float)) inv = R(1.0f / cell);
+        std::memcpy(D, dims, sizeof(D));
+        C = u32(D[0]) * u32(D[1]) * u32(D[2]);
+        cs.assign(size_t(C) + 1, 0);
+    }
+
+    u32 key_of(const V3<R>& p) const {
+        return cell_key(cell_coord(p.x, ox, inv, D[0]), cell_coord(p.y, oy, inv, D[1]),
+                        cell_coord(p.z, oz, inv, D[2]), D);
+    }
+
+    void build(const std::vector<V3<R>>& x, size_t n) {
+        keys.resize(n);
+        sorted_keys.resize(n);
+        perm.resize(n);
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)n; ++i) keys[i] = key_of(x[i]);
+        // Stable counting sort on (key, id): identical order to a stable LSD
+        // radix sort of (key, iota) pairs (cell_grid.cu:98-110).
+        std::vector<u32> start(size_t(C) + 2, 0);
+        for (size_t i = 0; i < n; ++i) start[size_t(keys[i]) + 1]++;
+        for (size_t c = 0; c <= C; ++c) start[c + 1] += start[c];
+        // cell_start[c] = number of particles with key < c, for c in [0, C]
+        // (fill + scatter + right-to-left min, cell_grid.cu:119-137)
+        for (size_t c = 0; c <= C; ++c) cs[c] = start[c];
+        for (size_t i = 0; i < n; ++i) {
+            const u32 slot = start[keys[i]]++;
+            perm[slot] = u32(i);
+            sorted_keys[slot] = keys[i];
+        }
+    }
+
+    // ITER_NEIGHBORS (dfsph_kernels.cu:13-35): dz, dy, dx in {-1,0,1}.
+    template <class F> void for_neighbors(const V3<R>& p, F&& f) const {
+        const int cx = cell_coord(p.x, ox, inv, D[0]);
+        const int cy = cell_coord(p.y, oy, inv, D[1]);
+        const int cz = cell_coord(p.z, oz, inv, D[2]);
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int nx = cx + dx, ny = cy + dy, nz = cz + dz;
+                    if (nx < 0 || nx >= D[0] || ny < 0 || ny >= D[1] || nz < 0 || nz >= D[2])
+                        continue;
+                    const u32 c = (u32(nx) * u32(D[1]) + u32(ny)) * u32(D[2]) + u32(nz);
+                    if (c >= C) continue;
+                    for (u32 s = cs[c]; s < cs[c + 1]; ++s) f(perm[s]);
+                }
+    }
+};
+
+// dfsph.cu:55-65
+void make_grid_params(const float dmin[3], const float dmax[3], float h, float origin[3],
+                      int32_t dims[3]) {
+    for (int a = 0; a < 3; ++a) {
+        dims[a] = static_cast<int32_t>(std::ceil((dmax[a] - dmin[a]) / h)) + 2;
+        origin[a] = dmin[a] - h;
+    }
+}
+
+template <class R> inline V3<R> load3(const float* p, size_t i) {
+    return {R(p[3 * i]), R(p[3 * i + 1]), R(p[3 * i + 2])};
+}
+
+// ---------------------------------------------------------------------------
+// DFSPHSolver restatement (dfsph.cu:71-133, dfsph_kernels.cu:40-400)
+template <class R> struct Solver {
+    orc_config cfg;
+    size_t n = 0, nb = 0;
+    std::vector<V3<R>> x, v, vtmp, xb;
+    std::vector<R> rho, alpha, kappa, rho_adv, kappa_v, mb;
+    Grid<R> fg, bg;
+    R mass = 0, h = 0;
+    orc_stats st{};
+
+    Solver(const orc_config& c, const float* px, const float* pv, size_t n_, const float* pxb,
+           size_t nb_)
+        : cfg(c), n(n_), nb(nb_) {
+        x.resize(n);
+        v.resize(n);
+        vtmp.resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            x[i] = load3<R>(px, i);
+            v[i] = pv ? load3<R>(pv, i) : V3<R>{0, 0, 0};
+        }
+        rho.assign(n, 0); alpha.assign(n, 0); kappa.assign(n, 0);
+        rho_adv.assign(n, 0); kappa_v.assign(n, 0);
+        h = R(cfg.smoothing_length);
+        // dfsph.cu:76-78
+        if (sizeof(R) == sizeof(float)) {
+            const float s = 2.0f * cfg.particle_radius;
+            const float v0 = s * s * s;
+            mass = R(cfg.rest_density * v0);
+        } else {
+            const R s = R(2) * R(cfg.particle_radius);
+            const R v0 = s * s * s;
+            mass = R(cfg.rest_density) * v0;
+        }
+        float origin[3];
+        int32_t dims[3];
+        make_grid_params(cfg.domain_min, cfg.domain_max, cfg.smoothing_length, origin, dims);
+        fg.init(origin, dims, cfg.smoothing_length);
+        bg.init(origin, dims, cfg.smoothing_length);
+        if (nb > 0 && pxb) {
+            xb.resize(nb);
+            for (size_t i = 0; i < nb; ++i) xb[i] = load3<R>(pxb, i);
+            mb.assign(nb, 0);
+            bg.build(xb, nb);  // dfsph.cu:96-97, built once
+            boundary_mass();   // dfsph.cu:98
+        } else {
+            nb = 0;
+        }
+    }
+
+    // dfsph_kernels.cu:40-65
+    void boundary_mass() {
+        const R rho0 = R(cfg.rest_density);
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)nb; ++i) {
+            const V3<R> pi = xb[i];
+            R sum_w = 0;
+            bg.for_neighbors(pi, [&](u32 j) {
+                const V3<R> pj = xb[j];
+                const V3<R> r{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
+                const R d = std::sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+                sum_w += W(d, h);
+            });
+            mb[i] = rho0 / std::fmax(sum_w, k1em9<R>());
+        }
+    }
+
+    // dfsph_kernels.cu:68-112
+    void density() {
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)n; ++i) {
+            const V3<R> pi = x[i];
+            R r_sum = 0;
+            fg.for_neighbors(pi, [&](u32 j) {
+                const V3<R> pj = x[j];
+                const V3<R> r{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
+                const R d = std::sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+                r_sum += mass * W(d, h);
+            });
+            if (nb > 0) {
+                bg.for_neighbors(pi, [&](u32 j) {
+                    const V3<R> pj = xb[j];
+                    const V3<R> r{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
+                    const R d = std::sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+                    r_sum += mb[j] * W(d, h);
+                });
+            }
+            rho[i] = r_sum;
+        }
+    }
+
+    // dfsph_kernels.cu:117-169
+    void alpha_factor() {
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)n; ++i) {
+            const V3<R> pi = x[i];
+            V3<R> sg{0, 0, 0};
+            R sq = 0;
+            fg.for_neighbors(pi, [&](u32 j) {
+                if (j == u32(i)) return;
+                const V3<R> pj = x[j];
+                const V3<R> g = gradW(V3<R>{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z}, h);
+                const V3<R> mg{mass * g.x, mass * g.y, mass * g.z};
+                sg.x += mg.x; sg.y += mg.y; sg.z += mg.z;
+                sq += mg.x * mg.x + mg.y * mg.y + mg.z * mg.z;
+            });
+            if (nb > 0) {
+                bg.for_neighbors(pi, [&](u32 j) {
+                    const V3<R> pj = xb[j];
+                    const V3<R> g = gradW(V3<R>{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z}, h);
+                    const V3<R> mg{mb[j] * g.x, mb[j] * g.y, mb[j] * g.z};
+                    sg.x += mg.x; sg.y += mg.y; sg.z += mg.z;
+                    sq += mg.x * mg.x + mg.y * mg.y + mg.z * mg.z;
+                });
+            }
+            const R denom = sg.x * sg.x + sg.y * sg.y + sg.z * sg.z + sq;
+            alpha[i] = (denom > k1em9<R>()) ? (rho[i] / denom) : R(0);
+        }
+    }
+
+    // dfsph_kernels.cu:347-355 with exp(-damping*dt) evaluated on the host
+    void gravity(float dt) {
+        R decay;
+        if (sizeof(R) == sizeof(float)) decay = R(std::exp(-cfg.damping * dt));
+        else decay = std::exp(-R(cfg.damping) * R(dt));
+        const R gx = R(cfg.gravity[0]), gy = R(cfg.gravity[1]), gz = R(cfg.gravity[2]);
+        const R dtr = R(dt);
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)n; ++i) {
+            v[i].x = v[i].x * decay + gx * dtr;
+            v[i].y = v[i].y * decay + gy * dtr;
+            v[i].z = v[i].z * decay + gz * dtr;
+        }
+    }
+
+    // dfsph_kernels.cu:305-341, as a Jacobi sweep
+    void xsph() {
+        if (cfg.viscosity <= 0.0f) return;  // dfsph.cu:198
+        const R eps = R(cfg.viscosity);
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)n; ++i) {
+            const V3<R> pi = x[i];
+            const V3<R> vi = v[i];
+            const R rho_i = rho[i];
+            V3<R> dl{0, 0, 0};
+            fg.for_neighbors(pi, [&](u32 j) {
+                if (j == u32(i)) return;
+                const V3<R> pj = x[j];
+                const V3<R> vj = v[j];
+                const V3<R> r{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
+                const R d = std::sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+                const R w = W(d, h);
+                const R rho_avg = R(0.5) * (rho_i + rho[j]);
+                const R c = eps * (mass / std::fmax(rho_avg, k1em9<R>())) * w;
+                dl.x += c * (vj.x - vi.x);
+                dl.y += c * (vj.y - vi.y);
+                dl.z += c * (vj.z - vi.z);
+            });
+            vtmp[i] = {vi.x + dl.x, vi.y + dl.y, vi.z + dl.z};
+        }
+        v.swap(vtmp);
+    }
+
+    // rho_dot_i (dfsph_kernels.cu:196-224)
+    R rho_dot(long long i) const {
+        const V3<R> pi = x[i];
+        const V3<R> vi = v[i];
+        R rd = 0;
+        fg.for_neighbors(pi, [&](u32 j) {
+            if (j == u32(i)) return;
+            const V3<R> pj = x[j];
+            const V3<R> vj = v[j];
+            const V3<R> g = gradW(V3<R>{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z}, h);
+            const V3<R> dv{vi.x - vj.x, vi.y - vj.y, vi.z - vj.z};
+            rd += mass * (dv.x * g.x + dv.y * g.y + dv.z * g.z);
+        });
+        if (nb > 0) {
+            bg.for_neighbors(pi, [&](u32 j) {
+                const V3<R> pj = xb[j];
+                const V3<R> g = gradW(V3<R>{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z}, h);
+                rd += mb[j] * (vi.x * g.x + vi.y * g.y + vi.z * g.z);
+            });
+        }
+        return rd;
+    }
+
+    // dfsph_kernels.cu:176-232
+    void density_change(float dt) {
+        const R dtr = R(dt), rho0 = R(cfg.rest_density);
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)n; ++i) {
+            const R rd = rho_dot(i);
+            const R rho_pred = rho[i] + dtr * rd;
+            rho_adv[i] = rho_pred;
+            const R over = std::fmax(R(0), rho_pred - rho0);
+            kappa[i] = over * alpha[i] / (dtr * dtr);
+        }
+    }
+
+    // divergence_change_kernel, phase-2 plan text (plan :1862-1907); boundary
+    // term as in the shipped density_change_kernel (dfsph_kernels.cu:213-224).
+    void divergence_change(float dt) {
+        const R dtr = R(dt);
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)n; ++i) {
+            const R rd = rho_dot(i);
+            rho_adv[i] = rd;
+            const R s = std::fmax(R(0), rd);
+            kappa_v[i] = s * alpha[i] / dtr;
+        }
+    }
+
+    // dfsph_kernels.cu:237-301 (pressure_form 1 replaces kappa/rho^2 by kappa/rho)
+    void apply_kappa(const std::vector<R>& kap, float dt) {
+        const R dtr = R(dt), rho0 = R(cfg.rest_density);
+        const bool bk = cfg.pressure_form == 1;
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)n; ++i) {
+            const V3<R> pi = x[i];
+            const R rho_i = rho[i];
+            const R k_i = kap[i];
+            const R ki = bk ? k_i / std::fmax(rho_i, k1em9<R>())
+                            : k_i / std::fmax(rho_i * rho_i, k1em9<R>());
+            V3<R> dl{0, 0, 0};
+            fg.for_neighbors(pi, [&](u32 j) {
+                if (j == u32(i)) return;
+                const V3<R> pj = x[j];
+                const R rho_j = rho[j];
+                const R k_j = kap[j];
+                const V3<R> g = gradW(V3<R>{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z}, h);
+                const R kj = bk ? k_j / std::fmax(rho_j, k1em9<R>())
+                                : k_j / std::fmax(rho_j * rho_j, k1em9<R>());
+                const R c2 = mass * (ki + kj);
+                dl.x -= dtr * c2 * g.x;
+                dl.y -= dtr * c2 * g.y;
+                dl.z -= dtr * c2 * g.z;
+            });
+            if (nb > 0) {
+                const R bf = bk ? k_i / std::fmax(rho0, k1em9<R>())
+                                : k_i / std::fmax(rho0 * rho0, k1em9<R>());
+                bg.for_neighbors(pi, [&](u32 j) {
+                    const V3<R> pj = xb[j];
+                    const V3<R> g = gradW(V3<R>{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z}, h);
+                    const R c2 = mb[j] * (ki + bf);
+                    dl.x -= dtr * c2 * g.x;
+                    dl.y -= dtr * c2 * g.y;
+                    dl.z -= dtr * c2 * g.z;
+                });
+            }
+            v[i].x += dl.x;
+            v[i].y += dl.y;
+            v[i].z += dl.z;
+        }
+    }
+
+    // mean(max(0, a_i - ref)), accumulated in binary64 (dfsph.cu:277-301)
+    double mean_excess(const std::vector<R>& a, R ref) const {
+        double s = 0;
+        for (size_t i = 0; i < n; ++i) s += double(std::fmax(R(0), a[i] - ref));
+        return s / double(n);
+    }
+
+    // dfsph.cu:215-260
+    void density_solve(float dt) {
+        const double thr = double(cfg.eta_density * cfg.rest_density);
+        st.density_iters = 0;
+        for (u32 it = 0; it < cfg.max_density_iters; ++it) {
+            density_change(dt);
+            apply_kappa(kappa, dt);
+            st.density_iters = it + 1;
+            const double err = mean_excess(rho_adv, R(cfg.rest_density));
+            st.mean_density_error = err;
+            if (it >= 1 && err < thr) break;
+        }
+    }
+
+    // divergence_solve, plan text :1967-1990
+    void divergence_solve(float dt) {
+        const double thr = double(cfg.eta_divergence * cfg.rest_density);
+        st.divergence_iters = 0;
+        for (u32 it = 0; it < cfg.max_divergence_iters; ++it) {
+            divergence_change(dt);
+            apply_kappa(kappa_v, dt);
+            st.divergence_iters = it + 1;
+            const double err = mean_excess(rho_adv, R(0));
+            st.mean_divergence_error = err;
+            if (err < thr) break;
+        }
+    }
+
+    // dfsph_kernels.cu:363-386 (dfsph.cu:262-273: |v| clamp 5 m/s)
+    void advect(float dt) {
+        const R dtr = R(dt), vmax = R(5.0f), eps = R(1e-3f);
+        const R lox = R(cfg.domain_min[0]) + eps, loy = R(cfg.domain_min[1]) + eps,
+                loz = R(cfg.domain_min[2]) + eps;
+        const R hix = R(cfg.domain_max[0]) - eps, hiy = R(cfg.domain_max[1]) - eps,
+                hiz = R(cfg.domain_max[2]) - eps;
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)n; ++i) {
+            V3<R> vv = v[i];
+            const R sp = std::sqrt(vv.x * vv.x + vv.y * vv.y + vv.z * vv.z);
+            if (sp > vmax) {
+                const R s = vmax / sp;
+                vv.x *= s; vv.y *= s; vv.z *= s;
+            }
+            V3<R> p{x[i].x + vv.x * dtr, x[i].y + vv.y * dtr, x[i].z + vv.z * dtr};
+            if (p.x < lox) { p.x = lox; if (vv.x < 0) vv.x = 0; }
+            if (p.y < loy) { p.y = loy; if (vv.y < 0) vv.y = 0; }
+            if (p.z < loz) { p.z = loz; if (vv.z < 0) vv.z = 0; }
+            if (p.x > hix) { p.x = hix; if (vv.x > 0) vv.x = 0; }
+            if (p.y > hiy) { p.y = hiy; if (vv.y > 0) vv.y = 0; }
+            if (p.z > hiz) { p.z = hiz; if (vv.z > 0) vv.z = 0; }
+            v[i] = vv;
+            x[i] = p;
+        }
+    }
+
+    // dfsph_kernels.cu:396-400 + CUB max (dfsph.cu:303-326)
+    void finish_stats() {
+        R m = 0;
+        u32 nan_count = 0;
+        for (size_t i = 0; i < n; ++i) {
+            const V3<R>& vv = v[i];
+            const R s = std::sqrt(vv.x * vv.x + vv.y * vv.y + vv.z * vv.z);
+            m = std::fmax(m, s);
+            if (!std::isfinite(x[i].x) || !std::isfinite(x[i].y) || !std::isfinite(x[i].z) ||
+                !std::isfinite(vv.x) || !std::isfinite(vv.y) || !std::isfinite(vv.z))
+                ++nan_count;
+        }
+        st.max_velocity = float(m);
+        st.nan_count = nan_count;
+    }
+
+    void density_alpha() {
+        fg.build(x, n);
+        density();
+        alpha_factor();
+    }
+
+    // dfsph.cu:120-133 (+ divergence solve after advect, plan text :1992-2003)
+    void step(float dt) {
+        if (n == 0) return;
+        density_alpha();
+        gravity(dt);
+        xsph();
+        density_solve(dt);
+        advect(dt);
+        if (cfg.enable_divergence_solve) {
+            density_alpha();
+            divergence_solve(dt);
+        }
+        finish_stats();
+    }
+};
+
+template <class R> void get_field(Solver<R>* s, int field, double* out) {
+    auto put3 = [&](const std::vector<V3<R>>& a) {
+        for (size_t i = 0; i < a.size(); ++i) {
+            out[3 * i] = double(a[i].x);
+            out[3 * i + 1] = double(a[i].y);
+            out[3 * i + 2] = double(a[i].z);
+        }
+    };
+    auto put1 = [&](const std::vector<R>& a) {
+        for (size_t i = 0; i < a.size(); ++i) out[i] = double(a[i]);
+    };
+    switch (field) {
+        case ORC_FIELD_X: put3(s->x); break;
+        case ORC_FIELD_V: put3(s->v); break;
+        case ORC_FIELD_RHO: put1(s->rho); break;
+        case ORC_FIELD_ALPHA: put1(s->alpha); break;
+        case ORC_FIELD_KAPPA: put1(s->kappa); break;
+        case ORC_FIELD_RHO_ADV: put1(s->rho_adv); break;
+        case ORC_FIELD_BMASS: put1(s->mb); break;
+        case ORC_FIELD_XB: put3(s->xb); break;
+        default: break;
+    }
+}
+
+struct Handle {
+    int precision;
+    void* impl;
+};
+
+// Bit spread of a 10-bit value: bit b moves to bit 3b (the expand step of
+// spatial_accel.cu:11-17, restated as a plain loop).
+u32 spread3(u32 v) {
+    u32 out = 0;
+    for (int b = 0; b < 10; ++b) out |= ((v >> b) & 1u) << (3 * b);
+    return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
+
+float orc_W_f32(float r, float l) { return W<float>(r, l); }
+double orc_W_f64(double r, double l) { return W<double>(r, l); }
+void orc_gradW_f32(const float r[3], float l, float out[3]) {
+    const V3<float> g = gradW<float>({r[0], r[1], r[2]}, l);
+    out[0] = g.x; out[1] = g.y; out[2] = g.z;
+}
+void orc_gradW_f64(const double r[3], double l, double out[3]) {
+    const V3<double> g = gradW<double>({r[0], r[1], r[2]}, l);
+    out[0] = g.x; out[1] = g.y; out[2] = g.z;
+}
+
+void orc_cell_keys(const float* xyz, size_t n, const float origin[3], const int32_t dims[3],
+                   float cell, uint32_t* keys) {
+    Grid<float> g;
+    g.init(origin, dims, cell);
+    for (size_t i = 0; i < n; ++i) keys[i] = g.key_of(load3<float>(xyz, i));
+}
+
+void orc_grid_build(const float* xyz, size_t n, const float origin[3], const int32_t dims[3],
+                    float cell, uint32_t* sorted_keys, uint32_t* perm, uint32_t* cell_start) {
+    Grid<float> g;
+    g.init(origin, dims, cell);
+    std::vector<V3<float>> x(n);
+    for (size_t i = 0; i < n; ++i) x[i] = load3<float>(xyz, i);
+    g.build(x, n);
+    if (n) {
+        std::memcpy(sorted_keys, g.sorted_keys.data(), n * sizeof(u32));
+        std::memcpy(perm, g.perm.data(), n * sizeof(u32));
+    }
+    std::memcpy(cell_start, g.cs.data(), g.cs.size() * sizeof(u32));
+}
+
+void orc_make_grid(const float dmin[3], const float dmax[3], float h, float origin[3],
+                   int32_t dims[3]) {
+    make_grid_params(dmin, dmax, h, origin, dims);
+}
+
+void orc_neighbor_counts(const float* xq, size_t nq, const float* xs, size_t ns,
+                         const float origin[3], const int32_t dims[3], float cell, float h,
+                         uint32_t* cand, uint32_t* nbr) {
+    Grid<float> g;
+    g.init(origin, dims, cell);
+    std::vector<V3<float>> src(ns);
+    for (size_t i = 0; i < ns; ++i) src[i] = load3<float>(xs, i);
+    g.build(src, ns);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)nq; ++i) {
+        const V3<float> p = load3<float>(xq, i);
+        u32 c = 0, k = 0;
+        g.for_neighbors(p, [&](u32 j) {
+            ++c;
+            const V3<float> r{p.x - src[j].x, p.y - src[j].y, p.z - src[j].z};
+            const float d = std::sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+            if (d / h < 2.0f) ++k;
+        });
+        cand[i] = c;
+        nbr[i] = k;
+    }
+}
+
+// spatial_accel.cu:30-40 (codes), :139-148 (host inverse extent), :158-171 (sort)
+void orc_morton(const float* xyz, size_t n, const float bmin[3], const float bmax[3],
+                uint32_t* codes, uint32_t* sorted_codes, uint32_t* perm) {
+    float inv[3];
+    for (int a = 0; a < 3; ++a) {
+        const float e = bmax[a] - bmin[a];
+        inv[a] = e > 0 ? 1.0f / e : 0.0f;
+    }
+    std::vector<u32> c(n);
+    for (size_t i = 0; i < n; ++i) {
+        u32 q[3];
+        for (int a = 0; a < 3; ++a) {
+            const float t = (xyz[3 * i + a] - bmin[a]) * inv[a];
+            const float s = std::fmin(std::fmax(t * 1024.0f, 0.0f), 1023.0f);
+            q[a] = static_cast<u32>(s);
+        }
+        c[i] = (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
+        if (codes) codes[i] = c[i];
+    }
+    std::vector<u32> idx(n);
+    for (size_t i = 0; i < n; ++i) idx[i] = u32(i);
+    std::stable_sort(idx.begin(), idx.end(), [&](u32 a, u32 b) { return c[a] < c[b]; });
+    for (size_t i = 0; i < n; ++i) {
+        if (perm) perm[i] = idx[i];
+        if (sorted_codes) sorted_codes[i] = c[idx[i]];
+    }
+}
+
+// boundary_sampler.cpp:6-32: six faces (-X,+X,-Y,+Y,-Z,+Z), n_layers inward
+// layers each, binary32 accumulating loops over the two in-face axes.
+size_t orc_sample_aabb_boundary(const float lo[3], const float hi[3], float spacing,
+                                int n_layers, float* out, size_t cap) {
+    size_t count = 0;
+    const int axes[6] = {0, 0, 1, 1, 2, 2};
+    for (int f = 0; f < 6; ++f) {
+        const int axis = axes[f];
+        const bool high = (f & 1) != 0;
+        const float face = high ? hi[axis] : lo[axis];
+        const float step = high ? -spacing : spacing;
+        const int ua = (axis + 1) % 3, va = (axis + 2) % 3;
+        for (int layer = 0; layer < n_layers; ++layer) {
+            const float apos = face + layer * step;
+            for (float u = lo[ua]; u <= hi[ua] + 0.5f * spacing; u += spacing)
+                for (float w = lo[va]; w <= hi[va] + 0.5f * spacing; w += spacing) {
+                    if (out && count < cap) {
+                        float* p = out + 3 * count;
+                        p[axis] = apos;
+                        p[ua] = u;
+                        p[va] = w;
+                    }
+                    ++count;
+                }
+        }
+    }
+    return count;
+}
+
+// main.cpp:65-74 / test_settled_puddle.cu:24-32: x-major, binary32 accumulation
+size_t orc_lattice_block(const float lo[3], const float hi[3], float spacing, float* out,
+                         size_t cap) {
+    size_t count = 0;
+    for (float a = lo[0]; a <= hi[0]; a += spacing)
+        for (float b = lo[1]; b <= hi[1]; b += spacing)
+            for (float c = lo[2]; c <= hi[2]; c += spacing) {
+                if (out && count < cap) {
+                    out[3 * count] = a;
+                    out[3 * count + 1] = b;
+                    out[3 * count + 2] = c;
+                }
+                ++count;
+            }
+    return count;
+}
+
+// timestep.h:23-28
+float orc_next_dt(float frame_dt, float dt_max, float cfl_lambda, uint32_t max_substeps,
+                  float max_velocity, float particle_radius, float remaining) {
+    const float v = std::max(max_velocity, 1e-6f);
+    const float dt_cfl = cfl_lambda * particle_radius / v;
+    const float dt_min = frame_dt / static_cast<float>(max_substeps);
+    return std::min({dt_max, dt_cfl, remaining, std::max(dt_cfl, dt_min)});
+}
+
+// dfsph.h:19-33 defaults
+void orc_config_default(orc_config* c) {
+    std::memset(c, 0, sizeof(*c));
+    c->rest_density = 1000.0f;
+    c->particle_radius = 0.0015f;
+    c->smoothing_length = 0.003f;
+    c->viscosity = 1e-3f;
+    c->damping = 0.5f;
+    c->surface_tension = 0.0f;
+    c->gravity[0] = 0.0f; c->gravity[1] = -9.81f; c->gravity[2] = 0.0f;
+    c->max_density_iters = 100;
+    c->max_divergence_iters = 100;
+    c->eta_density = 1e-3f;
+    c->eta_divergence = 1e-3f;
+    c->domain_min[0] = c->domain_min[1] = c->domain_min[2] = 0.0f;
+    c->domain_max[0] = c->domain_max[1] = c->domain_max[2] = 1.0f;
+    c->enable_divergence_solve = 0;
+    c->pressure_form = 0;
+}
+
+void* orc_solver_create(const orc_config* cfg, int precision, const float* x, const float* v,
+                        size_t n, const float* xb, size_t nb) {
+    Handle* h = new Handle{precision, nullptr};
+    if (precision == ORC_F64) h->impl = new Solver<double>(*cfg, x, v, n, xb, nb);
+    else h->impl = new Solver<float>(*cfg, x, v, n, xb, nb);
+    return h;
+}
+
+void orc_solver_destroy(void* s) {
+    Handle* h = static_cast<Handle*>(s);
+    if (!h) return;
+    if (h->precision == ORC_F64) delete static_cast<Solver<double>*>(h->impl);
+    else delete static_cast<Solver<float>*>(h->impl);
+    delete h;
+}
+
+#define ORC_DISPATCH(s, call)                                         \
+    do {                                                              \
+        Handle* h_ = static_cast<Handle*>(s);                         \
+        if (h_->precision == ORC_F64) {                               \
+            auto* S = static_cast<Solver<double>*>(h_->impl);         \
+            call;                                                     \
+        } else {                                                      \
+            auto* S = static_cast<Solver<float>*>(h_->impl);          \
+            call;                                                     \
+        }                                                             \
+    } while (0)
+
+void orc_solver_step(void* s, float dt) { ORC_DISPATCH(s, S->step(dt)); }
+void orc_solver_density_alpha(void* s) { ORC_DISPATCH(s, S->density_alpha()); }
+
+void orc_solver_set_state(void* s, const float* x, const float* v) {
+    ORC_DISPATCH(s, {
+        using RR = std::remove_reference_t<decltype(S->mass)>;
+        for (size_t i = 0; i < S->n; ++i) {
+            if (x) S->x[i] = load3<RR>(x, i);
+            if (v) S->v[i] = load3<RR>(v, i);
+        }
+    });
+}
+
+void orc_solver_get(void* s, int field, double* out) { ORC_DISPATCH(s, get_field(S, field, out)); }
+
+void orc_solver_get_f32(void* s, int field, float* out) {
+    Handle* h = static_cast<Handle*>(s);
+    if (h->precision != ORC_F32) return;
+    auto* S = static_cast<Solver<float>*>(h->impl);
+    auto put3 = [&](const std::vector<V3<float>>& a) {
+        std::memcpy(out, a.data(), a.size() * sizeof(V3<float>));
+    };
+    auto put1 = [&](const std::vector<float>& a) {
+        std::memcpy(out, a.data(), a.size() * sizeof(float));
+    };
+    switch (field) {
+        case ORC_FIELD_X: put3(S->x); break;
+        case ORC_FIELD_V: put3(S->v); break;
+        case ORC_FIELD_RHO: put1(S->rho); break;
+        case ORC_FIELD_ALPHA: put1(S->alpha); break;
+        case ORC_FIELD_KAPPA: put1(S->kappa); break;
+        case ORC_FIELD_RHO_ADV: put1(S->rho_adv); break;
+        case ORC_FIELD_BMASS: put1(S->mb); break;
+        case ORC_FIELD_XB: put3(S->xb); break;
+        default: break;
+    }
+}
+
+void orc_solver_stats(void* s, orc_stats* out) { ORC_DISPATCH(s, *out = S->st); }
+float orc_solver_particle_mass(void* s) {
+    float m = 0;
+    ORC_DISPATCH(s, m = float(S->mass));
+    return m;
+}
+size_t orc_solver_count(void* s) {
+    size_t n = 0;
+    ORC_DISPATCH(s, n = S->n);
+    return n;
+}
+
+}  // extern "C"

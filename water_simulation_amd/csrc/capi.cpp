@@ -1,0 +1,500 @@
+// capi.cpp -- the extern "C" boundary of libwsim (include/wsim.h).
+//
+// Every entry point converts exceptions into status codes and a thread-local
+// message; nothing C++ crosses the boundary.
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/wsim.h"
+#include "host_surface.hpp"
+#include "wsim_internal.hpp"
+
+struct wsim_store_t {
+    std::unique_ptr<wsim::ParticleStore> impl;
+};
+struct wsim_grid_t {
+    std::unique_ptr<wsim::CellGrid> owned;
+    const wsim::CellGrid* view = nullptr;
+    wsim::CellGrid* mut() { return owned.get(); }
+};
+struct wsim_accel_t {
+    wsim::SpatialAccel impl;
+};
+struct wsim_solver_t {
+    std::unique_ptr<wsim::DFSPHSolver> impl;
+    wsim_grid_t fluid_view;
+    wsim_grid_t boundary_view;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F> wsim_status guarded(F&& f) {
+    try {
+        f();
+        return WSIM_OK;
+    } catch (const wsim::Error& e) {
+        g_last_error = e.what();
+        return static_cast<wsim_status>(e.status);
+    } catch (const std::bad_alloc&) {
+        g_last_error = "out of host memory";
+        return WSIM_ERR_RUNTIME;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return WSIM_ERR_RUNTIME;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return WSIM_ERR_RUNTIME;
+    }
+}
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+void need(const void* p, const char* what) {
+    if (!p) wsim::fail(wsim::kInvalidArgument, std::string(what) + " is NULL");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* wsim_last_error(void) { return g_last_error.c_str(); }
+int wsim_abi_version(void) { return WSIM_ABI_VERSION; }
+
+// ---- device memory ----
+wsim_status wsim_device_count(int* count) {
+    return guarded([&] {
+        need(count, "count");
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+        *count = n;
+    });
+}
+wsim_status wsim_device_alloc(size_t bytes, void** out) {
+    return guarded([&] {
+        need(out, "out");
+        *out = nullptr;
+        if (bytes) WSIM_HIP(hipMalloc(out, bytes));
+    });
+}
+wsim_status wsim_device_free(void* ptr) {
+    return guarded([&] {
+        if (ptr) WSIM_HIP(hipFree(ptr));
+    });
+}
+wsim_status wsim_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+    return guarded([&] {
+        if (!bytes) return;
+        need(dst, "dst");
+        need(src, "src");
+        WSIM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+        WSIM_HIP(hipStreamSynchronize(as_stream(stream)));
+    });
+}
+wsim_status wsim_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+    return guarded([&] {
+        if (!bytes) return;
+        need(dst, "dst");
+        need(src, "src");
+        WSIM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+        WSIM_HIP(hipStreamSynchronize(as_stream(stream)));
+    });
+}
+wsim_status wsim_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
+    return guarded([&] {
+        if (!bytes) return;
+        need(dst, "dst");
+        need(src, "src");
+        WSIM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
+    });
+}
+wsim_status wsim_memset(void* dst, int value, size_t bytes, void* stream) {
+    return guarded([&] {
+        if (!bytes) return;
+        need(dst, "dst");
+        WSIM_HIP(hipMemsetAsync(dst, value, bytes, as_stream(stream)));
+    });
+}
+wsim_status wsim_stream_create(void** out_stream) {
+    return guarded([&] {
+        need(out_stream, "out_stream");
+        hipStream_t s;
+        WSIM_HIP(hipStreamCreate(&s));
+        *out_stream = reinterpret_cast<void*>(s);
+    });
+}
+wsim_status wsim_stream_destroy(void* stream) {
+    return guarded([&] {
+        if (stream) WSIM_HIP(hipStreamDestroy(as_stream(stream)));
+    });
+}
+wsim_status wsim_stream_synchronize(void* stream) {
+    return guarded([&] { WSIM_HIP(hipStreamSynchronize(as_stream(stream))); });
+}
+wsim_status wsim_device_synchronize(void) {
+    return guarded([&] { WSIM_HIP(hipDeviceSynchronize()); });
+}
+
+// ---- ParticleStore ----
+wsim_status wsim_store_create(size_t capacity, void* stream, wsim_store* out) {
+    return guarded([&] {
+        need(out, "out");
+        *out = nullptr;
+        auto h = std::make_unique<wsim_store_t>();
+        h->impl = std::make_unique<wsim::ParticleStore>(capacity, as_stream(stream));
+        *out = h.release();
+    });
+}
+void wsim_store_destroy(wsim_store store) { delete store; }
+wsim_status wsim_store_resize(wsim_store store, size_t count) {
+    return guarded([&] {
+        need(store, "store");
+        store->impl->resize(count);
+    });
+}
+size_t wsim_store_count(wsim_store store) { return store ? store->impl->count() : 0; }
+size_t wsim_store_capacity(wsim_store store) { return store ? store->impl->capacity() : 0; }
+wsim_status wsim_store_register_attribute(wsim_store store, const char* name, wsim_attrib_type type,
+                                          size_t* handle) {
+    return guarded([&] {
+        need(store, "store");
+        need(name, "name");
+        const size_t h = store->impl->register_attribute(name, static_cast<int>(type));
+        if (handle) *handle = h;
+    });
+}
+int wsim_store_has_attribute(wsim_store store, const char* name) {
+    return (store && name) ? (store->impl->has_attribute(name) ? 1 : 0) : 0;
+}
+wsim_status wsim_store_find_attribute(wsim_store store, const char* name, size_t* handle) {
+    return guarded([&] {
+        need(store, "store");
+        need(name, "name");
+        need(handle, "handle");
+        *handle = store->impl->find_attribute(name);
+    });
+}
+wsim_status wsim_store_attribute_data(wsim_store store, size_t handle, void** out) {
+    return guarded([&] {
+        need(store, "store");
+        need(out, "out");
+        *out = store->impl->attribute_data(handle);
+    });
+}
+wsim_status wsim_store_positions(wsim_store store, float** out) {
+    return guarded([&] {
+        need(store, "store");
+        need(out, "out");
+        *out = store->impl->positions();
+    });
+}
+wsim_status wsim_store_velocities(wsim_store store, float** out) {
+    return guarded([&] {
+        need(store, "store");
+        need(out, "out");
+        *out = store->impl->velocities();
+    });
+}
+size_t wsim_store_position_handle(wsim_store store) {
+    return store ? store->impl->position_handle() : 0;
+}
+size_t wsim_store_velocity_handle(wsim_store store) {
+    return store ? store->impl->velocity_handle() : 0;
+}
+
+// ---- CellGrid ----
+wsim_status wsim_grid_create(const float origin[3], const int32_t dims[3], float cell_length,
+                             void* stream, wsim_grid* out) {
+    return guarded([&] {
+        need(out, "out");
+        need(origin, "origin");
+        need(dims, "cells_per_axis");
+        *out = nullptr;
+        auto h = std::make_unique<wsim_grid_t>();
+        h->owned = std::make_unique<wsim::CellGrid>(origin, dims, cell_length, as_stream(stream));
+        h->view = h->owned.get();
+        *out = h.release();
+    });
+}
+void wsim_grid_destroy(wsim_grid grid) {
+    if (grid && grid->owned) delete grid;
+}
+wsim_status wsim_grid_reserve(wsim_grid grid, size_t capacity) {
+    return guarded([&] {
+        need(grid, "grid");
+        if (!grid->mut()) wsim::fail(wsim::kInvalidArgument, "grid is a read-only solver view");
+        grid->mut()->reserve(capacity);
+    });
+}
+wsim_status wsim_grid_build(wsim_grid grid, const float* positions_xyz, size_t count) {
+    return guarded([&] {
+        need(grid, "grid");
+        if (!grid->mut()) wsim::fail(wsim::kInvalidArgument, "grid is a read-only solver view");
+        if (count) need(positions_xyz, "positions");
+        grid->mut()->build(positions_xyz, count);
+    });
+}
+const uint32_t* wsim_grid_cell_start(wsim_grid g) { return g ? g->view->cell_start() : nullptr; }
+const uint32_t* wsim_grid_sorted_indices(wsim_grid g) {
+    return (g && g->view->capacity()) ? g->view->sorted_indices() : nullptr;
+}
+const uint32_t* wsim_grid_sorted_keys(wsim_grid g) {
+    return (g && g->view->capacity()) ? g->view->sorted_keys() : nullptr;
+}
+void wsim_grid_origin(wsim_grid g, float out[3]) {
+    if (!g || !out) return;
+    out[0] = g->view->geom().ox;
+    out[1] = g->view->geom().oy;
+    out[2] = g->view->geom().oz;
+}
+void wsim_grid_cells_per_axis(wsim_grid g, int32_t out[3]) {
+    if (!g || !out) return;
+    out[0] = g->view->geom().dx;
+    out[1] = g->view->geom().dy;
+    out[2] = g->view->geom().dz;
+}
+float wsim_grid_cell_length(wsim_grid g) { return g ? g->view->cell_length() : 0.0f; }
+uint32_t wsim_grid_total_cells(wsim_grid g) { return g ? g->view->total_cells() : 0u; }
+size_t wsim_grid_count(wsim_grid g) { return g ? g->view->count() : 0; }
+
+// ---- SpatialAccel (Morton) ----
+wsim_status wsim_accel_create(wsim_accel* out) {
+    return guarded([&] {
+        need(out, "out");
+        *out = new wsim_accel_t();
+    });
+}
+void wsim_accel_destroy(wsim_accel a) { delete a; }
+wsim_status wsim_accel_build(wsim_accel a, const float* positions_xyz, size_t count,
+                             const float bmin[3], const float bmax[3], void* stream) {
+    return guarded([&] {
+        need(a, "accel");
+        need(bmin, "bounds_min");
+        need(bmax, "bounds_max");
+        if (count) need(positions_xyz, "positions");
+        a->impl.build(positions_xyz, count, bmin, bmax, as_stream(stream));
+    });
+}
+size_t wsim_accel_leaf_count(wsim_accel a) { return a ? a->impl.leaf_count() : 0; }
+const uint32_t* wsim_accel_sorted_indices(wsim_accel a) {
+    return a ? a->impl.sorted_indices() : nullptr;
+}
+const uint32_t* wsim_accel_morton_codes(wsim_accel a) { return a ? a->impl.morton_codes() : nullptr; }
+
+// ---- DFSPHSolver ----
+void wsim_dfsph_config_default(wsim_dfsph_config* c) {
+    if (!c) return;
+    const wsim::DFSPHConfig d;
+    c->rest_density = d.rest_density;
+    c->particle_radius = d.particle_radius;
+    c->smoothing_length = d.smoothing_length;
+    c->viscosity = d.viscosity;
+    c->damping = d.damping;
+    c->surface_tension = d.surface_tension;
+    for (int a = 0; a < 3; ++a) {
+        c->gravity[a] = d.gravity[a];
+        c->domain_min[a] = d.domain_min[a];
+        c->domain_max[a] = d.domain_max[a];
+    }
+    c->max_density_iters = d.max_density_iters;
+    c->max_divergence_iters = d.max_divergence_iters;
+    c->eta_density = d.eta_density;
+    c->eta_divergence = d.eta_divergence;
+    c->enable_divergence_solve = d.enable_divergence_solve;
+    c->pressure_form = d.pressure_form;
+}
+
+wsim_status wsim_solver_create(wsim_store fluid, wsim_store boundary, const wsim_dfsph_config* c,
+                               void* stream, wsim_solver* out) {
+    return guarded([&] {
+        need(out, "out");
+        need(fluid, "fluid");
+        need(c, "cfg");
+        *out = nullptr;
+        if (c->pressure_form > 1) wsim::fail(wsim::kInvalidArgument, "pressure_form must be 0 or 1");
+        wsim::DFSPHConfig d;
+        d.rest_density = c->rest_density;
+        d.particle_radius = c->particle_radius;
+        d.smoothing_length = c->smoothing_length;
+        d.viscosity = c->viscosity;
+        d.damping = c->damping;
+        d.surface_tension = c->surface_tension;
+        for (int a = 0; a < 3; ++a) {
+            d.gravity[a] = c->gravity[a];
+            d.domain_min[a] = c->domain_min[a];
+            d.domain_max[a] = c->domain_max[a];
+        }
+        d.max_density_iters = c->max_density_iters;
+        d.max_divergence_iters = c->max_divergence_iters;
+        d.eta_density = c->eta_density;
+        d.eta_divergence = c->eta_divergence;
+        d.enable_divergence_solve = c->enable_divergence_solve;
+        d.pressure_form = c->pressure_form;
+        auto h = std::make_unique<wsim_solver_t>();
+        h->impl = std::make_unique<wsim::DFSPHSolver>(
+            *fluid->impl, boundary ? boundary->impl.get() : nullptr, d, as_stream(stream));
+        h->fluid_view.view = &h->impl->fluid_grid();
+        h->boundary_view.view = h->impl->boundary_grid();
+        *out = h.release();
+    });
+}
+void wsim_solver_destroy(wsim_solver s) { delete s; }
+wsim_status wsim_solver_step(wsim_solver s, float dt) {
+    return guarded([&] {
+        need(s, "solver");
+        s->impl->step(dt);
+    });
+}
+float wsim_solver_max_velocity(wsim_solver s) { return s ? s->impl->max_velocity() : 0.0f; }
+float wsim_solver_particle_mass(wsim_solver s) { return s ? s->impl->particle_mass() : 0.0f; }
+uint32_t wsim_solver_last_density_iters(wsim_solver s) {
+    return s ? s->impl->last_density_iters() : 0u;
+}
+static wsim_status attr_at(wsim_solver s, int which, size_t i, float* out) {
+    return guarded([&] {
+        need(s, "solver");
+        need(out, "out");
+        *out = s->impl->attribute_at(which, i);
+    });
+}
+wsim_status wsim_solver_density_at(wsim_solver s, size_t i, float* out) { return attr_at(s, 0, i, out); }
+wsim_status wsim_solver_alpha_at(wsim_solver s, size_t i, float* out) { return attr_at(s, 1, i, out); }
+wsim_status wsim_solver_boundary_mass_at(wsim_solver s, size_t i, float* out) {
+    return attr_at(s, 2, i, out);
+}
+wsim_status wsim_solver_stats(wsim_solver s, wsim_step_stats* out) {
+    return guarded([&] {
+        need(s, "solver");
+        need(out, "out");
+        const wsim::StepStats& st = s->impl->stats();
+        out->density_iters = st.density_iters;
+        out->divergence_iters = st.divergence_iters;
+        out->mean_density_error = st.mean_density_error;
+        out->mean_divergence_error = st.mean_divergence_error;
+        out->max_velocity = st.max_velocity;
+        out->nan_count = st.nan_count;
+        out->substeps = st.substeps;
+    });
+}
+wsim_grid wsim_solver_fluid_grid(wsim_solver s) { return s ? &s->fluid_view : nullptr; }
+wsim_grid wsim_solver_boundary_grid(wsim_solver s) {
+    return (s && s->boundary_view.view) ? &s->boundary_view : nullptr;
+}
+wsim_status wsim_solver_set_timing(wsim_solver s, int enable) {
+    return guarded([&] {
+        need(s, "solver");
+        s->impl->set_timing(enable != 0);
+    });
+}
+wsim_status wsim_solver_reset_timing(wsim_solver s) {
+    return guarded([&] {
+        need(s, "solver");
+        s->impl->timer().reset();
+    });
+}
+int wsim_solver_kernel_kinds(void) { return wsim::kKNumKinds; }
+const char* wsim_solver_kernel_name(int kind) { return wsim::kernel_kind_name(kind); }
+wsim_status wsim_solver_kernel_time(wsim_solver s, int kind, uint64_t* launches, double* total_ms) {
+    return guarded([&] {
+        need(s, "solver");
+        if (kind < 0 || kind >= wsim::kKNumKinds) wsim::fail(wsim::kOutOfRange, "kernel kind");
+        const auto& a = s->impl->timer().acc[kind];
+        if (launches) *launches = a.count;
+        if (total_ms) *total_ms = a.total_ms;
+    });
+}
+
+// ---- host surface ----
+wsim_status wsim_sample_aabb_boundary(const float lo[3], const float hi[3], float spacing,
+                                      int n_layers, float* out, size_t capacity, size_t* count) {
+    return guarded([&] {
+        need(lo, "lo");
+        need(hi, "hi");
+        const auto pts = wsim::sample_aabb_boundary(lo, hi, spacing, n_layers);
+        if (count) *count = pts.size();
+        if (out) {
+            const size_t m = pts.size() < capacity ? pts.size() : capacity;
+            std::memcpy(out, pts.data(), m * sizeof(wsim::Vec3));
+        }
+    });
+}
+void wsim_timestep_config_default(wsim_timestep_config* c) {
+    if (!c) return;
+    const wsim::TimeStepConfig d;
+    c->frame_dt = d.frame_dt;
+    c->dt_max = d.dt_max;
+    c->cfl_lambda = d.cfl_lambda;
+    c->max_substeps = d.max_substeps;
+}
+float wsim_timestep_next_dt(const wsim_timestep_config* c, float max_velocity,
+                            float particle_radius, float remaining) {
+    wsim::TimeStepConfig d;
+    if (c) {
+        d.frame_dt = c->frame_dt;
+        d.dt_max = c->dt_max;
+        d.cfl_lambda = c->cfl_lambda;
+        d.max_substeps = c->max_substeps;
+    }
+    return wsim::next_dt(d, max_velocity, particle_radius, remaining);
+}
+wsim_status wsim_lattice_block(const float lo[3], const float hi[3], float spacing, float* out,
+                               size_t capacity, size_t* count) {
+    return guarded([&] {
+        need(lo, "lo");
+        need(hi, "hi");
+        const auto pts = wsim::lattice_block(lo, hi, spacing);
+        if (count) *count = pts.size();
+        if (out) {
+            const size_t m = pts.size() < capacity ? pts.size() : capacity;
+            std::memcpy(out, pts.data(), m * sizeof(wsim::Vec3));
+        }
+    });
+}
+wsim_status wsim_write_frame(const char* path, const float* xyz, size_t count) {
+    return guarded([&] {
+        need(path, "path");
+        if (count) need(xyz, "positions");
+        wsim::write_frame(path, xyz, count);
+    });
+}
+
+static void copy_str(char* dst, size_t cap, const std::string& s) {
+    const size_t m = s.size() < cap - 1 ? s.size() : cap - 1;
+    std::memcpy(dst, s.data(), m);
+    dst[m] = '\0';
+}
+
+wsim_status wsim_scene_load(const char* path, wsim_scene* out) {
+    return guarded([&] {
+        need(path, "path");
+        need(out, "out");
+        const wsim::Scene s = wsim::load_scene(path);
+        std::memset(out, 0, sizeof(*out));
+        copy_str(out->name, sizeof(out->name), s.name);
+        out->width = s.width;
+        out->height = s.height;
+        out->fps = s.fps;
+        out->frame_start = s.frame_start;
+        out->frame_end = s.frame_end;
+        copy_str(out->format, sizeof(out->format), s.format);
+        copy_str(out->video, sizeof(out->video), s.video);
+        for (int a = 0; a < 3; ++a) {
+            out->cam_position[a] = s.cam_position[a];
+            out->cam_look_at[a] = s.cam_look_at[a];
+            out->cam_up[a] = s.cam_up[a];
+            out->gravity[a] = s.gravity[a];
+            out->initial_block_min[a] = s.initial_block_min[a];
+            out->initial_block_max[a] = s.initial_block_max[a];
+        }
+        out->cam_fov_y_deg = s.cam_fov_y_deg;
+        copy_str(out->solver, sizeof(out->solver), s.solver);
+        out->particle_radius = s.particle_radius;
+        out->rest_density = s.rest_density;
+        out->viscosity = s.viscosity;
+        out->surface_tension = s.surface_tension;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,188 @@
+// device_math.hpp -- device helpers shared by the grid, sort and solver kernels.
+//
+// Every floating-point expression keeps the operand order of the reference
+// (core/include/water/core/sph_kernels.cuh, solvers/dfsph/src/dfsph_kernels.cu)
+// and the library is compiled with -ffp-contract=off and IEEE divide/sqrt, so
+// each value equals the binary32 result of the same expression on the host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace wsim {
+
+using u32 = uint32_t;
+using u64 = uint64_t;
+
+constexpr int kWave = 64;
+constexpr int kXcds = 8;  // MI355X: 8 accelerator complex dies, one L2 each
+
+// Geometry of a uniform grid (cell_grid.h:16-50): origin, host-computed
+// inverse cell length (cell_grid.cu:81) and cells per axis.
+struct GridGeom {
+    float ox, oy, oz;
+    float inv;  // 1.0f / cell_length, IEEE, computed once on the host
+    int dx, dy, dz;
+    u32 C;      // total cells == sentinel key
+
+    // (int)floorf((p - o) * inv) (cell_grid.cu:11-13), made total: NaN and
+    // coordinates beyond the stencil's reach clamp to a value whose whole
+    // 3x3x3 stencil lies outside the grid.
+    __host__ __device__ static inline int coord(float p, float o, float inv, int D) {
+        const float f = floorf((p - o) * inv);
+        if (!(f >= -2.0f)) return -2;
+        if (f > static_cast<float>(D + 1)) return D + 1;
+        return static_cast<int>(f);
+    }
+    __host__ __device__ inline int cx(float p) const { return coord(p, ox, inv, dx); }
+    __host__ __device__ inline int cy(float p) const { return coord(p, oy, inv, dy); }
+    __host__ __device__ inline int cz(float p) const { return coord(p, oz, inv, dz); }
+
+    // Linear x-major key (cx*Dy + cy)*Dz + cz; out of range -> sentinel C
+    // (cell_grid.cu:14-21).
+    __host__ __device__ inline u32 key(int x, int y, int z) const {
+        if (x < 0 || x >= dx || y < 0 || y >= dy || z < 0 || z >= dz) return C;
+        return (static_cast<u32>(x) * static_cast<u32>(dy) + static_cast<u32>(y)) *
+                   static_cast<u32>(dz) +
+               static_cast<u32>(z);
+    }
+    __host__ __device__ inline u32 key_of(float px, float py, float pz) const {
+        return key(cx(px), cy(py), cz(pz));
+    }
+};
+
+// Constants of the cubic spline for one smoothing length l, hoisted out of
+// the pair loop.  sigma = 1/(pi*l*l*l) is the same binary32 value the
+// reference recomputes per call (sph_kernels.cuh:15,31).
+struct Spline {
+    float l;
+    float sigma;
+    float sigma_m075;  // sigma * (-0.75f), the left product of sph_kernels.cuh:37
+    float sigma_025;   // sigma * 0.25f, the left product of sph_kernels.cuh:20
+    static Spline make(float l) {
+        Spline s;
+        s.l = l;
+        s.sigma = 1.0f / (3.14159265358979323846f * l * l * l);
+        s.sigma_m075 = s.sigma * (-0.75f);
+        s.sigma_025 = s.sigma * 0.25f;
+        return s;
+    }
+};
+
+// W(d, l) for d = sqrtf(r.r) (sph_kernels.cuh:12-22).
+__device__ inline float spline_W(float d, const Spline& sp) {
+    const float q = d / sp.l;
+    if (q >= 2.0f) return 0.0f;
+    if (q < 1.0f) return sp.sigma * (1.0f - 1.5f * q * q + 0.75f * q * q * q);
+    const float t = 2.0f - q;
+    return sp.sigma_025 * t * t * t;
+}
+
+// Scalar factor c of grad W = r * c (sph_kernels.cuh:25-41) given d = sqrtf(r.r);
+// returns 0 where the reference returns the zero vector.
+__device__ inline float spline_gradc(float d, const Spline& sp) {
+    if (d < 1e-12f) return 0.0f;
+    const float q = d / sp.l;
+    if (q >= 2.0f) return 0.0f;
+    float dWdq;
+    if (q < 1.0f) {
+        dWdq = sp.sigma * (-3.0f * q + 2.25f * q * q);
+    } else {
+        const float t = 2.0f - q;
+        dWdq = sp.sigma_m075 * t * t;
+    }
+    return dWdq / (sp.l * d);
+}
+
+__device__ inline float norm3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
+
+// Workgroup id remap so that each XCD walks a contiguous run of tiles: the
+// dispatcher hands workgroup b to XCD (b % 8), so neighbouring tiles (which
+// gather the same cell-sorted particles) land in the same L2.
+__device__ inline u32 xcd_tile(u32 b, u32 nblocks) {
+    const u32 xcd = b % kXcds, k = b / kXcds;
+    const u32 q = nblocks / kXcds, rem = nblocks % kXcds;
+    return xcd < rem ? xcd * (q + 1) + k : rem * (q + 1) + (xcd - rem) * q + k;
+}
+
+// ---- wave64 reductions (shuffle based) ----
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ inline float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline u32 wave_sum_u32(u32 v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Block reductions for blockDim.x == NT (multiple of 64).  Deterministic:
+// fixed lane/wave tree.  Result valid in thread 0.
+template <int NT> __device__ inline double block_sum(double v, double* s) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) s[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < NT / 64; ++i) r += s[i];
+    }
+    __syncthreads();
+    return r;
+}
+template <int NT> __device__ inline float block_max(float v, float* s) {
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) s[w] = v;
+    __syncthreads();
+    float r = 0.0f;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < NT / 64; ++i) r = fmaxf(r, s[i]);
+    }
+    __syncthreads();
+    return r;
+}
+template <int NT> __device__ inline u32 block_sum_u32(u32 v, u32* s) {
+    v = wave_sum_u32(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) s[w] = v;
+    __syncthreads();
+    u32 r = 0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < NT / 64; ++i) r += s[i];
+    }
+    __syncthreads();
+    return r;
+}
+
+// Agent-scope loads that bypass the (non-coherent) per-CU vector cache; used
+// by the last workgroup of a launch to read the partials of the others.
+template <class T> __device__ inline T load_agent(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// "Last workgroup done" handshake: every workgroup publishes its partial,
+// fences, and takes a ticket; the workgroup that draws the last ticket
+// reduces all partials in a fixed order and resets the counter for the next
+// launch.  Returns true in all threads of the last workgroup.
+__device__ inline bool last_block_arrive(u32* counter, u32 nblocks, bool* s_flag) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const u32 t = atomicAdd(counter, 1u);
+        *s_flag = (t == nblocks - 1);
+    }
+    __syncthreads();
+    const bool last = *s_flag;
+    if (last) __threadfence();
+    return last;
+}
+
+}  // namespace wsim

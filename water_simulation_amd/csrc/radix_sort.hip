@@ -1,0 +1,240 @@
+// radix_sort.hip -- stable LSD radix sort of (u32 key, u32 value) pairs for
+// gfx950, 8-bit digits.
+//
+// Replaces cub::DeviceRadixSort::SortPairs at core/src/cell_grid.cu:98-110
+// and core/src/spatial_accel.cu:163-171.  With values = iota the output
+// permutation is the lexicographic (key, original index) order, i.e. exactly
+// the permutation a stable radix sort produces, bit for bit.
+//
+// Per digit pass (reduce-then-scan):
+//   rs_upsweep   -- per tile digit histogram; same-digit lanes of a wave are
+//                   grouped with ballots and counted by one LDS atomic;
+//   rs_scan      -- one workgroup scans the digit-major histogram matrix;
+//   rs_downsweep -- each wave ranks its keys stably (ballot match + per-wave
+//                   LDS counters), the tile is reordered in LDS, and written
+//                   out digit-run by digit-run so consecutive lanes store to
+//                   consecutive addresses.
+#include "device_math.hpp"
+#include "radix_sort.hpp"
+
+namespace wsim {
+namespace {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_WAVES = RS_THREADS / kWave;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_WAVE_TILE = kWave * RS_ITEMS;      // 1024 keys per wave
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;      // 4096 keys per workgroup
+constexpr int RS_SCAN_THREADS = 1024;
+static_assert(RS_THREADS == 256, "one digit per thread in the digit scans");
+
+// Lanes of the wave whose digit equals this lane's digit (nbits <= 8).
+__device__ inline u64 match_digit(u32 d, int nbits) {
+    u64 m = ~0ull;
+    for (int b = 0; b < nbits; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const u64 bal = __ballot(bit);
+        m &= bit ? bal : ~bal;
+    }
+    return m;
+}
+
+// Exclusive scan of one u32 per thread over a 256-thread workgroup.
+__device__ inline u32 block_exclusive_scan256(u32 v, u32* s_tmp) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    u32 x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const u32 y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_tmp[w] = x;
+    __syncthreads();
+    u32 off = 0;
+    for (int i = 0; i < w; ++i) off += s_tmp[i];
+    __syncthreads();
+    return off + x - v;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const u32* __restrict__ keys, u32 n,
+                                                         int shift, int nbits,
+                                                         u32* __restrict__ hist, u32 nblocks) {
+    __shared__ u32 cnt[256];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const u32 dmask = (1u << nbits) - 1u;
+    cnt[tid] = 0;
+    __syncthreads();
+    const u32 base = blockIdx.x * RS_TILE + w * RS_WAVE_TILE;
+#pragma unroll 4
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const u32 idx = base + r * kWave + lane;
+        const bool valid = idx < n;
+        const u32 d = valid ? (keys[idx] >> shift) & dmask : 0u;
+        const u64 m = match_digit(d, nbits) & __ballot(valid);
+        if (valid && lane == __builtin_ctzll(m)) atomicAdd(&cnt[d], static_cast<u32>(__popcll(m)));
+    }
+    __syncthreads();
+    if (static_cast<u32>(tid) <= dmask) hist[static_cast<u64>(tid) * nblocks + blockIdx.x] = cnt[tid];
+}
+
+// Exclusive scan, in place, of len u32 (digit-major histogram matrix).
+__global__ __launch_bounds__(RS_SCAN_THREADS) void rs_scan(u32* __restrict__ data, u32 len) {
+    __shared__ u32 s_w[RS_SCAN_THREADS / 64];
+    __shared__ u32 s_carry;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) s_carry = 0;
+    __syncthreads();
+    for (u32 base = 0; base < len; base += RS_SCAN_THREADS * 4) {
+        u32 v[4];
+        u32 t = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const u32 idx = base + tid * 4 + k;
+            v[k] = idx < len ? data[idx] : 0u;
+            t += v[k];
+        }
+        u32 x = t;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const u32 y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) s_w[w] = x;
+        __syncthreads();
+        u32 off = s_carry;
+        for (int i = 0; i < w; ++i) off += s_w[i];
+        u32 run = off + x - t;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const u32 idx = base + tid * 4 + k;
+            if (idx < len) data[idx] = run;
+            run += v[k];
+        }
+        __syncthreads();
+        if (tid == RS_SCAN_THREADS - 1) s_carry = run;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
+    const u32* __restrict__ kin, const u32* __restrict__ vin, u32* __restrict__ kout,
+    u32* __restrict__ vout, u32 n, int shift, int nbits, const u32* __restrict__ offs,
+    u32 nblocks) {
+    __shared__ u32 s_wc[RS_WAVES][256];
+    __shared__ u32 s_start[256];
+    __shared__ u32 s_goff[256];
+    __shared__ u32 s_tmp[RS_WAVES];
+    __shared__ u32 s_k[RS_TILE];
+    __shared__ u32 s_v[RS_TILE];
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const u32 dmask = (1u << nbits) - 1u;
+    for (int i = tid; i < RS_WAVES * 256; i += RS_THREADS) (&s_wc[0][0])[i] = 0;
+    __syncthreads();
+
+    const u32 tile0 = blockIdx.x * RS_TILE;
+    const u32 base = tile0 + w * RS_WAVE_TILE;
+    const u64 lanes_below = (1ull << lane) - 1ull;
+
+    u32 key[RS_ITEMS], val[RS_ITEMS], rank[RS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const u32 idx = base + r * kWave + lane;
+        const bool valid = idx < n;
+        key[r] = valid ? kin[idx] : 0u;
+        val[r] = valid ? vin[idx] : 0u;
+    }
+    // Stable ranks inside the wave's sub-tile: positions are ordered by
+    // (round, lane), so a key's rank among equal digits is the running count
+    // of its digit in this wave plus the number of lower lanes with it now.
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const u32 idx = base + r * kWave + lane;
+        const bool valid = idx < n;
+        const u32 d = (key[r] >> shift) & dmask;
+        const u64 m = match_digit(d, nbits) & __ballot(valid);
+        const u32 below = static_cast<u32>(__popcll(m & lanes_below));
+        u32 cur = 0;
+        if (valid) cur = s_wc[w][d];
+        __builtin_amdgcn_wave_barrier();
+        if (valid && below == 0) s_wc[w][d] = cur + static_cast<u32>(__popcll(m));
+        __builtin_amdgcn_wave_barrier();
+        rank[r] = cur + below;
+    }
+    __syncthreads();
+
+    // Per digit (one per thread): exclusive prefix over waves, tile total.
+    u32 total = 0;
+    {
+        const int d = tid;
+#pragma unroll
+        for (int ww = 0; ww < RS_WAVES; ++ww) {
+            const u32 c = s_wc[ww][d];
+            s_wc[ww][d] = total;
+            total += c;
+        }
+        s_goff[d] = (static_cast<u32>(d) <= dmask) ? offs[static_cast<u64>(d) * nblocks + blockIdx.x] : 0u;
+    }
+    const u32 start = block_exclusive_scan256(total, s_tmp);
+    s_start[tid] = start;
+    __syncthreads();
+
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const u32 idx = base + r * kWave + lane;
+        if (idx < n) {
+            const u32 d = (key[r] >> shift) & dmask;
+            const u32 pos = s_start[d] + s_wc[w][d] + rank[r];
+            s_k[pos] = key[r];
+            s_v[pos] = val[r];
+        }
+    }
+    __syncthreads();
+
+    const u32 cnt = (n - tile0 < static_cast<u32>(RS_TILE)) ? n - tile0 : static_cast<u32>(RS_TILE);
+    for (u32 e = tid; e < cnt; e += RS_THREADS) {
+        const u32 k = s_k[e];
+        const u32 d = (k >> shift) & dmask;
+        const u32 o = s_goff[d] + (e - s_start[d]);
+        kout[o] = k;
+        vout[o] = s_v[e];
+    }
+}
+
+}  // namespace
+
+size_t radix_hist_elems(size_t n) { return static_cast<size_t>(256) * div_up(n ? n : 1, RS_TILE); }
+
+int radix_sort_pairs(u32* k0, u32* v0, u32* k1, u32* v1, size_t n, int bits, u32* hist,
+                     hipStream_t stream) {
+    if (n <= 1 || bits <= 0) return 0;
+    if (n > 0xFFFFF000ull) fail(kInvalidArgument, "radix_sort_pairs: too many elements");
+    const u32 nn = static_cast<u32>(n);
+    const u32 nblocks = div_up(n, RS_TILE);
+    u32* kin = k0;
+    u32* vin = v0;
+    u32* kout = k1;
+    u32* vout = v1;
+    int where = 0;
+    for (int shift = 0; shift < bits; shift += 8) {
+        const int nbits = (bits - shift) < 8 ? (bits - shift) : 8;
+        const u32 len = (1u << nbits) * nblocks;
+        rs_upsweep<<<nblocks, RS_THREADS, 0, stream>>>(kin, nn, shift, nbits, hist, nblocks);
+        WSIM_LAUNCH_CHECK();
+        rs_scan<<<1, RS_SCAN_THREADS, 0, stream>>>(hist, len);
+        WSIM_LAUNCH_CHECK();
+        rs_downsweep<<<nblocks, RS_THREADS, 0, stream>>>(kin, vin, kout, vout, nn, shift, nbits,
+                                                         hist, nblocks);
+        WSIM_LAUNCH_CHECK();
+        u32* tk = kin;
+        u32* tv = vin;
+        kin = kout;
+        vin = vout;
+        kout = tk;
+        vout = tv;
+        where ^= 1;
+    }
+    return where;
+}
+
+}  // namespace wsim
