@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the DFSPH hot path on one MI355X.
+
+Workload: the large dam break -- 100 x 100 x 100 = 1,000,000 binary32 fluid
+particles (r = 0.0025 m, lattice spacing 0.005 m, the block one spacing
+inside the walls of a 2 x 1 x 1 m tank) and the tank's Akinci walls sampled
+at 0.005 m x 3 layers (1,209,618 boundary particles); rho0 = 1000, XSPH 0.01,
+damping 0.5, eta = 0.1 %, at most 100 density iterations, CFL time step
+(lambda = 0.4, velocity floor 0.5 m/s).  A "step" is one full solver substep
+(grid build, density/alpha, gravity, XSPH, density solve, advection, max |v|).
+
+Prints one JSON line.  value = particle-substeps/s with all inputs resident
+in HBM; roofline = the dominant kernel's algorithmic bytes per launch over
+its mean launch time (HIP events on the solver's stream); cpu_baseline = the
+binary32 CPU oracle on the host cores, started from the GPU state at the
+beginning of the timed region and run for a bounded number of substeps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+# Algorithmic bytes per fluid particle and launch: each per-particle array
+# element the kernel reads or writes by its interface, once (float3 = 12 B,
+# f32/u32 = 4 B); neighbour gathers, cell-table reads, scratch and
+# reduction partials count 0.
+ALG_BYTES = {
+    "reorder": 4 + 12 + 12 + 12 + 12,             # r perm, x, v; w sorted x, v
+    "density_alpha": 12 + 4 + 4,                  # r x; w rho, alpha
+    "xsph": 12 + 12 + 4 + 12,                     # r x, v, rho; w v
+    "density_change": 12 + 12 + 4 + 4 + 4 + 4,    # r x, v, rho, alpha; w rho*, kappa
+    "apply_kappa": 12 + 4 + 4 + 12 + 12,          # r x, rho, kappa, v; w v
+    "advect": 12 + 12 + 4 + 16 + 12 + 12 + 16,    # r x, v, perm, 4 attrs; w x, v, 4 attrs
+}
+
+
+def make_workload(n_side: int):
+    import water_simulation_amd as ws
+
+    r = 0.0025
+    i = np.arange(n_side, dtype=np.float32)
+    g = np.stack(np.meshgrid(i, i, i, indexing="ij"), -1).reshape(-1, 3)
+    x = (np.float32(0.015) + np.float32(0.005) * g).astype(np.float32)
+    xb = ws.sample_aabb_boundary((0, 0, 0), (2, 1, 1), 0.005, 3)
+    cfg = dict(rest_density=1000.0, particle_radius=r, smoothing_length=2 * r, viscosity=0.01,
+               damping=0.5, gravity=(0.0, -9.81, 0.0), max_density_iters=100,
+               eta_density=1e-3, domain_min=(0, 0, 0), domain_max=(2, 1, 1))
+    return x, xb, cfg
+
+
+def cpu_baseline(x, v, xb, cfg, dts, budget_s: float):
+    """Binary32 oracle on the host cores from the given state (bounded sample)."""
+    from oracle import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    O.set_threads(threads)
+    s = O.Solver(O.config(**cfg), x, v, xb, O.F32)
+    done, iters = 0, 0
+    t0 = time.perf_counter()
+    for dt in dts:
+        s.step(float(dt))
+        done += 1
+        iters += s.stats()["density_iters"]
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    return {
+        "value": len(x) * done / el,
+        "unit": "particle-substeps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{done} substeps of the timed trajectory from its first state "
+                  f"({len(x)} fluid + {len(xb)} boundary particles, {iters} density iterations, "
+                  f"{el:.1f} s, binary32 oracle, OpenMP)",
+    }, done, iters
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n-side", type=int, default=100)
+    ap.add_argument("--pressure-form", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--kernel-times", action="store_true", help="print per-kernel table to stderr")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one MI355X only (the path is single-GPU; "
+              "see the sharding design); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import water_simulation_amd as ws
+    from water_simulation_amd import TimeStepper
+
+    if ws.device_count() < 1:
+        print("bench.py: no HIP device", file=sys.stderr)
+        return 1
+
+    x, xb, cfg = make_workload(args.n_side)
+    cfg["pressure_form"] = args.pressure_form
+    n, nb = len(x), len(xb)
+    fluid = ws.ParticleStore(n)
+    fluid.resize(n)
+    fluid.set_positions(x)
+    bnd = ws.ParticleStore(nb)
+    bnd.resize(nb)
+    bnd.set_positions(xb)
+    stream = None
+    solver = ws.DFSPHSolver(fluid, bnd, ws.DFSPHConfig(**cfg), stream)
+    ts = TimeStepper()
+    r = cfg["particle_radius"]
+
+    def next_dt():
+        return ts.next_dt(max(solver.max_velocity(), 0.5), r, 1.0)
+
+    for _ in range(args.warmup):
+        solver.step(next_dt())
+    ws.device_synchronize()
+
+    x0 = fluid.get_positions()
+    v0 = fluid.get_velocities()
+    solver.set_timing(True)
+    solver.reset_timing()
+    dts, iters = [], []
+    ws.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dt = next_dt()
+        solver.step(dt)
+        dts.append(dt)
+        iters.append(solver.last_density_iters())
+    ws.device_synchronize()
+    elapsed = time.perf_counter() - t0
+    st = solver.stats()
+    kt = solver.kernel_times()
+    solver.set_timing(False)
+
+    value = n * args.steps / elapsed
+    # dominant kernel by total device time in the timed region
+    timed = {k: v for k, v in kt.items() if v[0] > 0 and k in ALG_BYTES}
+    dom = max(timed, key=lambda k: timed[k][1])
+    launches, total_ms = timed[dom]
+    avg_s = total_ms / launches / 1e3
+    achieved = ALG_BYTES[dom] * n / avg_s / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_{dom}.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+
+    if args.kernel_times:
+        tot = sum(v[1] for v in kt.values())
+        for k, (c, ms) in sorted(kt.items(), key=lambda kv: -kv[1][1]):
+            if c:
+                print(f"  {k:15s} launches {c:6d}  total {ms:9.2f} ms  mean {ms / c * 1e3:9.1f} us"
+                      f"  {100 * ms / tot:5.1f} %", file=sys.stderr)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu, _, _ = cpu_baseline(x0, v0, xb, cfg, dts, args.cpu_budget)
+
+    out = {
+        "metric": "particle-substeps/s at 1M particles, eta<=0.1%",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (lattice dam-break block, sampled tank walls)",
+        "config": {
+            "workload": f"large dam break: {n} fluid + {nb} boundary particles, "
+                        f"r=0.0025 m, 2x1x1 m tank, CFL substeps",
+            "n_fluid": n,
+            "n_boundary": nb,
+            "pressure_form": "reference kappa/rho^2" if args.pressure_form == 0 else "kappa/rho",
+            "max_density_iters": cfg["max_density_iters"],
+            "eta_density": cfg["eta_density"],
+            "mean_density_iters": float(np.mean(iters)),
+            "final_mean_density_error": st["mean_density_error"],
+            "eta_reached_last_substep": bool(st["mean_density_error"] <
+                                             cfg["eta_density"] * cfg["rest_density"]),
+            "mean_dt": float(np.mean(dts)),
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS,
+            "traffic": traffic,
+            "alg_bytes_per_launch": ALG_BYTES[dom] * n,
+            "mean_launch_us": avg_s * 1e6,
+            "launches": launches,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -129,15 +129,38 @@ __global__ __launch_bounds__(NT) void k_pack_boundary(u32 nb, const u32* __restr
     store_mb[perm[s]] = mb[s];
 }
 
+// Per-particle pair lists, ELL layout: entry k of particle i at [k * n + i]
+// holds (neighbour slot, bits of c) with grad W_ij = (x_i - x_j) * c, for
+// every pair the reference's stencil visits with c != 0, in the reference's
+// summation order.  Positions are frozen from the grid build to advection, so
+// the iteration kernels recompute r and g = r * c from the list and get the
+// same binary32 values the reference gets from grad_W (zero-gradient pairs
+// add +-0 to sums that never hold -0, so leaving them out is exact).
+// Particles with more pairs than the list depth use the stencil walk.
+// (PairList itself is declared in wsim_internal.hpp.)
+
+// Wave-blocked ELL: the K entries of the 64 particles of one wave occupy one
+// contiguous block, row k of it holding entry k of each lane.
+__device__ __forceinline__ u64 pair_at(u32 k, u32 i, u32 depth) {
+    return (static_cast<u64>(i >> 6) * depth + k) * 64u + (i & 63u);
+}
+
+__device__ __forceinline__ u32 pack_counts(u32 cf, u32 cb) {
+    return (cf < 0xFFFFu ? cf : 0xFFFFu) | ((cb < 0xFFFFu ? cb : 0xFFFFu) << 16);
+}
+
 // rho (dfsph_kernels.cu:68-112) and alpha (dfsph_kernels.cu:117-169) in one
-// stencil walk: they share r, |r| and q for every pair.
-__global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, const float4* __restrict__ xs,
+// stencil walk: they share r, |r| and q for every pair.  Also records the
+// pair lists used by every iteration of the pressure solves.
+__global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
+                                                      const float4* __restrict__ xs,
                                                       float* __restrict__ rho_out,
                                                       float* __restrict__ alpha_out) {
     const u32 i = tile_index(P.n);
     if (i >= P.n) return;
     const float4 p = xs[i];
     float rho = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f, sq = 0.0f;
+    u32 cf = 0, cb = 0;
     for_stencil(P.gf, P.csf, p.x, p.y, p.z, [&](u32 s) {
         const float4 q = xs[s];
         const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
@@ -150,6 +173,10 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, const float4* __res
             sy += my;
             sz += mz;
             sq += mx * mx + my * my + mz * mz;
+            if (c != 0.0f) {
+                if (cf < L.kf) L.f[pair_at(cf, i, L.kf)] = make_uint2(s, __float_as_uint(c));
+                ++cf;
+            }
         }
     });
     if (P.nb) {
@@ -164,8 +191,13 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, const float4* __res
             sy += my;
             sz += mz;
             sq += mx * mx + my * my + mz * mz;
+            if (c != 0.0f) {
+                if (cb < L.kb) L.b[pair_at(cb, i, L.kb)] = make_uint2(s, __float_as_uint(c));
+                ++cb;
+            }
         });
     }
+    L.cnt[i] = pack_counts(cf, cb);
     rho_out[i] = rho;
     const float denom = sx * sx + sy * sy + sz * sz + sq;
     alpha_out[i] = (denom > 1e-9f) ? (rho / denom) : 0.0f;
@@ -196,24 +228,37 @@ __global__ __launch_bounds__(NT) void k_xsph(Nbr P, const float4* __restrict__ x
     vout[i] = make_float4(vi.x + dx, vi.y + dy, vi.z + dz, 0.0f);
 }
 
+// rho_dot terms (dfsph_kernels.cu:210, 222) for one pair with gradient scalar c.
+__device__ __forceinline__ void rho_dot_fluid(float& rd, const float4& p, const float4& vi,
+                                              const float4& q, const float4& vj, float c,
+                                              float mass) {
+    const float gx = (p.x - q.x) * c, gy = (p.y - q.y) * c, gz = (p.z - q.z) * c;
+    const float ux = vi.x - vj.x, uy = vi.y - vj.y, uz = vi.z - vj.z;
+    rd += mass * (ux * gx + uy * gy + uz * gz);
+}
+__device__ __forceinline__ void rho_dot_boundary(float& rd, const float4& p, const float4& vi,
+                                                 const float4& q, float c) {
+    const float gx = (p.x - q.x) * c, gy = (p.y - q.y) * c, gz = (p.z - q.z) * c;
+    rd += q.w * (vi.x * gx + vi.y * gy + vi.z * gz);
+}
+
 // One predictor sweep.  MODE 0: density change + kappa (dfsph_kernels.cu:
 // 176-232).  MODE 1: divergence change + kappa_v (plan text :1862-1907, with
 // the boundary term of the density change).  Writes (x, kappa-hat) for the
 // following k_apply and reduces the mean error in binary64; the last
 // workgroup decides convergence on the device.
 template <int MODE>
-__global__ __launch_bounds__(NT) void k_change(Nbr P, const float4* __restrict__ xs,
+__global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* __restrict__ xs,
                                                const float4* __restrict__ vs,
                                                const float* __restrict__ rho,
                                                const float* __restrict__ alpha, float dt,
                                                float dt2, float rho0, int pform, u32 iter,
-                                               double thr, DeviceState* st,
+                                               const DeviceState* st,
                                                double* __restrict__ part,
                                                float* __restrict__ rho_adv,
                                                float* __restrict__ kappa,
                                                float4* __restrict__ xk) {
     __shared__ double s_red[NT / 64];
-    __shared__ bool s_last;
     const u32 conv = MODE == 0 ? st->conv_iter : st->div_conv_iter;
     if (conv < iter) return;  // converged earlier: grid-uniform no-op
     const u32 i = tile_index(P.n);
@@ -222,24 +267,58 @@ __global__ __launch_bounds__(NT) void k_change(Nbr P, const float4* __restrict__
         const float4 p = xs[i];
         const float4 vi = vs[i];
         float rd = 0.0f;
-        for_stencil(P.gf, P.csf, p.x, p.y, p.z, [&](u32 s) {
-            if (s == i) return;
-            const float4 q = xs[s];
-            const float4 vj = vs[s];
-            const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-            const float c = spline_gradc(norm3(rx, ry, rz), P.sp);
-            const float gx = rx * c, gy = ry * c, gz = rz * c;
-            const float ux = vi.x - vj.x, uy = vi.y - vj.y, uz = vi.z - vj.z;
-            rd += P.mass * (ux * gx + uy * gy + uz * gz);
-        });
-        if (P.nb) {
-            for_stencil(P.gb, P.csb, p.x, p.y, p.z, [&](u32 s) {
-                const float4 q = P.xb[s];
-                const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-                const float c = spline_gradc(norm3(rx, ry, rz), P.sp);
-                const float gx = rx * c, gy = ry * c, gz = rz * c;
-                rd += q.w * (vi.x * gx + vi.y * gy + vi.z * gz);
+        const u32 cw = L.cnt[i];
+        const u32 cf = cw & 0xFFFFu, cb = cw >> 16;
+        if (cf <= L.kf && cb <= L.kb) {
+            // four pairs per trip: loads issued together, terms summed in order
+            u32 k = 0;
+            for (; k + 4 <= cf; k += 4) {
+                uint2 e[4];
+                float4 q[4], w[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) e[u] = L.f[pair_at(k + u, i, L.kf)];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    q[u] = xs[e[u].x];
+                    w[u] = vs[e[u].x];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    rho_dot_fluid(rd, p, vi, q[u], w[u], __uint_as_float(e[u].y), P.mass);
+            }
+            for (; k < cf; ++k) {
+                const uint2 e = L.f[pair_at(k, i, L.kf)];
+                rho_dot_fluid(rd, p, vi, xs[e.x], vs[e.x], __uint_as_float(e.y), P.mass);
+            }
+            k = 0;
+            for (; k + 4 <= cb; k += 4) {
+                uint2 e[4];
+                float4 q[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) e[u] = L.b[pair_at(k + u, i, L.kb)];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) q[u] = P.xb[e[u].x];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) rho_dot_boundary(rd, p, vi, q[u], __uint_as_float(e[u].y));
+            }
+            for (; k < cb; ++k) {
+                const uint2 e = L.b[pair_at(k, i, L.kb)];
+                rho_dot_boundary(rd, p, vi, P.xb[e.x], __uint_as_float(e.y));
+            }
+        } else {
+            for_stencil(P.gf, P.csf, p.x, p.y, p.z, [&](u32 s) {
+                if (s == i) return;
+                const float4 q = xs[s];
+                const float c = spline_gradc(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
+                rho_dot_fluid(rd, p, vi, q, vs[s], c, P.mass);
             });
+            if (P.nb) {
+                for_stencil(P.gb, P.csb, p.x, p.y, p.z, [&](u32 s) {
+                    const float4 q = P.xb[s];
+                    const float c = spline_gradc(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
+                    rho_dot_boundary(rd, p, vi, q, c);
+                });
+            }
         }
         const float rho_i = rho[i];
         float kap, err;
@@ -258,64 +337,115 @@ __global__ __launch_bounds__(NT) void k_change(Nbr P, const float4* __restrict__
         xk[i] = make_float4(p.x, p.y, p.z, khat);
         contrib = static_cast<double>(err);
     }
+    // per-workgroup partial; the next launch (k_apply) finishes the sum, so
+    // no cross-workgroup fence is needed inside this kernel
     const double bsum = block_sum<NT>(contrib, s_red);
     if (threadIdx.x == 0) part[blockIdx.x] = bsum;
-    if (last_block_arrive(&st->ticket[MODE], gridDim.x, &s_last)) {
-        double s = 0.0;
-        for (u32 b = threadIdx.x; b < gridDim.x; b += NT) s += load_agent(&part[b]);
-        const double tot = block_sum<NT>(s, s_red);
-        if (threadIdx.x == 0) {
-            const double e = tot / static_cast<double>(P.n);
-            if (MODE == 0) {
-                st->err = e;
-                if (iter >= 1 && e < thr && st->conv_iter == kNone) st->conv_iter = iter;
-            } else {
-                st->div_err = e;
-                if (e < thr && st->div_conv_iter == kNone) st->div_conv_iter = iter;
-            }
-            st->ticket[MODE] = 0;
+}
+
+// Workgroup 0 of k_apply(iter) reduces the partials k_change(iter) wrote, in
+// a fixed order, and records the error and the convergence decision
+// (dfsph.cu:254-258; the divergence solve tests every iteration).
+__device__ __forceinline__ void finish_error(int mode, u32 iter, double thr, u32 n,
+                                             const double* __restrict__ part, u32 nparts,
+                                             DeviceState* st) {
+    __shared__ double s_red[NT / 64];
+    double s = 0.0;
+    for (u32 b = threadIdx.x; b < nparts; b += NT) s += part[b];
+    const double tot = block_sum<NT>(s, s_red);
+    if (threadIdx.x == 0) {
+        const double e = tot / static_cast<double>(n);
+        if (mode == 0) {
+            st->err = e;
+            if (iter >= 1 && e < thr && st->conv_iter == kNone) st->conv_iter = iter;
+        } else {
+            st->div_err = e;
+            if (e < thr && st->div_conv_iter == kNone) st->div_conv_iter = iter;
         }
     }
+}
+
+// Pressure velocity terms (dfsph_kernels.cu:275-278, 291-294) for one pair.
+__device__ __forceinline__ void kappa_term(float& dx, float& dy, float& dz, const float4& p,
+                                           const float4& q, float c, float c2, float dt) {
+    const float gx = (p.x - q.x) * c, gy = (p.y - q.y) * c, gz = (p.z - q.z) * c;
+    dx -= dt * c2 * gx;
+    dy -= dt * c2 * gy;
+    dz -= dt * c2 * gz;
 }
 
 // Pressure velocity update (dfsph_kernels.cu:237-301); kappa-hat of every
 // particle was precomputed by k_change with the same expression the
 // reference evaluates per pair.
-__global__ __launch_bounds__(NT) void k_apply(Nbr P, const float4* __restrict__ xk,
+__global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* __restrict__ xk,
                                               const float* __restrict__ kappa, float dt,
-                                              float rho0, int pform, u32 iter,
-                                              const u32* __restrict__ conv,
-                                              float4* __restrict__ vs) {
-    if (*conv < iter) return;
+                                              float rho0, int pform, u32 iter, int mode,
+                                              double thr, const double* __restrict__ part,
+                                              DeviceState* st, float4* __restrict__ vs) {
+    const u32 conv = mode == 0 ? st->conv_iter : st->div_conv_iter;
+    if (conv < iter) return;  // grid-uniform: this iteration's k_change did not run either
+    if (blockIdx.x == 0) finish_error(mode, iter, thr, P.n, part, gridDim.x, st);
     const u32 i = tile_index(P.n);
     if (i >= P.n) return;
     const float4 p = xk[i];
     const float ki = p.w;
     float dx = 0.0f, dy = 0.0f, dz = 0.0f;
-    for_stencil(P.gf, P.csf, p.x, p.y, p.z, [&](u32 s) {
-        if (s == i) return;
-        const float4 q = xk[s];
-        const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-        const float c = spline_gradc(norm3(rx, ry, rz), P.sp);
-        const float gx = rx * c, gy = ry * c, gz = rz * c;
-        const float c2 = P.mass * (ki + q.w);
-        dx -= dt * c2 * gx;
-        dy -= dt * c2 * gy;
-        dz -= dt * c2 * gz;
-    });
+    float bf = 0.0f;
     if (P.nb) {
         const float k_i = kappa[i];
-        const float bf = pform ? k_i / fmaxf(rho0, 1e-9f) : k_i / fmaxf(rho0 * rho0, 1e-9f);
-        for_stencil(P.gb, P.csb, p.x, p.y, p.z, [&](u32 s) {
-            const float4 q = P.xb[s];
-            const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-            const float c = spline_gradc(norm3(rx, ry, rz), P.sp);
-            const float gx = rx * c, gy = ry * c, gz = rz * c;
-            const float c2 = q.w * (ki + bf);
-            dx -= dt * c2 * gx;
-            dy -= dt * c2 * gy;
-            dz -= dt * c2 * gz;
+        bf = pform ? k_i / fmaxf(rho0, 1e-9f) : k_i / fmaxf(rho0 * rho0, 1e-9f);
+    }
+    const u32 cw = L.cnt[i];
+    const u32 cf = cw & 0xFFFFu, cb = cw >> 16;
+    if (cf <= L.kf && cb <= L.kb) {
+        u32 k = 0;
+        for (; k + 4 <= cf; k += 4) {
+            uint2 e[4];
+            float4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) e[u] = L.f[pair_at(k + u, i, L.kf)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = xk[e[u].x];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                kappa_term(dx, dy, dz, p, q[u], __uint_as_float(e[u].y), P.mass * (ki + q[u].w), dt);
+        }
+        for (; k < cf; ++k) {
+            const uint2 e = L.f[pair_at(k, i, L.kf)];
+            const float4 q = xk[e.x];
+            kappa_term(dx, dy, dz, p, q, __uint_as_float(e.y), P.mass * (ki + q.w), dt);
+        }
+        k = 0;
+        for (; k + 4 <= cb; k += 4) {
+            uint2 e[4];
+            float4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) e[u] = L.b[pair_at(k + u, i, L.kb)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = P.xb[e[u].x];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                kappa_term(dx, dy, dz, p, q[u], __uint_as_float(e[u].y), q[u].w * (ki + bf), dt);
+        }
+        for (; k < cb; ++k) {
+            const uint2 e = L.b[pair_at(k, i, L.kb)];
+            const float4 q = P.xb[e.x];
+            kappa_term(dx, dy, dz, p, q, __uint_as_float(e.y), q.w * (ki + bf), dt);
+        }
+    } else {
+        for_stencil(P.gf, P.csf, p.x, p.y, p.z, [&](u32 s) {
+            if (s == i) return;
+            const float4 q = xk[s];
+            const float c = spline_gradc(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
+            kappa_term(dx, dy, dz, p, q, c, P.mass * (ki + q.w), dt);
         });
+        if (P.nb) {
+            for_stencil(P.gb, P.csb, p.x, p.y, p.z, [&](u32 s) {
+                const float4 q = P.xb[s];
+                const float c = spline_gradc(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
+                kappa_term(dx, dy, dz, p, q, c, q.w * (ki + bf), dt);
+            });
+        }
     }
     float4 v = vs[i];
     v.x += dx;
@@ -328,32 +458,35 @@ __device__ __forceinline__ bool finite3(float a, float b, float c) {
     return isfinite(a) && isfinite(b) && isfinite(c);
 }
 
-// Reduce (max |v|, NaN count) partials; the last workgroup publishes them.
-__device__ __forceinline__ void finish_vstats(float mag, u32 bad, DeviceState* st,
-                                              float* __restrict__ pmax, u32* __restrict__ pnan) {
+// Per-workgroup (max |v|, NaN count) partials; k_finish_vstats reduces them.
+__device__ __forceinline__ void vstats_partial(float mag, u32 bad, float* __restrict__ pmax,
+                                               u32* __restrict__ pnan) {
     __shared__ float s_f[NT / 64];
     __shared__ u32 s_u[NT / 64];
-    __shared__ bool s_last;
     const float bm = block_max<NT>(mag, s_f);
     const u32 bn = block_sum_u32<NT>(bad, s_u);
     if (threadIdx.x == 0) {
         pmax[blockIdx.x] = bm;
         pnan[blockIdx.x] = bn;
     }
-    if (last_block_arrive(&st->ticket[2], gridDim.x, &s_last)) {
-        float m = 0.0f;
-        u32 c = 0;
-        for (u32 b = threadIdx.x; b < gridDim.x; b += NT) {
-            m = fmaxf(m, load_agent(&pmax[b]));
-            c += load_agent(&pnan[b]);
-        }
-        m = block_max<NT>(m, s_f);
-        c = block_sum_u32<NT>(c, s_u);
-        if (threadIdx.x == 0) {
-            st->max_v = m;
-            st->nan_count = c;
-            st->ticket[2] = 0;
-        }
+}
+
+__global__ __launch_bounds__(NT) void k_finish_vstats(const float* __restrict__ pmax,
+                                                      const u32* __restrict__ pnan, u32 nparts,
+                                                      DeviceState* st) {
+    __shared__ float s_f[NT / 64];
+    __shared__ u32 s_u[NT / 64];
+    float m = 0.0f;
+    u32 c = 0;
+    for (u32 b = threadIdx.x; b < nparts; b += NT) {
+        m = fmaxf(m, pmax[b]);
+        c += pnan[b];
+    }
+    m = block_max<NT>(m, s_f);
+    c = block_sum_u32<NT>(c, s_u);
+    if (threadIdx.x == 0) {
+        st->max_v = m;
+        st->nan_count = c;
     }
 }
 
@@ -377,7 +510,7 @@ __global__ __launch_bounds__(NT) void k_advect(u32 n, const float4* __restrict__
                                                const float* __restrict__ alpha,
                                                const float* __restrict__ kappa,
                                                const float* __restrict__ rho_adv, int wb_solve,
-                                               DeviceState* st, float* __restrict__ pmax,
+                                               float* __restrict__ pmax,
                                                u32* __restrict__ pnan) {
     const u32 s = blockIdx.x * NT + threadIdx.x;
     float mag = 0.0f;
@@ -416,7 +549,7 @@ __global__ __launch_bounds__(NT) void k_advect(u32 n, const float4* __restrict__
         mag = norm3(v.x, v.y, v.z);
         bad = finite3(px, py, pz) && finite3(v.x, v.y, v.z) ? 0u : 1u;
     }
-    finish_vstats(mag, bad, st, pmax, pnan);
+    vstats_partial(mag, bad, pmax, pnan);
 }
 
 // End of the divergence solve: velocities and the recomputed attributes back
@@ -428,7 +561,7 @@ __global__ __launch_bounds__(NT) void k_div_writeback(u32 n, const float4* __res
                                                       const float* __restrict__ alpha,
                                                       const float* __restrict__ kappa_v,
                                                       const float* __restrict__ rho_dot,
-                                                      int wb_solve, DeviceState* st,
+                                                      int wb_solve,
                                                       float* __restrict__ pmax,
                                                       u32* __restrict__ pnan) {
     const u32 s = blockIdx.x * NT + threadIdx.x;
@@ -451,7 +584,7 @@ __global__ __launch_bounds__(NT) void k_div_writeback(u32 n, const float4* __res
         mag = norm3(v.x, v.y, v.z);
         bad = finite3(x4.x, x4.y, x4.z) && finite3(v.x, v.y, v.z) ? 0u : 1u;
     }
-    finish_vstats(mag, bad, st, pmax, pnan);
+    vstats_partial(mag, bad, pmax, pnan);
 }
 
 __global__ void k_reset_solve(DeviceState* st, int which) {
@@ -567,6 +700,10 @@ DFSPHSolver::DFSPHSolver(ParticleStore& fluid, ParticleStore* boundary, const DF
     alpha_ = DeviceBuffer<float>(c1);
     kappa_ = DeviceBuffer<float>(c1);
     rho_adv_ = DeviceBuffer<float>(c1);
+    const size_t c64 = (c1 + 63) / 64 * 64;  // the wave-blocked lists pad to whole waves
+    pairs_f_ = DeviceBuffer<uint2>(static_cast<size_t>(kPairDepthFluid) * c64);
+    pairs_b_ = DeviceBuffer<uint2>(static_cast<size_t>(kPairDepthBoundary) * c64);
+    pair_cnt_ = DeviceBuffer<u32>(c1);
     const size_t nblk = div_up(c1, NT);
     part_d_ = DeviceBuffer<double>(nblk);
     part_f_ = DeviceBuffer<float>(nblk);
@@ -648,10 +785,21 @@ static Nbr make_nbr(const CellGrid& fg, const CellGrid* bg, const float4* xb, si
     return P;
 }
 
+PairList DFSPHSolver::pair_list() {
+    PairList L;
+    L.f = pairs_f_.data();
+    L.b = pairs_b_.data();
+    L.cnt = pair_cnt_.data();
+    L.kf = kPairDepthFluid;
+    L.kb = nb_ ? kPairDepthBoundary : 0u;
+    return L;
+}
+
 void DFSPHSolver::density_solve(float dt) {
     const u32 n = static_cast<u32>(fluid_.count());
     const u32 blocks = div_up(n, NT);
     const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, n);
+    const PairList L = pair_list();
     const double thr = static_cast<double>(cfg_.eta_density * cfg_.rest_density);
     const float dt2 = dt * dt;
     DeviceState* st = dstate_.data();
@@ -663,18 +811,18 @@ void DFSPHSolver::density_solve(float dt) {
         const u32 end = (maxit - it < chunk) ? maxit : it + chunk;
         for (; it < end; ++it) {
             tbegin(kKDensityChange, static_cast<int>(it));
-            k_change<0><<<blocks, NT, 0, stream_>>>(P, xs_.data(), vs_.data(), rho_.data(),
+            k_change<0><<<blocks, NT, 0, stream_>>>(P, L, xs_.data(), vs_.data(), rho_.data(),
                                                     alpha_.data(), dt, dt2, cfg_.rest_density,
-                                                    static_cast<int>(cfg_.pressure_form), it, thr,
+                                                    static_cast<int>(cfg_.pressure_form), it,
                                                     st, part_d_.data(), rho_adv_.data(),
                                                     kappa_.data(), xk_.data());
             WSIM_LAUNCH_CHECK();
             tend();
             tbegin(kKApplyKappa, static_cast<int>(it));
-            k_apply<<<blocks, NT, 0, stream_>>>(P, xk_.data(), kappa_.data(), dt,
+            k_apply<<<blocks, NT, 0, stream_>>>(P, L, xk_.data(), kappa_.data(), dt,
                                                 cfg_.rest_density,
-                                                static_cast<int>(cfg_.pressure_form), it,
-                                                &st->conv_iter, vs_.data());
+                                                static_cast<int>(cfg_.pressure_form), it, 0, thr,
+                                                part_d_.data(), st, vs_.data());
             WSIM_LAUNCH_CHECK();
             tend();
         }
@@ -691,6 +839,7 @@ void DFSPHSolver::divergence_solve(float dt) {
     const u32 n = static_cast<u32>(fluid_.count());
     const u32 blocks = div_up(n, NT);
     const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, n);
+    const PairList L = pair_list();
     const double thr = static_cast<double>(cfg_.eta_divergence * cfg_.rest_density);
     DeviceState* st = dstate_.data();
     k_reset_solve<<<1, 64, 0, stream_>>>(st, 1);
@@ -701,18 +850,18 @@ void DFSPHSolver::divergence_solve(float dt) {
         const u32 end = (maxit - it < chunk) ? maxit : it + chunk;
         for (; it < end; ++it) {
             tbegin(kKDivChange, static_cast<int>(it));
-            k_change<1><<<blocks, NT, 0, stream_>>>(P, xs_.data(), vs_.data(), rho_.data(),
+            k_change<1><<<blocks, NT, 0, stream_>>>(P, L, xs_.data(), vs_.data(), rho_.data(),
                                                     alpha_.data(), dt, dt * dt, cfg_.rest_density,
-                                                    static_cast<int>(cfg_.pressure_form), it, thr,
+                                                    static_cast<int>(cfg_.pressure_form), it,
                                                     st, part_d_.data(), rho_adv_.data(),
                                                     kappa_.data(), xk_.data());
             WSIM_LAUNCH_CHECK();
             tend();
             tbegin(kKDivApply, static_cast<int>(it));
-            k_apply<<<blocks, NT, 0, stream_>>>(P, xk_.data(), kappa_.data(), dt,
+            k_apply<<<blocks, NT, 0, stream_>>>(P, L, xk_.data(), kappa_.data(), dt,
                                                 cfg_.rest_density,
-                                                static_cast<int>(cfg_.pressure_form), it,
-                                                &st->div_conv_iter, vs_.data());
+                                                static_cast<int>(cfg_.pressure_form), it, 1, thr,
+                                                part_d_.data(), st, vs_.data());
             WSIM_LAUNCH_CHECK();
             tend();
         }
@@ -734,7 +883,8 @@ void DFSPHSolver::step(float dt) {
     sort_and_reorder(true, dt);
     Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, n);
     tbegin(kKDensityAlpha);
-    k_density_alpha<<<blocks, NT, 0, stream_>>>(P, xs_.data(), rho_.data(), alpha_.data());
+    k_density_alpha<<<blocks, NT, 0, stream_>>>(P, pair_list(), xs_.data(), rho_.data(),
+                                                alpha_.data());
     WSIM_LAUNCH_CHECK();
     tend();
     if (cfg_.viscosity > 0.0f) {  // dfsph.cu:198
@@ -761,15 +911,20 @@ void DFSPHSolver::step(float dt) {
         cfg_.domain_min[0] + eps, cfg_.domain_min[1] + eps, cfg_.domain_min[2] + eps,
         cfg_.domain_max[0] - eps, cfg_.domain_max[1] - eps, cfg_.domain_max[2] - eps, out,
         rho_.data(), alpha_.data(), kappa_.data(), rho_adv_.data(),
-        cfg_.max_density_iters > 0 ? 1 : 0, dstate_.data(), part_f_.data(), part_u_.data());
+        cfg_.max_density_iters > 0 ? 1 : 0, part_f_.data(), part_u_.data());
     WSIM_LAUNCH_CHECK();
     tend();
+    if (!cfg_.enable_divergence_solve) {
+        k_finish_vstats<<<1, NT, 0, stream_>>>(part_f_.data(), part_u_.data(), blocks, dstate_.data());
+        WSIM_LAUNCH_CHECK();
+    }
 
     if (cfg_.enable_divergence_solve) {
         sort_and_reorder(false, dt);
         P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, n);
         tbegin(kKDensityAlpha);
-        k_density_alpha<<<blocks, NT, 0, stream_>>>(P, xs_.data(), rho_.data(), alpha_.data());
+        k_density_alpha<<<blocks, NT, 0, stream_>>>(P, pair_list(), xs_.data(), rho_.data(),
+                                                    alpha_.data());
         WSIM_LAUNCH_CHECK();
         tend();
         divergence_solve(dt);
@@ -778,9 +933,11 @@ void DFSPHSolver::step(float dt) {
         k_div_writeback<<<blocks, NT, 0, stream_>>>(
             n, xs_.data(), vs_.data(), fluid_grid_.sorted_indices(), out, rho_.data(),
             alpha_.data(), kappa_.data(), rho_adv_.data(), cfg_.max_divergence_iters > 0 ? 1 : 0,
-            dstate_.data(), part_f_.data(), part_u_.data());
+            part_f_.data(), part_u_.data());
         WSIM_LAUNCH_CHECK();
         tend();
+        k_finish_vstats<<<1, NT, 0, stream_>>>(part_f_.data(), part_u_.data(), blocks, dstate_.data());
+        WSIM_LAUNCH_CHECK();
     }
     read_state();
 }

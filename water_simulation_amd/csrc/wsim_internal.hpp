@@ -184,6 +184,19 @@ private:
     int open_kind_ = 0, open_iter_ = 0;
 };
 
+// Per-particle pair lists of the pressure solves (see solver.hip).
+struct PairList {
+    uint2* f;   // fluid pairs (slot, bits of c), ELL [k * n + i]
+    uint2* b;   // boundary pairs
+    u32* cnt;   // fluid count | boundary count << 16 (saturating)
+    u32 kf, kb; // list depths
+};
+
+// Depth of the per-particle pair lists (pairs with a non-zero kernel
+// gradient); a particle with more pairs walks the stencil instead.
+constexpr u32 kPairDepthFluid = 48;
+constexpr u32 kPairDepthBoundary = 48;
+
 class DFSPHSolver {
 public:
     DFSPHSolver(ParticleStore& fluid, ParticleStore* boundary, const DFSPHConfig& cfg,
@@ -207,6 +220,7 @@ private:
     void divergence_solve(float dt);
     void sort_and_reorder(bool gravity, float dt);
     void read_state();
+    PairList pair_list();
     void tbegin(int kind, int iter = 0) {
         if (timing_) timer_.begin(kind, iter, stream_);
     }
@@ -230,6 +244,8 @@ private:
 
     // cell-sorted working set
     DeviceBuffer<float4> xs_, vs_, vs2_, xk_, xb_;
+    DeviceBuffer<uint2> pairs_f_, pairs_b_;
+    DeviceBuffer<u32> pair_cnt_;
     DeviceBuffer<float> rho_, alpha_, kappa_, rho_adv_;
     DeviceBuffer<double> part_d_;
     DeviceBuffer<float> part_f_;
