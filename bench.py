@@ -96,7 +96,22 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--kernel-times", action="store_true", help="print per-kernel table to stderr")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run libwsim's streaming copy kernel 5 times over BYTES bytes "
+                         "(the known-size workload for converting HBM byte counters)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import water_simulation_amd as ws
+        from water_simulation_amd import _ffi
+        a = ws.DeviceBuffer(args.calibrate_copy)
+        b = ws.DeviceBuffer(args.calibrate_copy)
+        ws.memcpy_h2d(a.ptr, np.zeros(args.calibrate_copy, np.uint8))
+        for _ in range(5):
+            _ffi.check(_ffi.lib().wsim_device_copy(b.ptr, a.ptr, args.calibrate_copy, None))
+        ws.device_synchronize()
+        print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy, "launches": 5}))
+        return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one MI355X only (the path is single-GPU; "

@@ -47,6 +47,9 @@ wsim_status wsim_memcpy_h2d(void* dst_device, const void* src_host, size_t bytes
 wsim_status wsim_memcpy_d2h(void* dst_host, const void* src_device, size_t bytes, void* stream);
 wsim_status wsim_memcpy_d2d(void* dst_device, const void* src_device, size_t bytes, void* stream);
 wsim_status wsim_memset(void* dst_device, int value, size_t bytes, void* stream);
+/* Device-to-device copy by libwsim's own streaming kernel (no reference
+ * counterpart): the known-size workload used to calibrate HBM byte counters. */
+wsim_status wsim_device_copy(void* dst_device, const void* src_device, size_t bytes, void* stream);
 wsim_status wsim_stream_create(void** out_stream);
 wsim_status wsim_stream_destroy(void* stream);
 wsim_status wsim_stream_synchronize(void* stream);

@@ -58,6 +58,7 @@ class OrcStats(C.Structure):
 vp, sz = C.c_void_p, C.c_size_t
 _SIGS = {
     "orc_set_threads": (None, [C.c_int]),
+    "orc_set_xsph_in_place": (None, [C.c_int]),
     "orc_get_threads": (C.c_int, []),
     "orc_W_f32": (C.c_float, [C.c_float, C.c_float]),
     "orc_W_f64": (C.c_double, [C.c_double, C.c_double]),

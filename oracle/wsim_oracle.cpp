@@ -37,6 +37,11 @@ using u32 = uint32_t;
 
 template <class R> struct V3 { R x, y, z; };
 
+// Test-only switch: XSPH as an in-place sequential sweep in original index
+// order (one of the schedules the reference's racy in-place kernel can
+// produce, dfsph_kernels.cu:317,338-340) instead of the Jacobi sweep.
+int g_xsph_in_place = 0;
+
 // Constants are the reference's binary32 literals, widened for R = double.
 template <class R> inline R kpi() { return R(3.14159265358979323846f); }
 template <class R> inline R k1em9() { return R(1e-9f); }
@@ -304,7 +309,8 @@ template <class R> struct Solver {
     void xsph() {
         if (cfg.viscosity <= 0.0f) return;  // dfsph.cu:198
         const R eps = R(cfg.viscosity);
-#pragma omp parallel for schedule(static)
+        if (g_xsph_in_place) vtmp = v;
+#pragma omp parallel for schedule(static) if (!g_xsph_in_place)
         for (long long i = 0; i < (long long)n; ++i) {
             const V3<R> pi = x[i];
             const V3<R> vi = v[i];
@@ -313,7 +319,7 @@ template <class R> struct Solver {
             fg.for_neighbors(pi, [&](u32 j) {
                 if (j == u32(i)) return;
                 const V3<R> pj = x[j];
-                const V3<R> vj = v[j];
+                const V3<R> vj = g_xsph_in_place ? vtmp[j] : v[j];
                 const V3<R> r{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
                 const R d = std::sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
                 const R w = W(d, h);
@@ -560,6 +566,8 @@ u32 spread3(u32 v) {
 }  // namespace
 
 extern "C" {
+
+void orc_set_xsph_in_place(int on) { g_xsph_in_place = on; }
 
 void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 int orc_get_threads(void) { return omp_get_max_threads(); }

@@ -77,6 +77,9 @@ typedef struct {
 } orc_stats;
 
 void orc_set_threads(int n);
+/* Test-only: XSPH as an in-place sequential sweep (one schedule of the
+ * reference's racy in-place kernel) instead of the Jacobi sweep. */
+void orc_set_xsph_in_place(int on);
 int  orc_get_threads(void);
 
 /* --- SPH kernel (core/include/water/core/sph_kernels.cuh:12-41) --- */

@@ -102,6 +102,7 @@ SIGNATURES = {
     "wsim_memcpy_d2h": (st, [vp, vp, sz, vp]),
     "wsim_memcpy_d2d": (st, [vp, vp, sz, vp]),
     "wsim_memset": (st, [vp, C.c_int, sz, vp]),
+    "wsim_device_copy": (st, [vp, vp, sz, vp]),
     "wsim_stream_create": (st, [C.POINTER(vp)]),
     "wsim_stream_destroy": (st, [vp]),
     "wsim_stream_synchronize": (st, [vp]),

@@ -110,6 +110,14 @@ wsim_status wsim_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stre
         WSIM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
     });
 }
+wsim_status wsim_device_copy(void* dst, const void* src, size_t bytes, void* stream) {
+    return guarded([&] {
+        if (!bytes) return;
+        need(dst, "dst");
+        need(src, "src");
+        wsim::device_copy(dst, src, bytes, as_stream(stream));
+    });
+}
 wsim_status wsim_memset(void* dst, int value, size_t bytes, void* stream) {
     return guarded([&] {
         if (!bytes) return;

@@ -78,6 +78,9 @@ private:
     int cur_ = 0;
 };
 
+// Plain 16-byte streaming copy kernel (copy.hip).
+void device_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
+
 // Cell-start table from sorted keys (one pass over the C+1 cells).
 void build_cell_start(const u32* sorted_keys, u32 n, u32 num_cells_plus_one, u32* cell_start,
                       hipStream_t stream);
