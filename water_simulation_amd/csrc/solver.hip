@@ -38,6 +38,7 @@ namespace {
 
 constexpr int NT = 256;
 constexpr u32 kNone = 0xFFFFFFFFu;
+constexpr int PU = 4;  // pairs per trip of the list loops (loads issued together)
 
 // Neighbour-search inputs shared by the stencil kernels.
 struct Nbr {
@@ -143,6 +144,13 @@ __global__ __launch_bounds__(NT) void k_pack_boundary(u32 nb, const u32* __restr
 // contiguous block, row k of it holding entry k of each lane.
 __device__ __forceinline__ u64 pair_at(u32 k, u32 i, u32 depth) {
     return (static_cast<u64>(i >> 6) * depth + k) * 64u + (i & 63u);
+}
+
+// List entries are streamed once per traversal: load them non-temporally so
+// they do not evict the gathered neighbour data from L2.
+__device__ __forceinline__ uint2 ld_pair(const uint2* p) {
+    const u64 v = __builtin_nontemporal_load(reinterpret_cast<const u64*>(p));
+    return make_uint2(static_cast<u32>(v), static_cast<u32>(v >> 32));
 }
 
 __device__ __forceinline__ u32 pack_counts(u32 cf, u32 cb) {
@@ -270,39 +278,39 @@ __global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* 
         const u32 cw = L.cnt[i];
         const u32 cf = cw & 0xFFFFu, cb = cw >> 16;
         if (cf <= L.kf && cb <= L.kb) {
-            // four pairs per trip: loads issued together, terms summed in order
+            // PU pairs per trip: loads issued together, terms summed in order
             u32 k = 0;
-            for (; k + 4 <= cf; k += 4) {
-                uint2 e[4];
-                float4 q[4], w[4];
+            for (; k + PU <= cf; k += PU) {
+                uint2 e[PU];
+                float4 q[PU], w[PU];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) e[u] = L.f[pair_at(k + u, i, L.kf)];
+                for (int u = 0; u < PU; ++u) e[u] = ld_pair(L.f + pair_at(k + u, i, L.kf));
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < PU; ++u) {
                     q[u] = xs[e[u].x];
                     w[u] = vs[e[u].x];
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
+                for (int u = 0; u < PU; ++u)
                     rho_dot_fluid(rd, p, vi, q[u], w[u], __uint_as_float(e[u].y), P.mass);
             }
             for (; k < cf; ++k) {
-                const uint2 e = L.f[pair_at(k, i, L.kf)];
+                const uint2 e = ld_pair(L.f + pair_at(k, i, L.kf));
                 rho_dot_fluid(rd, p, vi, xs[e.x], vs[e.x], __uint_as_float(e.y), P.mass);
             }
             k = 0;
-            for (; k + 4 <= cb; k += 4) {
-                uint2 e[4];
-                float4 q[4];
+            for (; k + PU <= cb; k += PU) {
+                uint2 e[PU];
+                float4 q[PU];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) e[u] = L.b[pair_at(k + u, i, L.kb)];
+                for (int u = 0; u < PU; ++u) e[u] = ld_pair(L.b + pair_at(k + u, i, L.kb));
 #pragma unroll
-                for (int u = 0; u < 4; ++u) q[u] = P.xb[e[u].x];
+                for (int u = 0; u < PU; ++u) q[u] = P.xb[e[u].x];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) rho_dot_boundary(rd, p, vi, q[u], __uint_as_float(e[u].y));
+                for (int u = 0; u < PU; ++u) rho_dot_boundary(rd, p, vi, q[u], __uint_as_float(e[u].y));
             }
             for (; k < cb; ++k) {
-                const uint2 e = L.b[pair_at(k, i, L.kb)];
+                const uint2 e = ld_pair(L.b + pair_at(k, i, L.kb));
                 rho_dot_boundary(rd, p, vi, P.xb[e.x], __uint_as_float(e.y));
             }
         } else {
@@ -399,36 +407,36 @@ __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* _
     const u32 cf = cw & 0xFFFFu, cb = cw >> 16;
     if (cf <= L.kf && cb <= L.kb) {
         u32 k = 0;
-        for (; k + 4 <= cf; k += 4) {
-            uint2 e[4];
-            float4 q[4];
+        for (; k + PU <= cf; k += PU) {
+            uint2 e[PU];
+            float4 q[PU];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) e[u] = L.f[pair_at(k + u, i, L.kf)];
+            for (int u = 0; u < PU; ++u) e[u] = ld_pair(L.f + pair_at(k + u, i, L.kf));
 #pragma unroll
-            for (int u = 0; u < 4; ++u) q[u] = xk[e[u].x];
+            for (int u = 0; u < PU; ++u) q[u] = xk[e[u].x];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < PU; ++u)
                 kappa_term(dx, dy, dz, p, q[u], __uint_as_float(e[u].y), P.mass * (ki + q[u].w), dt);
         }
         for (; k < cf; ++k) {
-            const uint2 e = L.f[pair_at(k, i, L.kf)];
+            const uint2 e = ld_pair(L.f + pair_at(k, i, L.kf));
             const float4 q = xk[e.x];
             kappa_term(dx, dy, dz, p, q, __uint_as_float(e.y), P.mass * (ki + q.w), dt);
         }
         k = 0;
-        for (; k + 4 <= cb; k += 4) {
-            uint2 e[4];
-            float4 q[4];
+        for (; k + PU <= cb; k += PU) {
+            uint2 e[PU];
+            float4 q[PU];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) e[u] = L.b[pair_at(k + u, i, L.kb)];
+            for (int u = 0; u < PU; ++u) e[u] = ld_pair(L.b + pair_at(k + u, i, L.kb));
 #pragma unroll
-            for (int u = 0; u < 4; ++u) q[u] = P.xb[e[u].x];
+            for (int u = 0; u < PU; ++u) q[u] = P.xb[e[u].x];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < PU; ++u)
                 kappa_term(dx, dy, dz, p, q[u], __uint_as_float(e[u].y), q[u].w * (ki + bf), dt);
         }
         for (; k < cb; ++k) {
-            const uint2 e = L.b[pair_at(k, i, L.kb)];
+            const uint2 e = ld_pair(L.b + pair_at(k, i, L.kb));
             const float4 q = P.xb[e.x];
             kappa_term(dx, dy, dz, p, q, __uint_as_float(e.y), q.w * (ki + bf), dt);
         }
