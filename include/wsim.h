@@ -179,6 +179,56 @@ const char* wsim_solver_kernel_name(int kind);
 wsim_status wsim_solver_kernel_time(wsim_solver solver, int kind, uint64_t* launches,
                                     double* total_ms);
 
+/* ---- launch shapes, autotuner, graph replay ----
+ * No counterpart in the reference (its kernels use fixed 128/256-thread
+ * blocks, dfsph.cu:137-326).  Every shape computes the same bits. */
+typedef struct {
+    uint32_t density_alpha_nt;   /* workgroup size: 64, 128, 256, 512 */
+    uint32_t xsph_nt;            /* 64, 128, 256, 512 */
+    uint32_t change_nt;          /* 256, 512, 1024 */
+    uint32_t change_pu;          /* pairs per trip of the list loops: 2, 4, 8 */
+    uint32_t apply_nt;           /* 64, 128, 256, 512 */
+    uint32_t apply_pu;           /* 2, 4, 8 */
+    uint32_t cell_cpt;           /* cells per thread of the cell_start pass: 4, 8, 16 */
+    uint32_t remap;              /* 1: XCD-aware workgroup order */
+} wsim_kernel_shapes;
+wsim_status wsim_solver_get_shapes(wsim_solver solver, wsim_kernel_shapes* out);
+wsim_status wsim_solver_set_shapes(wsim_solver solver, const wsim_kernel_shapes* shapes);
+/* Times every candidate shape on the current particle state (reps launches
+ * each, median) and keeps the fastest per kernel.  Leaves the stores and the
+ * results of the following steps unchanged. */
+wsim_status wsim_solver_autotune(wsim_solver solver, int reps);
+/* Candidates timed by the last autotune: entry i gives
+ * {kernel kind, workgroup size, pairs per trip, remap, cells per thread}
+ * and the median milliseconds per launch. */
+size_t wsim_solver_tune_count(wsim_solver solver);
+wsim_status wsim_solver_tune_entry(wsim_solver solver, size_t i, uint32_t shape[5], double* ms);
+/* Replays each substep phase as a captured HIP graph (one per phase and per
+ * iteration chunk).  Off by default for a plain solver, on for a batch;
+ * ignored while kernel timing is on. */
+wsim_status wsim_solver_set_graphs(wsim_solver solver, int enable);
+
+/* ---- batched scenes ----
+ * No counterpart in the reference, which steps one scene per solver.
+ * num_scenes independent scenes share the config (except the XSPH viscosity)
+ * and the boundary store.  Scene s owns the fluid store indices
+ * [offsets[s], offsets[s+1]), offsets being the prefix sums of counts; the
+ * store count must equal their sum.  Every scene steps with its own dt,
+ * convergence test and statistics and gets the same results, bit for bit, as
+ * a plain solver stepping that scene alone with the same dts.  The handle is
+ * a wsim_solver: destroy, stats (scene 0), shapes, autotune, timing and
+ * graphs work as for a plain solver. */
+wsim_status wsim_batch_create(wsim_store fluid, wsim_store boundary, const wsim_dfsph_config* cfg,
+                              uint32_t num_scenes, const uint32_t* counts,
+                              const float* viscosity /* num_scenes values, or NULL */,
+                              void* stream, wsim_solver* out);
+/* dt: num_scenes substep lengths. */
+wsim_status wsim_batch_step(wsim_solver batch, const float* dt);
+uint32_t wsim_batch_scene_count(wsim_solver batch);
+/* out: num_scenes + 1 offsets. */
+wsim_status wsim_batch_scene_offsets(wsim_solver batch, uint32_t* out);
+wsim_status wsim_batch_scene_stats(wsim_solver batch, uint32_t scene, wsim_step_stats* out);
+
 /* ---- host-side surface ---- */
 /* sample_aabb_boundary(lo, hi, spacing, n_layers)
  * (solvers/dfsph/include/water/solvers/boundary_sampler.h:24-25).  Writes up to

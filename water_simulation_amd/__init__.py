@@ -28,7 +28,7 @@ from ._ffi import DeviceError, InvalidArgument, OutOfRange, WsimError, check
 
 __all__ = [
     "AttribType", "ParticleStore", "CellGrid", "SpatialAccel", "DFSPHConfig", "DFSPHSolver",
-    "TimeStepper", "TimeStepperConfig", "Scene", "load_scene", "sample_aabb_boundary",
+    "BatchSolver", "TimeStepper", "TimeStepperConfig", "Scene", "load_scene", "sample_aabb_boundary",
     "lattice_block", "write_frame", "read_frame", "DeviceBuffer", "memcpy_h2d", "memcpy_d2h",
     "device_count", "device_synchronize", "WsimError", "OutOfRange", "InvalidArgument",
     "DeviceError", "library_path",
@@ -436,6 +436,88 @@ class DFSPHSolver:
             check(_L().wsim_solver_kernel_time(self._h, k, C.byref(n), C.byref(ms)))
             out[_L().wsim_solver_kernel_name(k).decode()] = (n.value, ms.value)
         return out
+
+    # launch shapes, autotuner and graph replay (no counterpart in the reference)
+    def shapes(self) -> dict:
+        c = _ffi.KernelShapesC()
+        check(_L().wsim_solver_get_shapes(self._h, C.byref(c)))
+        return {f[0]: getattr(c, f[0]) for f in c._fields_}
+
+    def set_shapes(self, **kw) -> None:
+        cur = self.shapes()
+        unknown = set(kw) - set(cur)
+        if unknown:
+            raise ValueError(f"unknown shape fields {sorted(unknown)}")
+        cur.update(kw)
+        c = _ffi.KernelShapesC(**cur)
+        check(_L().wsim_solver_set_shapes(self._h, C.byref(c)))
+
+    def autotune(self, reps: int = 5) -> list:
+        """Times every launch-shape candidate on the current state and keeps
+        the fastest; returns [(kernel, workgroup, pairs/trip, remap, cells/thread, ms)]."""
+        check(_L().wsim_solver_autotune(self._h, int(reps)))
+        out = []
+        shape = (C.c_uint32 * 5)()
+        ms = C.c_double()
+        for i in range(_L().wsim_solver_tune_count(self._h)):
+            check(_L().wsim_solver_tune_entry(self._h, i, shape, C.byref(ms)))
+            out.append((_L().wsim_solver_kernel_name(shape[0]).decode(), shape[1], shape[2],
+                        shape[3], shape[4], ms.value))
+        return out
+
+    def set_graphs(self, on: bool) -> None:
+        check(_L().wsim_solver_set_graphs(self._h, 1 if on else 0))
+
+
+class BatchSolver(DFSPHSolver):
+    """Many independent scenes of uneven sizes in one solver: one packed
+    working set, one segmented grid sort and one launch per phase, substeps
+    replayed as HIP graphs.  Scene s owns fluid-store indices
+    [offsets[s], offsets[s+1]); it steps with its own dt, viscosity,
+    convergence test and statistics, and matches a plain DFSPHSolver on that
+    scene alone bit for bit.  The boundary store is shared by all scenes."""
+
+    def __init__(self, fluid: ParticleStore, boundary: Optional[ParticleStore],
+                 cfg: DFSPHConfig, counts, viscosity=None, stream=None):
+        self._fluid = fluid
+        self._boundary = boundary
+        self.cfg = cfg
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        self.num_scenes = int(counts.size)
+        visc = None
+        if viscosity is not None:
+            visc = np.ascontiguousarray(viscosity, dtype=np.float32)
+            if visc.size != counts.size:
+                raise ValueError("one viscosity per scene")
+        h = C.c_void_p()
+        c = cfg.to_c()
+        self._h = None
+        check(_L().wsim_batch_create(
+            fluid.handle, boundary.handle if boundary else None, C.byref(c), self.num_scenes,
+            counts.ctypes.data_as(C.POINTER(C.c_uint32)),
+            visc.ctypes.data_as(C.POINTER(C.c_float)) if visc is not None else None,
+            _stream(stream), C.byref(h)))
+        self._h = h.value
+        off = (C.c_uint32 * (self.num_scenes + 1))()
+        check(_L().wsim_batch_scene_offsets(self._h, off))
+        self.offsets = np.array(off[:], dtype=np.int64)
+
+    def step(self, dt) -> None:
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float32), (self.num_scenes,)),
+                                 dtype=np.float32)
+        check(_L().wsim_batch_step(self._h, d.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def scene_stats(self, s: int) -> dict:
+        st = _ffi.StepStatsC()
+        check(_L().wsim_batch_scene_stats(self._h, int(s), C.byref(st)))
+        return {f[0]: getattr(st, f[0]) for f in st._fields_}
+
+    def all_stats(self) -> list:
+        return [self.scene_stats(s) for s in range(self.num_scenes)]
+
+    def max_velocities(self) -> np.ndarray:
+        return np.array([self.scene_stats(s)["max_velocity"] for s in range(self.num_scenes)],
+                        np.float32)
 
 
 # --------------------------------------------------------------------------

@@ -52,6 +52,19 @@ class StepStatsC(C.Structure):
     ]
 
 
+class KernelShapesC(C.Structure):
+    _fields_ = [
+        ("density_alpha_nt", C.c_uint32),
+        ("xsph_nt", C.c_uint32),
+        ("change_nt", C.c_uint32),
+        ("change_pu", C.c_uint32),
+        ("apply_nt", C.c_uint32),
+        ("apply_pu", C.c_uint32),
+        ("cell_cpt", C.c_uint32),
+        ("remap", C.c_uint32),
+    ]
+
+
 class TimeStepConfigC(C.Structure):
     _fields_ = [
         ("frame_dt", C.c_float),
@@ -156,6 +169,18 @@ SIGNATURES = {
     "wsim_solver_kernel_kinds": (C.c_int, []),
     "wsim_solver_kernel_name": (C.c_char_p, [C.c_int]),
     "wsim_solver_kernel_time": (st, [vp, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+    "wsim_solver_get_shapes": (st, [vp, C.POINTER(KernelShapesC)]),
+    "wsim_solver_set_shapes": (st, [vp, C.POINTER(KernelShapesC)]),
+    "wsim_solver_autotune": (st, [vp, C.c_int]),
+    "wsim_solver_tune_count": (sz, [vp]),
+    "wsim_solver_tune_entry": (st, [vp, sz, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
+    "wsim_solver_set_graphs": (st, [vp, C.c_int]),
+    "wsim_batch_create": (st, [vp, vp, C.POINTER(DFSPHConfigC), C.c_uint32,
+                               C.POINTER(C.c_uint32), C.POINTER(C.c_float), vp, C.POINTER(vp)]),
+    "wsim_batch_step": (st, [vp, C.POINTER(C.c_float)]),
+    "wsim_batch_scene_count": (C.c_uint32, [vp]),
+    "wsim_batch_scene_offsets": (st, [vp, C.POINTER(C.c_uint32)]),
+    "wsim_batch_scene_stats": (st, [vp, C.c_uint32, C.POINTER(StepStatsC)]),
     "wsim_sample_aabb_boundary": (st, [F3, F3, C.c_float, C.c_int, vp, sz, C.POINTER(sz)]),
     "wsim_timestep_config_default": (None, [C.POINTER(TimeStepConfigC)]),
     "wsim_timestep_next_dt": (C.c_float, [C.POINTER(TimeStepConfigC), C.c_float, C.c_float,

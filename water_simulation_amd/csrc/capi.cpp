@@ -5,6 +5,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/wsim.h"
 #include "host_surface.hpp"
@@ -314,6 +315,39 @@ void wsim_dfsph_config_default(wsim_dfsph_config* c) {
     c->pressure_form = d.pressure_form;
 }
 
+static wsim::DFSPHConfig to_config(const wsim_dfsph_config* c) {
+    if (c->pressure_form > 1) wsim::fail(wsim::kInvalidArgument, "pressure_form must be 0 or 1");
+    wsim::DFSPHConfig d;
+    d.rest_density = c->rest_density;
+    d.particle_radius = c->particle_radius;
+    d.smoothing_length = c->smoothing_length;
+    d.viscosity = c->viscosity;
+    d.damping = c->damping;
+    d.surface_tension = c->surface_tension;
+    for (int a = 0; a < 3; ++a) {
+        d.gravity[a] = c->gravity[a];
+        d.domain_min[a] = c->domain_min[a];
+        d.domain_max[a] = c->domain_max[a];
+    }
+    d.max_density_iters = c->max_density_iters;
+    d.max_divergence_iters = c->max_divergence_iters;
+    d.eta_density = c->eta_density;
+    d.eta_divergence = c->eta_divergence;
+    d.enable_divergence_solve = c->enable_divergence_solve;
+    d.pressure_form = c->pressure_form;
+    return d;
+}
+
+static void fill_stats(const wsim::StepStats& st, wsim_step_stats* out) {
+    out->density_iters = st.density_iters;
+    out->divergence_iters = st.divergence_iters;
+    out->mean_density_error = st.mean_density_error;
+    out->mean_divergence_error = st.mean_divergence_error;
+    out->max_velocity = st.max_velocity;
+    out->nan_count = st.nan_count;
+    out->substeps = st.substeps;
+}
+
 wsim_status wsim_solver_create(wsim_store fluid, wsim_store boundary, const wsim_dfsph_config* c,
                                void* stream, wsim_solver* out) {
     return guarded([&] {
@@ -321,25 +355,7 @@ wsim_status wsim_solver_create(wsim_store fluid, wsim_store boundary, const wsim
         need(fluid, "fluid");
         need(c, "cfg");
         *out = nullptr;
-        if (c->pressure_form > 1) wsim::fail(wsim::kInvalidArgument, "pressure_form must be 0 or 1");
-        wsim::DFSPHConfig d;
-        d.rest_density = c->rest_density;
-        d.particle_radius = c->particle_radius;
-        d.smoothing_length = c->smoothing_length;
-        d.viscosity = c->viscosity;
-        d.damping = c->damping;
-        d.surface_tension = c->surface_tension;
-        for (int a = 0; a < 3; ++a) {
-            d.gravity[a] = c->gravity[a];
-            d.domain_min[a] = c->domain_min[a];
-            d.domain_max[a] = c->domain_max[a];
-        }
-        d.max_density_iters = c->max_density_iters;
-        d.max_divergence_iters = c->max_divergence_iters;
-        d.eta_density = c->eta_density;
-        d.eta_divergence = c->eta_divergence;
-        d.enable_divergence_solve = c->enable_divergence_solve;
-        d.pressure_form = c->pressure_form;
+        const wsim::DFSPHConfig d = to_config(c);
         auto h = std::make_unique<wsim_solver_t>();
         h->impl = std::make_unique<wsim::DFSPHSolver>(
             *fluid->impl, boundary ? boundary->impl.get() : nullptr, d, as_stream(stream));
@@ -376,14 +392,7 @@ wsim_status wsim_solver_stats(wsim_solver s, wsim_step_stats* out) {
     return guarded([&] {
         need(s, "solver");
         need(out, "out");
-        const wsim::StepStats& st = s->impl->stats();
-        out->density_iters = st.density_iters;
-        out->divergence_iters = st.divergence_iters;
-        out->mean_density_error = st.mean_density_error;
-        out->mean_divergence_error = st.mean_divergence_error;
-        out->max_velocity = st.max_velocity;
-        out->nan_count = st.nan_count;
-        out->substeps = st.substeps;
+        fill_stats(s->impl->stats(), out);
     });
 }
 wsim_grid wsim_solver_fluid_grid(wsim_solver s) { return s ? &s->fluid_view : nullptr; }
@@ -411,6 +420,122 @@ wsim_status wsim_solver_kernel_time(wsim_solver s, int kind, uint64_t* launches,
         const auto& a = s->impl->timer().acc[kind];
         if (launches) *launches = a.count;
         if (total_ms) *total_ms = a.total_ms;
+    });
+}
+
+// ---- launch shapes, autotuner, graphs ----
+static wsim::KernelShapes from_c(const wsim_kernel_shapes* c) {
+    wsim::KernelShapes k;
+    k.density_alpha_nt = c->density_alpha_nt;
+    k.xsph_nt = c->xsph_nt;
+    k.change_nt = c->change_nt;
+    k.change_pu = c->change_pu;
+    k.apply_nt = c->apply_nt;
+    k.apply_pu = c->apply_pu;
+    k.cell_cpt = c->cell_cpt;
+    k.remap = c->remap;
+    return k;
+}
+wsim_status wsim_solver_get_shapes(wsim_solver s, wsim_kernel_shapes* out) {
+    return guarded([&] {
+        need(s, "solver");
+        need(out, "out");
+        const wsim::KernelShapes& k = s->impl->shapes();
+        out->density_alpha_nt = k.density_alpha_nt;
+        out->xsph_nt = k.xsph_nt;
+        out->change_nt = k.change_nt;
+        out->change_pu = k.change_pu;
+        out->apply_nt = k.apply_nt;
+        out->apply_pu = k.apply_pu;
+        out->cell_cpt = k.cell_cpt;
+        out->remap = k.remap;
+    });
+}
+wsim_status wsim_solver_set_shapes(wsim_solver s, const wsim_kernel_shapes* shapes) {
+    return guarded([&] {
+        need(s, "solver");
+        need(shapes, "shapes");
+        s->impl->set_shapes(from_c(shapes));
+    });
+}
+wsim_status wsim_solver_autotune(wsim_solver s, int reps) {
+    return guarded([&] {
+        need(s, "solver");
+        s->impl->autotune(reps);
+    });
+}
+size_t wsim_solver_tune_count(wsim_solver s) { return s ? s->impl->tune_log().size() : 0; }
+wsim_status wsim_solver_tune_entry(wsim_solver s, size_t i, uint32_t shape[5], double* ms) {
+    return guarded([&] {
+        need(s, "solver");
+        need(shape, "shape");
+        const auto& log = s->impl->tune_log();
+        if (i >= log.size()) wsim::fail(wsim::kOutOfRange, "tune entry out of range");
+        const wsim::TuneRecord& r = log[i];
+        shape[0] = static_cast<uint32_t>(r.kernel);
+        shape[1] = r.nt;
+        shape[2] = r.pu;
+        shape[3] = r.remap;
+        shape[4] = r.cpt;
+        if (ms) *ms = r.ms;
+    });
+}
+wsim_status wsim_solver_set_graphs(wsim_solver s, int enable) {
+    return guarded([&] {
+        need(s, "solver");
+        s->impl->set_graphs(enable != 0);
+    });
+}
+
+// ---- batched scenes ----
+wsim_status wsim_batch_create(wsim_store fluid, wsim_store boundary, const wsim_dfsph_config* c,
+                              uint32_t num_scenes, const uint32_t* counts, const float* viscosity,
+                              void* stream, wsim_solver* out) {
+    return guarded([&] {
+        need(out, "out");
+        need(fluid, "fluid");
+        need(c, "cfg");
+        need(counts, "counts");
+        *out = nullptr;
+        if (num_scenes == 0) wsim::fail(wsim::kInvalidArgument, "num_scenes must be positive");
+        const wsim::DFSPHConfig d = to_config(c);
+        std::vector<uint32_t> cnt(counts, counts + num_scenes);
+        std::vector<float> visc;
+        if (viscosity) visc.assign(viscosity, viscosity + num_scenes);
+        auto h = std::make_unique<wsim_solver_t>();
+        h->impl = std::make_unique<wsim::DFSPHSolver>(*fluid->impl,
+                                                      boundary ? boundary->impl.get() : nullptr, d,
+                                                      as_stream(stream), cnt, visc);
+        h->fluid_view.view = &h->impl->fluid_grid();
+        h->boundary_view.view = h->impl->boundary_grid();
+        *out = h.release();
+    });
+}
+wsim_status wsim_batch_step(wsim_solver b, const float* dt) {
+    return guarded([&] {
+        need(b, "batch");
+        need(dt, "dt");
+        if (!b->impl->batched()) wsim::fail(wsim::kInvalidArgument, "not a batched solver");
+        b->impl->step_scenes(dt);
+    });
+}
+uint32_t wsim_batch_scene_count(wsim_solver b) {
+    return b ? static_cast<uint32_t>(b->impl->scene_count()) : 0u;
+}
+wsim_status wsim_batch_scene_offsets(wsim_solver b, uint32_t* out) {
+    return guarded([&] {
+        need(b, "batch");
+        need(out, "out");
+        if (!b->impl->batched()) wsim::fail(wsim::kInvalidArgument, "not a batched solver");
+        const auto& st = b->impl->scene_starts();
+        std::memcpy(out, st.data(), st.size() * sizeof(uint32_t));
+    });
+}
+wsim_status wsim_batch_scene_stats(wsim_solver b, uint32_t scene, wsim_step_stats* out) {
+    return guarded([&] {
+        need(b, "batch");
+        need(out, "out");
+        fill_stats(b->impl->scene_stats(scene), out);
     });
 }
 

@@ -162,27 +162,4 @@ template <int NT> __device__ inline u32 block_sum_u32(u32 v, u32* s) {
     return r;
 }
 
-// Agent-scope loads that bypass the (non-coherent) per-CU vector cache; used
-// by the last workgroup of a launch to read the partials of the others.
-template <class T> __device__ inline T load_agent(const T* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// "Last workgroup done" handshake: every workgroup publishes its partial,
-// fences, and takes a ticket; the workgroup that draws the last ticket
-// reduces all partials in a fixed order and resets the counter for the next
-// launch.  Returns true in all threads of the last workgroup.
-__device__ inline bool last_block_arrive(u32* counter, u32 nblocks, bool* s_flag) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const u32 t = atomicAdd(counter, 1u);
-        *s_flag = (t == nblocks - 1);
-    }
-    __syncthreads();
-    const bool last = *s_flag;
-    if (last) __threadfence();
-    return last;
-}
-
 }  // namespace wsim

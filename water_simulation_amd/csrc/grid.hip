@@ -16,6 +16,11 @@
 //                      suffix-min scans the tile and stores it coalesced.
 // Per-build allocations of the reference (two temporaries per call,
 // cell_grid.cu:94-95) are replaced by buffers sized once by reserve().
+//
+// A segmented build (batched scenes) offsets each scene's keys by
+// scene * (C + 1): one sort then orders the particles scene-major and
+// cell-sorted within each scene, and one cell_start pass over
+// scenes * (C + 1) + 1 entries serves every scene.
 #include "radix_sort.hpp"
 #include "wsim_internal.hpp"
 
@@ -23,8 +28,6 @@ namespace wsim {
 namespace {
 
 constexpr int CS_THREADS = 256;
-constexpr int CS_PER_THREAD = 16;
-constexpr u32 CS_TILE = CS_THREADS * CS_PER_THREAD;  // 4096 cells per workgroup
 
 __device__ inline u32 cs_pad(u32 i) { return i + (i >> 4); }  // odd stride: no bank conflicts
 
@@ -58,8 +61,35 @@ __global__ __launch_bounds__(256) void k_cell_keys(const float* __restrict__ xyz
     vals[i] = i;
 }
 
+// Scene of original index i: the last s with starts[s] <= i (empty scenes
+// share their start with the next scene and are skipped).
+__device__ inline u32 scene_of_index(const u32* __restrict__ starts, u32 scenes, u32 i) {
+    u32 lo = 0, hi = scenes - 1;
+    while (lo < hi) {
+        const u32 mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_seg_keys(const float* __restrict__ xyz, u32 n, GridGeom g,
+                                                  const u32* __restrict__ starts, u32 scenes,
+                                                  u32* __restrict__ keys, u32* __restrict__ vals) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const u64 o = 3ull * i;
+    const u32 s = scene_of_index(starts, scenes, i);
+    keys[i] = s * (g.C + 1u) + g.key_of(xyz[o], xyz[o + 1], xyz[o + 2]);
+    vals[i] = i;
+}
+
+// Each workgroup owns CPT * 256 consecutive cells.
+template <int CPT>
 __global__ __launch_bounds__(CS_THREADS) void k_cell_start(const u32* __restrict__ keys, u32 n,
                                                            u32 ncs, u32* __restrict__ cs) {
+    constexpr int CS_PER_THREAD = CPT;
+    constexpr u32 CS_TILE = CS_THREADS * CPT;
     __shared__ u32 s_mark[CS_TILE + CS_TILE / 16];
     __shared__ u32 s_k[2];
     __shared__ u32 s_wmin[CS_THREADS / 64];
@@ -112,9 +142,24 @@ __global__ __launch_bounds__(CS_THREADS) void k_cell_start(const u32* __restrict
 
 }  // namespace
 
-void build_cell_start(const u32* sorted_keys, u32 n, u32 ncs, u32* cell_start, hipStream_t stream) {
-    const u32 blocks = div_up(ncs, CS_TILE);
-    k_cell_start<<<blocks, CS_THREADS, 0, stream>>>(sorted_keys, n, ncs, cell_start);
+void build_cell_start(const u32* sorted_keys, u32 n, u32 ncs, u32* cell_start, hipStream_t stream,
+                      u32 cpt) {
+    switch (cpt) {
+    case 4:
+        k_cell_start<4><<<div_up(ncs, CS_THREADS * 4), CS_THREADS, 0, stream>>>(sorted_keys, n, ncs,
+                                                                                cell_start);
+        break;
+    case 8:
+        k_cell_start<8><<<div_up(ncs, CS_THREADS * 8), CS_THREADS, 0, stream>>>(sorted_keys, n, ncs,
+                                                                                cell_start);
+        break;
+    case 16:
+        k_cell_start<16><<<div_up(ncs, CS_THREADS * 16), CS_THREADS, 0, stream>>>(sorted_keys, n,
+                                                                                  ncs, cell_start);
+        break;
+    default:
+        fail(kInvalidArgument, "build_cell_start: cells per thread must be 4, 8 or 16");
+    }
     WSIM_LAUNCH_CHECK();
 }
 
@@ -165,7 +210,43 @@ void CellGrid::build(const float* positions_xyz, size_t count) {
     WSIM_LAUNCH_CHECK();
     cur_ = radix_sort_pairs(k_[0].data(), v_[0].data(), k_[1].data(), v_[1].data(), count,
                             key_bits(g_.C), hist_.data(), stream_);
-    build_cell_start(k_[cur_].data(), n, g_.C + 1, cs_.data(), stream_);
+    build_cell_start(k_[cur_].data(), n, g_.C + 1, cs_.data(), stream_, cpt_);
+}
+
+void CellGrid::reserve_segments(u32 scenes) {
+    if (scenes == 0) fail(kInvalidArgument, "CellGrid: at least one scene");
+    if (scenes == segments_) return;
+    const u64 ncs = static_cast<u64>(scenes) * (static_cast<u64>(g_.C) + 1) + 1;
+    if (ncs >= 0xFFFFF000ull) fail(kInvalidArgument, "CellGrid: too many cells over all scenes");
+    cs_ = DeviceBuffer<u32>(ncs);
+    cs_.zero(stream_);
+    segments_ = scenes;
+}
+
+void CellGrid::build_segmented(const float* positions_xyz, size_t count, const u32* starts,
+                               u32 scenes) {
+    if (scenes != segments_) fail(kInvalidArgument, "CellGrid: scene count differs from reserve_segments");
+    if (scenes == 1) {
+        build(positions_xyz, count);
+        return;
+    }
+    count_ = count;
+    if (count == 0) return;
+    if (count > capacity_) reserve(count);
+    const u32 n = static_cast<u32>(count);
+    k_seg_keys<<<div_up(n, 256), 256, 0, stream_>>>(positions_xyz, n, g_, starts, scenes,
+                                                    k_[0].data(), v_[0].data());
+    WSIM_LAUNCH_CHECK();
+    const u32 ncs = scenes * (g_.C + 1) + 1;
+    cur_ = radix_sort_pairs(k_[0].data(), v_[0].data(), k_[1].data(), v_[1].data(), count,
+                            key_bits(ncs - 2), hist_.data(), stream_);
+    build_cell_start(k_[cur_].data(), n, ncs, cs_.data(), stream_, cpt_);
+}
+
+void CellGrid::set_cells_per_thread(u32 cpt) {
+    if (cpt != 4 && cpt != 8 && cpt != 16)
+        fail(kInvalidArgument, "CellGrid: cells per thread must be 4, 8 or 16");
+    cpt_ = cpt;
 }
 
 }  // namespace wsim

@@ -1,19 +1,21 @@
 // solver.hip -- DFSPH substep on cell-sorted particle data for gfx950.
 //
 // Replaces DFSPHSolver::step (solvers/dfsph/src/dfsph.cu:120-133) and its 11
-// kernels (solvers/dfsph/src/dfsph_kernels.cu:40-400).
+// kernels (solvers/dfsph/src/dfsph_kernels.cu:40-400), for one scene or for
+// a batch of independent scenes of uneven sizes.
 //
-// Data flow per substep (all launches on the caller's stream):
+// Data flow per substep (all launches on the solver's stream):
 //   grid build         keys -> stable radix sort -> dense cell_start      (grid.hip)
 //   k_reorder          gather position/velocity into cell-sorted float4
 //                      arrays; gravity + damping applied on the way
 //                      (dfsph_kernels.cu:347-355; density and alpha do not
 //                      read velocity, so applying it first is equivalent)
 //   k_density_alpha    rho and alpha in one stencil walk (dfsph_kernels.cu:68-169)
-//   k_xsph             Jacobi XSPH into a second velocity buffer (dfsph_kernels.cu:305-341)
-//   density solve      k_change<0> / k_apply per iteration; the mean error is
-//                      reduced in binary64 by the last workgroup of k_change
-//                      and the early exit is decided on the device
+//   k_xsph             Jacobi XSPH into the working velocity (dfsph_kernels.cu:305-341)
+//   density solve      k_change<0> / k_apply per iteration; binary64 error
+//                      partials per tile, reduced by the first wave of each
+//                      scene in the following k_apply, which also decides the
+//                      scene's early exit on the device
 //                      (dfsph.cu:215-260, dfsph_kernels.cu:176-301, 389-394)
 //   k_advect           clamp, integrate, domain clamp, scatter position,
 //                      velocity and the solver attributes back to the caller's
@@ -22,12 +24,23 @@
 //   [divergence solve] grid, reorder, density/alpha, k_change<1> / k_apply,
 //                      k_div_writeback (phase-2 plan text :1858-2003)
 //
+// Scenes.  Scene s owns store indices [start_s, start_s + n_s) and working
+// slots [off_s, off_s + n_s), off_s a multiple of kTile, so every tile and
+// every wave belongs to one scene (looked up per tile, read with scalar
+// loads).  The segmented grid gives scene s the keys s*(C+1) + key; its
+// cells' sorted ranges shift by off_s - start_s into working slots.  dt,
+// damping factor, gravity step and XSPH viscosity are per scene; the error
+// sum, the early exit and max |v| are per scene.  A plain solver is the one
+// scene case.
+//
 // Every per-pair expression keeps the reference's operand order, the stencil
 // is walked dz, dy, dx with ascending original id inside each cell (the
 // sorted slots of a cell are in id order), and the code is compiled without
 // FP contraction and with IEEE divide/sqrt: each field equals the binary32
-// oracle bit for bit.
+// oracle bit for bit, for every launch shape and in every batch.
+#include <algorithm>
 #include <cmath>
+#include <initializer_list>
 #include <limits>
 
 #include "radix_sort.hpp"
@@ -36,28 +49,31 @@
 namespace wsim {
 namespace {
 
-constexpr int NT = 256;
 constexpr u32 kNone = 0xFFFFFFFFu;
-constexpr int PU = 4;  // pairs per trip of the list loops (loads issued together)
+constexpr int NS = 256;  // workgroup size of the streaming kernels
 
-// Neighbour-search inputs shared by the stencil kernels.
+// Neighbour-search and scene inputs shared by the per-particle kernels.
 struct Nbr {
     GridGeom gf;
-    const u32* csf;
+    const u32* csf;    // fluid cell_start; scene s's table starts at key_base
     GridGeom gb;
     const u32* csb;
     const float4* xb;  // cell-sorted boundary (x, y, z, mass)
     u32 nb;            // 0: no boundary
     Spline sp;
     float mass;
-    u32 n;
+    u32 nslots;        // working-set slots (multiple of kTile)
+    const SceneSeg* seg;
+    const u32* tile_scene;  // scene of every tile; nullptr: one scene
+    int remap;              // XCD-aware workgroup order
 };
 
 // 27-cell stencil of ITER_NEIGHBORS (dfsph_kernels.cu:13-35): dz, dy, dx in
-// {-1, 0, 1}, out-of-grid cells skipped, then every slot of the cell.
+// {-1, 0, 1}, out-of-grid cells skipped, then every slot of the cell.  Sorted
+// positions map to working slots by adding `shift`.
 template <class F>
-__device__ __forceinline__ void for_stencil(const GridGeom& g, const u32* __restrict__ cs, float px,
-                                            float py, float pz, F&& f) {
+__device__ __forceinline__ void for_stencil(const GridGeom& g, const u32* __restrict__ cs,
+                                            u32 shift, float px, float py, float pz, F&& f) {
     const int cx = g.cx(px), cy = g.cy(py), cz = g.cz(pz);
     for (int dz = -1; dz <= 1; ++dz) {
         const int nz = cz + dz;
@@ -73,73 +89,91 @@ __device__ __forceinline__ void for_stencil(const GridGeom& g, const u32* __rest
                         static_cast<u32>(g.dz) +
                     static_cast<u32>(nz);
                 const u32 b = cs[c], e = cs[c + 1];
-                for (u32 s = b; s < e; ++s) f(s);
+                for (u32 s = b; s < e; ++s) f(s + shift);
             }
         }
     }
 }
 
-__device__ __forceinline__ u32 tile_index(u32 n) {
-    return xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+template <int NT>
+__device__ __forceinline__ u32 slot_index(int remap) {
+    const u32 t = remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+    return t * NT + threadIdx.x;
 }
 
-__global__ __launch_bounds__(NT) void k_reorder(const u32* __restrict__ perm,
+// Scene of working slot i (< nslots).  Waves never straddle tiles, so the
+// result is wave-uniform and the scene table is read with scalar loads.
+__device__ __forceinline__ u32 scene_of_slot(const Nbr& P, u32 i) {
+    if (!P.tile_scene) return 0u;
+    const u32 t = __builtin_amdgcn_readfirstlane(i / kTile);
+    return P.tile_scene[t];
+}
+
+__global__ __launch_bounds__(NS) void k_reorder(Nbr P, const u32* __restrict__ perm,
                                                 const float* __restrict__ x,
-                                                const float* __restrict__ v, u32 n,
+                                                const float* __restrict__ v,
                                                 float4* __restrict__ xs, float4* __restrict__ vs,
-                                                int gravity, float decay, float gdx, float gdy,
-                                                float gdz) {
-    const u32 s = blockIdx.x * NT + threadIdx.x;
-    if (s >= n) return;
-    const u64 o = 3ull * perm[s];
-    xs[s] = make_float4(x[o], x[o + 1], x[o + 2], 0.0f);
-    if (vs) {
-        float a = v[o], b = v[o + 1], c = v[o + 2];
-        if (gravity) {  // v * exp(-damping*dt) + g*dt (dfsph_kernels.cu:351-354)
-            a = a * decay + gdx;
-            b = b * decay + gdy;
-            c = c * decay + gdz;
-        }
-        vs[s] = make_float4(a, b, c, 0.0f);
+                                                int gravity) {
+    const u32 i = blockIdx.x * NS + threadIdx.x;
+    if (i >= P.nslots) return;
+    const SceneSeg& G = P.seg[scene_of_slot(P, i)];
+    const u32 l = i - G.off;
+    if (l >= G.n) return;
+    const u64 o = 3ull * perm[G.start + l];
+    xs[i] = make_float4(x[o], x[o + 1], x[o + 2], 0.0f);
+    float a = v[o], b = v[o + 1], c = v[o + 2];
+    if (gravity) {  // v * exp(-damping*dt) + g*dt (dfsph_kernels.cu:351-354)
+        a = a * G.decay + G.gdx;
+        b = b * G.decay + G.gdy;
+        c = c * G.decay + G.gdz;
     }
+    vs[i] = make_float4(a, b, c, 0.0f);
+}
+
+__global__ __launch_bounds__(NS) void k_gather_boundary(const u32* __restrict__ perm,
+                                                        const float* __restrict__ x, u32 nb,
+                                                        float4* __restrict__ xb) {
+    const u32 s = blockIdx.x * NS + threadIdx.x;
+    if (s >= nb) return;
+    const u64 o = 3ull * perm[s];
+    xb[s] = make_float4(x[o], x[o + 1], x[o + 2], 0.0f);
 }
 
 // Akinci boundary mass, in cell-sorted order (dfsph_kernels.cu:40-65).
-__global__ __launch_bounds__(NT) void k_boundary_mass(GridGeom g, const u32* __restrict__ cs,
+__global__ __launch_bounds__(NS) void k_boundary_mass(GridGeom g, const u32* __restrict__ cs,
                                                       const float4* __restrict__ xb, u32 nb,
                                                       Spline sp, float rho0,
                                                       float* __restrict__ mb) {
-    const u32 i = tile_index(nb);
+    const u32 i = blockIdx.x * NS + threadIdx.x;
     if (i >= nb) return;
     const float4 p = xb[i];
     float sum_w = 0.0f;
-    for_stencil(g, cs, p.x, p.y, p.z, [&](u32 s) {
+    for_stencil(g, cs, 0u, p.x, p.y, p.z, [&](u32 s) {
         const float4 q = xb[s];
         sum_w += spline_W(norm3(p.x - q.x, p.y - q.y, p.z - q.z), sp);
     });
     mb[i] = rho0 / fmaxf(sum_w, 1e-9f);
 }
 
-__global__ __launch_bounds__(NT) void k_pack_boundary(u32 nb, const u32* __restrict__ perm,
+__global__ __launch_bounds__(NS) void k_pack_boundary(u32 nb, const u32* __restrict__ perm,
                                                       const float* __restrict__ mb,
                                                       float4* __restrict__ xb,
                                                       float* __restrict__ store_mb) {
-    const u32 s = blockIdx.x * NT + threadIdx.x;
+    const u32 s = blockIdx.x * NS + threadIdx.x;
     if (s >= nb) return;
     xb[s].w = mb[s];
     store_mb[perm[s]] = mb[s];
 }
 
-// Per-particle pair lists, ELL layout: entry k of particle i at [k * n + i]
-// holds (neighbour slot, bits of c) with grad W_ij = (x_i - x_j) * c, for
-// every pair the reference's stencil visits with c != 0, in the reference's
-// summation order.  Positions are frozen from the grid build to advection, so
-// the iteration kernels recompute r and g = r * c from the list and get the
-// same binary32 values the reference gets from grad_W (zero-gradient pairs
-// add +-0 to sums that never hold -0, so leaving them out is exact).
-// Particles with more pairs than the list depth use the stencil walk.
-// (PairList itself is declared in wsim_internal.hpp.)
-
+// Per-particle pair lists: entry k of particle i holds (neighbour slot, bits
+// of c) with grad W_ij = (x_i - x_j) * c, for every pair the reference's
+// stencil visits with c != 0, in the reference's summation order.  Positions
+// are frozen from the grid build to advection, so the iteration kernels
+// recompute r and g = r * c from the list and get the same binary32 values
+// the reference gets from grad_W (zero-gradient pairs add +-0 to sums that
+// never hold -0, so leaving them out is exact).  Particles with more pairs
+// than the list depth use the stencil walk.
+//
 // Wave-blocked ELL: the K entries of the 64 particles of one wave occupy one
 // contiguous block, row k of it holding entry k of each lane.
 __device__ __forceinline__ u64 pair_at(u32 k, u32 i, u32 depth) {
@@ -160,16 +194,21 @@ __device__ __forceinline__ u32 pack_counts(u32 cf, u32 cb) {
 // rho (dfsph_kernels.cu:68-112) and alpha (dfsph_kernels.cu:117-169) in one
 // stencil walk: they share r, |r| and q for every pair.  Also records the
 // pair lists used by every iteration of the pressure solves.
+template <int NT>
 __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
                                                       const float4* __restrict__ xs,
                                                       float* __restrict__ rho_out,
                                                       float* __restrict__ alpha_out) {
-    const u32 i = tile_index(P.n);
-    if (i >= P.n) return;
+    const u32 i = slot_index<NT>(P.remap);
+    if (i >= P.nslots) return;
+    const SceneSeg& G = P.seg[scene_of_slot(P, i)];
+    if (i - G.off >= G.n) return;
+    const u32* cs = P.csf + G.key_base;
+    const u32 shift = G.off - G.start;
     const float4 p = xs[i];
     float rho = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f, sq = 0.0f;
     u32 cf = 0, cb = 0;
-    for_stencil(P.gf, P.csf, p.x, p.y, p.z, [&](u32 s) {
+    for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
         const float4 q = xs[s];
         const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
         const float d = norm3(rx, ry, rz);
@@ -188,7 +227,7 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
         }
     });
     if (P.nb) {
-        for_stencil(P.gb, P.csb, p.x, p.y, p.z, [&](u32 s) {
+        for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
             const float4 q = P.xb[s];
             const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
             const float d = norm3(rx, ry, rz);
@@ -211,18 +250,29 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
     alpha_out[i] = (denom > 1e-9f) ? (rho / denom) : 0.0f;
 }
 
-// XSPH as a Jacobi sweep: every neighbour velocity is the pre-XSPH one.
+// XSPH as a Jacobi sweep: every neighbour velocity is the pre-XSPH one.  A
+// scene without viscosity copies its velocities (dfsph.cu:198 skips XSPH).
+template <int NT>
 __global__ __launch_bounds__(NT) void k_xsph(Nbr P, const float4* __restrict__ xs,
                                              const float4* __restrict__ vin,
-                                             const float* __restrict__ rho, float eps,
+                                             const float* __restrict__ rho,
                                              float4* __restrict__ vout) {
-    const u32 i = tile_index(P.n);
-    if (i >= P.n) return;
-    const float4 p = xs[i];
+    const u32 i = slot_index<NT>(P.remap);
+    if (i >= P.nslots) return;
+    const SceneSeg& G = P.seg[scene_of_slot(P, i)];
+    if (i - G.off >= G.n) return;
     const float4 vi = vin[i];
+    const float eps = G.eps;
+    if (!(eps > 0.0f)) {
+        vout[i] = vi;
+        return;
+    }
+    const u32* cs = P.csf + G.key_base;
+    const u32 shift = G.off - G.start;
+    const float4 p = xs[i];
     const float rho_i = rho[i];
     float dx = 0.0f, dy = 0.0f, dz = 0.0f;
-    for_stencil(P.gf, P.csf, p.x, p.y, p.z, [&](u32 s) {
+    for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
         if (s == i) return;
         const float4 q = xs[s];
         const float4 vj = vin[s];
@@ -253,116 +303,131 @@ __device__ __forceinline__ void rho_dot_boundary(float& rd, const float4& p, con
 // One predictor sweep.  MODE 0: density change + kappa (dfsph_kernels.cu:
 // 176-232).  MODE 1: divergence change + kappa_v (plan text :1862-1907, with
 // the boundary term of the density change).  Writes (x, kappa-hat) for the
-// following k_apply and reduces the mean error in binary64; the last
-// workgroup decides convergence on the device.
-template <int MODE>
+// following k_apply and one binary64 error partial per tile: the four waves
+// of a tile are summed in a fixed order, so the partials do not depend on
+// the workgroup size.  Scenes that converged before `iter` skip the work.
+template <int MODE, int NT, int PU>
 __global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* __restrict__ xs,
                                                const float4* __restrict__ vs,
                                                const float* __restrict__ rho,
-                                               const float* __restrict__ alpha, float dt,
-                                               float dt2, float rho0, int pform, u32 iter,
-                                               const DeviceState* st,
+                                               const float* __restrict__ alpha, float rho0,
+                                               int pform, u32 iter, const DeviceState* st,
                                                double* __restrict__ part,
                                                float* __restrict__ rho_adv,
                                                float* __restrict__ kappa,
                                                float4* __restrict__ xk) {
+    static_assert(NT % kTile == 0, "k_change workgroups hold whole tiles");
     __shared__ double s_red[NT / 64];
-    const u32 conv = MODE == 0 ? st->conv_iter : st->div_conv_iter;
-    if (conv < iter) return;  // converged earlier: grid-uniform no-op
-    const u32 i = tile_index(P.n);
+    const u32 i = slot_index<NT>(P.remap);
     double contrib = 0.0;
-    if (i < P.n) {
-        const float4 p = xs[i];
-        const float4 vi = vs[i];
-        float rd = 0.0f;
-        const u32 cw = L.cnt[i];
-        const u32 cf = cw & 0xFFFFu, cb = cw >> 16;
-        if (cf <= L.kf && cb <= L.kb) {
-            // PU pairs per trip: loads issued together, terms summed in order
-            u32 k = 0;
-            for (; k + PU <= cf; k += PU) {
-                uint2 e[PU];
-                float4 q[PU], w[PU];
+    bool active = false;  // wave-uniform
+    if (i < P.nslots) {
+        const u32 sc = scene_of_slot(P, i);
+        const u32 conv = MODE == 0 ? st[sc].conv_iter : st[sc].div_conv_iter;
+        active = !(conv < iter);
+        const SceneSeg& G = P.seg[sc];
+        if (active && i - G.off < G.n) {
+            const u32* cs = P.csf + G.key_base;
+            const u32 shift = G.off - G.start;
+            const float4 p = xs[i];
+            const float4 vi = vs[i];
+            float rd = 0.0f;
+            const u32 cw = L.cnt[i];
+            const u32 cf = cw & 0xFFFFu, cb = cw >> 16;
+            if (cf <= L.kf && cb <= L.kb) {
+                // PU pairs per trip: loads issued together, terms summed in order
+                u32 k = 0;
+                for (; k + PU <= cf; k += PU) {
+                    uint2 e[PU];
+                    float4 q[PU], w[PU];
 #pragma unroll
-                for (int u = 0; u < PU; ++u) e[u] = ld_pair(L.f + pair_at(k + u, i, L.kf));
+                    for (int u = 0; u < PU; ++u) e[u] = ld_pair(L.f + pair_at(k + u, i, L.kf));
 #pragma unroll
-                for (int u = 0; u < PU; ++u) {
-                    q[u] = xs[e[u].x];
-                    w[u] = vs[e[u].x];
+                    for (int u = 0; u < PU; ++u) {
+                        q[u] = xs[e[u].x];
+                        w[u] = vs[e[u].x];
+                    }
+#pragma unroll
+                    for (int u = 0; u < PU; ++u)
+                        rho_dot_fluid(rd, p, vi, q[u], w[u], __uint_as_float(e[u].y), P.mass);
                 }
+                for (; k < cf; ++k) {
+                    const uint2 e = ld_pair(L.f + pair_at(k, i, L.kf));
+                    rho_dot_fluid(rd, p, vi, xs[e.x], vs[e.x], __uint_as_float(e.y), P.mass);
+                }
+                k = 0;
+                for (; k + PU <= cb; k += PU) {
+                    uint2 e[PU];
+                    float4 q[PU];
 #pragma unroll
-                for (int u = 0; u < PU; ++u)
-                    rho_dot_fluid(rd, p, vi, q[u], w[u], __uint_as_float(e[u].y), P.mass);
-            }
-            for (; k < cf; ++k) {
-                const uint2 e = ld_pair(L.f + pair_at(k, i, L.kf));
-                rho_dot_fluid(rd, p, vi, xs[e.x], vs[e.x], __uint_as_float(e.y), P.mass);
-            }
-            k = 0;
-            for (; k + PU <= cb; k += PU) {
-                uint2 e[PU];
-                float4 q[PU];
+                    for (int u = 0; u < PU; ++u) e[u] = ld_pair(L.b + pair_at(k + u, i, L.kb));
 #pragma unroll
-                for (int u = 0; u < PU; ++u) e[u] = ld_pair(L.b + pair_at(k + u, i, L.kb));
+                    for (int u = 0; u < PU; ++u) q[u] = P.xb[e[u].x];
 #pragma unroll
-                for (int u = 0; u < PU; ++u) q[u] = P.xb[e[u].x];
-#pragma unroll
-                for (int u = 0; u < PU; ++u) rho_dot_boundary(rd, p, vi, q[u], __uint_as_float(e[u].y));
-            }
-            for (; k < cb; ++k) {
-                const uint2 e = ld_pair(L.b + pair_at(k, i, L.kb));
-                rho_dot_boundary(rd, p, vi, P.xb[e.x], __uint_as_float(e.y));
-            }
-        } else {
-            for_stencil(P.gf, P.csf, p.x, p.y, p.z, [&](u32 s) {
-                if (s == i) return;
-                const float4 q = xs[s];
-                const float c = spline_gradc(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
-                rho_dot_fluid(rd, p, vi, q, vs[s], c, P.mass);
-            });
-            if (P.nb) {
-                for_stencil(P.gb, P.csb, p.x, p.y, p.z, [&](u32 s) {
-                    const float4 q = P.xb[s];
+                    for (int u = 0; u < PU; ++u)
+                        rho_dot_boundary(rd, p, vi, q[u], __uint_as_float(e[u].y));
+                }
+                for (; k < cb; ++k) {
+                    const uint2 e = ld_pair(L.b + pair_at(k, i, L.kb));
+                    rho_dot_boundary(rd, p, vi, P.xb[e.x], __uint_as_float(e.y));
+                }
+            } else {
+                for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
+                    if (s == i) return;
+                    const float4 q = xs[s];
                     const float c = spline_gradc(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
-                    rho_dot_boundary(rd, p, vi, q, c);
+                    rho_dot_fluid(rd, p, vi, q, vs[s], c, P.mass);
                 });
+                if (P.nb) {
+                    for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
+                        const float4 q = P.xb[s];
+                        const float c = spline_gradc(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
+                        rho_dot_boundary(rd, p, vi, q, c);
+                    });
+                }
             }
+            const float rho_i = rho[i];
+            float kap, err;
+            if (MODE == 0) {
+                const float rho_pred = rho_i + G.dt * rd;
+                rho_adv[i] = rho_pred;
+                err = fmaxf(0.0f, rho_pred - rho0);
+                kap = err * alpha[i] / G.dt2;
+            } else {
+                rho_adv[i] = rd;
+                err = fmaxf(0.0f, rd);
+                kap = err * alpha[i] / G.dt;
+            }
+            kappa[i] = kap;
+            const float khat =
+                pform ? kap / fmaxf(rho_i, 1e-9f) : kap / fmaxf(rho_i * rho_i, 1e-9f);
+            xk[i] = make_float4(p.x, p.y, p.z, khat);
+            contrib = static_cast<double>(err);
         }
-        const float rho_i = rho[i];
-        float kap, err;
-        if (MODE == 0) {
-            const float rho_pred = rho_i + dt * rd;
-            rho_adv[i] = rho_pred;
-            err = fmaxf(0.0f, rho_pred - rho0);
-            kap = err * alpha[i] / dt2;
-        } else {
-            rho_adv[i] = rd;
-            err = fmaxf(0.0f, rd);
-            kap = err * alpha[i] / dt;
-        }
-        kappa[i] = kap;
-        const float khat = pform ? kap / fmaxf(rho_i, 1e-9f) : kap / fmaxf(rho_i * rho_i, 1e-9f);
-        xk[i] = make_float4(p.x, p.y, p.z, khat);
-        contrib = static_cast<double>(err);
     }
-    // per-workgroup partial; the next launch (k_apply) finishes the sum, so
-    // no cross-workgroup fence is needed inside this kernel
-    const double bsum = block_sum<NT>(contrib, s_red);
-    if (threadIdx.x == 0) part[blockIdx.x] = bsum;
+    // tile partial; the next launch (k_apply) finishes the scene's sum, so no
+    // cross-workgroup fence is needed inside this kernel
+    const double ws = wave_sum(contrib);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) s_red[w] = ws;
+    __syncthreads();
+    if (lane == 0 && (w & 3) == 0 && active)
+        part[i / kTile] = ((s_red[w] + s_red[w + 1]) + s_red[w + 2]) + s_red[w + 3];
 }
 
-// Workgroup 0 of k_apply(iter) reduces the partials k_change(iter) wrote, in
-// a fixed order, and records the error and the convergence decision
-// (dfsph.cu:254-258; the divergence solve tests every iteration).
-__device__ __forceinline__ void finish_error(int mode, u32 iter, double thr, u32 n,
-                                             const double* __restrict__ part, u32 nparts,
-                                             DeviceState* st) {
-    __shared__ double s_red[NT / 64];
+// The first wave of each scene in k_apply(iter) sums the scene's tile
+// partials written by k_change(iter), in a fixed order, and records the
+// error and the convergence decision (dfsph.cu:254-258; the divergence solve
+// tests every iteration).
+__device__ __forceinline__ void finish_error(int mode, u32 iter, double thr, const SceneSeg& G,
+                                             const double* __restrict__ part, DeviceState* st) {
+    const int lane = threadIdx.x & 63;
     double s = 0.0;
-    for (u32 b = threadIdx.x; b < nparts; b += NT) s += part[b];
-    const double tot = block_sum<NT>(s, s_red);
-    if (threadIdx.x == 0) {
-        const double e = tot / static_cast<double>(n);
+#pragma unroll 8
+    for (u32 t = lane; t < G.ntiles; t += 64) s += part[G.tile0 + t];
+    const double tot = wave_sum(s);
+    if (lane == 0) {
+        const double e = tot / static_cast<double>(G.n);
         if (mode == 0) {
             st->err = e;
             if (iter >= 1 && e < thr && st->conv_iter == kNone) st->conv_iter = iter;
@@ -385,16 +450,25 @@ __device__ __forceinline__ void kappa_term(float& dx, float& dy, float& dz, cons
 // Pressure velocity update (dfsph_kernels.cu:237-301); kappa-hat of every
 // particle was precomputed by k_change with the same expression the
 // reference evaluates per pair.
+template <int NT, int PU>
 __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* __restrict__ xk,
-                                              const float* __restrict__ kappa, float dt,
-                                              float rho0, int pform, u32 iter, int mode,
-                                              double thr, const double* __restrict__ part,
-                                              DeviceState* st, float4* __restrict__ vs) {
-    const u32 conv = mode == 0 ? st->conv_iter : st->div_conv_iter;
-    if (conv < iter) return;  // grid-uniform: this iteration's k_change did not run either
-    if (blockIdx.x == 0) finish_error(mode, iter, thr, P.n, part, gridDim.x, st);
-    const u32 i = tile_index(P.n);
-    if (i >= P.n) return;
+                                              const float* __restrict__ kappa, float rho0,
+                                              int pform, u32 iter, int mode, double thr,
+                                              const double* __restrict__ part, DeviceState* st,
+                                              float4* __restrict__ vs) {
+    const u32 i = slot_index<NT>(P.remap);
+    if (i >= P.nslots) return;
+    const u32 sc = scene_of_slot(P, i);
+    DeviceState* sst = st + sc;
+    const u32 conv = mode == 0 ? sst->conv_iter : sst->div_conv_iter;
+    if (conv < iter) return;  // scene-uniform: this iteration's k_change did not run either
+    const SceneSeg& G = P.seg[sc];
+    const u32 l = i - G.off;
+    if (l < 64) finish_error(mode, iter, thr, G, part, sst);
+    if (l >= G.n) return;
+    const u32* cs = P.csf + G.key_base;
+    const u32 shift = G.off - G.start;
+    const float dt = G.dt;
     const float4 p = xk[i];
     const float ki = p.w;
     float dx = 0.0f, dy = 0.0f, dz = 0.0f;
@@ -441,14 +515,14 @@ __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* _
             kappa_term(dx, dy, dz, p, q, __uint_as_float(e.y), q.w * (ki + bf), dt);
         }
     } else {
-        for_stencil(P.gf, P.csf, p.x, p.y, p.z, [&](u32 s) {
+        for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
             if (s == i) return;
             const float4 q = xk[s];
             const float c = spline_gradc(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
             kappa_term(dx, dy, dz, p, q, c, P.mass * (ki + q.w), dt);
         });
         if (P.nb) {
-            for_stencil(P.gb, P.csb, p.x, p.y, p.z, [&](u32 s) {
+            for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
                 const float4 q = P.xb[s];
                 const float c = spline_gradc(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
                 kappa_term(dx, dy, dz, p, q, c, q.w * (ki + bf), dt);
@@ -466,35 +540,37 @@ __device__ __forceinline__ bool finite3(float a, float b, float c) {
     return isfinite(a) && isfinite(b) && isfinite(c);
 }
 
-// Per-workgroup (max |v|, NaN count) partials; k_finish_vstats reduces them.
-__device__ __forceinline__ void vstats_partial(float mag, u32 bad, float* __restrict__ pmax,
-                                               u32* __restrict__ pnan) {
-    __shared__ float s_f[NT / 64];
-    __shared__ u32 s_u[NT / 64];
-    const float bm = block_max<NT>(mag, s_f);
-    const u32 bn = block_sum_u32<NT>(bad, s_u);
-    if (threadIdx.x == 0) {
-        pmax[blockIdx.x] = bm;
-        pnan[blockIdx.x] = bn;
+// Per-wave (max |v|, NaN count) partials; k_finish_vstats reduces them per scene.
+__device__ __forceinline__ void vstats_partial(u32 i, u32 nslots, float mag, u32 bad,
+                                               float* __restrict__ pmax, u32* __restrict__ pnan) {
+    const float m = wave_max(mag);
+    const u32 c = wave_sum_u32(bad);
+    if ((threadIdx.x & 63) == 0 && i < nslots) {
+        pmax[i >> 6] = m;
+        pnan[i >> 6] = c;
     }
 }
 
-__global__ __launch_bounds__(NT) void k_finish_vstats(const float* __restrict__ pmax,
-                                                      const u32* __restrict__ pnan, u32 nparts,
+// One workgroup per scene.
+__global__ __launch_bounds__(NS) void k_finish_vstats(const SceneSeg* __restrict__ seg,
+                                                      const float* __restrict__ pmax,
+                                                      const u32* __restrict__ pnan,
                                                       DeviceState* st) {
-    __shared__ float s_f[NT / 64];
-    __shared__ u32 s_u[NT / 64];
+    __shared__ float s_f[NS / 64];
+    __shared__ u32 s_u[NS / 64];
+    const SceneSeg& G = seg[blockIdx.x];
+    const u32 w0 = G.off >> 6, nw = (G.n + 63u) >> 6;
     float m = 0.0f;
     u32 c = 0;
-    for (u32 b = threadIdx.x; b < nparts; b += NT) {
-        m = fmaxf(m, pmax[b]);
-        c += pnan[b];
+    for (u32 w = threadIdx.x; w < nw; w += NS) {
+        m = fmaxf(m, pmax[w0 + w]);
+        c += pnan[w0 + w];
     }
-    m = block_max<NT>(m, s_f);
-    c = block_sum_u32<NT>(c, s_u);
+    m = block_max<NS>(m, s_f);
+    c = block_sum_u32<NS>(c, s_u);
     if (threadIdx.x == 0) {
-        st->max_v = m;
-        st->nan_count = c;
+        st[blockIdx.x].max_v = m;
+        st[blockIdx.x].nan_count = c;
     }
 }
 
@@ -509,9 +585,9 @@ struct StoreOut {
 
 // Advection with the |v| clamp and the domain clamp (dfsph_kernels.cu:363-386),
 // scatter back to original order, then max |v| (dfsph_kernels.cu:396-400).
-__global__ __launch_bounds__(NT) void k_advect(u32 n, const float4* __restrict__ xs,
+__global__ __launch_bounds__(NS) void k_advect(Nbr P, const float4* __restrict__ xs,
                                                const float4* __restrict__ vs,
-                                               const u32* __restrict__ perm, float dt, float vmax,
+                                               const u32* __restrict__ perm, float vmax,
                                                float lx, float ly, float lz, float hx, float hy,
                                                float hz, StoreOut out,
                                                const float* __restrict__ rho,
@@ -520,86 +596,211 @@ __global__ __launch_bounds__(NT) void k_advect(u32 n, const float4* __restrict__
                                                const float* __restrict__ rho_adv, int wb_solve,
                                                float* __restrict__ pmax,
                                                u32* __restrict__ pnan) {
-    const u32 s = blockIdx.x * NT + threadIdx.x;
+    const u32 i = blockIdx.x * NS + threadIdx.x;
     float mag = 0.0f;
     u32 bad = 0;
-    if (s < n) {
-        const float4 x4 = xs[s];
-        float4 v = vs[s];
-        const float sp = norm3(v.x, v.y, v.z);
-        if (sp > vmax) {
-            const float f = vmax / sp;
-            v.x *= f;
-            v.y *= f;
-            v.z *= f;
+    if (i < P.nslots) {
+        const SceneSeg& G = P.seg[scene_of_slot(P, i)];
+        const u32 l = i - G.off;
+        if (l < G.n) {
+            const float dt = G.dt;
+            const float4 x4 = xs[i];
+            float4 v = vs[i];
+            const float sp = norm3(v.x, v.y, v.z);
+            if (sp > vmax) {
+                const float f = vmax / sp;
+                v.x *= f;
+                v.y *= f;
+                v.z *= f;
+            }
+            float px = x4.x + v.x * dt, py = x4.y + v.y * dt, pz = x4.z + v.z * dt;
+            if (px < lx) { px = lx; if (v.x < 0) v.x = 0; }
+            if (py < ly) { py = ly; if (v.y < 0) v.y = 0; }
+            if (pz < lz) { pz = lz; if (v.z < 0) v.z = 0; }
+            if (px > hx) { px = hx; if (v.x > 0) v.x = 0; }
+            if (py > hy) { py = hy; if (v.y > 0) v.y = 0; }
+            if (pz > hz) { pz = hz; if (v.z > 0) v.z = 0; }
+            const u32 id = perm[G.start + l];
+            const u64 o = 3ull * id;
+            out.x[o] = px;
+            out.x[o + 1] = py;
+            out.x[o + 2] = pz;
+            out.v[o] = v.x;
+            out.v[o + 1] = v.y;
+            out.v[o + 2] = v.z;
+            out.rho[id] = rho[i];
+            out.alpha[id] = alpha[i];
+            if (wb_solve) {
+                out.kappa[id] = kappa[i];
+                out.rho_adv[id] = rho_adv[i];
+            }
+            mag = norm3(v.x, v.y, v.z);
+            bad = finite3(px, py, pz) && finite3(v.x, v.y, v.z) ? 0u : 1u;
         }
-        float px = x4.x + v.x * dt, py = x4.y + v.y * dt, pz = x4.z + v.z * dt;
-        if (px < lx) { px = lx; if (v.x < 0) v.x = 0; }
-        if (py < ly) { py = ly; if (v.y < 0) v.y = 0; }
-        if (pz < lz) { pz = lz; if (v.z < 0) v.z = 0; }
-        if (px > hx) { px = hx; if (v.x > 0) v.x = 0; }
-        if (py > hy) { py = hy; if (v.y > 0) v.y = 0; }
-        if (pz > hz) { pz = hz; if (v.z > 0) v.z = 0; }
-        const u32 id = perm[s];
-        const u64 o = 3ull * id;
-        out.x[o] = px;
-        out.x[o + 1] = py;
-        out.x[o + 2] = pz;
-        out.v[o] = v.x;
-        out.v[o + 1] = v.y;
-        out.v[o + 2] = v.z;
-        out.rho[id] = rho[s];
-        out.alpha[id] = alpha[s];
-        if (wb_solve) {
-            out.kappa[id] = kappa[s];
-            out.rho_adv[id] = rho_adv[s];
-        }
-        mag = norm3(v.x, v.y, v.z);
-        bad = finite3(px, py, pz) && finite3(v.x, v.y, v.z) ? 0u : 1u;
     }
-    vstats_partial(mag, bad, pmax, pnan);
+    vstats_partial(i, P.nslots, mag, bad, pmax, pnan);
 }
 
 // End of the divergence solve: velocities and the recomputed attributes back
 // to original order, then max |v|.
-__global__ __launch_bounds__(NT) void k_div_writeback(u32 n, const float4* __restrict__ xs,
+__global__ __launch_bounds__(NS) void k_div_writeback(Nbr P, const float4* __restrict__ xs,
                                                       const float4* __restrict__ vs,
                                                       const u32* __restrict__ perm, StoreOut out,
                                                       const float* __restrict__ rho,
                                                       const float* __restrict__ alpha,
                                                       const float* __restrict__ kappa_v,
                                                       const float* __restrict__ rho_dot,
-                                                      int wb_solve,
-                                                      float* __restrict__ pmax,
+                                                      int wb_solve, float* __restrict__ pmax,
                                                       u32* __restrict__ pnan) {
-    const u32 s = blockIdx.x * NT + threadIdx.x;
+    const u32 i = blockIdx.x * NS + threadIdx.x;
     float mag = 0.0f;
     u32 bad = 0;
-    if (s < n) {
-        const float4 x4 = xs[s];
-        const float4 v = vs[s];
-        const u32 id = perm[s];
-        const u64 o = 3ull * id;
-        out.v[o] = v.x;
-        out.v[o + 1] = v.y;
-        out.v[o + 2] = v.z;
-        out.rho[id] = rho[s];
-        out.alpha[id] = alpha[s];
-        if (wb_solve) {
-            out.kappa[id] = kappa_v[s];
-            out.rho_adv[id] = rho_dot[s];
+    if (i < P.nslots) {
+        const SceneSeg& G = P.seg[scene_of_slot(P, i)];
+        const u32 l = i - G.off;
+        if (l < G.n) {
+            const float4 x4 = xs[i];
+            const float4 v = vs[i];
+            const u32 id = perm[G.start + l];
+            const u64 o = 3ull * id;
+            out.v[o] = v.x;
+            out.v[o + 1] = v.y;
+            out.v[o + 2] = v.z;
+            out.rho[id] = rho[i];
+            out.alpha[id] = alpha[i];
+            if (wb_solve) {
+                out.kappa[id] = kappa_v[i];
+                out.rho_adv[id] = rho_dot[i];
+            }
+            mag = norm3(v.x, v.y, v.z);
+            bad = finite3(x4.x, x4.y, x4.z) && finite3(v.x, v.y, v.z) ? 0u : 1u;
         }
-        mag = norm3(v.x, v.y, v.z);
-        bad = finite3(x4.x, x4.y, x4.z) && finite3(v.x, v.y, v.z) ? 0u : 1u;
     }
-    vstats_partial(mag, bad, pmax, pnan);
+    vstats_partial(i, P.nslots, mag, bad, pmax, pnan);
 }
 
-__global__ void k_reset_solve(DeviceState* st, int which) {
-    if (threadIdx.x == 0) {
-        if (which == 0) st->conv_iter = kNone;
-        else st->div_conv_iter = kNone;
+// Opens a solve: every non-empty scene unconverged, empty scenes done.
+__global__ __launch_bounds__(NS) void k_reset_solve(const SceneSeg* __restrict__ seg,
+                                                    DeviceState* st, u32 scenes, int which) {
+    const u32 s = blockIdx.x * NS + threadIdx.x;
+    if (s >= scenes) return;
+    const u32 v = seg[s].n ? kNone : 0u;
+    if (which == 0) st[s].conv_iter = v;
+    else st[s].div_conv_iter = v;
+}
+
+// ---- launch-shape dispatch ------------------------------------------------
+u32 grid_for(u32 nslots, u32 nt) { return div_up(nslots, nt); }
+
+void launch_density_alpha(u32 nt, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
+                          const float4* xs, float* rho, float* alpha) {
+    switch (nt) {
+    case 64: k_density_alpha<64><<<grid_for(nslots, 64), 64, 0, s>>>(P, L, xs, rho, alpha); break;
+    case 128: k_density_alpha<128><<<grid_for(nslots, 128), 128, 0, s>>>(P, L, xs, rho, alpha); break;
+    case 256: k_density_alpha<256><<<grid_for(nslots, 256), 256, 0, s>>>(P, L, xs, rho, alpha); break;
+    case 512: k_density_alpha<512><<<grid_for(nslots, 512), 512, 0, s>>>(P, L, xs, rho, alpha); break;
+    default: fail(kInvalidArgument, "density_alpha workgroup size");
     }
+    WSIM_LAUNCH_CHECK();
+}
+
+void launch_xsph(u32 nt, hipStream_t s, u32 nslots, const Nbr& P, const float4* xs,
+                 const float4* vin, const float* rho, float4* vout) {
+    switch (nt) {
+    case 64: k_xsph<64><<<grid_for(nslots, 64), 64, 0, s>>>(P, xs, vin, rho, vout); break;
+    case 128: k_xsph<128><<<grid_for(nslots, 128), 128, 0, s>>>(P, xs, vin, rho, vout); break;
+    case 256: k_xsph<256><<<grid_for(nslots, 256), 256, 0, s>>>(P, xs, vin, rho, vout); break;
+    case 512: k_xsph<512><<<grid_for(nslots, 512), 512, 0, s>>>(P, xs, vin, rho, vout); break;
+    default: fail(kInvalidArgument, "xsph workgroup size");
+    }
+    WSIM_LAUNCH_CHECK();
+}
+
+struct ChangeArgs {
+    const float4* xs;
+    const float4* vs;
+    const float* rho;
+    const float* alpha;
+    float rho0;
+    int pform;
+    u32 iter;
+    const DeviceState* st;
+    double* part;
+    float* rho_adv;
+    float* kappa;
+    float4* xk;
+};
+
+template <int MODE, int NT, int PU>
+void change_one(hipStream_t s, u32 nslots, const Nbr& P, const PairList& L, const ChangeArgs& a) {
+    k_change<MODE, NT, PU><<<grid_for(nslots, NT), NT, 0, s>>>(P, L, a.xs, a.vs, a.rho, a.alpha,
+                                                               a.rho0, a.pform, a.iter, a.st,
+                                                               a.part, a.rho_adv, a.kappa, a.xk);
+}
+
+template <int MODE, int NT>
+void change_pu(u32 pu, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
+               const ChangeArgs& a) {
+    switch (pu) {
+    case 2: change_one<MODE, NT, 2>(s, nslots, P, L, a); break;
+    case 4: change_one<MODE, NT, 4>(s, nslots, P, L, a); break;
+    case 8: change_one<MODE, NT, 8>(s, nslots, P, L, a); break;
+    default: fail(kInvalidArgument, "change pairs per trip");
+    }
+}
+
+template <int MODE>
+void launch_change(u32 nt, u32 pu, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
+                   const ChangeArgs& a) {
+    switch (nt) {
+    case 256: change_pu<MODE, 256>(pu, s, nslots, P, L, a); break;
+    case 512: change_pu<MODE, 512>(pu, s, nslots, P, L, a); break;
+    case 1024: change_pu<MODE, 1024>(pu, s, nslots, P, L, a); break;
+    default: fail(kInvalidArgument, "change workgroup size");
+    }
+    WSIM_LAUNCH_CHECK();
+}
+
+struct ApplyArgs {
+    const float4* xk;
+    const float* kappa;
+    float rho0;
+    int pform;
+    u32 iter;
+    int mode;
+    double thr;
+    const double* part;
+    DeviceState* st;
+    float4* vs;
+};
+
+template <int NT, int PU>
+void apply_one(hipStream_t s, u32 nslots, const Nbr& P, const PairList& L, const ApplyArgs& a) {
+    k_apply<NT, PU><<<grid_for(nslots, NT), NT, 0, s>>>(P, L, a.xk, a.kappa, a.rho0, a.pform,
+                                                        a.iter, a.mode, a.thr, a.part, a.st, a.vs);
+}
+
+template <int NT>
+void apply_pu(u32 pu, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
+              const ApplyArgs& a) {
+    switch (pu) {
+    case 2: apply_one<NT, 2>(s, nslots, P, L, a); break;
+    case 4: apply_one<NT, 4>(s, nslots, P, L, a); break;
+    case 8: apply_one<NT, 8>(s, nslots, P, L, a); break;
+    default: fail(kInvalidArgument, "apply pairs per trip");
+    }
+}
+
+void launch_apply(u32 nt, u32 pu, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
+                  const ApplyArgs& a) {
+    switch (nt) {
+    case 64: apply_pu<64>(pu, s, nslots, P, L, a); break;
+    case 128: apply_pu<128>(pu, s, nslots, P, L, a); break;
+    case 256: apply_pu<256>(pu, s, nslots, P, L, a); break;
+    case 512: apply_pu<512>(pu, s, nslots, P, L, a); break;
+    default: fail(kInvalidArgument, "apply workgroup size");
+    }
+    WSIM_LAUNCH_CHECK();
 }
 
 // dfsph.cu:55-65
@@ -614,6 +815,25 @@ CellGrid make_grid(const DFSPHConfig& c, hipStream_t s) {
     return CellGrid(origin, dims, c.smoothing_length, s);
 }
 
+hipStream_t own_stream_if_null(hipStream_t s, hipStream_t* own) {
+    if (s) return s;
+    // a blocking stream: ordered with work the caller puts on the null stream
+    WSIM_HIP(hipStreamCreate(own));
+    return *own;
+}
+
+u32 round_tile(u64 n) { return static_cast<u32>((n + kTile - 1) / kTile * kTile); }
+
+enum PhaseKey : u64 {
+    kPhPrefix = 1,
+    kPhDivPrefix = 2,
+    kPhAdvect = 3,
+    kPhDivTail = 4,
+    kPhDensityIters = 5,
+    kPhDivIters = 6,
+};
+u64 phase_key(u64 kind, u64 b = 0, u64 e = 0) { return (kind << 48) | (b << 24) | e; }
+
 }  // namespace
 
 const char* kernel_kind_name(int k) {
@@ -622,6 +842,16 @@ const char* kernel_kind_name(int k) {
                                             "advect",         "div_change",  "div_apply",
                                             "div_writeback"};
     return (k >= 0 && k < kKNumKinds) ? names[k] : "?";
+}
+
+bool valid_shapes(const KernelShapes& s) {
+    auto in = [](u32 v, std::initializer_list<u32> set) {
+        return std::find(set.begin(), set.end(), v) != set.end();
+    };
+    return in(s.density_alpha_nt, {64, 128, 256, 512}) && in(s.xsph_nt, {64, 128, 256, 512}) &&
+           in(s.change_nt, {256, 512, 1024}) && in(s.change_pu, {2, 4, 8}) &&
+           in(s.apply_nt, {64, 128, 256, 512}) && in(s.apply_pu, {2, 4, 8}) &&
+           in(s.cell_cpt, {4, 8, 16}) && in(s.remap, {0, 1});
 }
 
 // ---------------------------------------------------------------------------
@@ -678,10 +908,53 @@ void KernelTimer::reset() { acc = {}; }
 // ---------------------------------------------------------------------------
 DFSPHSolver::DFSPHSolver(ParticleStore& fluid, ParticleStore* boundary, const DFSPHConfig& cfg,
                          hipStream_t stream)
-    : fluid_(fluid), boundary_(boundary), cfg_(cfg), stream_(stream),
-      fluid_grid_(make_grid(cfg, stream)) {
+    : fluid_(fluid), boundary_(boundary), cfg_(cfg),
+      stream_(own_stream_if_null(stream, &own_stream_)), fluid_grid_(make_grid(cfg, stream_)) {
+    init({}, {});
+}
+
+DFSPHSolver::DFSPHSolver(ParticleStore& fluid, ParticleStore* boundary, const DFSPHConfig& cfg,
+                         hipStream_t stream, const std::vector<u32>& counts,
+                         const std::vector<float>& viscosity)
+    : fluid_(fluid), boundary_(boundary), cfg_(cfg),
+      stream_(own_stream_if_null(stream, &own_stream_)), fluid_grid_(make_grid(cfg, stream_)) {
+    batched_ = true;
+    use_graphs_ = true;
+    init(counts, viscosity);
+}
+
+DFSPHSolver::~DFSPHSolver() {
+    (void)hipStreamSynchronize(stream_);
+    drop_graphs();
+    if (own_stream_) (void)hipStreamDestroy(own_stream_);
+}
+
+void DFSPHSolver::init(const std::vector<u32>& counts, const std::vector<float>& viscosity) {
     if (!(cfg_.smoothing_length > 0.0f))
         fail(kInvalidArgument, "DFSPHSolver: smoothing_length must be positive");
+    size_t slots_cap = 0;
+    u32 scenes = 1;
+    if (batched_) {
+        if (counts.empty()) fail(kInvalidArgument, "batched DFSPHSolver: at least one scene");
+        if (counts.size() > 0xFFFFu) fail(kInvalidArgument, "batched DFSPHSolver: too many scenes");
+        if (!viscosity.empty() && viscosity.size() != counts.size())
+            fail(kInvalidArgument, "batched DFSPHSolver: one viscosity per scene");
+        u64 total = 0;
+        for (u32 c : counts) {
+            total += c;
+            slots_cap += round_tile(c);
+        }
+        if (total != fluid_.count())
+            fail(kInvalidArgument, "batched DFSPHSolver: scene counts must add up to the store count");
+        scenes = static_cast<u32>(counts.size());
+        viscosity_ = viscosity.empty() ? std::vector<float>(counts.size(), cfg_.viscosity) : viscosity;
+    } else {
+        slots_cap = round_tile(fluid_.capacity());
+        viscosity_ = {cfg_.viscosity};
+    }
+    if (slots_cap == 0) slots_cap = kTile;
+    if (slots_cap >= 0xFFFFF000ull) fail(kInvalidArgument, "DFSPHSolver: too many particles");
+
     // dfsph.cu:76-78
     const float spacing = 2.0f * cfg_.particle_radius;
     const float v0 = spacing * spacing * spacing;
@@ -692,33 +965,30 @@ DFSPHSolver::DFSPHSolver(ParticleStore& fluid, ParticleStore* boundary, const DF
     h_alpha_ = fluid_.register_attribute("dfsph_alpha", kF32);
     h_kappa_ = fluid_.register_attribute("dfsph_kappa", kF32);
     h_density_adv_ = fluid_.register_attribute("dfsph_density_adv", kF32);
-    if (cfg_.enable_divergence_solve) {
-        h_kappa_v_ = fluid_.register_attribute("dfsph_kappa_v", kF32);
-        has_kappa_v_ = true;
-    }
+    if (cfg_.enable_divergence_solve) h_kappa_v_ = fluid_.register_attribute("dfsph_kappa_v", kF32);
 
-    const size_t cap = fluid_.capacity();
-    fluid_grid_.reserve(cap);
-    const size_t c1 = cap ? cap : 1;
-    xs_ = DeviceBuffer<float4>(c1);
-    vs_ = DeviceBuffer<float4>(c1);
-    vs2_ = DeviceBuffer<float4>(c1);
-    xk_ = DeviceBuffer<float4>(c1);
-    rho_ = DeviceBuffer<float>(c1);
-    alpha_ = DeviceBuffer<float>(c1);
-    kappa_ = DeviceBuffer<float>(c1);
-    rho_adv_ = DeviceBuffer<float>(c1);
-    const size_t c64 = (c1 + 63) / 64 * 64;  // the wave-blocked lists pad to whole waves
-    pairs_f_ = DeviceBuffer<uint2>(static_cast<size_t>(kPairDepthFluid) * c64);
-    pairs_b_ = DeviceBuffer<uint2>(static_cast<size_t>(kPairDepthBoundary) * c64);
-    pair_cnt_ = DeviceBuffer<u32>(c1);
-    const size_t nblk = div_up(c1, NT);
-    part_d_ = DeviceBuffer<double>(nblk);
-    part_f_ = DeviceBuffer<float>(nblk);
-    part_u_ = DeviceBuffer<u32>(nblk);
-    dstate_ = DeviceBuffer<DeviceState>(1);
+    fluid_grid_.reserve(batched_ ? fluid_.count() : fluid_.capacity());
+    fluid_grid_.reserve_segments(scenes);
+    xs_ = DeviceBuffer<float4>(slots_cap);
+    vs_ = DeviceBuffer<float4>(slots_cap);
+    vpre_ = DeviceBuffer<float4>(slots_cap);
+    xk_ = DeviceBuffer<float4>(slots_cap);
+    rho_ = DeviceBuffer<float>(slots_cap);
+    alpha_ = DeviceBuffer<float>(slots_cap);
+    kappa_ = DeviceBuffer<float>(slots_cap);
+    rho_adv_ = DeviceBuffer<float>(slots_cap);
+    pairs_f_ = DeviceBuffer<uint2>(static_cast<size_t>(kPairDepthFluid) * slots_cap);
+    pairs_b_ = DeviceBuffer<uint2>(static_cast<size_t>(kPairDepthBoundary) * slots_cap);
+    pair_cnt_ = DeviceBuffer<u32>(slots_cap);
+    part_d_ = DeviceBuffer<double>(slots_cap / kTile);
+    part_f_ = DeviceBuffer<float>(slots_cap / 64);
+    part_u_ = DeviceBuffer<u32>(slots_cap / 64);
+    hseg_ = PinnedBuffer<SceneSeg>(scenes);
+    dseg_ = DeviceBuffer<SceneSeg>(scenes);
+    dstate_ = DeviceBuffer<DeviceState>(scenes);
     dstate_.zero(stream_);
-    hstate_ = PinnedBuffer<DeviceState>(1);
+    hstate_ = PinnedBuffer<DeviceState>(scenes);
+    stats_.assign(scenes, StepStats{});
 
     // dfsph.cu:90-99: boundary grid built once, Akinci masses computed once
     if (boundary_) {
@@ -731,48 +1001,105 @@ DFSPHSolver::DFSPHSolver(ParticleStore& fluid, ParticleStore* boundary, const DF
             xb_ = DeviceBuffer<float4>(nb_);
             DeviceBuffer<float> mb(nb_);
             const u32 nb = static_cast<u32>(nb_);
-            const u32 blocks = div_up(nb, NT);
-            k_reorder<<<blocks, NT, 0, stream_>>>(boundary_grid_->sorted_indices(),
-                                                  boundary_->positions(), nullptr, nb, xb_.data(),
-                                                  nullptr, 0, 0.0f, 0.0f, 0.0f, 0.0f);
+            const u32 blocks = div_up(nb, NS);
+            k_gather_boundary<<<blocks, NS, 0, stream_>>>(boundary_grid_->sorted_indices(),
+                                                          boundary_->positions(), nb, xb_.data());
             WSIM_LAUNCH_CHECK();
-            k_boundary_mass<<<blocks, NT, 0, stream_>>>(boundary_grid_->geom(),
+            k_boundary_mass<<<blocks, NS, 0, stream_>>>(boundary_grid_->geom(),
                                                         boundary_grid_->cell_start(), xb_.data(), nb,
                                                         Spline::make(cfg_.smoothing_length),
                                                         cfg_.rest_density, mb.data());
             WSIM_LAUNCH_CHECK();
-            k_pack_boundary<<<blocks, NT, 0, stream_>>>(
+            k_pack_boundary<<<blocks, NS, 0, stream_>>>(
                 nb, boundary_grid_->sorted_indices(), mb.data(), xb_.data(),
                 static_cast<float*>(boundary_->attribute_data(h_bmass_)));
             WSIM_LAUNCH_CHECK();
             WSIM_HIP(hipStreamSynchronize(stream_));
         }
     }
+    if (batched_) layout(counts);
     WSIM_HIP(hipStreamSynchronize(stream_));
 }
 
-void DFSPHSolver::sort_and_reorder(bool gravity, float dt) {
-    const u32 n = static_cast<u32>(fluid_.count());
-    tbegin(kKGrid);
-    fluid_grid_.build(fluid_.positions(), n);
-    tend();
-    float decay = 1.0f, gdx = 0.0f, gdy = 0.0f, gdz = 0.0f;
-    if (gravity) {
-        decay = std::exp(-cfg_.damping * dt);  // dfsph_kernels.cu:351, once on the host
-        gdx = cfg_.gravity[0] * dt;
-        gdy = cfg_.gravity[1] * dt;
-        gdz = cfg_.gravity[2] * dt;
+void DFSPHSolver::layout(const std::vector<u32>& counts) {
+    drop_graphs();
+    const u32 scenes = static_cast<u32>(counts.size());
+    counts_ = counts;
+    starts_.assign(scenes + 1, 0);
+    u32 off = 0;
+    any_xsph_ = false;
+    std::vector<u32> tile_scene;
+    for (u32 s = 0; s < scenes; ++s) {
+        starts_[s + 1] = starts_[s] + counts[s];
+        SceneSeg& g = hseg_[s];
+        g = SceneSeg{};
+        g.off = off;
+        g.n = counts[s];
+        g.start = starts_[s];
+        g.tile0 = off / kTile;
+        g.ntiles = div_up(counts[s], kTile);
+        g.key_base = s * (fluid_grid_.total_cells() + 1u);
+        g.eps = viscosity_[s];
+        if (viscosity_[s] > 0.0f && counts[s] > 0) any_xsph_ = true;
+        for (u32 t = 0; t < g.ntiles; ++t) tile_scene.push_back(s);
+        off += g.ntiles * kTile;
     }
-    tbegin(kKReorder);
-    k_reorder<<<div_up(n, NT), NT, 0, stream_>>>(fluid_grid_.sorted_indices(), fluid_.positions(),
-                                                 fluid_.velocities(), n, xs_.data(), vs_.data(),
-                                                 gravity ? 1 : 0, decay, gdx, gdy, gdz);
-    WSIM_LAUNCH_CHECK();
-    tend();
+    total_ = starts_[scenes];
+    nslots_ = off;
+    if (static_cast<size_t>(nslots_) > xs_.size())
+        fail(kRuntimeError, "DFSPHSolver: working set smaller than the scene layout");
+    if (batched_) {
+        tile_scene_ = DeviceBuffer<u32>(tile_scene.empty() ? 1 : tile_scene.size());
+        dstarts_ = DeviceBuffer<u32>(scenes + 1);
+        if (!tile_scene.empty())
+            WSIM_HIP(hipMemcpy(tile_scene_.data(), tile_scene.data(), tile_scene.size() * sizeof(u32),
+                               hipMemcpyHostToDevice));
+        WSIM_HIP(hipMemcpy(dstarts_.data(), starts_.data(), (scenes + 1) * sizeof(u32),
+                           hipMemcpyHostToDevice));
+    }
+}
+
+void DFSPHSolver::drop_graphs() {
+    for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+    graphs_.clear();
+}
+
+void DFSPHSolver::set_graphs(bool on) {
+    if (!on) drop_graphs();
+    use_graphs_ = on;
+}
+
+void DFSPHSolver::set_shapes(const KernelShapes& s) {
+    if (!valid_shapes(s)) fail(kInvalidArgument, "DFSPHSolver: invalid kernel shapes");
+    shapes_ = s;
+    fluid_grid_.set_cells_per_thread(s.cell_cpt);
+    drop_graphs();
+}
+
+const StepStats& DFSPHSolver::scene_stats(size_t s) const {
+    if (s >= stats_.size()) fail(kOutOfRange, "DFSPHSolver: scene index out of range");
+    return stats_[s];
+}
+
+float DFSPHSolver::max_velocity() const {
+    float m = 0.0f;
+    for (const auto& s : stats_) m = std::max(m, s.max_velocity);
+    return m;
+}
+
+PairList DFSPHSolver::pair_list() {
+    PairList L;
+    L.f = pairs_f_.data();
+    L.b = pairs_b_.data();
+    L.cnt = pair_cnt_.data();
+    L.kf = kPairDepthFluid;
+    L.kb = nb_ ? kPairDepthBoundary : 0u;
+    return L;
 }
 
 static Nbr make_nbr(const CellGrid& fg, const CellGrid* bg, const float4* xb, size_t nb,
-                    const DFSPHConfig& cfg, float mass, u32 n) {
+                    const DFSPHConfig& cfg, float mass, u32 nslots, const SceneSeg* seg,
+                    const u32* tile_scene, u32 remap) {
     Nbr P;
     P.gf = fg.geom();
     P.csf = fg.cell_start();
@@ -789,188 +1116,253 @@ static Nbr make_nbr(const CellGrid& fg, const CellGrid* bg, const float4* xb, si
     }
     P.sp = Spline::make(cfg.smoothing_length);
     P.mass = mass;
-    P.n = n;
+    P.nslots = nslots;
+    P.seg = seg;
+    P.tile_scene = tile_scene;
+    P.remap = static_cast<int>(remap);
     return P;
 }
 
-PairList DFSPHSolver::pair_list() {
-    PairList L;
-    L.f = pairs_f_.data();
-    L.b = pairs_b_.data();
-    L.cnt = pair_cnt_.data();
-    L.kf = kPairDepthFluid;
-    L.kb = nb_ ? kPairDepthBoundary : 0u;
-    return L;
-}
-
-void DFSPHSolver::density_solve(float dt) {
-    const u32 n = static_cast<u32>(fluid_.count());
-    const u32 blocks = div_up(n, NT);
-    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, n);
-    const PairList L = pair_list();
-    const double thr = static_cast<double>(cfg_.eta_density * cfg_.rest_density);
-    const float dt2 = dt * dt;
-    DeviceState* st = dstate_.data();
-    k_reset_solve<<<1, 64, 0, stream_>>>(st, 0);
-    WSIM_LAUNCH_CHECK();
-    const u32 maxit = cfg_.max_density_iters;
-    u32 it = 0, chunk = 2;
-    while (it < maxit) {
-        const u32 end = (maxit - it < chunk) ? maxit : it + chunk;
-        for (; it < end; ++it) {
-            tbegin(kKDensityChange, static_cast<int>(it));
-            k_change<0><<<blocks, NT, 0, stream_>>>(P, L, xs_.data(), vs_.data(), rho_.data(),
-                                                    alpha_.data(), dt, dt2, cfg_.rest_density,
-                                                    static_cast<int>(cfg_.pressure_form), it,
-                                                    st, part_d_.data(), rho_adv_.data(),
-                                                    kappa_.data(), xk_.data());
-            WSIM_LAUNCH_CHECK();
-            tend();
-            tbegin(kKApplyKappa, static_cast<int>(it));
-            k_apply<<<blocks, NT, 0, stream_>>>(P, L, xk_.data(), kappa_.data(), dt,
-                                                cfg_.rest_density,
-                                                static_cast<int>(cfg_.pressure_form), it, 0, thr,
-                                                part_d_.data(), st, vs_.data());
-            WSIM_LAUNCH_CHECK();
-            tend();
-        }
-        if (it >= maxit) break;
-        WSIM_HIP(hipMemcpyAsync(&hstate_[0].conv_iter, &st->conv_iter, sizeof(u32),
-                                hipMemcpyDeviceToHost, stream_));
-        WSIM_HIP(hipStreamSynchronize(stream_));
-        if (hstate_[0].conv_iter != kNone) break;
-        chunk = chunk * 2 > 32 ? 32 : chunk * 2;
+// Per-substep scene parameters, computed on the host once per scene exactly
+// as the plain solver computes them (dfsph_kernels.cu:351 exp(-damping*dt)).
+void DFSPHSolver::set_substep(const float* dt) {
+    const u32 scenes = static_cast<u32>(counts_.size());
+    for (u32 s = 0; s < scenes; ++s) {
+        SceneSeg& g = hseg_[s];
+        const float d = dt[s];
+        g.dt = d;
+        g.dt2 = d * d;
+        g.decay = std::exp(-cfg_.damping * d);
+        g.gdx = cfg_.gravity[0] * d;
+        g.gdy = cfg_.gravity[1] * d;
+        g.gdz = cfg_.gravity[2] * d;
+        g.eps = viscosity_[s];
     }
-}
-
-void DFSPHSolver::divergence_solve(float dt) {
-    const u32 n = static_cast<u32>(fluid_.count());
-    const u32 blocks = div_up(n, NT);
-    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, n);
-    const PairList L = pair_list();
-    const double thr = static_cast<double>(cfg_.eta_divergence * cfg_.rest_density);
-    DeviceState* st = dstate_.data();
-    k_reset_solve<<<1, 64, 0, stream_>>>(st, 1);
-    WSIM_LAUNCH_CHECK();
-    const u32 maxit = cfg_.max_divergence_iters;
-    u32 it = 0, chunk = 1;
-    while (it < maxit) {
-        const u32 end = (maxit - it < chunk) ? maxit : it + chunk;
-        for (; it < end; ++it) {
-            tbegin(kKDivChange, static_cast<int>(it));
-            k_change<1><<<blocks, NT, 0, stream_>>>(P, L, xs_.data(), vs_.data(), rho_.data(),
-                                                    alpha_.data(), dt, dt * dt, cfg_.rest_density,
-                                                    static_cast<int>(cfg_.pressure_form), it,
-                                                    st, part_d_.data(), rho_adv_.data(),
-                                                    kappa_.data(), xk_.data());
-            WSIM_LAUNCH_CHECK();
-            tend();
-            tbegin(kKDivApply, static_cast<int>(it));
-            k_apply<<<blocks, NT, 0, stream_>>>(P, L, xk_.data(), kappa_.data(), dt,
-                                                cfg_.rest_density,
-                                                static_cast<int>(cfg_.pressure_form), it, 1, thr,
-                                                part_d_.data(), st, vs_.data());
-            WSIM_LAUNCH_CHECK();
-            tend();
-        }
-        if (it >= maxit) break;
-        WSIM_HIP(hipMemcpyAsync(&hstate_[0].div_conv_iter, &st->div_conv_iter, sizeof(u32),
-                                hipMemcpyDeviceToHost, stream_));
-        WSIM_HIP(hipStreamSynchronize(stream_));
-        if (hstate_[0].div_conv_iter != kNone) break;
-        chunk = chunk * 2 > 32 ? 32 : chunk * 2;
-    }
+    WSIM_HIP(hipMemcpyAsync(dseg_.data(), hseg_.data(), scenes * sizeof(SceneSeg),
+                            hipMemcpyHostToDevice, stream_));
 }
 
 void DFSPHSolver::step(float dt) {
-    const size_t nsz = fluid_.count();
-    if (nsz == 0) return;  // dfsph.cu:121-122
-    const u32 n = static_cast<u32>(nsz);
-    const u32 blocks = div_up(n, NT);
+    std::vector<float> d(batched_ ? counts_.size() : 1, dt);
+    step_scenes(d.data());
+}
 
-    sort_and_reorder(true, dt);
-    Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, n);
-    tbegin(kKDensityAlpha);
-    k_density_alpha<<<blocks, NT, 0, stream_>>>(P, pair_list(), xs_.data(), rho_.data(),
-                                                alpha_.data());
+void DFSPHSolver::step_scenes(const float* dt) {
+    if (!batched_) {
+        const size_t nsz = fluid_.count();
+        if (nsz == 0) return;  // dfsph.cu:121-122
+        if (counts_.empty() || counts_[0] != nsz) layout({static_cast<u32>(nsz)});
+    } else {
+        if (fluid_.count() != total_)
+            fail(kInvalidArgument, "batched DFSPHSolver: the store count changed");
+        if (total_ == 0) return;
+    }
+    set_substep(dt);
+    run_substep();
+}
+
+void DFSPHSolver::run_phase(u64 key, const std::function<void()>& body) {
+    if (!use_graphs_ || timing_) {
+        body();
+        return;
+    }
+    auto it = graphs_.find(key);
+    if (it == graphs_.end()) {
+        hipGraph_t g = nullptr;
+        WSIM_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        capturing_ = true;
+        try {
+            body();
+        } catch (...) {
+            capturing_ = false;
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        capturing_ = false;
+        WSIM_HIP(hipStreamEndCapture(stream_, &g));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        WSIM_HIP(e);
+        it = graphs_.emplace(key, exec).first;
+    }
+    WSIM_HIP(hipGraphLaunch(it->second, stream_));
+}
+
+void DFSPHSolver::phase_prefix(bool gravity) {
+    const u32 scenes = static_cast<u32>(counts_.size());
+    tbegin(kKGrid);
+    if (batched_) fluid_grid_.build_segmented(fluid_.positions(), total_, dstarts_.data(), scenes);
+    else fluid_grid_.build(fluid_.positions(), total_);
+    tend();
+    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, nslots_,
+                           dseg_.data(), batched_ ? tile_scene_.data() : nullptr, shapes_.remap);
+    const bool xsph = gravity && any_xsph_;
+    float4* vtarget = xsph ? vpre_.data() : vs_.data();
+    tbegin(kKReorder);
+    k_reorder<<<nslots_ / NS, NS, 0, stream_>>>(P, fluid_grid_.sorted_indices(), fluid_.positions(),
+                                                 fluid_.velocities(), xs_.data(), vtarget,
+                                                 gravity ? 1 : 0);
     WSIM_LAUNCH_CHECK();
     tend();
-    if (cfg_.viscosity > 0.0f) {  // dfsph.cu:198
+    tbegin(kKDensityAlpha);
+    launch_density_alpha(shapes_.density_alpha_nt, stream_, nslots_, P, pair_list(), xs_.data(),
+                         rho_.data(), alpha_.data());
+    tend();
+    if (xsph) {
         tbegin(kKXsph);
-        k_xsph<<<blocks, NT, 0, stream_>>>(P, xs_.data(), vs_.data(), rho_.data(), cfg_.viscosity,
-                                           vs2_.data());
-        WSIM_LAUNCH_CHECK();
+        launch_xsph(shapes_.xsph_nt, stream_, nslots_, P, xs_.data(), vpre_.data(), rho_.data(),
+                    vs_.data());
         tend();
-        std::swap(vs_, vs2_);
     }
-    density_solve(dt);
+    k_reset_solve<<<div_up(scenes, NS), NS, 0, stream_>>>(dseg_.data(), dstate_.data(), scenes,
+                                                          gravity ? 0 : 1);
+    WSIM_LAUNCH_CHECK();
+}
 
+void DFSPHSolver::phase_iterations(int mode, u32 begin, u32 end) {
+    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, nslots_,
+                           dseg_.data(), batched_ ? tile_scene_.data() : nullptr, shapes_.remap);
+    const PairList L = pair_list();
+    const double thr = mode == 0 ? static_cast<double>(cfg_.eta_density * cfg_.rest_density)
+                                 : static_cast<double>(cfg_.eta_divergence * cfg_.rest_density);
+    ChangeArgs ca{xs_.data(), vs_.data(), rho_.data(), alpha_.data(), cfg_.rest_density,
+                  static_cast<int>(cfg_.pressure_form), 0, dstate_.data(), part_d_.data(),
+                  rho_adv_.data(), kappa_.data(), xk_.data()};
+    ApplyArgs aa{xk_.data(), kappa_.data(), cfg_.rest_density, static_cast<int>(cfg_.pressure_form),
+                 0, mode, thr, part_d_.data(), dstate_.data(), vs_.data()};
+    for (u32 it = begin; it < end; ++it) {
+        ca.iter = aa.iter = it;
+        tbegin(mode == 0 ? kKDensityChange : kKDivChange, static_cast<int>(it));
+        if (mode == 0) launch_change<0>(shapes_.change_nt, shapes_.change_pu, stream_, nslots_, P, L, ca);
+        else launch_change<1>(shapes_.change_nt, shapes_.change_pu, stream_, nslots_, P, L, ca);
+        tend();
+        tbegin(mode == 0 ? kKApplyKappa : kKDivApply, static_cast<int>(it));
+        launch_apply(shapes_.apply_nt, shapes_.apply_pu, stream_, nslots_, P, L, aa);
+        tend();
+    }
+}
+
+static StoreOut store_out(ParticleStore& f, size_t hd, size_t ha, size_t hk, size_t hr) {
     StoreOut out;
-    out.x = fluid_.positions();
-    out.v = fluid_.velocities();
-    out.rho = static_cast<float*>(fluid_.attribute_data(h_density_));
-    out.alpha = static_cast<float*>(fluid_.attribute_data(h_alpha_));
-    out.kappa = static_cast<float*>(fluid_.attribute_data(h_kappa_));
-    out.rho_adv = static_cast<float*>(fluid_.attribute_data(h_density_adv_));
+    out.x = f.positions();
+    out.v = f.velocities();
+    out.rho = static_cast<float*>(f.attribute_data(hd));
+    out.alpha = static_cast<float*>(f.attribute_data(ha));
+    out.kappa = static_cast<float*>(f.attribute_data(hk));
+    out.rho_adv = static_cast<float*>(f.attribute_data(hr));
+    return out;
+}
+
+void DFSPHSolver::phase_advect() {
+    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, nslots_,
+                           dseg_.data(), batched_ ? tile_scene_.data() : nullptr, shapes_.remap);
+    const StoreOut out = store_out(fluid_, h_density_, h_alpha_, h_kappa_, h_density_adv_);
     const float eps = 1e-3f;  // dfsph_kernels.cu:377
     tbegin(kKAdvect);
-    k_advect<<<blocks, NT, 0, stream_>>>(
-        n, xs_.data(), vs_.data(), fluid_grid_.sorted_indices(), dt, 5.0f,
-        cfg_.domain_min[0] + eps, cfg_.domain_min[1] + eps, cfg_.domain_min[2] + eps,
-        cfg_.domain_max[0] - eps, cfg_.domain_max[1] - eps, cfg_.domain_max[2] - eps, out,
-        rho_.data(), alpha_.data(), kappa_.data(), rho_adv_.data(),
-        cfg_.max_density_iters > 0 ? 1 : 0, part_f_.data(), part_u_.data());
+    k_advect<<<nslots_ / NS, NS, 0, stream_>>>(
+        P, xs_.data(), vs_.data(), fluid_grid_.sorted_indices(), 5.0f, cfg_.domain_min[0] + eps,
+        cfg_.domain_min[1] + eps, cfg_.domain_min[2] + eps, cfg_.domain_max[0] - eps,
+        cfg_.domain_max[1] - eps, cfg_.domain_max[2] - eps, out, rho_.data(), alpha_.data(),
+        kappa_.data(), rho_adv_.data(), cfg_.max_density_iters > 0 ? 1 : 0, part_f_.data(),
+        part_u_.data());
     WSIM_LAUNCH_CHECK();
     tend();
     if (!cfg_.enable_divergence_solve) {
-        k_finish_vstats<<<1, NT, 0, stream_>>>(part_f_.data(), part_u_.data(), blocks, dstate_.data());
+        k_finish_vstats<<<static_cast<u32>(counts_.size()), NS, 0, stream_>>>(
+            dseg_.data(), part_f_.data(), part_u_.data(), dstate_.data());
         WSIM_LAUNCH_CHECK();
     }
+}
 
+void DFSPHSolver::phase_div_tail() {
+    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, nslots_,
+                           dseg_.data(), batched_ ? tile_scene_.data() : nullptr, shapes_.remap);
+    const StoreOut out = store_out(fluid_, h_density_, h_alpha_, h_kappa_v_, h_density_adv_);
+    tbegin(kKDivWriteback);
+    k_div_writeback<<<nslots_ / NS, NS, 0, stream_>>>(
+        P, xs_.data(), vs_.data(), fluid_grid_.sorted_indices(), out, rho_.data(), alpha_.data(),
+        kappa_.data(), rho_adv_.data(), cfg_.max_divergence_iters > 0 ? 1 : 0, part_f_.data(),
+        part_u_.data());
+    WSIM_LAUNCH_CHECK();
+    tend();
+    k_finish_vstats<<<static_cast<u32>(counts_.size()), NS, 0, stream_>>>(
+        dseg_.data(), part_f_.data(), part_u_.data(), dstate_.data());
+    WSIM_LAUNCH_CHECK();
+}
+
+bool DFSPHSolver::all_converged(int mode) {
+    const u32 scenes = static_cast<u32>(counts_.size());
+    WSIM_HIP(hipMemcpyAsync(hstate_.data(), dstate_.data(), scenes * sizeof(DeviceState),
+                            hipMemcpyDeviceToHost, stream_));
+    WSIM_HIP(hipStreamSynchronize(stream_));
+    for (u32 s = 0; s < scenes; ++s) {
+        const u32 c = mode == 0 ? hstate_[s].conv_iter : hstate_[s].div_conv_iter;
+        if (c == kNone) return false;
+    }
+    return true;
+}
+
+// Iterations run in chunks of 2, 2, 4, ... 32 between host checks; inside a
+// chunk, scenes that converged turn the remaining launches into no-ops.
+void DFSPHSolver::solve(int mode) {
+    const u32 maxit = mode == 0 ? cfg_.max_density_iters : cfg_.max_divergence_iters;
+    u32 it = 0, chunk = mode == 0 ? 2 : 1;
+    while (it < maxit) {
+        const u32 end = (maxit - it < chunk) ? maxit : it + chunk;
+        run_phase(phase_key(mode == 0 ? kPhDensityIters : kPhDivIters, it, end),
+                  [&] { phase_iterations(mode, it, end); });
+        it = end;
+        if (it >= maxit) break;
+        if (all_converged(mode)) break;
+        chunk = chunk * 2 > 32 ? 32 : chunk * 2;
+    }
+}
+
+void DFSPHSolver::run_substep() {
+    run_phase(phase_key(kPhPrefix), [&] { phase_prefix(true); });
+    solve(0);
+    run_phase(phase_key(kPhAdvect), [&] { phase_advect(); });
     if (cfg_.enable_divergence_solve) {
-        sort_and_reorder(false, dt);
-        P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, n);
-        tbegin(kKDensityAlpha);
-        k_density_alpha<<<blocks, NT, 0, stream_>>>(P, pair_list(), xs_.data(), rho_.data(),
-                                                    alpha_.data());
-        WSIM_LAUNCH_CHECK();
-        tend();
-        divergence_solve(dt);
-        out.kappa = static_cast<float*>(fluid_.attribute_data(h_kappa_v_));
-        tbegin(kKDivWriteback);
-        k_div_writeback<<<blocks, NT, 0, stream_>>>(
-            n, xs_.data(), vs_.data(), fluid_grid_.sorted_indices(), out, rho_.data(),
-            alpha_.data(), kappa_.data(), rho_adv_.data(), cfg_.max_divergence_iters > 0 ? 1 : 0,
-            part_f_.data(), part_u_.data());
-        WSIM_LAUNCH_CHECK();
-        tend();
-        k_finish_vstats<<<1, NT, 0, stream_>>>(part_f_.data(), part_u_.data(), blocks, dstate_.data());
-        WSIM_LAUNCH_CHECK();
+        run_phase(phase_key(kPhDivPrefix), [&] { phase_prefix(false); });
+        solve(1);
+        run_phase(phase_key(kPhDivTail), [&] { phase_div_tail(); });
     }
     read_state();
 }
 
 void DFSPHSolver::read_state() {
-    WSIM_HIP(hipMemcpyAsync(hstate_.data(), dstate_.data(), sizeof(DeviceState),
+    const u32 scenes = static_cast<u32>(counts_.size());
+    WSIM_HIP(hipMemcpyAsync(hstate_.data(), dstate_.data(), scenes * sizeof(DeviceState),
                             hipMemcpyDeviceToHost, stream_));
     WSIM_HIP(hipStreamSynchronize(stream_));
-    const DeviceState& h = hstate_[0];
-    const u32 maxit = cfg_.max_density_iters;
-    stats_.density_iters = maxit == 0 ? 0 : (h.conv_iter == kNone ? maxit : h.conv_iter + 1);
-    stats_.mean_density_error = maxit == 0 ? 0.0 : h.err;
-    if (cfg_.enable_divergence_solve) {
-        const u32 mv = cfg_.max_divergence_iters;
-        stats_.divergence_iters = mv == 0 ? 0 : (h.div_conv_iter == kNone ? mv : h.div_conv_iter + 1);
-        stats_.mean_divergence_error = mv == 0 ? 0.0 : h.div_err;
+    u32 max_d = 0, max_v = 0;
+    for (u32 s = 0; s < scenes; ++s) {
+        const DeviceState& h = hstate_[s];
+        StepStats& st = stats_[s];
+        const bool empty = counts_[s] == 0;
+        const u32 maxit = cfg_.max_density_iters;
+        st.density_iters =
+            (maxit == 0 || empty) ? 0 : (h.conv_iter == kNone ? maxit : h.conv_iter + 1);
+        st.mean_density_error = (maxit == 0 || empty) ? 0.0 : h.err;
+        if (cfg_.enable_divergence_solve) {
+            const u32 mv = cfg_.max_divergence_iters;
+            st.divergence_iters =
+                (mv == 0 || empty) ? 0 : (h.div_conv_iter == kNone ? mv : h.div_conv_iter + 1);
+            st.mean_divergence_error = (mv == 0 || empty) ? 0.0 : h.div_err;
+        }
+        st.max_velocity = h.max_v;
+        st.nan_count = h.nan_count;
+        st.substeps += 1;
+        max_d = std::max(max_d, st.density_iters);
+        max_v = std::max(max_v, st.divergence_iters);
     }
-    stats_.max_velocity = h.max_v;
-    stats_.nan_count = h.nan_count;
-    stats_.substeps += 1;
     if (timing_) {
         int valid[kKNumKinds];
         for (int k = 0; k < kKNumKinds; ++k) valid[k] = std::numeric_limits<int>::max();
-        valid[kKDensityChange] = valid[kKApplyKappa] = static_cast<int>(stats_.density_iters);
-        valid[kKDivChange] = valid[kKDivApply] = static_cast<int>(stats_.divergence_iters);
+        valid[kKDensityChange] = valid[kKApplyKappa] = static_cast<int>(max_d);
+        valid[kKDivChange] = valid[kKDivApply] = static_cast<int>(max_v);
         timer_.collect(valid);
     }
 }
@@ -986,6 +1378,132 @@ float DFSPHSolver::attribute_at(int which, size_t i) const {
                             hipMemcpyDeviceToHost, stream_));
     WSIM_HIP(hipStreamSynchronize(stream_));
     return v;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner.  Every candidate computes the same bits, so the search runs on
+// the solver's own data: one substep prefix on the current store state (no
+// gravity step is written back, the store is not touched), then each kernel
+// is launched `reps` times per candidate between HIP events and the median
+// launch time decides.  Outputs of the timed launches land in solver buffers
+// that the next substep rebuilds from the store.
+void DFSPHSolver::autotune(int reps) {
+    if (reps < 1) fail(kInvalidArgument, "autotune: reps must be positive");
+    if (!batched_) {
+        const size_t nsz = fluid_.count();
+        if (nsz == 0) return;
+        if (counts_.empty() || counts_[0] != nsz) layout({static_cast<u32>(nsz)});
+    }
+    if (total_ == 0) return;
+    drop_graphs();
+    const bool saved_timing = timing_;
+    timing_ = false;
+    const u32 scenes = static_cast<u32>(counts_.size());
+    std::vector<float> dt(scenes);
+    for (u32 s = 0; s < scenes; ++s) dt[s] = hseg_[s].dt > 0.0f ? hseg_[s].dt : 1e-3f;
+    set_substep(dt.data());
+    phase_prefix(true);
+
+    hipEvent_t ev[2];
+    WSIM_HIP(hipEventCreate(&ev[0]));
+    WSIM_HIP(hipEventCreate(&ev[1]));
+    auto time_ms = [&](const std::function<void()>& launch) {
+        launch();  // warm-up
+        std::vector<float> t(reps);
+        for (int r = 0; r < reps; ++r) {
+            WSIM_HIP(hipEventRecord(ev[0], stream_));
+            launch();
+            WSIM_HIP(hipEventRecord(ev[1], stream_));
+            WSIM_HIP(hipEventSynchronize(ev[1]));
+            WSIM_HIP(hipEventElapsedTime(&t[r], ev[0], ev[1]));
+        }
+        std::nth_element(t.begin(), t.begin() + reps / 2, t.end());
+        return static_cast<double>(t[reps / 2]);
+    };
+    tune_log_.clear();
+    KernelShapes best = shapes_;
+    const PairList L = pair_list();
+    const u32* tsc = batched_ ? tile_scene_.data() : nullptr;
+    auto nbr = [&](u32 remap) {
+        return make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, nslots_,
+                        dseg_.data(), tsc, remap);
+    };
+    ChangeArgs ca{xs_.data(), vs_.data(), rho_.data(), alpha_.data(), cfg_.rest_density,
+                  static_cast<int>(cfg_.pressure_form), 0, dstate_.data(), part_d_.data(),
+                  rho_adv_.data(), kappa_.data(), xk_.data()};
+    ApplyArgs aa{xk_.data(), kappa_.data(), cfg_.rest_density, static_cast<int>(cfg_.pressure_form),
+                 0, 0, static_cast<double>(cfg_.eta_density * cfg_.rest_density), part_d_.data(),
+                 dstate_.data(), vs_.data()};
+    try {
+        // cell_start pass
+        const u32 ncs = scenes * (fluid_grid_.total_cells() + 1u) + 1u;
+        double bt = 1e300;
+        for (u32 cpt : {4u, 8u, 16u}) {
+            const double ms = time_ms([&] {
+                build_cell_start(fluid_grid_.sorted_keys(), total_, ncs,
+                                 const_cast<u32*>(fluid_grid_.cell_start()), stream_, cpt);
+            });
+            tune_log_.push_back({kKGrid, 256, 0, 0, cpt, ms});
+            if (ms < bt) bt = ms, best.cell_cpt = cpt;
+        }
+        double best_total = 0.0;
+        for (u32 remap : {0u, 1u}) {
+            const Nbr P = nbr(remap);
+            double b_da = 1e300, b_xs = 1e300, b_ch = 1e300, b_ap = 1e300;
+            KernelShapes cand = best;
+            for (u32 nt : {64u, 128u, 256u, 512u}) {
+                const double ms = time_ms([&] {
+                    launch_density_alpha(nt, stream_, nslots_, P, L, xs_.data(), rho_.data(),
+                                         alpha_.data());
+                });
+                tune_log_.push_back({kKDensityAlpha, nt, 0, remap, 0, ms});
+                if (ms < b_da) b_da = ms, cand.density_alpha_nt = nt;
+            }
+            if (any_xsph_) {
+                for (u32 nt : {64u, 128u, 256u, 512u}) {
+                    const double ms = time_ms([&] {
+                        launch_xsph(nt, stream_, nslots_, P, xs_.data(), vpre_.data(), rho_.data(),
+                                    vs_.data());
+                    });
+                    tune_log_.push_back({kKXsph, nt, 0, remap, 0, ms});
+                    if (ms < b_xs) b_xs = ms, cand.xsph_nt = nt;
+                }
+            } else {
+                b_xs = 0.0;
+            }
+            for (u32 nt : {256u, 512u, 1024u})
+                for (u32 pu : {2u, 4u, 8u}) {
+                    const double ms = time_ms(
+                        [&] { launch_change<0>(nt, pu, stream_, nslots_, P, L, ca); });
+                    tune_log_.push_back({kKDensityChange, nt, pu, remap, 0, ms});
+                    if (ms < b_ch) b_ch = ms, cand.change_nt = nt, cand.change_pu = pu;
+                }
+            for (u32 nt : {64u, 128u, 256u, 512u})
+                for (u32 pu : {2u, 4u, 8u}) {
+                    const double ms =
+                        time_ms([&] { launch_apply(nt, pu, stream_, nslots_, P, L, aa); });
+                    tune_log_.push_back({kKApplyKappa, nt, pu, remap, 0, ms});
+                    if (ms < b_ap) b_ap = ms, cand.apply_nt = nt, cand.apply_pu = pu;
+                }
+            const double total = b_da + b_xs + b_ch + b_ap;
+            if (remap == 0 || total < best_total) {
+                best_total = total;
+                cand.remap = remap;
+                cand.cell_cpt = best.cell_cpt;
+                best = cand;
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        timing_ = saved_timing;
+        throw;
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    timing_ = saved_timing;
+    WSIM_HIP(hipStreamSynchronize(stream_));
+    set_shapes(best);
 }
 
 }  // namespace wsim

@@ -2,6 +2,8 @@
 #pragma once
 
 #include <array>
+#include <functional>
+#include <map>
 #include <memory>
 #include <string>
 #include <unordered_map>
@@ -59,6 +61,19 @@ public:
     void reserve(size_t capacity);
     void build(const float* positions_xyz, size_t count);
 
+    // Scene-segmented grid: particles [starts[s], starts[s+1]) (device array
+    // of scenes+1 offsets) belong to scene s and take the key s*(C+1) + key,
+    // scene-local sentinel included, so one sort orders every scene's cells
+    // and cell_start holds scenes*(C+1)+1 entries.  One scene is build().
+    void reserve_segments(u32 scenes);
+    void build_segmented(const float* positions_xyz, size_t count, const u32* starts,
+                         u32 scenes);
+    u32 segments() const { return segments_; }
+    // Cells per thread of the cell_start pass (4, 8 or 16); any value gives
+    // the same table.
+    void set_cells_per_thread(u32 cpt);
+    u32 cells_per_thread() const { return cpt_; }
+
     const u32* cell_start() const { return cs_.data(); }
     const u32* sorted_indices() const { return v_[cur_].data(); }
     const u32* sorted_keys() const { return k_[cur_].data(); }
@@ -76,6 +91,8 @@ private:
     size_t capacity_ = 0, count_ = 0;
     DeviceBuffer<u32> k_[2], v_[2], hist_, cs_;
     int cur_ = 0;
+    u32 segments_ = 1;
+    u32 cpt_ = 16;
 };
 
 // Plain 16-byte streaming copy kernel (copy.hip).
@@ -83,7 +100,7 @@ void device_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
 
 // Cell-start table from sorted keys (one pass over the C+1 cells).
 void build_cell_start(const u32* sorted_keys, u32 n, u32 num_cells_plus_one, u32* cell_start,
-                      hipStream_t stream);
+                      hipStream_t stream, u32 cells_per_thread = 16);
 
 // ---------------------------------------------------------------------------
 // Morton path of SpatialAccel (spatial_accel.h:19-53, spatial_accel.cu:11-40,
@@ -101,6 +118,7 @@ private:
     DeviceBuffer<u32> k_[2], v_[2], hist_;
     int cur_ = 0;
 };
+
 
 // ---------------------------------------------------------------------------
 // DFSPH solver (dfsph.h:17-84, dfsph.cu, dfsph_kernels.cu).
@@ -132,11 +150,33 @@ struct StepStats {
     u64 substeps = 0;
 };
 
-// Device-resident solve state; counters self-reset so no memset is needed
-// between launches (all of it is valid under graph capture).
+// Working-set slots are grouped in tiles of kTile; each scene starts on a
+// tile boundary, so every tile (and every wave) belongs to one scene and the
+// binary64 error partials are per tile whatever the workgroup size.
+constexpr u32 kTile = 256;
+
+// One scene of the packed working set.  The static part is set when the
+// scene layout is fixed, the per-substep part before every substep; the
+// device copy is read with scalar loads by every kernel.
+struct SceneSeg {
+    u32 off;       // first working-set slot (multiple of kTile)
+    u32 n;         // fluid particles
+    u32 start;     // first original index == first position in the sorted order
+    u32 tile0;     // off / kTile
+    u32 ntiles;    // ceil(n / kTile)
+    u32 key_base;  // scene * (C + 1)
+    u32 pad0, pad1;
+    float dt, dt2, decay;  // dt, dt*dt, exp(-damping*dt)
+    float gdx, gdy, gdz;   // gravity * dt
+    float eps;             // XSPH viscosity
+    float pad2;
+};
+static_assert(sizeof(SceneSeg) == 64, "SceneSeg layout");
+
+// Device-resident solve state of one scene; kernels keep it current, the
+// host reads it once per substep (and between iteration chunks).
 struct DeviceState {
-    u32 ticket[4];
-    u32 conv_iter;      // density solve: iteration whose error test passed, or kNone
+    u32 conv_iter;      // density solve: iteration whose error test passed, or ~0u
     u32 div_conv_iter;  // divergence solve: same
     u32 nan_count;
     float max_v;
@@ -189,7 +229,7 @@ private:
 
 // Per-particle pair lists of the pressure solves (see solver.hip).
 struct PairList {
-    uint2* f;   // fluid pairs (slot, bits of c), ELL [k * n + i]
+    uint2* f;   // fluid pairs (slot, bits of c), wave-blocked ELL
     uint2* b;   // boundary pairs
     u32* cnt;   // fluid count | boundary count << 16 (saturating)
     u32 kf, kb; // list depths
@@ -200,53 +240,128 @@ struct PairList {
 constexpr u32 kPairDepthFluid = 48;
 constexpr u32 kPairDepthBoundary = 48;
 
+// Launch shapes of the stencil kernels.  Every choice computes the same bits;
+// the autotuner times them on the solver's own data and keeps the fastest.
+struct KernelShapes {
+    u32 density_alpha_nt = 256;  // workgroup size: 64, 128, 256, 512
+    u32 xsph_nt = 256;           // 64, 128, 256, 512
+    u32 change_nt = 256;         // 256, 512, 1024
+    u32 change_pu = 4;           // pairs per trip of the list loops: 2, 4, 8
+    u32 apply_nt = 256;          // 64, 128, 256, 512
+    u32 apply_pu = 4;            // 2, 4, 8
+    u32 cell_cpt = 16;           // cells per thread of the cell_start pass: 4, 8, 16
+    u32 remap = 1;               // XCD-aware workgroup -> tile order
+};
+bool valid_shapes(const KernelShapes& s);
+
+// Timing of one autotuner candidate (milliseconds per launch, median).
+struct TuneRecord {
+    int kernel;  // KernelKind (kKGrid stands for the cell_start pass)
+    u32 nt, pu, remap, cpt;
+    double ms;
+};
+
+// DFSPH substep over one or many independent scenes.  A plain solver is one
+// scene holding every particle of the fluid store; a batched solver packs
+// scenes of uneven sizes (consecutive index ranges of the store) into one
+// working set, one grid sort and one launch per phase, with per-scene dt,
+// viscosity, convergence test and statistics.  Scene s of a batch produces
+// the same bits as a plain solver run on that scene alone.
 class DFSPHSolver {
 public:
     DFSPHSolver(ParticleStore& fluid, ParticleStore* boundary, const DFSPHConfig& cfg,
                 hipStream_t stream);
-    void step(float dt);
+    // Batched: scene s holds `counts[s]` particles (store indices in order),
+    // XSPH viscosity viscosity[s] (nullptr: cfg.viscosity for all).
+    DFSPHSolver(ParticleStore& fluid, ParticleStore* boundary, const DFSPHConfig& cfg,
+                hipStream_t stream, const std::vector<u32>& counts,
+                const std::vector<float>& viscosity);
+    ~DFSPHSolver();
+    DFSPHSolver(const DFSPHSolver&) = delete;
+    DFSPHSolver& operator=(const DFSPHSolver&) = delete;
 
-    float max_velocity() const { return stats_.max_velocity; }
+    void step(float dt);               // every scene advances by dt
+    void step_scenes(const float* dt);  // scene s advances by dt[s]
+
+    float max_velocity() const;  // over all scenes
     float particle_mass() const { return mass_; }
-    u32 last_density_iters() const { return stats_.density_iters; }
-    const StepStats& stats() const { return stats_; }
+    u32 last_density_iters() const { return stats_[0].density_iters; }
+    const StepStats& stats() const { return stats_[0]; }
+    const StepStats& scene_stats(size_t s) const;
+    size_t scene_count() const { return counts_.size(); }
+    bool batched() const { return batched_; }
+    const std::vector<u32>& scene_starts() const { return starts_; }
     const CellGrid& fluid_grid() const { return fluid_grid_; }
     const CellGrid* boundary_grid() const { return boundary_grid_.get(); }
     float attribute_at(int which, size_t i) const;  // 0 density, 1 alpha, 2 boundary mass
     const DFSPHConfig& config() const { return cfg_; }
+    hipStream_t stream() const { return stream_; }
 
     void set_timing(bool on) { timing_ = on; }
     KernelTimer& timer() { return timer_; }
 
+    // Substep phases as captured HIP graphs (one per phase and per iteration
+    // chunk), replayed while the scene layout and the shapes are unchanged.
+    void set_graphs(bool on);
+    bool graphs() const { return use_graphs_; }
+
+    const KernelShapes& shapes() const { return shapes_; }
+    void set_shapes(const KernelShapes& s);
+    // Times every shape candidate of the stencil kernels on the current
+    // particle state (solver buffers only: the store and the results of the
+    // next step are untouched) and keeps the fastest per kernel.
+    void autotune(int reps);
+    const std::vector<TuneRecord>& tune_log() const { return tune_log_; }
+
 private:
-    void density_solve(float dt);
-    void divergence_solve(float dt);
-    void sort_and_reorder(bool gravity, float dt);
+    void init(const std::vector<u32>& counts, const std::vector<float>& viscosity);
+    void layout(const std::vector<u32>& counts);
+    void set_substep(const float* dt);
+    void run_substep();
+    void phase_prefix(bool gravity);
+    void phase_iterations(int mode, u32 begin, u32 end);
+    void phase_advect();
+    void phase_div_tail();
+    void solve(int mode);
+    bool all_converged(int mode);
     void read_state();
+    void run_phase(u64 key, const std::function<void()>& body);
+    void drop_graphs();
     PairList pair_list();
     void tbegin(int kind, int iter = 0) {
-        if (timing_) timer_.begin(kind, iter, stream_);
+        if (timing_ && !capturing_) timer_.begin(kind, iter, stream_);
     }
     void tend() {
-        if (timing_) timer_.end(stream_);
+        if (timing_ && !capturing_) timer_.end(stream_);
     }
 
     ParticleStore& fluid_;
     ParticleStore* boundary_;
     DFSPHConfig cfg_;
+    hipStream_t own_stream_ = nullptr;  // created when the caller passes the null stream
     hipStream_t stream_;
     CellGrid fluid_grid_;
     std::unique_ptr<CellGrid> boundary_grid_;
     float mass_ = 0.0f;
     size_t nb_ = 0;
-    StepStats stats_;
+    bool batched_ = false;
+
+    // scene layout
+    std::vector<u32> counts_, starts_;
+    std::vector<float> viscosity_;
+    u32 nslots_ = 0, total_ = 0;
+    bool any_xsph_ = false;
+    PinnedBuffer<SceneSeg> hseg_;
+    DeviceBuffer<SceneSeg> dseg_;
+    DeviceBuffer<u32> dstarts_;
+    DeviceBuffer<u32> tile_scene_;
+    std::vector<StepStats> stats_;
 
     size_t h_density_ = 0, h_alpha_ = 0, h_kappa_ = 0, h_density_adv_ = 0, h_kappa_v_ = 0;
     size_t h_bmass_ = 0;
-    bool has_kappa_v_ = false;
 
-    // cell-sorted working set
-    DeviceBuffer<float4> xs_, vs_, vs2_, xk_, xb_;
+    // cell-sorted working set (nslots_ entries, scenes tile-aligned)
+    DeviceBuffer<float4> xs_, vs_, vpre_, xk_, xb_;
     DeviceBuffer<uint2> pairs_f_, pairs_b_;
     DeviceBuffer<u32> pair_cnt_;
     DeviceBuffer<float> rho_, alpha_, kappa_, rho_adv_;
@@ -255,6 +370,11 @@ private:
     DeviceBuffer<u32> part_u_;
     DeviceBuffer<DeviceState> dstate_;
     PinnedBuffer<DeviceState> hstate_;
+
+    KernelShapes shapes_;
+    std::vector<TuneRecord> tune_log_;
+    bool use_graphs_ = false, capturing_ = false;
+    std::map<u64, hipGraphExec_t> graphs_;
 
     bool timing_ = false;
     KernelTimer timer_;
