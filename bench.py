@@ -86,6 +86,102 @@ def cpu_baseline(x, v, xb, cfg, dts, budget_s: float):
     }, done, iters
 
 
+def batched_leg(steps: int, warmup: int, tune_reps: int):
+    """64 uneven box scenes (seeded mix, ~1.05M fluid particles in total): one
+    BatchSolver against 64 plain DFSPHSolvers run one after another, both on
+    the same per-scene dt schedule; the sequential side is timed with and
+    without graph replay and the speedup is taken against the faster."""
+    import water_simulation_amd as ws
+    from water_simulation_amd import scenes as SC
+
+    mix = SC.uneven_mix(64)
+    counts = [len(s.positions) for s in mix]
+    total = int(sum(counts))
+    xb = ws.sample_aabb_boundary((0, 0, 0), (SC.BOX,) * 3, SC.WALL_SPACING, SC.WALL_LAYERS)
+    cfg = ws.DFSPHConfig(**SC.box_config())
+    ts = ws.TimeStepper()
+
+    def walls():
+        b = ws.ParticleStore(len(xb))
+        b.resize(len(xb))
+        b.set_positions(xb)
+        return b
+
+    def dt_of(vmax):
+        return ts.next_dt(max(vmax, 0.5), SC.RADIUS, 1.0)
+
+    fluid = ws.ParticleStore(total)
+    fluid.resize(total)
+    fluid.set_positions(np.concatenate([s.positions for s in mix]))
+    fluid.set_velocities(np.concatenate([s.velocities for s in mix]))
+    bw = walls()
+    batch = ws.BatchSolver(fluid, bw, cfg, counts, [s.viscosity for s in mix])
+    tune = batch.autotune(reps=tune_reps) if tune_reps > 0 else []
+    vmax = [SC.initial_max_speed(s.velocities) for s in mix]
+    sched, iters = [], []
+    ws.device_synchronize()
+    t_b = 0.0
+    for k in range(warmup + steps):
+        dt = np.array([dt_of(v) for v in vmax], np.float32)
+        sched.append(dt)
+        t0 = time.perf_counter()
+        batch.step(dt)
+        el = time.perf_counter() - t0
+        if k >= warmup:
+            t_b += el
+        st = batch.all_stats()
+        vmax = [s["max_velocity"] for s in st]
+        if k >= warmup:
+            iters.append(np.mean([s["density_iters"] for s in st]))
+    xb_final = fluid.get_positions()
+    sched = np.array(sched)
+
+    def sequential(graphs: bool):
+        t, same = 0.0, True
+        for s, scene in enumerate(mix):
+            f = ws.ParticleStore(counts[s])
+            f.resize(counts[s])
+            f.set_positions(scene.positions)
+            f.set_velocities(scene.velocities)
+            c = ws.DFSPHConfig(**SC.box_config(viscosity=scene.viscosity))
+            p = ws.DFSPHSolver(f, walls(), c)
+            p.set_graphs(graphs)
+            for k in range(warmup + steps):
+                t0 = time.perf_counter()
+                p.step(float(sched[k, s]))
+                el = time.perf_counter() - t0
+                if k >= warmup:
+                    t += el
+            a, b = batch.offsets[s], batch.offsets[s + 1]
+            same &= bool(np.array_equal(f.get_positions().view(np.uint32),
+                                        xb_final[a:b].view(np.uint32)))
+            p.close()
+        return t, same
+
+    t_eager, same_e = sequential(False)
+    t_graph, same_g = sequential(True)
+    t_seq = min(t_eager, t_graph)
+    return {
+        "workload": f"uneven batched mix: 64 box scenes (0.6 m, r=0.005 m, walls 0.01 m x 3 "
+                    f"layers), {min(counts)}..{max(counts)} particles per scene, {total} total, "
+                    f"v ~ N(0, 0.5 m/s), XSPH in [0.005, 0.05], eta=0.1%",
+        "scenes": 64,
+        "particles": total,
+        "substeps": steps,
+        "warmup": warmup,
+        "batched_s": t_b,
+        "sequential_eager_s": t_eager,
+        "sequential_graphs_s": t_graph,
+        "speedup": t_seq / t_b,
+        "batched_particle_substeps_per_s": total * steps / t_b,
+        "sequential_particle_substeps_per_s": total * steps / t_seq,
+        "mean_density_iters": float(np.mean(iters)),
+        "bitwise_equal_to_sequential": bool(same_e and same_g),
+        "shapes": batch.shapes(),
+        "autotune_candidates": len(tune),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +192,11 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--kernel-times", action="store_true", help="print per-kernel table to stderr")
+    ap.add_argument("--batch-steps", type=int, default=20,
+                    help="substeps per scene in the batched-vs-sequential leg (0: skip it)")
+    ap.add_argument("--batch-warmup", type=int, default=3)
+    ap.add_argument("--tune-reps", type=int, default=3)
+    ap.add_argument("--only-batch", action="store_true", help="run only the batched leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run libwsim's streaming copy kernel 5 times over BYTES bytes "
                          "(the known-size workload for converting HBM byte counters)")
@@ -124,6 +225,11 @@ def main() -> int:
     if ws.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
+
+    if args.only_batch:
+        print(json.dumps({"batched": batched_leg(args.batch_steps, args.batch_warmup,
+                                                 args.tune_reps)}))
+        return 0
 
     x, xb, cfg = make_workload(args.n_side)
     cfg["pressure_form"] = args.pressure_form
@@ -183,12 +289,18 @@ def main() -> int:
                 print(f"  {k:15s} launches {c:6d}  total {ms:9.2f} ms  mean {ms / c * 1e3:9.1f} us"
                       f"  {100 * ms / tot:5.1f} %", file=sys.stderr)
 
+    solver.close()
+    batched = None
+    if args.batch_steps > 0:
+        batched = batched_leg(args.batch_steps, args.batch_warmup, args.tune_reps)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu, _, _ = cpu_baseline(x0, v0, xb, cfg, dts, args.cpu_budget)
 
     out = {
-        "metric": "particle-substeps/s at 1M particles, eta<=0.1%",
+        "metric": "particle-substeps/s at 1M particles, eta<=0.1%; batched/seq speedup; "
+                  "adjoint/fwd time",
         "value": value,
         "unit": "particle-substeps/s",
         "n_gpus": 1,
@@ -228,6 +340,7 @@ def main() -> int:
             "launches": launches,
         },
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out))
     return 0
