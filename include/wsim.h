@@ -229,6 +229,35 @@ uint32_t wsim_batch_scene_count(wsim_solver batch);
 wsim_status wsim_batch_scene_offsets(wsim_solver batch, uint32_t* out);
 wsim_status wsim_batch_scene_stats(wsim_solver batch, uint32_t scene, wsim_step_stats* out);
 
+/* ---- differentiation ----
+ * No counterpart in the reference.  Reverse-mode derivatives of plain-solver
+ * substeps (density solve; the divergence solve must be off) with the
+ * discrete structure of the forward pass frozen: neighbour sets, iteration
+ * count, max(0, .) and clamp masks.  Adjoint arrays are binary64 device
+ * arrays of 3n values in original particle order. */
+wsim_status wsim_solver_set_viscosity(wsim_solver solver, float viscosity);
+/* One substep from the store's current state, recorded, then transposed:
+ * d(loss)/d(output x, v) -> d(loss)/d(input x, v) and d(loss)/d(viscosity).
+ * The store is left at the substep's input state. */
+wsim_status wsim_adjoint_step(wsim_solver solver, float dt, const double* gx_out,
+                              const double* gv_out, double* gx_in, double* gv_in,
+                              double* g_viscosity);
+/* Checkpointed rollout: forward keeps the state every checkpoint_every
+ * substeps; backward recomputes one segment at a time from its checkpoint,
+ * recording every pressure iterate, and runs the substep adjoints last to
+ * first. */
+typedef struct wsim_rollout_t* wsim_rollout;
+wsim_status wsim_rollout_create(wsim_solver solver, uint32_t checkpoint_every, wsim_rollout* out);
+void wsim_rollout_destroy(wsim_rollout rollout);
+/* x0, v0: device, 3n floats; dt: host, substeps values.  Leaves the final
+ * state in the solver's fluid store. */
+wsim_status wsim_rollout_forward(wsim_rollout rollout, const float* x0, const float* v0,
+                                 const float* dt, uint32_t substeps);
+/* gx, gv: d(loss)/d(final x, v); writes d(loss)/d(x0, v0) and
+ * d(loss)/d(viscosity).  Leaves the store at the initial state. */
+wsim_status wsim_rollout_backward(wsim_rollout rollout, const double* gx, const double* gv,
+                                  double* gx0, double* gv0, double* g_viscosity);
+
 /* ---- host-side surface ---- */
 /* sample_aabb_boundary(lo, hi, spacing, n_layers)
  * (solvers/dfsph/include/water/solvers/boundary_sampler.h:24-25).  Writes up to

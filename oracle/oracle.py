@@ -16,7 +16,7 @@ HERE = Path(__file__).resolve().parent
 ORACLE_LIB = HERE / "liboracle.so"
 REF_LIB = HERE / "_ref" / "libref_host.so"
 
-F32, F64 = 32, 64
+F32, F64, DUAL = 32, 64, 33
 FIELD_X, FIELD_V, FIELD_RHO, FIELD_ALPHA, FIELD_KAPPA, FIELD_RHO_ADV, FIELD_BMASS, FIELD_XB = range(8)
 
 F3 = C.c_float * 3
@@ -58,6 +58,14 @@ class OrcStats(C.Structure):
 vp, sz = C.c_void_p, C.c_size_t
 _SIGS = {
     "orc_set_threads": (None, [C.c_int]),
+    "orc_solver_seed": (None, [vp, vp, vp, C.c_double]),
+    "orc_solver_set_state_f64": (None, [vp, vp, vp]),
+    "orc_solver_set_viscosity_f64": (None, [vp, C.c_double]),
+    "orc_solver_get_tangent": (None, [vp, C.c_int, vp]),
+    "orc_tape_create": (vp, []),
+    "orc_tape_destroy": (None, [vp]),
+    "orc_tape_builds": (sz, [vp]),
+    "orc_solver_set_tape": (None, [vp, vp, C.c_int]),
     "orc_set_xsph_in_place": (None, [C.c_int]),
     "orc_get_threads": (C.c_int, []),
     "orc_W_f32": (C.c_float, [C.c_float, C.c_float]),
@@ -264,6 +272,15 @@ class Solver:
         lib().orc_solver_set_state(self._h, None if xa is None else xa.ctypes.data,
                                    None if va is None else va.ctypes.data)
 
+    def set_state_f64(self, x=None, v=None) -> None:
+        xa = None if x is None else np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        va = None if v is None else np.ascontiguousarray(v, np.float64).reshape(-1, 3)
+        lib().orc_solver_set_state_f64(self._h, None if xa is None else xa.ctypes.data,
+                                       None if va is None else va.ctypes.data)
+
+    def set_viscosity_f64(self, viscosity: float) -> None:
+        lib().orc_solver_set_viscosity_f64(self._h, float(viscosity))
+
     def _size(self, field: int) -> tuple:
         if field in (FIELD_X, FIELD_V):
             return (self.n, 3)
@@ -288,5 +305,43 @@ class Solver:
         lib().orc_solver_stats(self._h, C.byref(s))
         return {f[0]: getattr(s, f[0]) for f in s._fields_}
 
+    # gradient checks
+    def seed(self, dx=None, dv=None, d_viscosity: float = 0.0) -> None:
+        """Tangent seeds of a DUAL solver (initial positions, velocities, viscosity)."""
+        assert self.precision == DUAL
+        ax = None if dx is None else np.ascontiguousarray(dx, np.float64).reshape(-1, 3)
+        av = None if dv is None else np.ascontiguousarray(dv, np.float64).reshape(-1, 3)
+        lib().orc_solver_seed(self._h, None if ax is None else ax.ctypes.data,
+                              None if av is None else av.ctypes.data, float(d_viscosity))
+
+    def tangent(self, field: int) -> np.ndarray:
+        assert self.precision == DUAL
+        out = np.empty(self._size(field), np.float64)
+        lib().orc_solver_get_tangent(self._h, field, out.ctypes.data)
+        return out
+
+    def set_tape(self, tape: "Tape", mode: int) -> None:
+        """mode 1: record the neighbour structure and iteration counts; 2: replay."""
+        self._tape = tape
+        lib().orc_solver_set_tape(self._h, tape._h if tape else None, int(mode))
+
     def particle_mass(self) -> float:
         return lib().orc_solver_particle_mass(self._h)
+
+
+class Tape:
+    """Frozen structure (per grid build: home cells; per solve: iteration count)."""
+
+    def __init__(self):
+        self._h = lib().orc_tape_create()
+
+    def builds(self) -> int:
+        return lib().orc_tape_builds(self._h)
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().orc_tape_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass

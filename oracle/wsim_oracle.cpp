@@ -29,6 +29,7 @@
 #include <cstring>
 #include <limits>
 #include <vector>
+#include <type_traits>
 #include <omp.h>
 
 namespace {
@@ -36,6 +37,78 @@ namespace {
 using u32 = uint32_t;
 
 template <class R> struct V3 { R x, y, z; };
+
+// Forward-mode tangent number for the gradient checks.  The value part is
+// computed in binary32 by exactly the operations, in exactly the order, of
+// the float instance, so a DualF solver follows the binary32 trajectory bit
+// for bit; the tangent part carries d(value)/d(seed) in binary64 by the chain
+// rule at those binary32 values.  Branches (stencil membership, clamps,
+// max(0, .), the iteration count) follow the value, which is the frozen
+// structure the adjoint differentiates.
+struct DualF {
+    float v = 0.0f;
+    double d = 0.0;
+    DualF() = default;
+    DualF(float a) : v(a) {}
+    DualF(double a) : v(static_cast<float>(a)) {}
+    DualF(int a) : v(static_cast<float>(a)) {}
+    DualF(float a, double t) : v(a), d(t) {}
+    explicit operator int() const { return static_cast<int>(v); }
+    explicit operator float() const { return v; }
+    explicit operator double() const { return v; }
+};
+inline DualF operator+(DualF a, DualF b) { return {a.v + b.v, a.d + b.d}; }
+inline DualF operator-(DualF a, DualF b) { return {a.v - b.v, a.d - b.d}; }
+inline DualF operator-(DualF a) { return {-a.v, -a.d}; }
+inline DualF operator*(DualF a, DualF b) {
+    return {a.v * b.v, a.d * double(b.v) + double(a.v) * b.d};
+}
+inline DualF operator/(DualF a, DualF b) {
+    const double bv = b.v;
+    return {a.v / b.v, (a.d * bv - double(a.v) * b.d) / (bv * bv)};
+}
+inline DualF& operator+=(DualF& a, DualF b) { return a = a + b; }
+inline DualF& operator-=(DualF& a, DualF b) { return a = a - b; }
+inline DualF& operator*=(DualF& a, DualF b) { return a = a * b; }
+inline bool operator<(DualF a, DualF b) { return a.v < b.v; }
+inline bool operator<=(DualF a, DualF b) { return a.v <= b.v; }
+inline bool operator>(DualF a, DualF b) { return a.v > b.v; }
+inline bool operator>=(DualF a, DualF b) { return a.v >= b.v; }
+inline bool operator==(DualF a, DualF b) { return a.v == b.v; }
+inline bool operator!=(DualF a, DualF b) { return a.v != b.v; }
+
+// Math used by the solver templates, for float, double and DualF.
+inline float o_sqrt(float a) { return std::sqrt(a); }
+inline double o_sqrt(double a) { return std::sqrt(a); }
+inline DualF o_sqrt(DualF a) {
+    return {std::sqrt(a.v), a.v > 0.0f ? a.d / (2.0 * std::sqrt(double(a.v))) : 0.0};
+}
+inline float o_fmax(float a, float b) { return std::fmax(a, b); }
+inline double o_fmax(double a, double b) { return std::fmax(a, b); }
+inline DualF o_fmax(DualF a, DualF b) {
+    // the operand std::fmax returns (NaN loses; ties keep the first)
+    return (a.v >= b.v || std::isnan(b.v)) ? a : b;
+}
+inline float o_floor(float a) { return std::floor(a); }
+inline double o_floor(double a) { return std::floor(a); }
+inline DualF o_floor(DualF a) { return {std::floor(a.v), 0.0}; }
+inline bool o_isfinite(float a) { return std::isfinite(a); }
+inline bool o_isfinite(double a) { return std::isfinite(a); }
+inline bool o_isfinite(DualF a) { return std::isfinite(a.v); }
+
+// Instances whose values are binary32 (float itself and DualF).
+template <class R> struct F32Values { static constexpr bool value = std::is_same<R, float>::value; };
+template <> struct F32Values<DualF> { static constexpr bool value = true; };
+
+// Frozen structure for the finite-difference checks: every grid build's
+// per-particle home cells and every solve's iteration count, in call order.
+// Recorded by one solver and replayed by others (any precision), so the
+// perturbed runs keep the recording run's neighbour sets and iteration
+// counts: the map the adjoint differentiates.
+struct StructTape {
+    std::vector<std::vector<int16_t>> homes;
+    std::vector<u32> iters;
+};
 
 // Test-only switch: XSPH as an in-place sequential sweep in original index
 // order (one of the schedules the reference's racy in-place kernel can
@@ -61,7 +134,7 @@ template <class R> inline R W(R r, R l) {
 // sph_kernels.cuh:25-41  grad_i W(x_i - x_j, l)
 template <class R> inline V3<R> gradW(const V3<R>& rij, R l) {
     const R r2 = rij.x * rij.x + rij.y * rij.y + rij.z * rij.z;
-    const R r = std::sqrt(r2);
+    const R r = o_sqrt(r2);
     if (r < k1em12<R>()) return {R(0), R(0), R(0)};
     const R q = r / l;
     if (q >= R(2)) return {R(0), R(0), R(0)};
@@ -85,7 +158,7 @@ template <class R> inline V3<R> gradW(const V3<R>& rij, R l) {
 // made total: NaN and coordinates beyond the stencil's reach are clamped to
 // values whose whole stencil lies outside the grid.
 template <class R> inline int cell_coord(R p, R o, R inv, int D) {
-    const R f = std::floor((p - o) * inv);
+    const R f = o_floor((p - o) * inv);
     if (!(f >= R(-2))) return -2;
     if (f > R(D + 1)) return D + 1;
     return static_cast<int>(f);
@@ -107,7 +180,7 @@ template <class R> struct Grid {
         ox = R(origin[0]); oy = R(origin[1]); oz = R(origin[2]);
         // cell_grid.cu:81: inverse cell length computed once on the host
         inv = R(1) / R(cell);
-        if (sizeof(R) == sizeof(float)) inv = R(1.0f / cell);
+        if constexpr (F32Values<R>::value) inv = R(1.0f / cell);
         std::memcpy(D, dims, sizeof(D));
         C = u32(D[0]) * u32(D[1]) * u32(D[2]);
         cs.assign(size_t(C) + 1, 0);
@@ -118,12 +191,16 @@ template <class R> struct Grid {
                         cell_coord(p.z, oz, inv, D[2]), D);
     }
 
-    void build(const std::vector<V3<R>>& x, size_t n) {
+    // home: optional taped home cells (3 per particle) used instead of the
+    // positions' cells (frozen-structure replays).
+    void build(const std::vector<V3<R>>& x, size_t n, const int16_t* home = nullptr) {
         keys.resize(n);
         sorted_keys.resize(n);
         perm.resize(n);
 #pragma omp parallel for schedule(static)
-        for (long long i = 0; i < (long long)n; ++i) keys[i] = key_of(x[i]);
+        for (long long i = 0; i < (long long)n; ++i)
+            keys[i] = home ? cell_key(home[3 * i], home[3 * i + 1], home[3 * i + 2], D)
+                           : key_of(x[i]);
         // Stable counting sort on (key, id): identical order to a stable LSD
         // radix sort of (key, iota) pairs (cell_grid.cu:98-110).
         std::vector<u32> start(size_t(C) + 2, 0);
@@ -139,11 +216,18 @@ template <class R> struct Grid {
         }
     }
 
-    // ITER_NEIGHBORS (dfsph_kernels.cu:13-35): dz, dy, dx in {-1,0,1}.
-    template <class F> void for_neighbors(const V3<R>& p, F&& f) const {
-        const int cx = cell_coord(p.x, ox, inv, D[0]);
-        const int cy = cell_coord(p.y, oy, inv, D[1]);
-        const int cz = cell_coord(p.z, oz, inv, D[2]);
+    void home_of(const V3<R>& p, int16_t* out) const {
+        out[0] = int16_t(cell_coord(p.x, ox, inv, D[0]));
+        out[1] = int16_t(cell_coord(p.y, oy, inv, D[1]));
+        out[2] = int16_t(cell_coord(p.z, oz, inv, D[2]));
+    }
+
+    // ITER_NEIGHBORS (dfsph_kernels.cu:13-35): dz, dy, dx in {-1,0,1}.  The
+    // query's home cell comes from its position or, in a replay, from `hc`.
+    template <class F> void for_neighbors(const V3<R>& p, F&& f, const int16_t* hc = nullptr) const {
+        const int cx = hc ? hc[0] : cell_coord(p.x, ox, inv, D[0]);
+        const int cy = hc ? hc[1] : cell_coord(p.y, oy, inv, D[1]);
+        const int cz = hc ? hc[2] : cell_coord(p.z, oz, inv, D[2]);
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy)
                 for (int dx = -1; dx <= 1; ++dx) {
@@ -179,7 +263,16 @@ template <class R> struct Solver {
     std::vector<R> rho, alpha, kappa, rho_adv, kappa_v, mb;
     Grid<R> fg, bg;
     R mass = 0, h = 0;
+    R eps_param = 0;  // XSPH viscosity (a tangent seed for DualF)
     orc_stats st{};
+    // frozen-structure tape: 0 off, 1 record, 2 replay
+    StructTape* tape = nullptr;
+    int tape_mode = 0;
+    size_t tape_home = 0, tape_iter = 0;
+    const int16_t* home = nullptr;  // home cells of the current grid build
+    std::vector<int16_t> home_buf;
+
+    const int16_t* hc(long long i) const { return home ? home + 3 * i : nullptr; }
 
     Solver(const orc_config& c, const float* px, const float* pv, size_t n_, const float* pxb,
            size_t nb_)
@@ -194,8 +287,9 @@ template <class R> struct Solver {
         rho.assign(n, 0); alpha.assign(n, 0); kappa.assign(n, 0);
         rho_adv.assign(n, 0); kappa_v.assign(n, 0);
         h = R(cfg.smoothing_length);
+        eps_param = R(cfg.viscosity);
         // dfsph.cu:76-78
-        if (sizeof(R) == sizeof(float)) {
+        if constexpr (F32Values<R>::value) {
             const float s = 2.0f * cfg.particle_radius;
             const float v0 = s * s * s;
             mass = R(cfg.rest_density * v0);
@@ -230,10 +324,10 @@ template <class R> struct Solver {
             bg.for_neighbors(pi, [&](u32 j) {
                 const V3<R> pj = xb[j];
                 const V3<R> r{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
-                const R d = std::sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+                const R d = o_sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
                 sum_w += W(d, h);
             });
-            mb[i] = rho0 / std::fmax(sum_w, k1em9<R>());
+            mb[i] = rho0 / o_fmax(sum_w, k1em9<R>());
         }
     }
 
@@ -246,16 +340,16 @@ template <class R> struct Solver {
             fg.for_neighbors(pi, [&](u32 j) {
                 const V3<R> pj = x[j];
                 const V3<R> r{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
-                const R d = std::sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+                const R d = o_sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
                 r_sum += mass * W(d, h);
-            });
+            }, hc(i));
             if (nb > 0) {
                 bg.for_neighbors(pi, [&](u32 j) {
                     const V3<R> pj = xb[j];
                     const V3<R> r{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
-                    const R d = std::sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+                    const R d = o_sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
                     r_sum += mb[j] * W(d, h);
-                });
+                }, hc(i));
             }
             rho[i] = r_sum;
         }
@@ -275,7 +369,7 @@ template <class R> struct Solver {
                 const V3<R> mg{mass * g.x, mass * g.y, mass * g.z};
                 sg.x += mg.x; sg.y += mg.y; sg.z += mg.z;
                 sq += mg.x * mg.x + mg.y * mg.y + mg.z * mg.z;
-            });
+            }, hc(i));
             if (nb > 0) {
                 bg.for_neighbors(pi, [&](u32 j) {
                     const V3<R> pj = xb[j];
@@ -283,7 +377,7 @@ template <class R> struct Solver {
                     const V3<R> mg{mb[j] * g.x, mb[j] * g.y, mb[j] * g.z};
                     sg.x += mg.x; sg.y += mg.y; sg.z += mg.z;
                     sq += mg.x * mg.x + mg.y * mg.y + mg.z * mg.z;
-                });
+                }, hc(i));
             }
             const R denom = sg.x * sg.x + sg.y * sg.y + sg.z * sg.z + sq;
             alpha[i] = (denom > k1em9<R>()) ? (rho[i] / denom) : R(0);
@@ -293,7 +387,7 @@ template <class R> struct Solver {
     // dfsph_kernels.cu:347-355 with exp(-damping*dt) evaluated on the host
     void gravity(float dt) {
         R decay;
-        if (sizeof(R) == sizeof(float)) decay = R(std::exp(-cfg.damping * dt));
+        if constexpr (F32Values<R>::value) decay = R(std::exp(-cfg.damping * dt));
         else decay = std::exp(-R(cfg.damping) * R(dt));
         const R gx = R(cfg.gravity[0]), gy = R(cfg.gravity[1]), gz = R(cfg.gravity[2]);
         const R dtr = R(dt);
@@ -308,7 +402,7 @@ template <class R> struct Solver {
     // dfsph_kernels.cu:305-341, as a Jacobi sweep
     void xsph() {
         if (cfg.viscosity <= 0.0f) return;  // dfsph.cu:198
-        const R eps = R(cfg.viscosity);
+        const R eps = eps_param;
         if (g_xsph_in_place) vtmp = v;
 #pragma omp parallel for schedule(static) if (!g_xsph_in_place)
         for (long long i = 0; i < (long long)n; ++i) {
@@ -321,14 +415,14 @@ template <class R> struct Solver {
                 const V3<R> pj = x[j];
                 const V3<R> vj = g_xsph_in_place ? vtmp[j] : v[j];
                 const V3<R> r{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
-                const R d = std::sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+                const R d = o_sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
                 const R w = W(d, h);
                 const R rho_avg = R(0.5) * (rho_i + rho[j]);
-                const R c = eps * (mass / std::fmax(rho_avg, k1em9<R>())) * w;
+                const R c = eps * (mass / o_fmax(rho_avg, k1em9<R>())) * w;
                 dl.x += c * (vj.x - vi.x);
                 dl.y += c * (vj.y - vi.y);
                 dl.z += c * (vj.z - vi.z);
-            });
+            }, hc(i));
             vtmp[i] = {vi.x + dl.x, vi.y + dl.y, vi.z + dl.z};
         }
         v.swap(vtmp);
@@ -346,13 +440,13 @@ template <class R> struct Solver {
             const V3<R> g = gradW(V3<R>{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z}, h);
             const V3<R> dv{vi.x - vj.x, vi.y - vj.y, vi.z - vj.z};
             rd += mass * (dv.x * g.x + dv.y * g.y + dv.z * g.z);
-        });
+        }, hc(i));
         if (nb > 0) {
             bg.for_neighbors(pi, [&](u32 j) {
                 const V3<R> pj = xb[j];
                 const V3<R> g = gradW(V3<R>{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z}, h);
                 rd += mb[j] * (vi.x * g.x + vi.y * g.y + vi.z * g.z);
-            });
+            }, hc(i));
         }
         return rd;
     }
@@ -365,7 +459,7 @@ template <class R> struct Solver {
             const R rd = rho_dot(i);
             const R rho_pred = rho[i] + dtr * rd;
             rho_adv[i] = rho_pred;
-            const R over = std::fmax(R(0), rho_pred - rho0);
+            const R over = o_fmax(R(0), rho_pred - rho0);
             kappa[i] = over * alpha[i] / (dtr * dtr);
         }
     }
@@ -378,7 +472,7 @@ template <class R> struct Solver {
         for (long long i = 0; i < (long long)n; ++i) {
             const R rd = rho_dot(i);
             rho_adv[i] = rd;
-            const R s = std::fmax(R(0), rd);
+            const R s = o_fmax(R(0), rd);
             kappa_v[i] = s * alpha[i] / dtr;
         }
     }
@@ -392,8 +486,8 @@ template <class R> struct Solver {
             const V3<R> pi = x[i];
             const R rho_i = rho[i];
             const R k_i = kap[i];
-            const R ki = bk ? k_i / std::fmax(rho_i, k1em9<R>())
-                            : k_i / std::fmax(rho_i * rho_i, k1em9<R>());
+            const R ki = bk ? k_i / o_fmax(rho_i, k1em9<R>())
+                            : k_i / o_fmax(rho_i * rho_i, k1em9<R>());
             V3<R> dl{0, 0, 0};
             fg.for_neighbors(pi, [&](u32 j) {
                 if (j == u32(i)) return;
@@ -401,16 +495,16 @@ template <class R> struct Solver {
                 const R rho_j = rho[j];
                 const R k_j = kap[j];
                 const V3<R> g = gradW(V3<R>{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z}, h);
-                const R kj = bk ? k_j / std::fmax(rho_j, k1em9<R>())
-                                : k_j / std::fmax(rho_j * rho_j, k1em9<R>());
+                const R kj = bk ? k_j / o_fmax(rho_j, k1em9<R>())
+                                : k_j / o_fmax(rho_j * rho_j, k1em9<R>());
                 const R c2 = mass * (ki + kj);
                 dl.x -= dtr * c2 * g.x;
                 dl.y -= dtr * c2 * g.y;
                 dl.z -= dtr * c2 * g.z;
-            });
+            }, hc(i));
             if (nb > 0) {
-                const R bf = bk ? k_i / std::fmax(rho0, k1em9<R>())
-                                : k_i / std::fmax(rho0 * rho0, k1em9<R>());
+                const R bf = bk ? k_i / o_fmax(rho0, k1em9<R>())
+                                : k_i / o_fmax(rho0 * rho0, k1em9<R>());
                 bg.for_neighbors(pi, [&](u32 j) {
                     const V3<R> pj = xb[j];
                     const V3<R> g = gradW(V3<R>{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z}, h);
@@ -418,7 +512,7 @@ template <class R> struct Solver {
                     dl.x -= dtr * c2 * g.x;
                     dl.y -= dtr * c2 * g.y;
                     dl.z -= dtr * c2 * g.z;
-                });
+                }, hc(i));
             }
             v[i].x += dl.x;
             v[i].y += dl.y;
@@ -429,36 +523,52 @@ template <class R> struct Solver {
     // mean(max(0, a_i - ref)), accumulated in binary64 (dfsph.cu:277-301)
     double mean_excess(const std::vector<R>& a, R ref) const {
         double s = 0;
-        for (size_t i = 0; i < n; ++i) s += double(std::fmax(R(0), a[i] - ref));
+        for (size_t i = 0; i < n; ++i) s += double(o_fmax(R(0), a[i] - ref));
         return s / double(n);
     }
 
     // dfsph.cu:215-260
+    // Iteration count of a solve in a replay (kNone: decide by the error test).
+    static constexpr u32 kFree = 0xFFFFFFFFu;
+    u32 replay_iters() {
+        if (tape_mode != 2) return kFree;
+        return tape_iter < tape->iters.size() ? tape->iters[tape_iter++] : 0u;
+    }
+    void record_iters(u32 k) {
+        if (tape_mode == 1) tape->iters.push_back(k);
+    }
+
     void density_solve(float dt) {
         const double thr = double(cfg.eta_density * cfg.rest_density);
+        const u32 fixed = replay_iters();
         st.density_iters = 0;
         for (u32 it = 0; it < cfg.max_density_iters; ++it) {
+            if (fixed != kFree && it >= fixed) break;
             density_change(dt);
             apply_kappa(kappa, dt);
             st.density_iters = it + 1;
             const double err = mean_excess(rho_adv, R(cfg.rest_density));
             st.mean_density_error = err;
-            if (it >= 1 && err < thr) break;
+            if (fixed == kFree && it >= 1 && err < thr) break;
         }
+        record_iters(st.density_iters);
     }
 
     // divergence_solve, plan text :1967-1990
     void divergence_solve(float dt) {
         const double thr = double(cfg.eta_divergence * cfg.rest_density);
+        const u32 fixed = replay_iters();
         st.divergence_iters = 0;
         for (u32 it = 0; it < cfg.max_divergence_iters; ++it) {
+            if (fixed != kFree && it >= fixed) break;
             divergence_change(dt);
             apply_kappa(kappa_v, dt);
             st.divergence_iters = it + 1;
             const double err = mean_excess(rho_adv, R(0));
             st.mean_divergence_error = err;
-            if (err < thr) break;
+            if (fixed == kFree && err < thr) break;
         }
+        record_iters(st.divergence_iters);
     }
 
     // dfsph_kernels.cu:363-386 (dfsph.cu:262-273: |v| clamp 5 m/s)
@@ -471,7 +581,7 @@ template <class R> struct Solver {
 #pragma omp parallel for schedule(static)
         for (long long i = 0; i < (long long)n; ++i) {
             V3<R> vv = v[i];
-            const R sp = std::sqrt(vv.x * vv.x + vv.y * vv.y + vv.z * vv.z);
+            const R sp = o_sqrt(vv.x * vv.x + vv.y * vv.y + vv.z * vv.z);
             if (sp > vmax) {
                 const R s = vmax / sp;
                 vv.x *= s; vv.y *= s; vv.z *= s;
@@ -494,10 +604,10 @@ template <class R> struct Solver {
         u32 nan_count = 0;
         for (size_t i = 0; i < n; ++i) {
             const V3<R>& vv = v[i];
-            const R s = std::sqrt(vv.x * vv.x + vv.y * vv.y + vv.z * vv.z);
-            m = std::fmax(m, s);
-            if (!std::isfinite(x[i].x) || !std::isfinite(x[i].y) || !std::isfinite(x[i].z) ||
-                !std::isfinite(vv.x) || !std::isfinite(vv.y) || !std::isfinite(vv.z))
+            const R s = o_sqrt(vv.x * vv.x + vv.y * vv.y + vv.z * vv.z);
+            m = o_fmax(m, s);
+            if (!o_isfinite(x[i].x) || !o_isfinite(x[i].y) || !o_isfinite(x[i].z) ||
+                !o_isfinite(vv.x) || !o_isfinite(vv.y) || !o_isfinite(vv.z))
                 ++nan_count;
         }
         st.max_velocity = float(m);
@@ -505,7 +615,16 @@ template <class R> struct Solver {
     }
 
     void density_alpha() {
-        fg.build(x, n);
+        home = nullptr;
+        if (tape_mode == 1) {
+            home_buf.resize(3 * n);
+            for (size_t i = 0; i < n; ++i) fg.home_of(x[i], home_buf.data() + 3 * i);
+            tape->homes.push_back(home_buf);
+            home = home_buf.data();
+        } else if (tape_mode == 2 && tape_home < tape->homes.size()) {
+            home = tape->homes[tape_home++].data();
+        }
+        fg.build(x, n, home);
         density();
         alpha_factor();
     }
@@ -551,7 +670,7 @@ template <class R> void get_field(Solver<R>* s, int field, double* out) {
 }
 
 struct Handle {
-    int precision;
+    int precision;  // ORC_F32, ORC_F64 or ORC_DUAL
     void* impl;
 };
 
@@ -739,6 +858,7 @@ void* orc_solver_create(const orc_config* cfg, int precision, const float* x, co
                         size_t n, const float* xb, size_t nb) {
     Handle* h = new Handle{precision, nullptr};
     if (precision == ORC_F64) h->impl = new Solver<double>(*cfg, x, v, n, xb, nb);
+    else if (precision == ORC_DUAL) h->impl = new Solver<DualF>(*cfg, x, v, n, xb, nb);
     else h->impl = new Solver<float>(*cfg, x, v, n, xb, nb);
     return h;
 }
@@ -747,6 +867,7 @@ void orc_solver_destroy(void* s) {
     Handle* h = static_cast<Handle*>(s);
     if (!h) return;
     if (h->precision == ORC_F64) delete static_cast<Solver<double>*>(h->impl);
+    else if (h->precision == ORC_DUAL) delete static_cast<Solver<DualF>*>(h->impl);
     else delete static_cast<Solver<float>*>(h->impl);
     delete h;
 }
@@ -757,11 +878,63 @@ void orc_solver_destroy(void* s) {
         if (h_->precision == ORC_F64) {                               \
             auto* S = static_cast<Solver<double>*>(h_->impl);         \
             call;                                                     \
+        } else if (h_->precision == ORC_DUAL) {                       \
+            auto* S = static_cast<Solver<DualF>*>(h_->impl);          \
+            call;                                                     \
         } else {                                                      \
             auto* S = static_cast<Solver<float>*>(h_->impl);          \
             call;                                                     \
         }                                                             \
     } while (0)
+
+void orc_solver_seed(void* s, const double* dx, const double* dv, double d_viscosity) {
+    Handle* h = static_cast<Handle*>(s);
+    if (h->precision != ORC_DUAL) return;
+    auto* S = static_cast<Solver<DualF>*>(h->impl);
+    for (size_t i = 0; i < S->n; ++i) {
+        if (dx) { S->x[i].x.d = dx[3 * i]; S->x[i].y.d = dx[3 * i + 1]; S->x[i].z.d = dx[3 * i + 2]; }
+        if (dv) { S->v[i].x.d = dv[3 * i]; S->v[i].y.d = dv[3 * i + 1]; S->v[i].z.d = dv[3 * i + 2]; }
+    }
+    S->eps_param.d = d_viscosity;
+}
+
+void orc_solver_get_tangent(void* s, int field, double* out) {
+    Handle* h = static_cast<Handle*>(s);
+    if (h->precision != ORC_DUAL) return;
+    auto* S = static_cast<Solver<DualF>*>(h->impl);
+    auto put3 = [&](const std::vector<V3<DualF>>& a) {
+        for (size_t i = 0; i < a.size(); ++i) {
+            out[3 * i] = a[i].x.d;
+            out[3 * i + 1] = a[i].y.d;
+            out[3 * i + 2] = a[i].z.d;
+        }
+    };
+    auto put1 = [&](const std::vector<DualF>& a) {
+        for (size_t i = 0; i < a.size(); ++i) out[i] = a[i].d;
+    };
+    switch (field) {
+        case ORC_FIELD_X: put3(S->x); break;
+        case ORC_FIELD_V: put3(S->v); break;
+        case ORC_FIELD_RHO: put1(S->rho); break;
+        case ORC_FIELD_ALPHA: put1(S->alpha); break;
+        case ORC_FIELD_KAPPA: put1(S->kappa); break;
+        case ORC_FIELD_RHO_ADV: put1(S->rho_adv); break;
+        default: break;
+    }
+}
+
+void* orc_tape_create(void) { return new StructTape(); }
+void orc_tape_destroy(void* t) { delete static_cast<StructTape*>(t); }
+size_t orc_tape_builds(void* t) { return static_cast<StructTape*>(t)->homes.size(); }
+
+void orc_solver_set_tape(void* s, void* tape, int mode) {
+    ORC_DISPATCH(s, {
+        S->tape = static_cast<StructTape*>(tape);
+        S->tape_mode = tape ? mode : 0;
+        S->tape_home = 0;
+        S->tape_iter = 0;
+    });
+}
 
 void orc_solver_step(void* s, float dt) { ORC_DISPATCH(s, S->step(dt)); }
 void orc_solver_density_alpha(void* s) { ORC_DISPATCH(s, S->density_alpha()); }
@@ -773,6 +946,32 @@ void orc_solver_set_state(void* s, const float* x, const float* v) {
             if (x) S->x[i] = load3<RR>(x, i);
             if (v) S->v[i] = load3<RR>(v, i);
         }
+    });
+}
+
+void orc_solver_set_state_f64(void* s, const double* x, const double* v) {
+    ORC_DISPATCH(s, {
+        using RR = std::remove_reference_t<decltype(S->mass)>;
+        for (size_t i = 0; i < S->n; ++i) {
+            if (x) {
+                S->x[i].x = RR(x[3 * i]);
+                S->x[i].y = RR(x[3 * i + 1]);
+                S->x[i].z = RR(x[3 * i + 2]);
+            }
+            if (v) {
+                S->v[i].x = RR(v[3 * i]);
+                S->v[i].y = RR(v[3 * i + 1]);
+                S->v[i].z = RR(v[3 * i + 2]);
+            }
+        }
+    });
+}
+
+void orc_solver_set_viscosity_f64(void* s, double viscosity) {
+    ORC_DISPATCH(s, {
+        using RR = std::remove_reference_t<decltype(S->mass)>;
+        S->cfg.viscosity = float(viscosity);
+        S->eps_param = RR(viscosity);
     });
 }
 

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-enum { ORC_F32 = 32, ORC_F64 = 64 };
+enum { ORC_F32 = 32, ORC_F64 = 64, ORC_DUAL = 33 };
 
 typedef struct {
     float    rest_density;
@@ -128,9 +128,29 @@ void  orc_solver_step(void* s, float dt);
 /* Grid + density + alpha only, on the current positions (dfsph.cu:124-126). */
 void  orc_solver_density_alpha(void* s);
 void  orc_solver_set_state(void* s, const float* x, const float* v);
+/* Same in binary64 (rounded to binary32 for F32 and DUAL solvers). */
+void  orc_solver_set_state_f64(void* s, const double* x, const double* v);
+/* XSPH viscosity in binary64 (finite differences in the viscosity). */
+void  orc_solver_set_viscosity_f64(void* s, double viscosity);
 void  orc_solver_get(void* s, int field, double* out);
 void  orc_solver_get_f32(void* s, int field, float* out); /* F32 solvers only */
 void  orc_solver_stats(void* s, orc_stats* out);
+
+/* --- gradient checks ---
+ * ORC_DUAL solvers carry forward-mode tangents: values bit-identical to the
+ * ORC_F32 instance, tangents d(value)/d(seed) in binary64.  Seeds: initial
+ * positions, initial velocities (3n each, may be NULL) and the XSPH
+ * viscosity. */
+void  orc_solver_seed(void* s, const double* dx, const double* dv, double d_viscosity);
+void  orc_solver_get_tangent(void* s, int field, double* out); /* X, V, RHO, ALPHA, KAPPA, RHO_ADV */
+/* Frozen structure: mode 1 records every grid build's home cells and every
+ * solve's iteration count; mode 2 replays a recorded tape (from its start),
+ * so perturbed runs of any precision keep the recording run's neighbour sets
+ * and iteration counts. */
+void* orc_tape_create(void);
+void  orc_tape_destroy(void* tape);
+size_t orc_tape_builds(void* tape);
+void  orc_solver_set_tape(void* s, void* tape, int mode);
 float orc_solver_particle_mass(void* s);
 size_t orc_solver_count(void* s);
 

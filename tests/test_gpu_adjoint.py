@@ -1,0 +1,190 @@
+"""The HIP adjoint against the host reference gradient (forward-mode DUAL
+oracle along the same binary32 trajectory) and against binary64 central
+finite differences with the neighbour structure frozen, for single substeps
+(every branch: no iterations, XSPH off, both pressure forms, |v| and domain
+clamps) and for checkpointed rollouts, plus the torch.autograd.Function with
+the cell-density loss and a viscosity recovery.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import water_simulation_amd as ws
+from water_simulation_amd import adjoint as A
+from helpers import random_box
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def make_solver(x, v, xb, cfg):
+    n = len(x)
+    f = ws.ParticleStore(n)
+    f.resize(n)
+    f.set_positions(x)
+    f.set_velocities(v)
+    b = None
+    if xb is not None:
+        b = ws.ParticleStore(len(xb))
+        b.resize(len(xb))
+        b.set_positions(xb)
+    s = ws.DFSPHSolver(f, b, ws.DFSPHConfig(**cfg))
+    s._keep = (f, b)
+    return s, f
+
+
+def dual_directional(x, v, xb, cfg, dts, gx, gv, ux, uv, ue):
+    d = O.Solver(O.config(**cfg), x, v, xb, O.DUAL)
+    d.seed(dx=ux, dv=uv, d_viscosity=ue)
+    for dt in dts:
+        d.step(float(dt))
+    return float((gx * d.tangent(O.FIELD_X)).sum() + (gv * d.tangent(O.FIELD_V)).sum()), d
+
+
+def fd_directional(x, v, xb, cfg, dts, gx, gv, ux, uv, ue, eps=1e-6):
+    rec = O.Solver(O.config(**cfg), x, v, xb, O.F32)
+    tape = O.Tape()
+    rec.set_tape(tape, 1)
+    for dt in dts:
+        rec.step(float(dt))
+    val = []
+    for sgn in (1.0, -1.0):
+        s = O.Solver(O.config(**cfg), x, v, xb, O.F64)
+        s.set_state_f64(x.astype(np.float64) + sgn * eps * ux, v.astype(np.float64) + sgn * eps * uv)
+        s.set_viscosity_f64(cfg["viscosity"] + sgn * eps * ue)
+        s.set_tape(tape, 2)
+        for dt in dts:
+            s.step(float(dt))
+        val.append(float((gx * s.get(O.FIELD_X)).sum() + (gv * s.get(O.FIELD_V)).sum()))
+    return (val[0] - val[1]) / (2 * eps)
+
+
+CASES = {
+    "no-iterations": dict(max_density_iters=0, viscosity=0.01),
+    "xsph-off": dict(max_density_iters=5, viscosity=0.0),
+    "reference-form": dict(max_density_iters=12, viscosity=0.01),
+    "kappa-over-rho": dict(max_density_iters=12, viscosity=0.02, pressure_form=1),
+    "no-boundary": dict(max_density_iters=6, viscosity=0.01),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_adjoint_substep_matches_dual(case):
+    over = dict(CASES[case])
+    x, v, xb, cfg = random_box(n=343, seed=11, boundary=(case != "no-boundary"), **over)
+    dt = 0.002
+    rng = np.random.default_rng(3)
+    gx = rng.normal(size=x.shape)
+    gv = rng.normal(size=x.shape)
+    solver, fluid = make_solver(x, v, xb, cfg)
+    gxi, gvi, ge = A.adjoint_step(solver, dt, gx, gv)
+    assert np.array_equal(fluid.get_positions(), x)  # store back at the input state
+    for k in range(3):
+        ux = 1e-2 * rng.normal(size=x.shape)
+        uv = rng.normal(size=x.shape)
+        ue = 0.3 if cfg["viscosity"] > 0 else 0.0
+        want, _ = dual_directional(x, v, xb, cfg, [dt], gx, gv, ux, uv, ue)
+        got = float((gxi * ux).sum() + (gvi * uv).sum() + ge * ue)
+        assert abs(got - want) <= 1e-6 * abs(want), (case, k, got, want)
+
+
+def test_adjoint_clamps_match_dual():
+    # fast particles (|v| clamp at 5 m/s) next to the walls (domain clamp)
+    x, v, xb, cfg = random_box(n=343, seed=4, max_density_iters=4)
+    rng = np.random.default_rng(9)
+    v = (v + rng.normal(0, 4.0, v.shape)).astype(np.float32)
+    x = x.copy()
+    # the lowest lattice plane (49 particles, distinct y and z) moved next to
+    # x = domain_min + 1e-3 and sent outward
+    x[:49, 0] = np.float32(0.0015)
+    v[:49, 0] = np.float32(-3.0)
+    dt = 0.002
+    speed = np.sqrt((v.astype(np.float64) ** 2).sum(1))
+    assert (speed > 5.0).sum() > 10
+    gx = rng.normal(size=x.shape)
+    gv = rng.normal(size=x.shape)
+    solver, _ = make_solver(x, v, xb, cfg)
+    gxi, gvi, ge = A.adjoint_step(solver, dt, gx, gv)
+    for k in range(3):
+        ux = 1e-3 * rng.normal(size=x.shape)
+        uv = rng.normal(size=x.shape)
+        want, _ = dual_directional(x, v, xb, cfg, [dt], gx, gv, ux, uv, 0.2)
+        got = float((gxi * ux).sum() + (gvi * uv).sum() + ge * 0.2)
+        assert abs(got - want) <= 1e-6 * abs(want), (k, got, want)
+
+
+def test_rollout_matches_dual_and_fd():
+    x, v, xb, cfg = random_box(n=343, seed=21, max_density_iters=8, viscosity=0.02)
+    dts = np.full(25, 0.002, np.float32)
+    rng = np.random.default_rng(5)
+    gx = rng.normal(size=x.shape)
+    gv = rng.normal(size=x.shape)
+    solver, fluid = make_solver(x, v, xb, cfg)
+    ro = A.Rollout(solver, checkpoint_every=7)  # 4 segments, the last partial
+    ro.forward(x, v, dts)
+    ref = O.Solver(O.config(**cfg), x, v, xb, O.F32)
+    for dt in dts:
+        ref.step(float(dt))
+    assert np.array_equal(fluid.get_positions(), ref.get(O.FIELD_X))
+    gx0, gv0, ge = ro.backward(gx, gv)
+    for k in range(3):
+        ux = 1e-2 * rng.normal(size=x.shape)
+        uv = rng.normal(size=x.shape)
+        want, _ = dual_directional(x, v, xb, cfg, dts, gx, gv, ux, uv, 0.3)
+        got = float((gx0 * ux).sum() + (gv0 * uv).sum() + ge * 0.3)
+        assert abs(got - want) <= 1e-6 * abs(want), (k, got, want)
+    ux = 1e-2 * rng.normal(size=x.shape)
+    uv = rng.normal(size=x.shape)
+    fd = fd_directional(x, v, xb, cfg, dts, gx, gv, ux, uv, 0.3)
+    got = float((gx0 * ux).sum() + (gv0 * uv).sum() + ge * 0.3)
+    assert abs(got - fd) <= 1e-4 * abs(fd), (got, fd)
+    print(f"\nrollout: adjoint {got:.9e}  FD(binary64, frozen) {fd:.9e}  rel {abs(got - fd) / abs(fd):.2e}")
+
+
+def small_inverse(side=8, substeps=40):
+    p = A.inverse_problem(side=side, substeps=substeps)
+    return p
+
+
+def test_torch_rollout_density_loss_matches_dual():
+    # torch must own the process's HIP runtime, so this runs in a fresh
+    # interpreter that imports torch first (see tests/torch_interop.py)
+    import json
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    script = Path(__file__).with_name("torch_interop.py")
+    r = subprocess.run([sys.executable, "-u", str(script)], capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    print("\ntorch interop:", res)
+    assert res["same_trajectory"]
+    assert res["loss_rel_err"] <= 1e-9
+    for got, want in res["directional"]:
+        assert abs(got - want) <= 1e-5 * abs(want), (got, want)
+
+
+def test_viscosity_recovery():
+    p = small_inverse(side=8, substeps=30)
+    cfg = dict(p.cfg, max_density_iters=10, viscosity=0.05)
+    solver, fluid = make_solver(p.x0, p.v_true, p.xb, cfg)
+    mass = solver.particle_mass()
+    ro = A.Rollout(solver, checkpoint_every=10)
+    ro.forward(p.x0, p.v_true, p.dts)
+    target = A.density_field_np(fluid.get_positions(), mass)
+    eps, losses = 0.01, []
+    for it in range(12):
+        solver.set_viscosity(eps)
+        ro.forward(p.x0, p.v_true, p.dts)
+        loss, gxl = A.density_loss_np(fluid.get_positions(), target, mass)
+        _, _, ge = ro.backward(gxl, np.zeros_like(gxl))
+        losses.append(loss)
+        # half a Newton step of the quadratic model L = a (eps - eps*)^2, kept in (0, 0.1)
+        step = -loss / ge if ge != 0 else 0.0
+        eps = float(np.clip(eps + step, 1e-4, 0.1))
+    print(f"\nviscosity recovery: losses {losses[0]:.3e} -> {losses[-1]:.3e}, eps {eps:.5f} (true 0.05)")
+    assert losses[-1] < 1e-2 * losses[0]
+    assert abs(eps - 0.05) < 0.01

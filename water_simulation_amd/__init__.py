@@ -468,6 +468,11 @@ class DFSPHSolver:
     def set_graphs(self, on: bool) -> None:
         check(_L().wsim_solver_set_graphs(self._h, 1 if on else 0))
 
+    def set_viscosity(self, viscosity: float) -> None:
+        """XSPH viscosity of a plain solver (the differentiable parameter)."""
+        check(_L().wsim_solver_set_viscosity(self._h, float(viscosity)))
+        self.cfg.viscosity = float(viscosity)
+
 
 class BatchSolver(DFSPHSolver):
     """Many independent scenes of uneven sizes in one solver: one packed

@@ -175,6 +175,12 @@ SIGNATURES = {
     "wsim_solver_tune_count": (sz, [vp]),
     "wsim_solver_tune_entry": (st, [vp, sz, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
     "wsim_solver_set_graphs": (st, [vp, C.c_int]),
+    "wsim_solver_set_viscosity": (st, [vp, C.c_float]),
+    "wsim_adjoint_step": (st, [vp, C.c_float, vp, vp, vp, vp, C.POINTER(C.c_double)]),
+    "wsim_rollout_create": (st, [vp, C.c_uint32, C.POINTER(vp)]),
+    "wsim_rollout_destroy": (None, [vp]),
+    "wsim_rollout_forward": (st, [vp, vp, vp, C.POINTER(C.c_float), C.c_uint32]),
+    "wsim_rollout_backward": (st, [vp, vp, vp, vp, vp, C.POINTER(C.c_double)]),
     "wsim_batch_create": (st, [vp, vp, C.POINTER(DFSPHConfigC), C.c_uint32,
                                C.POINTER(C.c_uint32), C.POINTER(C.c_float), vp, C.POINTER(vp)]),
     "wsim_batch_step": (st, [vp, C.POINTER(C.c_float)]),

@@ -39,7 +39,8 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 
-HIP_SOURCES = ["radix_sort.hip", "grid.hip", "morton.hip", "store.hip", "solver.hip", "copy.hip"]
+HIP_SOURCES = ["radix_sort.hip", "grid.hip", "morton.hip", "store.hip", "solver.hip", "adjoint.hip",
+               "copy.hip"]
 HOST_SOURCES = ["host_surface.cpp", "capi.cpp"]
 
 

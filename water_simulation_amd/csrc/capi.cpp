@@ -26,6 +26,10 @@ struct wsim_solver_t {
     std::unique_ptr<wsim::DFSPHSolver> impl;
     wsim_grid_t fluid_view;
     wsim_grid_t boundary_view;
+    std::unique_ptr<wsim::SubstepTape> tape;  // wsim_adjoint_step's record
+};
+struct wsim_rollout_t {
+    std::unique_ptr<wsim::Rollout> impl;
 };
 
 namespace {
@@ -536,6 +540,50 @@ wsim_status wsim_batch_scene_stats(wsim_solver b, uint32_t scene, wsim_step_stat
         need(b, "batch");
         need(out, "out");
         fill_stats(b->impl->scene_stats(scene), out);
+    });
+}
+
+// ---- differentiation ----
+wsim_status wsim_solver_set_viscosity(wsim_solver s, float viscosity) {
+    return guarded([&] {
+        need(s, "solver");
+        s->impl->set_viscosity(viscosity);
+    });
+}
+wsim_status wsim_adjoint_step(wsim_solver s, float dt, const double* gx_out, const double* gv_out,
+                              double* gx_in, double* gv_in, double* g_viscosity) {
+    return guarded([&] {
+        need(s, "solver");
+        if (!s->tape) s->tape = std::make_unique<wsim::SubstepTape>();
+        s->impl->step_taped(dt, *s->tape);
+        const double g = s->impl->adjoint_step(*s->tape, gx_out, gv_out, gx_in, gv_in);
+        if (g_viscosity) *g_viscosity = g;
+    });
+}
+wsim_status wsim_rollout_create(wsim_solver s, uint32_t every, wsim_rollout* out) {
+    return guarded([&] {
+        need(s, "solver");
+        need(out, "out");
+        *out = nullptr;
+        auto h = std::make_unique<wsim_rollout_t>();
+        h->impl = std::make_unique<wsim::Rollout>(*s->impl, s->impl->fluid_store(), every);
+        *out = h.release();
+    });
+}
+void wsim_rollout_destroy(wsim_rollout r) { delete r; }
+wsim_status wsim_rollout_forward(wsim_rollout r, const float* x0, const float* v0, const float* dt,
+                                 uint32_t substeps) {
+    return guarded([&] {
+        need(r, "rollout");
+        r->impl->forward(x0, v0, dt, substeps);
+    });
+}
+wsim_status wsim_rollout_backward(wsim_rollout r, const double* gx, const double* gv, double* gx0,
+                                  double* gv0, double* g_viscosity) {
+    return guarded([&] {
+        need(r, "rollout");
+        const double g = r->impl->backward(gx, gv, gx0, gv0);
+        if (g_viscosity) *g_viscosity = g;
     });
 }
 

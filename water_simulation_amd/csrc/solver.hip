@@ -44,70 +44,13 @@
 #include <limits>
 
 #include "radix_sort.hpp"
+#include "stencil.hpp"
 #include "wsim_internal.hpp"
 
 namespace wsim {
 namespace {
 
-constexpr u32 kNone = 0xFFFFFFFFu;
-constexpr int NS = 256;  // workgroup size of the streaming kernels
-
-// Neighbour-search and scene inputs shared by the per-particle kernels.
-struct Nbr {
-    GridGeom gf;
-    const u32* csf;    // fluid cell_start; scene s's table starts at key_base
-    GridGeom gb;
-    const u32* csb;
-    const float4* xb;  // cell-sorted boundary (x, y, z, mass)
-    u32 nb;            // 0: no boundary
-    Spline sp;
-    float mass;
-    u32 nslots;        // working-set slots (multiple of kTile)
-    const SceneSeg* seg;
-    const u32* tile_scene;  // scene of every tile; nullptr: one scene
-    int remap;              // XCD-aware workgroup order
-};
-
-// 27-cell stencil of ITER_NEIGHBORS (dfsph_kernels.cu:13-35): dz, dy, dx in
-// {-1, 0, 1}, out-of-grid cells skipped, then every slot of the cell.  Sorted
-// positions map to working slots by adding `shift`.
-template <class F>
-__device__ __forceinline__ void for_stencil(const GridGeom& g, const u32* __restrict__ cs,
-                                            u32 shift, float px, float py, float pz, F&& f) {
-    const int cx = g.cx(px), cy = g.cy(py), cz = g.cz(pz);
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int nz = cz + dz;
-        if (nz < 0 || nz >= g.dz) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int ny = cy + dy;
-            if (ny < 0 || ny >= g.dy) continue;
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int nx = cx + dx;
-                if (nx < 0 || nx >= g.dx) continue;
-                const u32 c =
-                    (static_cast<u32>(nx) * static_cast<u32>(g.dy) + static_cast<u32>(ny)) *
-                        static_cast<u32>(g.dz) +
-                    static_cast<u32>(nz);
-                const u32 b = cs[c], e = cs[c + 1];
-                for (u32 s = b; s < e; ++s) f(s + shift);
-            }
-        }
-    }
-}
-
-template <int NT>
-__device__ __forceinline__ u32 slot_index(int remap) {
-    const u32 t = remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
-    return t * NT + threadIdx.x;
-}
-
-// Scene of working slot i (< nslots).  Waves never straddle tiles, so the
-// result is wave-uniform and the scene table is read with scalar loads.
-__device__ __forceinline__ u32 scene_of_slot(const Nbr& P, u32 i) {
-    if (!P.tile_scene) return 0u;
-    const u32 t = __builtin_amdgcn_readfirstlane(i / kTile);
-    return P.tile_scene[t];
-}
+using namespace dev;
 
 __global__ __launch_bounds__(NS) void k_reorder(Nbr P, const u32* __restrict__ perm,
                                                 const float* __restrict__ x,
@@ -163,28 +106,6 @@ __global__ __launch_bounds__(NS) void k_pack_boundary(u32 nb, const u32* __restr
     if (s >= nb) return;
     xb[s].w = mb[s];
     store_mb[perm[s]] = mb[s];
-}
-
-// Per-particle pair lists: entry k of particle i holds (neighbour slot, bits
-// of c) with grad W_ij = (x_i - x_j) * c, for every pair the reference's
-// stencil visits with c != 0, in the reference's summation order.  Positions
-// are frozen from the grid build to advection, so the iteration kernels
-// recompute r and g = r * c from the list and get the same binary32 values
-// the reference gets from grad_W (zero-gradient pairs add +-0 to sums that
-// never hold -0, so leaving them out is exact).  Particles with more pairs
-// than the list depth use the stencil walk.
-//
-// Wave-blocked ELL: the K entries of the 64 particles of one wave occupy one
-// contiguous block, row k of it holding entry k of each lane.
-__device__ __forceinline__ u64 pair_at(u32 k, u32 i, u32 depth) {
-    return (static_cast<u64>(i >> 6) * depth + k) * 64u + (i & 63u);
-}
-
-// List entries are streamed once per traversal: load them non-temporally so
-// they do not evict the gathered neighbour data from L2.
-__device__ __forceinline__ uint2 ld_pair(const uint2* p) {
-    const u64 v = __builtin_nontemporal_load(reinterpret_cast<const u64*>(p));
-    return make_uint2(static_cast<u32>(v), static_cast<u32>(v >> 32));
 }
 
 __device__ __forceinline__ u32 pack_counts(u32 cf, u32 cb) {
@@ -455,7 +376,7 @@ __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* _
                                               const float* __restrict__ kappa, float rho0,
                                               int pform, u32 iter, int mode, double thr,
                                               const double* __restrict__ part, DeviceState* st,
-                                              float4* __restrict__ vs) {
+                                              const float4* vin, float4* vout) {
     const u32 i = slot_index<NT>(P.remap);
     if (i >= P.nslots) return;
     const u32 sc = scene_of_slot(P, i);
@@ -529,15 +450,11 @@ __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* _
             });
         }
     }
-    float4 v = vs[i];
+    float4 v = vin[i];
     v.x += dx;
     v.y += dy;
     v.z += dz;
-    vs[i] = v;
-}
-
-__device__ __forceinline__ bool finite3(float a, float b, float c) {
-    return isfinite(a) && isfinite(b) && isfinite(c);
+    vout[i] = v;
 }
 
 // Per-wave (max |v|, NaN count) partials; k_finish_vstats reduces them per scene.
@@ -771,13 +688,15 @@ struct ApplyArgs {
     double thr;
     const double* part;
     DeviceState* st;
-    float4* vs;
+    const float4* vin;
+    float4* vout;
 };
 
 template <int NT, int PU>
 void apply_one(hipStream_t s, u32 nslots, const Nbr& P, const PairList& L, const ApplyArgs& a) {
     k_apply<NT, PU><<<grid_for(nslots, NT), NT, 0, s>>>(P, L, a.xk, a.kappa, a.rho0, a.pform,
-                                                        a.iter, a.mode, a.thr, a.part, a.st, a.vs);
+                                                        a.iter, a.mode, a.thr, a.part, a.st, a.vin,
+                                                        a.vout);
 }
 
 template <int NT>
@@ -1097,32 +1016,6 @@ PairList DFSPHSolver::pair_list() {
     return L;
 }
 
-static Nbr make_nbr(const CellGrid& fg, const CellGrid* bg, const float4* xb, size_t nb,
-                    const DFSPHConfig& cfg, float mass, u32 nslots, const SceneSeg* seg,
-                    const u32* tile_scene, u32 remap) {
-    Nbr P;
-    P.gf = fg.geom();
-    P.csf = fg.cell_start();
-    if (bg && nb) {
-        P.gb = bg->geom();
-        P.csb = bg->cell_start();
-        P.xb = xb;
-        P.nb = static_cast<u32>(nb);
-    } else {
-        P.gb = fg.geom();
-        P.csb = nullptr;
-        P.xb = nullptr;
-        P.nb = 0;
-    }
-    P.sp = Spline::make(cfg.smoothing_length);
-    P.mass = mass;
-    P.nslots = nslots;
-    P.seg = seg;
-    P.tile_scene = tile_scene;
-    P.remap = static_cast<int>(remap);
-    return P;
-}
-
 // Per-substep scene parameters, computed on the host once per scene exactly
 // as the plain solver computes them (dfsph_kernels.cu:351 exp(-damping*dt)).
 void DFSPHSolver::set_substep(const float* dt) {
@@ -1231,7 +1124,7 @@ void DFSPHSolver::phase_iterations(int mode, u32 begin, u32 end) {
                   static_cast<int>(cfg_.pressure_form), 0, dstate_.data(), part_d_.data(),
                   rho_adv_.data(), kappa_.data(), xk_.data()};
     ApplyArgs aa{xk_.data(), kappa_.data(), cfg_.rest_density, static_cast<int>(cfg_.pressure_form),
-                 0, mode, thr, part_d_.data(), dstate_.data(), vs_.data()};
+                 0, mode, thr, part_d_.data(), dstate_.data(), vs_.data(), vs_.data()};
     for (u32 it = begin; it < end; ++it) {
         ca.iter = aa.iter = it;
         tbegin(mode == 0 ? kKDensityChange : kKDivChange, static_cast<int>(it));
@@ -1381,6 +1274,87 @@ float DFSPHSolver::attribute_at(int which, size_t i) const {
 }
 
 // ---------------------------------------------------------------------------
+// Taped substep for the adjoint: the same kernels as step(), with every
+// iteration reading its velocity from slab k of the tape and writing slab
+// k+1, and its (x, kappa-hat) and rho* into slab k.
+void DFSPHSolver::iterations_taped(u32 begin, u32 end, SubstepTape& t) {
+    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, nslots_,
+                           dseg_.data(), nullptr, shapes_.remap);
+    const PairList L = pair_list();
+    const double thr = static_cast<double>(cfg_.eta_density * cfg_.rest_density);
+    const size_t ns = nslots_;
+    ChangeArgs ca{xs_.data(), nullptr, rho_.data(), alpha_.data(), cfg_.rest_density,
+                  static_cast<int>(cfg_.pressure_form), 0, dstate_.data(), part_d_.data(),
+                  nullptr, kappa_.data(), nullptr};
+    ApplyArgs aa{nullptr, kappa_.data(), cfg_.rest_density, static_cast<int>(cfg_.pressure_form), 0, 0,
+                 thr, part_d_.data(), dstate_.data(), nullptr, nullptr};
+    for (u32 it = begin; it < end; ++it) {
+        ca.iter = aa.iter = it;
+        ca.vs = t.w.data() + it * ns;
+        ca.rho_adv = t.p.data() + it * ns;
+        ca.xk = t.xk.data() + it * ns;
+        aa.xk = ca.xk;
+        aa.vin = ca.vs;
+        aa.vout = t.w.data() + (it + 1) * ns;
+        tbegin(kKDensityChange, static_cast<int>(it));
+        launch_change<0>(shapes_.change_nt, shapes_.change_pu, stream_, nslots_, P, L, ca);
+        tend();
+        tbegin(kKApplyKappa, static_cast<int>(it));
+        launch_apply(shapes_.apply_nt, shapes_.apply_pu, stream_, nslots_, P, L, aa);
+        tend();
+    }
+}
+
+void DFSPHSolver::step_taped(float dt, SubstepTape& t) {
+    if (batched_) fail(kInvalidArgument, "step_taped: plain solver only");
+    if (cfg_.enable_divergence_solve)
+        fail(kInvalidArgument, "step_taped: the adjoint covers the density solve only");
+    const size_t nsz = fluid_.count();
+    if (nsz == 0) fail(kInvalidArgument, "step_taped: empty store");
+    if (counts_.empty() || counts_[0] != nsz) layout({static_cast<u32>(nsz)});
+    const u32 maxit = cfg_.max_density_iters;
+    const size_t ns = nslots_;
+    t.x.ensure(3 * nsz);
+    t.v.ensure(3 * nsz);
+    t.w.ensure((maxit + 1) * ns);
+    t.xk.ensure(std::max<u32>(maxit, 1) * ns);
+    t.p.ensure(std::max<u32>(maxit, 1) * ns);
+    WSIM_HIP(hipMemcpyAsync(t.x.data(), fluid_.positions(), 3 * nsz * sizeof(float),
+                            hipMemcpyDeviceToDevice, stream_));
+    WSIM_HIP(hipMemcpyAsync(t.v.data(), fluid_.velocities(), 3 * nsz * sizeof(float),
+                            hipMemcpyDeviceToDevice, stream_));
+    set_substep(&dt);
+    phase_prefix(true);
+    WSIM_HIP(hipMemcpyAsync(t.w.data(), vs_.data(), ns * sizeof(float4), hipMemcpyDeviceToDevice,
+                            stream_));
+    u32 it = 0, chunk = 2;
+    while (it < maxit) {
+        const u32 end = (maxit - it < chunk) ? maxit : it + chunk;
+        iterations_taped(it, end, t);
+        it = end;
+        if (it >= maxit) break;
+        if (all_converged(0)) break;
+        chunk = chunk * 2 > 32 ? 32 : chunk * 2;
+    }
+    u32 k = 0;
+    if (maxit > 0) {
+        all_converged(0);
+        k = hstate_[0].conv_iter == kNone ? maxit : hstate_[0].conv_iter + 1;
+    }
+    WSIM_HIP(hipMemcpyAsync(vs_.data(), t.w.data() + k * ns, ns * sizeof(float4),
+                            hipMemcpyDeviceToDevice, stream_));
+    if (k > 0)
+        WSIM_HIP(hipMemcpyAsync(rho_adv_.data(), t.p.data() + (k - 1) * ns, ns * sizeof(float),
+                                hipMemcpyDeviceToDevice, stream_));
+    phase_advect();
+    read_state();
+    t.dt = dt;
+    t.iters = k;
+    t.n = static_cast<u32>(nsz);
+    t.nslots = nslots_;
+}
+
+// ---------------------------------------------------------------------------
 // Autotuner.  Every candidate computes the same bits, so the search runs on
 // the solver's own data: one substep prefix on the current store state (no
 // gravity step is written back, the store is not touched), then each kernel
@@ -1433,7 +1407,7 @@ void DFSPHSolver::autotune(int reps) {
                   rho_adv_.data(), kappa_.data(), xk_.data()};
     ApplyArgs aa{xk_.data(), kappa_.data(), cfg_.rest_density, static_cast<int>(cfg_.pressure_form),
                  0, 0, static_cast<double>(cfg_.eta_density * cfg_.rest_density), part_d_.data(),
-                 dstate_.data(), vs_.data()};
+                 dstate_.data(), vs_.data(), vs_.data()};
     try {
         // cell_start pass
         const u32 ncs = scenes * (fluid_grid_.total_cells() + 1u) + 1u;

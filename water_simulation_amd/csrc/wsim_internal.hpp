@@ -261,6 +261,25 @@ struct TuneRecord {
     double ms;
 };
 
+// Forward record of one substep for the adjoint: the input state and every
+// pressure iterate.  Grids, pair lists, rho and alpha are recomputed from the
+// input state, which reproduces them bit for bit.
+struct SubstepTape {
+    DeviceBuffer<float> x, v;  // input state, original order (3n)
+    DeviceBuffer<float4> w;    // K+1 velocity slabs: w[k] enters iteration k, w[K] advection
+    DeviceBuffer<float4> xk;   // K slabs of (x, kappa-hat)
+    DeviceBuffer<float> p;     // K slabs of rho*
+    float dt = 0.0f;
+    u32 iters = 0, n = 0, nslots = 0;
+};
+
+// Adjoint variables in working-slot order (binary64).
+struct AdjointWork {
+    DeviceBuffer<double4> xbar, wbar, v0bar, gbar;
+    DeviceBuffer<double> rdbar, rhobar, alphabar, part;
+    PinnedBuffer<double> hpart;
+};
+
 // DFSPH substep over one or many independent scenes.  A plain solver is one
 // scene holding every particle of the fluid store; a batched solver packs
 // scenes of uneven sizes (consecutive index ranges of the store) into one
@@ -296,6 +315,7 @@ public:
     float attribute_at(int which, size_t i) const;  // 0 density, 1 alpha, 2 boundary mass
     const DFSPHConfig& config() const { return cfg_; }
     hipStream_t stream() const { return stream_; }
+    ParticleStore& fluid_store() { return fluid_; }
 
     void set_timing(bool on) { timing_ = on; }
     KernelTimer& timer() { return timer_; }
@@ -313,7 +333,21 @@ public:
     void autotune(int reps);
     const std::vector<TuneRecord>& tune_log() const { return tune_log_; }
 
+    // XSPH viscosity of a plain solver (the parameter the adjoint differentiates).
+    void set_viscosity(float eps);
+    // Plain solver only: a substep that records `tape` (eager launches).
+    void step_taped(float dt, SubstepTape& tape);
+    // Adjoint of the substep recorded in `tape` (solvers/dfsph/src/dfsph.cu:
+    // 120-133 transposed, neighbour sets, iteration count and clamp masks of
+    // the recorded pass): from d(loss)/d(output positions, velocities) to
+    // d(loss)/d(input positions, velocities), all binary64 device arrays of
+    // 3n in original order; returns this substep's d(loss)/d(viscosity).
+    // Leaves the store at the substep's input state.
+    double adjoint_step(const SubstepTape& tape, const double* gx_out, const double* gv_out,
+                        double* gx_in, double* gv_in);
+
 private:
+    void iterations_taped(u32 begin, u32 end, SubstepTape& t);
     void init(const std::vector<u32>& counts, const std::vector<float>& viscosity);
     void layout(const std::vector<u32>& counts);
     void set_substep(const float* dt);
@@ -378,6 +412,32 @@ private:
 
     bool timing_ = false;
     KernelTimer timer_;
+    AdjointWork adj_;
+};
+
+// Differentiable rollout over a plain solver: forward substeps with a state
+// checkpoint every `every` substeps; backward recomputes one segment at a
+// time from its checkpoint, recording every substep's tape, and runs the
+// substep adjoints in reverse.
+class Rollout {
+public:
+    Rollout(DFSPHSolver& solver, ParticleStore& fluid, u32 every);
+    // x0, v0: device, 3n, original order.  Leaves the final state in the store.
+    void forward(const float* x0, const float* v0, const float* dts, u32 substeps);
+    // gx, gv: d(loss)/d(final positions, velocities); writes d(loss)/d(x0, v0)
+    // and returns d(loss)/d(viscosity).  Device binary64 arrays of 3n.
+    double backward(const double* gx, const double* gv, double* gx0, double* gv0);
+    u32 substeps() const { return static_cast<u32>(dts_.size()); }
+
+private:
+    DFSPHSolver& solver_;
+    ParticleStore& fluid_;
+    u32 every_;
+    size_t n_ = 0;
+    std::vector<float> dts_;
+    std::vector<DeviceBuffer<float>> ckx_, ckv_;
+    std::vector<SubstepTape> tapes_;
+    DeviceBuffer<double> g_[4];
 };
 
 }  // namespace wsim
