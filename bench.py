@@ -182,6 +182,56 @@ def batched_leg(steps: int, warmup: int, tune_reps: int):
     }
 
 
+def adjoint_leg(substeps: int, every: int):
+    """Inverse-problem configuration: 32^3 particles (r = 0.005 m) in the 0.6 m
+    Akinci box, fixed dt = 0.002 s, seeded smooth initial velocities, 16^3
+    cell-density loss.  Times one checkpointed forward rollout and its backward
+    pass (segment recomputation with taped pressure iterates + adjoint kernels)."""
+    import water_simulation_amd as ws
+    from water_simulation_amd import adjoint as A
+
+    p = A.inverse_problem(side=32, substeps=substeps)
+    n = len(p.x0)
+    f = ws.ParticleStore(n)
+    f.resize(n)
+    f.set_positions(p.x0)
+    b = ws.ParticleStore(len(p.xb))
+    b.resize(len(p.xb))
+    b.set_positions(p.xb)
+    solver = ws.DFSPHSolver(f, b, ws.DFSPHConfig(**p.cfg))
+    mass = solver.particle_mass()
+    ro = A.Rollout(solver, checkpoint_every=every)
+    ro.forward(p.x0, p.v_true, p.dts)
+    target = A.density_field_np(f.get_positions(), mass)
+    ro.forward(p.x0, p.v_guess, p.dts[:every])  # warm-up
+    ws.device_synchronize()
+    t0 = time.perf_counter()
+    ro.forward(p.x0, p.v_guess, p.dts)
+    ws.device_synchronize()
+    t_fwd = time.perf_counter() - t0
+    iters = solver.stats()["density_iters"]
+    loss, gx = A.density_loss_np(f.get_positions(), target, mass)
+    t0 = time.perf_counter()
+    gx0, gv0, ge = ro.backward(gx, np.zeros_like(gx))
+    ws.device_synchronize()
+    t_bwd = time.perf_counter() - t0
+    return {
+        "workload": f"inverse problem: {n} particles (32^3, r=0.005 m) in a 0.6 m Akinci box "
+                    f"({len(p.xb)} wall particles), {substeps} substeps of 0.002 s, checkpoint "
+                    f"every {every}, 16^3 cell-density loss",
+        "substeps": substeps,
+        "checkpoint_every": every,
+        "forward_s": t_fwd,
+        "backward_s": t_bwd,
+        "adjoint_over_forward": t_bwd / t_fwd,
+        "last_density_iters": iters,
+        "loss": loss,
+        "grad_v0_norm": float(np.sqrt((gv0 ** 2).sum())),
+        "dloss_dviscosity": ge,
+        "backward_includes": "segment recompute with taped iterates + adjoint kernels",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,6 +247,9 @@ def main() -> int:
     ap.add_argument("--batch-warmup", type=int, default=3)
     ap.add_argument("--tune-reps", type=int, default=3)
     ap.add_argument("--only-batch", action="store_true", help="run only the batched leg")
+    ap.add_argument("--adjoint-substeps", type=int, default=200,
+                    help="substeps of the adjoint leg (0: skip it)")
+    ap.add_argument("--only-adjoint", action="store_true", help="run only the adjoint leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run libwsim's streaming copy kernel 5 times over BYTES bytes "
                          "(the known-size workload for converting HBM byte counters)")
@@ -226,6 +279,9 @@ def main() -> int:
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
 
+    if args.only_adjoint:
+        print(json.dumps({"adjoint": adjoint_leg(args.adjoint_substeps, 20)}))
+        return 0
     if args.only_batch:
         print(json.dumps({"batched": batched_leg(args.batch_steps, args.batch_warmup,
                                                  args.tune_reps)}))
@@ -294,6 +350,10 @@ def main() -> int:
     if args.batch_steps > 0:
         batched = batched_leg(args.batch_steps, args.batch_warmup, args.tune_reps)
 
+    adjoint = None
+    if args.adjoint_substeps > 0:
+        adjoint = adjoint_leg(args.adjoint_substeps, 20)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu, _, _ = cpu_baseline(x0, v0, xb, cfg, dts, args.cpu_budget)
@@ -341,6 +401,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "batched": batched,
+        "adjoint": adjoint,
     }
     print(json.dumps(out))
     return 0

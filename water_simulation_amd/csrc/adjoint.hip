@@ -39,6 +39,7 @@ namespace {
 using namespace dev;
 
 constexpr int NA = 256;  // workgroup size of the adjoint kernels (one tile)
+constexpr int kAdjointLanes = 4;  // lanes per particle in the per-iteration kernels
 
 struct D3 {
     double x, y, z;
@@ -126,6 +127,40 @@ __device__ __forceinline__ void for_pairs(const Nbr& P, const PairList& L, const
     }
 }
 
+// Pairs of particle i split over a group of G lanes: lane g takes pairs
+// g, g + G, ... of the lists; an overflowed list is walked by lane 0 alone.
+template <int G, class FF, class FB>
+__device__ __forceinline__ void for_pairs_split(const Nbr& P, const PairList& L, const SceneSeg& Sg,
+                                                u32 i, u32 g, const float4& p,
+                                                const float4* __restrict__ xs, FF&& ff, FB&& fb) {
+    const u32 cw = L.cnt[i];
+    const u32 cf = cw & 0xFFFFu, cb = cw >> 16;
+    if (cf <= L.kf && cb <= L.kb) {
+        for (u32 k = g; k < cf; k += G) {
+            const uint2 e = ld_pair(L.f + pair_at(k, i, L.kf));
+            ff(e.x, __uint_as_float(e.y));
+        }
+        for (u32 k = g; k < cb; k += G) {
+            const uint2 e = ld_pair(L.b + pair_at(k, i, L.kb));
+            fb(e.x, __uint_as_float(e.y));
+        }
+    } else if (g == 0) {
+        for_pairs(P, L, Sg, i, p, xs, ff, fb);
+    }
+}
+
+// Sum over the G lanes of a group (fixed butterfly order).
+template <int G> __device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o);
+    return v;
+}
+template <int G> __device__ __forceinline__ void group_sum3(D3& a) {
+    a.x = group_sum<G>(a.x);
+    a.y = group_sum<G>(a.y);
+    a.z = group_sum<G>(a.z);
+}
+
 // Advection (dfsph_kernels.cu:363-386) transposed with the recorded clamp
 // masks; gathers the output adjoints from original order and opens the
 // per-particle accumulators.
@@ -187,7 +222,9 @@ __global__ __launch_bounds__(NA) void k_adj_advect(Nbr P, const u32* __restrict_
 
 // Transposed apply-kappa of one iteration (dfsph_kernels.cu:237-301) and the
 // pointwise part of its kappa (dfsph_kernels.cu:226-231): from w_bar^(k+1)
-// to rho_dot_bar^k, with the rho, alpha and H-term sensitivities.
+// to rho_dot_bar^k, with the rho, alpha and H-term sensitivities.  G lanes
+// per particle share its pairs.
+template <int G>
 __global__ __launch_bounds__(NA) void k_adj_kappa(Nbr P, PairList L, const float4* __restrict__ xs,
                                                   const double4* __restrict__ wbar,
                                                   const float4* __restrict__ xk,
@@ -198,42 +235,56 @@ __global__ __launch_bounds__(NA) void k_adj_kappa(Nbr P, PairList L, const float
                                                   double* __restrict__ rhobar,
                                                   double* __restrict__ alphabar,
                                                   double4* __restrict__ xbar) {
-    const u32 i = slot_index<NA>(P.remap);
-    if (i >= P.nslots) return;
-    const SceneSeg& G = P.seg[scene_of_slot(P, i)];
-    if (i - G.off >= G.n) return;
-    const float dt = G.dt;
-    const double ddt = dt;
-    const float4 p = xk[i];
+    const u32 t = slot_index<NA>(P.remap);
+    const u32 i = t / G, g = t % G;
+    double kbar = 0.0, bbar = 0.0;
+    D3 xa = {0.0, 0.0, 0.0};
+    bool valid = false;
+    const SceneSeg* Gp = nullptr;
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    float bf = 0.0f;
+    if (i < P.nslots) {
+        Gp = &P.seg[scene_of_slot(P, i)];
+        valid = i - Gp->off < Gp->n;
+    }
+    if (valid) {
+        const SceneSeg& S = *Gp;
+        const double ddt = S.dt;
+        p = xk[i];
+        const float kap = fmaxf(0.0f, pk[i] - rho0) * alpha[i] / S.dt2;
+        bf = pform ? kap / fmaxf(rho0, 1e-9f) : kap / fmaxf(rho0 * rho0, 1e-9f);
+        const double ki = p.w;
+        const D3 wi = d3(wbar[i]);
+        const double m = P.mass;
+        for_pairs_split<G>(
+            P, L, S, i, g, p, xs,
+            [&](u32 j, float) {
+                const float4 q = xk[j];
+                const Geo ge = pair_geo(p, q, P.sp);
+                const D3 dw = sub(wi, d3(wbar[j]));
+                kbar -= ddt * m * dot(dw, ge.grad());
+                axpy(xa, -ddt * m * (ki + static_cast<double>(q.w)), ge.hess(dw));
+            },
+            [&](u32 b, float) {
+                const float4 q = P.xb[b];
+                const Geo ge = pair_geo(p, q, P.sp);
+                const double mb = q.w;
+                const double tt = -ddt * mb * dot(wi, ge.grad());
+                kbar += tt;
+                bbar += tt;
+                axpy(xa, -ddt * mb * (ki + static_cast<double>(bf)), ge.hess(wi));
+            });
+    }
+    kbar = group_sum<G>(kbar);
+    bbar = group_sum<G>(bbar);
+    group_sum3<G>(xa);
+    if (!valid || g != 0) return;
+    const SceneSeg& S = *Gp;
     const float rho_i = rho[i];
     const float a_i = alpha[i];
     const float pr = pk[i];
     const float over = fmaxf(0.0f, pr - rho0);
-    const float kap = over * a_i / G.dt2;
-    const float bf = pform ? kap / fmaxf(rho0, 1e-9f) : kap / fmaxf(rho0 * rho0, 1e-9f);
-    const double ki = p.w;
-    const D3 wi = d3(wbar[i]);
-    double kbar = 0.0, bbar = 0.0;
-    D3 xa = {0.0, 0.0, 0.0};
-    const double m = P.mass;
-    for_pairs(
-        P, L, G, i, p, xs,
-        [&](u32 j, float) {
-            const float4 q = xk[j];
-            const Geo g = pair_geo(p, q, P.sp);
-            const D3 dw = sub(wi, d3(wbar[j]));
-            kbar -= ddt * m * dot(dw, g.grad());
-            axpy(xa, -ddt * m * (ki + static_cast<double>(q.w)), g.hess(dw));
-        },
-        [&](u32 b, float) {
-            const float4 q = P.xb[b];
-            const Geo g = pair_geo(p, q, P.sp);
-            const double mb = q.w;
-            const double t = -ddt * mb * dot(wi, g.grad());
-            kbar += t;
-            bbar += t;
-            axpy(xa, -ddt * mb * (ki + static_cast<double>(bf)), g.hess(wi));
-        });
+    const float kap = over * a_i / S.dt2;
     // kappa-hat = kappa / max(rho^2, 1e-9) (or / max(rho, 1e-9)); boundary factor
     // kappa / max(rho0^2, 1e-9) (or / max(rho0, 1e-9))
     const double dr = rho_i;
@@ -249,11 +300,11 @@ __global__ __launch_bounds__(NA) void k_adj_kappa(Nbr P, PairList L, const float
     }
     double pbar = 0.0, ab = 0.0;
     if (pr - rho0 > 0.0f) {
-        const double dt2 = G.dt2;
+        const double dt2 = S.dt2;
         pbar = kapbar * static_cast<double>(a_i) / dt2;
         ab = kapbar * static_cast<double>(over) / dt2;
     }
-    rdbar[i] = ddt * pbar;
+    rdbar[i] = static_cast<double>(S.dt) * pbar;
     rhobar[i] += rb + pbar;
     alphabar[i] += ab;
     const double4 x0 = xbar[i];
@@ -261,36 +312,45 @@ __global__ __launch_bounds__(NA) void k_adj_kappa(Nbr P, PairList L, const float
 }
 
 // Transposed density change (dfsph_kernels.cu:196-224): w_bar^k from
-// w_bar^(k+1) and rho_dot_bar^k, with its H terms.
+// w_bar^(k+1) and rho_dot_bar^k, with its H terms.  G lanes per particle.
+template <int G>
 __global__ __launch_bounds__(NA) void k_adj_rhodot(Nbr P, PairList L, const float4* __restrict__ xs,
                                                    const float4* __restrict__ wk,
                                                    const double* __restrict__ rdbar,
                                                    double4* __restrict__ wbar,
                                                    double4* __restrict__ xbar) {
-    const u32 i = slot_index<NA>(P.remap);
-    if (i >= P.nslots) return;
-    const SceneSeg& G = P.seg[scene_of_slot(P, i)];
-    if (i - G.off >= G.n) return;
-    const float4 p = xs[i];
-    const D3 wi = d3f(wk[i]);
-    const double ri = rdbar[i];
-    const double m = P.mass;
+    const u32 t = slot_index<NA>(P.remap);
+    const u32 i = t / G, g = t % G;
     D3 acc = {0.0, 0.0, 0.0}, xa = {0.0, 0.0, 0.0};
-    for_pairs(
-        P, L, G, i, p, xs,
-        [&](u32 j, float) {
-            const Geo g = pair_geo(p, xs[j], P.sp);
-            const double s = m * (ri + rdbar[j]);
-            axpy(acc, s, g.grad());
-            axpy(xa, s, g.hess(sub(wi, d3f(wk[j]))));
-        },
-        [&](u32 b, float) {
-            const float4 q = P.xb[b];
-            const Geo g = pair_geo(p, q, P.sp);
-            const double s = static_cast<double>(q.w) * ri;
-            axpy(acc, s, g.grad());
-            axpy(xa, s, g.hess(wi));
-        });
+    bool valid = false;
+    if (i < P.nslots) {
+        const SceneSeg& S = P.seg[scene_of_slot(P, i)];
+        valid = i - S.off < S.n;
+        if (valid) {
+            const float4 p = xs[i];
+            const D3 wi = d3f(wk[i]);
+            const double ri = rdbar[i];
+            const double m = P.mass;
+            for_pairs_split<G>(
+                P, L, S, i, g, p, xs,
+                [&](u32 j, float) {
+                    const Geo ge = pair_geo(p, xs[j], P.sp);
+                    const double sc = m * (ri + rdbar[j]);
+                    axpy(acc, sc, ge.grad());
+                    axpy(xa, sc, ge.hess(sub(wi, d3f(wk[j]))));
+                },
+                [&](u32 b, float) {
+                    const float4 q = P.xb[b];
+                    const Geo ge = pair_geo(p, q, P.sp);
+                    const double sc = static_cast<double>(q.w) * ri;
+                    axpy(acc, sc, ge.grad());
+                    axpy(xa, sc, ge.hess(wi));
+                });
+        }
+    }
+    group_sum3<G>(acc);
+    group_sum3<G>(xa);
+    if (!valid || g != 0) return;
     const double4 w0 = wbar[i];
     wbar[i] = make_double4(w0.x + acc.x, w0.y + acc.y, w0.z + acc.z, 0.0);
     const double4 x0 = xbar[i];
@@ -505,16 +565,18 @@ double DFSPHSolver::adjoint_step(const SubstepTape& t, const double* gx_out, con
         adj_.xbar.data(), adj_.wbar.data(), adj_.rhobar.data(), adj_.alphabar.data());
     WSIM_LAUNCH_CHECK();
     const int pform = static_cast<int>(cfg_.pressure_form);
+    constexpr int GL = kAdjointLanes;
+    const u32 gblocks = static_cast<u32>(ns * GL / NA);
     for (u32 k = K; k-- > 0;) {
-        k_adj_kappa<<<blocks, NA, 0, stream_>>>(P, L, xs_.data(), adj_.wbar.data(),
-                                                 t.xk.data() + k * ns, t.p.data() + k * ns,
-                                                 rho_.data(), alpha_.data(), cfg_.rest_density,
-                                                 pform, adj_.rdbar.data(), adj_.rhobar.data(),
-                                                 adj_.alphabar.data(), adj_.xbar.data());
+        k_adj_kappa<GL><<<gblocks, NA, 0, stream_>>>(P, L, xs_.data(), adj_.wbar.data(),
+                                                      t.xk.data() + k * ns, t.p.data() + k * ns,
+                                                      rho_.data(), alpha_.data(), cfg_.rest_density,
+                                                      pform, adj_.rdbar.data(), adj_.rhobar.data(),
+                                                      adj_.alphabar.data(), adj_.xbar.data());
         WSIM_LAUNCH_CHECK();
-        k_adj_rhodot<<<blocks, NA, 0, stream_>>>(P, L, xs_.data(), t.w.data() + k * ns,
-                                                  adj_.rdbar.data(), adj_.wbar.data(),
-                                                  adj_.xbar.data());
+        k_adj_rhodot<GL><<<gblocks, NA, 0, stream_>>>(P, L, xs_.data(), t.w.data() + k * ns,
+                                                       adj_.rdbar.data(), adj_.wbar.data(),
+                                                       adj_.xbar.data());
         WSIM_LAUNCH_CHECK();
     }
     const float4* v0 = any_xsph_ ? vpre_.data() : vs_.data();
