@@ -108,6 +108,18 @@ template <> struct F32Values<DualF> { static constexpr bool value = true; };
 struct StructTape {
     std::vector<std::vector<int16_t>> homes;
     std::vector<u32> iters;
+    // per particle: max(0, .) branch of every sweep; clamp decisions of every advection
+    std::vector<std::vector<uint8_t>> masks;
+    std::vector<std::vector<uint16_t>> clamps;
+};
+
+// Clamp decisions of one particle's advection (dfsph_kernels.cu:363-386).
+enum : uint16_t {
+    kClampSpeed = 1,        // |v| > vmax: rescaled
+    kClampLo = 2,           // << axis: position clamped at the low bound
+    kClampHi = 16,          // << axis: position clamped at the high bound
+    kZeroLo = 128,          // << axis: velocity component zeroed at the low bound
+    kZeroHi = 1024,         // << axis: at the high bound
 };
 
 // Test-only switch: XSPH as an in-place sequential sweep in original index
@@ -157,11 +169,19 @@ template <class R> inline V3<R> gradW(const V3<R>& rij, R l) {
 // (int)floorf((p - o) * inv) (cell_grid.cu:11-13, dfsph_kernels.cu:15-17),
 // made total: NaN and coordinates beyond the stencil's reach are clamped to
 // values whose whole stencil lies outside the grid.
-template <class R> inline int cell_coord(R p, R o, R inv, int D) {
-    const R f = o_floor((p - o) * inv);
-    if (!(f >= R(-2))) return -2;
-    if (f > R(D + 1)) return D + 1;
+//
+// Cells are integer structure: every instance computes them in binary32 from
+// the binary32-rounded position, as the reference does, so the float, double
+// and dual solvers see the same grids for the same inputs (wall particles sit
+// exactly on cell faces, where a binary64 evaluation can pick the neighbour).
+inline int cell_coord_f(float p, float o, float inv, int D) {
+    const float f = std::floor((p - o) * inv);
+    if (!(f >= -2.0f)) return -2;
+    if (f > float(D + 1)) return D + 1;
     return static_cast<int>(f);
+}
+template <class R> inline int cell_coord(R p, float o, float inv, int D) {
+    return cell_coord_f(static_cast<float>(p), o, inv, D);
 }
 
 inline u32 cell_key(int cx, int cy, int cz, const int32_t D[3]) {
@@ -171,16 +191,14 @@ inline u32 cell_key(int cx, int cy, int cz, const int32_t D[3]) {
 }
 
 template <class R> struct Grid {
-    R ox = 0, oy = 0, oz = 0, inv = 0;
+    float ox = 0, oy = 0, oz = 0, inv = 0;
     int32_t D[3] = {0, 0, 0};
     u32 C = 0;
     std::vector<u32> keys, sorted_keys, perm, cs;
 
     void init(const float origin[3], const int32_t dims[3], float cell) {
-        ox = R(origin[0]); oy = R(origin[1]); oz = R(origin[2]);
-        // cell_grid.cu:81: inverse cell length computed once on the host
-        inv = R(1) / R(cell);
-        if constexpr (F32Values<R>::value) inv = R(1.0f / cell);
+        ox = origin[0]; oy = origin[1]; oz = origin[2];
+        inv = 1.0f / cell;  // cell_grid.cu:81: inverse cell length computed once on the host
         std::memcpy(D, dims, sizeof(D));
         C = u32(D[0]) * u32(D[1]) * u32(D[2]);
         cs.assign(size_t(C) + 1, 0);
@@ -268,7 +286,32 @@ template <class R> struct Solver {
     // frozen-structure tape: 0 off, 1 record, 2 replay
     StructTape* tape = nullptr;
     int tape_mode = 0;
-    size_t tape_home = 0, tape_iter = 0;
+    size_t tape_home = 0, tape_iter = 0, tape_mask = 0, tape_clamp = 0;
+    std::vector<uint8_t> mask_buf;
+    std::vector<uint16_t> clamp_buf;
+
+    // max(0, a) with its branch recorded or replayed (frozen-structure tapes).
+    const uint8_t* mask_in = nullptr;
+    uint8_t* mask_out = nullptr;
+    void open_masks() {
+        mask_in = nullptr;
+        mask_out = nullptr;
+        if (tape_mode == 1) {
+            mask_buf.assign(n, 0);
+            mask_out = mask_buf.data();
+        } else if (tape_mode == 2 && tape_mask < tape->masks.size()) {
+            mask_in = tape->masks[tape_mask++].data();
+        }
+    }
+    void close_masks() {
+        if (tape_mode == 1) tape->masks.push_back(mask_buf);
+    }
+    R relu(long long i, R a) const {
+        if (mask_in) return mask_in[i] ? a : R(0);
+        const R r = o_fmax(R(0), a);
+        if (mask_out) mask_out[i] = (r != R(0)) ? 1 : 0;
+        return r;
+    }
     const int16_t* home = nullptr;  // home cells of the current grid build
     std::vector<int16_t> home_buf;
 
@@ -454,27 +497,31 @@ template <class R> struct Solver {
     // dfsph_kernels.cu:176-232
     void density_change(float dt) {
         const R dtr = R(dt), rho0 = R(cfg.rest_density);
+        open_masks();
 #pragma omp parallel for schedule(static)
         for (long long i = 0; i < (long long)n; ++i) {
             const R rd = rho_dot(i);
             const R rho_pred = rho[i] + dtr * rd;
             rho_adv[i] = rho_pred;
-            const R over = o_fmax(R(0), rho_pred - rho0);
+            const R over = relu(i, rho_pred - rho0);
             kappa[i] = over * alpha[i] / (dtr * dtr);
         }
+        close_masks();
     }
 
     // divergence_change_kernel, phase-2 plan text (plan :1862-1907); boundary
     // term as in the shipped density_change_kernel (dfsph_kernels.cu:213-224).
     void divergence_change(float dt) {
         const R dtr = R(dt);
+        open_masks();
 #pragma omp parallel for schedule(static)
         for (long long i = 0; i < (long long)n; ++i) {
             const R rd = rho_dot(i);
             rho_adv[i] = rd;
-            const R s = o_fmax(R(0), rd);
+            const R s = relu(i, rd);
             kappa_v[i] = s * alpha[i] / dtr;
         }
+        close_masks();
     }
 
     // dfsph_kernels.cu:237-301 (pressure_form 1 replaces kappa/rho^2 by kappa/rho)
@@ -578,24 +625,53 @@ template <class R> struct Solver {
                 loz = R(cfg.domain_min[2]) + eps;
         const R hix = R(cfg.domain_max[0]) - eps, hiy = R(cfg.domain_max[1]) - eps,
                 hiz = R(cfg.domain_max[2]) - eps;
+        const uint16_t* cin = nullptr;
+        uint16_t* cout = nullptr;
+        if (tape_mode == 1) {
+            clamp_buf.assign(n, 0);
+            cout = clamp_buf.data();
+        } else if (tape_mode == 2 && tape_clamp < tape->clamps.size()) {
+            cin = tape->clamps[tape_clamp++].data();
+        }
+        const R lo[3] = {lox, loy, loz}, hi[3] = {hix, hiy, hiz};
 #pragma omp parallel for schedule(static)
         for (long long i = 0; i < (long long)n; ++i) {
             V3<R> vv = v[i];
+            R* vc[3] = {&vv.x, &vv.y, &vv.z};
+            uint16_t f = 0;
             const R sp = o_sqrt(vv.x * vv.x + vv.y * vv.y + vv.z * vv.z);
-            if (sp > vmax) {
+            if (cin ? (cin[i] & kClampSpeed) : (sp > vmax)) {
                 const R s = vmax / sp;
                 vv.x *= s; vv.y *= s; vv.z *= s;
+                f |= kClampSpeed;
             }
             V3<R> p{x[i].x + vv.x * dtr, x[i].y + vv.y * dtr, x[i].z + vv.z * dtr};
-            if (p.x < lox) { p.x = lox; if (vv.x < 0) vv.x = 0; }
-            if (p.y < loy) { p.y = loy; if (vv.y < 0) vv.y = 0; }
-            if (p.z < loz) { p.z = loz; if (vv.z < 0) vv.z = 0; }
-            if (p.x > hix) { p.x = hix; if (vv.x > 0) vv.x = 0; }
-            if (p.y > hiy) { p.y = hiy; if (vv.y > 0) vv.y = 0; }
-            if (p.z > hiz) { p.z = hiz; if (vv.z > 0) vv.z = 0; }
+            R* pc[3] = {&p.x, &p.y, &p.z};
+            for (int a = 0; a < 3; ++a) {
+                if (cin ? (cin[i] & (kClampLo << a)) : (*pc[a] < lo[a])) {
+                    *pc[a] = lo[a];
+                    f |= kClampLo << a;
+                    if (cin ? (cin[i] & (kZeroLo << a)) : (*vc[a] < R(0))) {
+                        *vc[a] = R(0);
+                        f |= kZeroLo << a;
+                    }
+                }
+            }
+            for (int a = 0; a < 3; ++a) {
+                if (cin ? (cin[i] & (kClampHi << a)) : (*pc[a] > hi[a])) {
+                    *pc[a] = hi[a];
+                    f |= kClampHi << a;
+                    if (cin ? (cin[i] & (kZeroHi << a)) : (*vc[a] > R(0))) {
+                        *vc[a] = R(0);
+                        f |= kZeroHi << a;
+                    }
+                }
+            }
+            if (cout) cout[i] = f;
             v[i] = vv;
             x[i] = p;
         }
+        if (tape_mode == 1) tape->clamps.push_back(clamp_buf);
     }
 
     // dfsph_kernels.cu:396-400 + CUB max (dfsph.cu:303-326)
@@ -933,6 +1009,8 @@ void orc_solver_set_tape(void* s, void* tape, int mode) {
         S->tape_mode = tape ? mode : 0;
         S->tape_home = 0;
         S->tape_iter = 0;
+        S->tape_mask = 0;
+        S->tape_clamp = 0;
     });
 }
 

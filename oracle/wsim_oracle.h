@@ -143,10 +143,11 @@ void  orc_solver_stats(void* s, orc_stats* out);
  * viscosity. */
 void  orc_solver_seed(void* s, const double* dx, const double* dv, double d_viscosity);
 void  orc_solver_get_tangent(void* s, int field, double* out); /* X, V, RHO, ALPHA, KAPPA, RHO_ADV */
-/* Frozen structure: mode 1 records every grid build's home cells and every
- * solve's iteration count; mode 2 replays a recorded tape (from its start),
- * so perturbed runs of any precision keep the recording run's neighbour sets
- * and iteration counts. */
+/* Frozen structure: mode 1 records every grid build's home cells, every
+ * solve's iteration count, every sweep's max(0, .) branches and every
+ * advection's clamp decisions; mode 2 replays a recorded tape (from its
+ * start), so perturbed runs of any precision take the recording run's
+ * branches -- the piecewise-smooth map the adjoint differentiates. */
 void* orc_tape_create(void);
 void  orc_tape_destroy(void* tape);
 size_t orc_tape_builds(void* tape);
