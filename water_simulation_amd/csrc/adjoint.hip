@@ -115,13 +115,17 @@ __device__ __forceinline__ void for_pairs(const Nbr& P, const PairList& L, const
     for_stencil(P.gf, P.csf + G.key_base, G.off - G.start, p.x, p.y, p.z, [&](u32 s) {
         if (s == i) return;
         const float4 q = xs[s];
-        const float c = spline_gradc(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
+        const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+        if (r2 >= P.sp.far2) return;
+        const float c = spline_gradc(sqrtf(r2), P.sp);
         if (c != 0.0f) ff(s, c);
     });
     if (P.nb) {
         for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
             const float4 q = P.xb[s];
-            const float c = spline_gradc(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
+            const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+            if (r2 >= P.sp.far2) return;
+            const float c = spline_gradc(sqrtf(r2), P.sp);
             if (c != 0.0f) fb(s, c);
         });
     }
@@ -387,7 +391,9 @@ __global__ __launch_bounds__(NA) void k_adj_xsph(Nbr P, const float4* __restrict
             for_stencil(P.gf, P.csf + G.key_base, G.off - G.start, p.x, p.y, p.z, [&](u32 s) {
                 if (s == i) return;
                 const float4 q = xs[s];
-                const float w = spline_W(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
+                const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+                if (r2 >= P.sp.far2) return;
+                const float w = spline_W(sqrtf(r2), P.sp);
                 if (w == 0.0f) return;
                 const float avg = 0.5f * (rho_i + rho[s]);
                 const float am = fmaxf(avg, 1e-9f);

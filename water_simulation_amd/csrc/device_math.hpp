@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 
 namespace wsim {
@@ -60,12 +61,22 @@ struct Spline {
     float sigma;
     float sigma_m075;  // sigma * (-0.75f), the left product of sph_kernels.cuh:37
     float sigma_025;   // sigma * 0.25f, the left product of sph_kernels.cuh:20
+    // Smallest squared distance r2 with sqrtf(r2) / l >= 2 in binary32: every
+    // r2 >= far2 has W = 0 and grad W = 0 (sqrtf and the IEEE quotient are
+    // monotone), so stencil candidates that far are skipped without the
+    // square root and divisions.  Their terms would add +-0 to sums that
+    // never hold -0, so skipping them changes no bit.
+    float far2;
     static Spline make(float l) {
         Spline s;
         s.l = l;
         s.sigma = 1.0f / (3.14159265358979323846f * l * l * l);
         s.sigma_m075 = s.sigma * (-0.75f);
         s.sigma_025 = s.sigma * 0.25f;
+        // start a little below (2l)^2 and walk up ulp by ulp
+        float x = (2.0f * l) * (2.0f * l) * 0.99999f;
+        for (int k = 0; k < 4096 && !(sqrtf(x) / l >= 2.0f); ++k) x = nextafterf(x, 3.0e38f);
+        s.far2 = (sqrtf(x) / l >= 2.0f) ? x : 3.0e38f;
         return s;
     }
 };
@@ -95,7 +106,29 @@ __device__ inline float spline_gradc(float d, const Spline& sp) {
     return dWdq / (sp.l * d);
 }
 
+// W and the gradient factor for a candidate known to lie inside the support
+// (r2 < Spline::far2, so q < 2): both polynomial pieces are evaluated and one
+// is selected, which gives the same bits as spline_W / spline_gradc without
+// their branches.
+__device__ inline float spline_W_near(float d, const Spline& sp) {
+    const float q = d / sp.l;
+    const float inner = sp.sigma * (1.0f - 1.5f * q * q + 0.75f * q * q * q);
+    const float t = 2.0f - q;
+    const float outer = sp.sigma_025 * t * t * t;
+    return q < 1.0f ? inner : outer;
+}
+__device__ inline float spline_gradc_near(float d, const Spline& sp) {
+    const float q = d / sp.l;
+    const float inner = sp.sigma * (-3.0f * q + 2.25f * q * q);
+    const float t = 2.0f - q;
+    const float outer = sp.sigma_m075 * t * t;
+    const float c = (q < 1.0f ? inner : outer) / (sp.l * d);
+    return d < 1e-12f ? 0.0f : c;
+}
+
 __device__ inline float norm3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
+// The radicand of norm3, in the same operation order.
+__device__ inline float dist2(float x, float y, float z) { return x * x + y * y + z * z; }
 
 // Workgroup id remap so that each XCD walks a contiguous run of tiles: the
 // dispatcher hands workgroup b to XCD (b % 8), so neighbouring tiles (which

@@ -9,7 +9,8 @@
 // Per digit pass (reduce-then-scan):
 //   rs_upsweep   -- per tile digit histogram; same-digit lanes of a wave are
 //                   grouped with ballots and counted by one LDS atomic;
-//   rs_scan      -- one workgroup scans the digit-major histogram matrix;
+//   rs_scan_rows -- one workgroup per digit scans that digit's per-tile counts
+//                   and records the digit total;
 //   rs_downsweep -- each wave ranks its keys stably (ballot match + per-wave
 //                   LDS counters), the tile is reordered in LDS, and written
 //                   out digit-run by digit-run so consecutive lanes store to
@@ -25,7 +26,7 @@ constexpr int RS_WAVES = RS_THREADS / kWave;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_WAVE_TILE = kWave * RS_ITEMS;      // 1024 keys per wave
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;      // 4096 keys per workgroup
-constexpr int RS_SCAN_THREADS = 1024;
+constexpr int RS_ROW_PER = 4;  // tiles per thread and round of the row scan
 static_assert(RS_THREADS == 256, "one digit per thread in the digit scans");
 
 // Lanes of the wave whose digit equals this lane's digit (nbits <= 8).
@@ -77,49 +78,45 @@ __global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const u32* __restrict__
     if (static_cast<u32>(tid) <= dmask) hist[static_cast<u64>(tid) * nblocks + blockIdx.x] = cnt[tid];
 }
 
-// Exclusive scan, in place, of len u32 (digit-major histogram matrix).
-__global__ __launch_bounds__(RS_SCAN_THREADS) void rs_scan(u32* __restrict__ data, u32 len) {
-    __shared__ u32 s_w[RS_SCAN_THREADS / 64];
+// Row scan of the digit-major histogram matrix: workgroup d turns row d (the
+// per-tile counts of digit d) into row-local exclusive offsets, in place, and
+// writes the digit's total.  Each thread owns up to RS_ROW_PER consecutive
+// tiles of the row; rows longer than RS_THREADS * RS_ROW_PER loop with a
+// carry.  The downsweep adds the exclusive scan of the digit totals.
+__global__ __launch_bounds__(RS_THREADS) void rs_scan_rows(u32* __restrict__ hist, u32 nblocks,
+                                                           u32* __restrict__ totals) {
+    __shared__ u32 s_tmp[RS_WAVES];
     __shared__ u32 s_carry;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    u32* row = hist + static_cast<u64>(blockIdx.x) * nblocks;
+    const int tid = threadIdx.x;
     if (tid == 0) s_carry = 0;
     __syncthreads();
-    for (u32 base = 0; base < len; base += RS_SCAN_THREADS * 4) {
-        u32 v[4];
+    for (u32 base = 0; base < nblocks; base += RS_THREADS * RS_ROW_PER) {
+        const u32 first = base + static_cast<u32>(tid) * RS_ROW_PER;
+        u32 v[RS_ROW_PER];
         u32 t = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const u32 idx = base + tid * 4 + k;
-            v[k] = idx < len ? data[idx] : 0u;
+        for (int k = 0; k < RS_ROW_PER; ++k) {
+            v[k] = first + k < nblocks ? row[first + k] : 0u;
             t += v[k];
         }
-        u32 x = t;
+        u32 run = s_carry + block_exclusive_scan256(t, s_tmp);
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const u32 y = __shfl_up(x, o);
-            if (lane >= o) x += y;
-        }
-        if (lane == 63) s_w[w] = x;
-        __syncthreads();
-        u32 off = s_carry;
-        for (int i = 0; i < w; ++i) off += s_w[i];
-        u32 run = off + x - t;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const u32 idx = base + tid * 4 + k;
-            if (idx < len) data[idx] = run;
+        for (int k = 0; k < RS_ROW_PER; ++k) {
+            if (first + k < nblocks) row[first + k] = run;
             run += v[k];
         }
         __syncthreads();
-        if (tid == RS_SCAN_THREADS - 1) s_carry = run;
+        if (tid == RS_THREADS - 1) s_carry = run;
         __syncthreads();
     }
+    if (tid == 0) totals[blockIdx.x] = s_carry;
 }
 
 __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
     const u32* __restrict__ kin, const u32* __restrict__ vin, u32* __restrict__ kout,
     u32* __restrict__ vout, u32 n, int shift, int nbits, const u32* __restrict__ offs,
-    u32 nblocks) {
+    u32 nblocks, const u32* __restrict__ totals) {
     __shared__ u32 s_wc[RS_WAVES][256];
     __shared__ u32 s_start[256];
     __shared__ u32 s_goff[256];
@@ -175,6 +172,10 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
         }
         s_goff[d] = (static_cast<u32>(d) <= dmask) ? offs[static_cast<u64>(d) * nblocks + blockIdx.x] : 0u;
     }
+    // digit base: exclusive scan of the digit totals over the whole input
+    const u32 dtot = (static_cast<u32>(tid) <= dmask) ? totals[tid] : 0u;
+    const u32 dbase = block_exclusive_scan256(dtot, s_tmp);
+    s_goff[tid] += dbase;
     const u32 start = block_exclusive_scan256(total, s_tmp);
     s_start[tid] = start;
     __syncthreads();
@@ -203,7 +204,10 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
 
 }  // namespace
 
-size_t radix_hist_elems(size_t n) { return static_cast<size_t>(256) * div_up(n ? n : 1, RS_TILE); }
+// Per-tile digit counts plus the 256 digit totals.
+size_t radix_hist_elems(size_t n) {
+    return static_cast<size_t>(256) * div_up(n ? n : 1, RS_TILE) + 256;
+}
 
 int radix_sort_pairs(u32* k0, u32* v0, u32* k1, u32* v1, size_t n, int bits, u32* hist,
                      hipStream_t stream) {
@@ -218,13 +222,13 @@ int radix_sort_pairs(u32* k0, u32* v0, u32* k1, u32* v1, size_t n, int bits, u32
     int where = 0;
     for (int shift = 0; shift < bits; shift += 8) {
         const int nbits = (bits - shift) < 8 ? (bits - shift) : 8;
-        const u32 len = (1u << nbits) * nblocks;
         rs_upsweep<<<nblocks, RS_THREADS, 0, stream>>>(kin, nn, shift, nbits, hist, nblocks);
         WSIM_LAUNCH_CHECK();
-        rs_scan<<<1, RS_SCAN_THREADS, 0, stream>>>(hist, len);
+        u32* totals = hist + static_cast<size_t>(256) * nblocks;
+        rs_scan_rows<<<1u << nbits, RS_THREADS, 0, stream>>>(hist, nblocks, totals);
         WSIM_LAUNCH_CHECK();
         rs_downsweep<<<nblocks, RS_THREADS, 0, stream>>>(kin, vin, kout, vout, nn, shift, nbits,
-                                                         hist, nblocks);
+                                                         hist, nblocks, totals);
         WSIM_LAUNCH_CHECK();
         u32* tk = kin;
         u32* tv = vin;

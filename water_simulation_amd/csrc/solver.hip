@@ -93,7 +93,9 @@ __global__ __launch_bounds__(NS) void k_boundary_mass(GridGeom g, const u32* __r
     float sum_w = 0.0f;
     for_stencil(g, cs, 0u, p.x, p.y, p.z, [&](u32 s) {
         const float4 q = xb[s];
-        sum_w += spline_W(norm3(p.x - q.x, p.y - q.y, p.z - q.z), sp);
+        const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+        if (r2 >= sp.far2) return;  // W = 0
+        sum_w += spline_W_near(sqrtf(r2), sp);
     });
     mb[i] = rho0 / fmaxf(sum_w, 1e-9f);
 }
@@ -129,31 +131,41 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
     const float4 p = xs[i];
     float rho = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f, sq = 0.0f;
     u32 cf = 0, cb = 0;
+    bool unlisted = false;  // an in-support fluid pair with a zero gradient
     for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
         const float4 q = xs[s];
         const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-        const float d = norm3(rx, ry, rz);
-        rho += P.mass * spline_W(d, P.sp);
-        if (s != i) {
-            const float c = spline_gradc(d, P.sp);
-            const float mx = P.mass * (rx * c), my = P.mass * (ry * c), mz = P.mass * (rz * c);
-            sx += mx;
-            sy += my;
-            sz += mz;
-            sq += mx * mx + my * my + mz * mz;
-            if (c != 0.0f) {
-                if (cf < L.kf) L.f[pair_at(cf, i, L.kf)] = make_uint2(s, __float_as_uint(c));
-                ++cf;
-            }
+        const float r2 = dist2(rx, ry, rz);
+        if (r2 >= P.sp.far2) return;  // W = 0 and c = 0
+        const float d = sqrtf(r2);
+        rho += P.mass * spline_W_near(d, P.sp);
+        // the self pair (d = 0) has c = 0: its terms add +0, and it is not listed
+        const float c = spline_gradc_near(d, P.sp);
+        const float mx = P.mass * (rx * c), my = P.mass * (ry * c), mz = P.mass * (rz * c);
+        sx += mx;
+        sy += my;
+        sz += mz;
+        sq += mx * mx + my * my + mz * mz;
+        if (c != 0.0f) {
+            if (cf < L.kf) L.f[pair_at(cf, i, L.kf)] = make_uint2(s, __float_as_uint(c));
+            ++cf;
+        } else if (s != i) {
+            unlisted = true;
         }
     });
+    // XSPH sums over every in-support pair (W != 0) and reads the fluid list;
+    // a pair the list leaves out (coincident particles) sends this particle
+    // to the stencil walk in every list kernel
+    if (unlisted) cf = 0xFFFFu;
     if (P.nb) {
         for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
             const float4 q = P.xb[s];
             const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-            const float d = norm3(rx, ry, rz);
-            rho += q.w * spline_W(d, P.sp);
-            const float c = spline_gradc(d, P.sp);
+            const float r2 = dist2(rx, ry, rz);
+            if (r2 >= P.sp.far2) return;
+            const float d = sqrtf(r2);
+            rho += q.w * spline_W_near(d, P.sp);
+            const float c = spline_gradc_near(d, P.sp);
             const float mx = q.w * (rx * c), my = q.w * (ry * c), mz = q.w * (rz * c);
             sx += mx;
             sy += my;
@@ -171,10 +183,26 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
     alpha_out[i] = (denom > 1e-9f) ? (rho / denom) : 0.0f;
 }
 
+// One XSPH pair term (dfsph_kernels.cu:326-337).
+__device__ __forceinline__ void xsph_term(float& dx, float& dy, float& dz, const float4& p,
+                                          const float4& vi, float rho_i, const float4& q,
+                                          const float4& vj, float rho_j, float eps,
+                                          const Nbr& P) {
+    const float w = spline_W_near(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
+    const float rho_avg = 0.5f * (rho_i + rho_j);
+    const float c = eps * (P.mass / fmaxf(rho_avg, 1e-9f)) * w;
+    dx += c * (vj.x - vi.x);
+    dy += c * (vj.y - vi.y);
+    dz += c * (vj.z - vi.z);
+}
+
 // XSPH as a Jacobi sweep: every neighbour velocity is the pre-XSPH one.  A
 // scene without viscosity copies its velocities (dfsph.cu:198 skips XSPH).
-template <int NT>
-__global__ __launch_bounds__(NT) void k_xsph(Nbr P, const float4* __restrict__ xs,
+// The fluid pair list holds every in-support pair in stencil order (pairs
+// outside the support add +-0); PU pairs per trip have their loads issued
+// together.  Particles without a complete list walk the stencil.
+template <int NT, int PU>
+__global__ __launch_bounds__(NT) void k_xsph(Nbr P, PairList L, const float4* __restrict__ xs,
                                              const float4* __restrict__ vin,
                                              const float* __restrict__ rho,
                                              float4* __restrict__ vout) {
@@ -188,16 +216,43 @@ __global__ __launch_bounds__(NT) void k_xsph(Nbr P, const float4* __restrict__ x
         vout[i] = vi;
         return;
     }
-    const u32* cs = P.csf + G.key_base;
-    const u32 shift = G.off - G.start;
     const float4 p = xs[i];
     const float rho_i = rho[i];
     float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+    const u32 cf = L.cnt[i] & 0xFFFFu;
+    if (cf <= L.kf) {
+        u32 k = 0;
+        for (; k + PU <= cf; k += PU) {
+            u32 j[PU];
+            float4 q[PU], vj[PU];
+            float rj[PU];
+#pragma unroll
+            for (int u = 0; u < PU; ++u) j[u] = ld_pair(L.f + pair_at(k + u, i, L.kf)).x;
+#pragma unroll
+            for (int u = 0; u < PU; ++u) {
+                q[u] = xs[j[u]];
+                vj[u] = vin[j[u]];
+                rj[u] = rho[j[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < PU; ++u) xsph_term(dx, dy, dz, p, vi, rho_i, q[u], vj[u], rj[u], eps, P);
+        }
+        for (; k < cf; ++k) {
+            const u32 j = ld_pair(L.f + pair_at(k, i, L.kf)).x;
+            xsph_term(dx, dy, dz, p, vi, rho_i, xs[j], vin[j], rho[j], eps, P);
+        }
+        vout[i] = make_float4(vi.x + dx, vi.y + dy, vi.z + dz, 0.0f);
+        return;
+    }
+    const u32* cs = P.csf + G.key_base;
+    const u32 shift = G.off - G.start;
     for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
         if (s == i) return;
         const float4 q = xs[s];
+        const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+        if (r2 >= P.sp.far2) return;  // W = 0
         const float4 vj = vin[s];
-        const float w = spline_W(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
+        const float w = spline_W_near(sqrtf(r2), P.sp);
         const float rho_avg = 0.5f * (rho_i + rho[s]);
         const float c = eps * (P.mass / fmaxf(rho_avg, 1e-9f)) * w;
         dx += c * (vj.x - vi.x);
@@ -296,13 +351,17 @@ __global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* 
                 for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
                     if (s == i) return;
                     const float4 q = xs[s];
-                    const float c = spline_gradc(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
+                    const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+                    if (r2 >= P.sp.far2) return;
+                    const float c = spline_gradc(sqrtf(r2), P.sp);
                     rho_dot_fluid(rd, p, vi, q, vs[s], c, P.mass);
                 });
                 if (P.nb) {
                     for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
                         const float4 q = P.xb[s];
-                        const float c = spline_gradc(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
+                        const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+                        if (r2 >= P.sp.far2) return;
+                        const float c = spline_gradc(sqrtf(r2), P.sp);
                         rho_dot_boundary(rd, p, vi, q, c);
                     });
                 }
@@ -439,13 +498,17 @@ __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* _
         for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
             if (s == i) return;
             const float4 q = xk[s];
-            const float c = spline_gradc(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
+            const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+            if (r2 >= P.sp.far2) return;
+            const float c = spline_gradc(sqrtf(r2), P.sp);
             kappa_term(dx, dy, dz, p, q, c, P.mass * (ki + q.w), dt);
         });
         if (P.nb) {
             for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
                 const float4 q = P.xb[s];
-                const float c = spline_gradc(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
+                const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+                if (r2 >= P.sp.far2) return;
+                const float c = spline_gradc(sqrtf(r2), P.sp);
                 kappa_term(dx, dy, dz, p, q, c, q.w * (ki + bf), dt);
             });
         }
@@ -621,13 +684,14 @@ void launch_density_alpha(u32 nt, hipStream_t s, u32 nslots, const Nbr& P, const
     WSIM_LAUNCH_CHECK();
 }
 
-void launch_xsph(u32 nt, hipStream_t s, u32 nslots, const Nbr& P, const float4* xs,
-                 const float4* vin, const float* rho, float4* vout) {
+void launch_xsph(u32 nt, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
+                 const float4* xs, const float4* vin, const float* rho, float4* vout) {
+    constexpr int XPU = 4;
     switch (nt) {
-    case 64: k_xsph<64><<<grid_for(nslots, 64), 64, 0, s>>>(P, xs, vin, rho, vout); break;
-    case 128: k_xsph<128><<<grid_for(nslots, 128), 128, 0, s>>>(P, xs, vin, rho, vout); break;
-    case 256: k_xsph<256><<<grid_for(nslots, 256), 256, 0, s>>>(P, xs, vin, rho, vout); break;
-    case 512: k_xsph<512><<<grid_for(nslots, 512), 512, 0, s>>>(P, xs, vin, rho, vout); break;
+    case 64: k_xsph<64, XPU><<<grid_for(nslots, 64), 64, 0, s>>>(P, L, xs, vin, rho, vout); break;
+    case 128: k_xsph<128, XPU><<<grid_for(nslots, 128), 128, 0, s>>>(P, L, xs, vin, rho, vout); break;
+    case 256: k_xsph<256, XPU><<<grid_for(nslots, 256), 256, 0, s>>>(P, L, xs, vin, rho, vout); break;
+    case 512: k_xsph<512, XPU><<<grid_for(nslots, 512), 512, 0, s>>>(P, L, xs, vin, rho, vout); break;
     default: fail(kInvalidArgument, "xsph workgroup size");
     }
     WSIM_LAUNCH_CHECK();
@@ -1105,8 +1169,8 @@ void DFSPHSolver::phase_prefix(bool gravity) {
     tend();
     if (xsph) {
         tbegin(kKXsph);
-        launch_xsph(shapes_.xsph_nt, stream_, nslots_, P, xs_.data(), vpre_.data(), rho_.data(),
-                    vs_.data());
+        launch_xsph(shapes_.xsph_nt, stream_, nslots_, P, pair_list(), xs_.data(), vpre_.data(),
+                    rho_.data(), vs_.data());
         tend();
     }
     k_reset_solve<<<div_up(scenes, NS), NS, 0, stream_>>>(dseg_.data(), dstate_.data(), scenes,
@@ -1197,11 +1261,15 @@ bool DFSPHSolver::all_converged(int mode) {
     return true;
 }
 
-// Iterations run in chunks of 2, 2, 4, ... 32 between host checks; inside a
-// chunk, scenes that converged turn the remaining launches into no-ops.
+// Iterations run in chunks between host checks; inside a chunk, scenes that
+// converged turn the remaining launches into no-ops.  The first chunk is the
+// previous substep's iteration count rounded up to a power of two (at least
+// 2 for the density solve, which tests from its second iteration on), then
+// chunks double up to 32.
 void DFSPHSolver::solve(int mode) {
     const u32 maxit = mode == 0 ? cfg_.max_density_iters : cfg_.max_divergence_iters;
     u32 it = 0, chunk = mode == 0 ? 2 : 1;
+    while (chunk < last_iters_[mode] && chunk < maxit) chunk <<= 1;
     while (it < maxit) {
         const u32 end = (maxit - it < chunk) ? maxit : it + chunk;
         run_phase(phase_key(mode == 0 ? kPhDensityIters : kPhDivIters, it, end),
@@ -1251,6 +1319,8 @@ void DFSPHSolver::read_state() {
         max_d = std::max(max_d, st.density_iters);
         max_v = std::max(max_v, st.divergence_iters);
     }
+    last_iters_[0] = max_d;
+    last_iters_[1] = max_v;
     if (timing_) {
         int valid[kKNumKinds];
         for (int k = 0; k < kKNumKinds; ++k) valid[k] = std::numeric_limits<int>::max();
@@ -1436,8 +1506,8 @@ void DFSPHSolver::autotune(int reps) {
             if (any_xsph_) {
                 for (u32 nt : {64u, 128u, 256u, 512u}) {
                     const double ms = time_ms([&] {
-                        launch_xsph(nt, stream_, nslots_, P, xs_.data(), vpre_.data(), rho_.data(),
-                                    vs_.data());
+                        launch_xsph(nt, stream_, nslots_, P, L, xs_.data(), vpre_.data(),
+                                    rho_.data(), vs_.data());
                     });
                     tune_log_.push_back({kKXsph, nt, 0, remap, 0, ms});
                     if (ms < b_xs) b_xs = ms, cand.xsph_nt = nt;

@@ -407,6 +407,7 @@ private:
 
     KernelShapes shapes_;
     std::vector<TuneRecord> tune_log_;
+    u32 last_iters_[2] = {0, 0};  // iterations of the previous substep's solves
     bool use_graphs_ = false, capturing_ = false;
     std::map<u64, hipGraphExec_t> graphs_;
 
