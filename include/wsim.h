@@ -185,9 +185,9 @@ wsim_status wsim_solver_kernel_time(wsim_solver solver, int kind, uint64_t* laun
 typedef struct {
     uint32_t density_alpha_nt;   /* workgroup size: 64, 128, 256, 512 */
     uint32_t xsph_nt;            /* 64, 128, 256, 512 */
-    uint32_t change_nt;          /* 256, 512, 1024 */
+    uint32_t change_nt;          /* 0 (neighbourhood staged in LDS, 256 threads), 256, 512, 1024 */
     uint32_t change_pu;          /* pairs per trip of the list loops: 2, 4, 8 */
-    uint32_t apply_nt;           /* 64, 128, 256, 512 */
+    uint32_t apply_nt;           /* 0 (staged in LDS), 64, 128, 256, 512 */
     uint32_t apply_pu;           /* 2, 4, 8 */
     uint32_t cell_cpt;           /* cells per thread of the cell_start pass: 4, 8, 16 */
     uint32_t remap;              /* 1: XCD-aware workgroup order */

@@ -102,9 +102,12 @@ __device__ __forceinline__ void for_pairs(const Nbr& P, const PairList& L, const
     const u32 cw = L.cnt[i];
     const u32 cf = cw & 0xFFFFu, cb = cw >> 16;
     if (cf <= L.kf && cb <= L.kb) {
+        const StageRanges R = L.plan[i / kTile];
         for (u32 k = 0; k < cf; ++k) {
-            const uint2 e = ld_pair(L.f + pair_at(k, i, L.kf));
-            ff(e.x, __uint_as_float(e.y));
+            u32 j;
+            float c;
+            fluid_entry(L, R, k, i, j, c);
+            ff(j, R.total ? pair_c(p, xs[j], P.sp) : c);
         }
         for (u32 k = 0; k < cb; ++k) {
             const uint2 e = ld_pair(L.b + pair_at(k, i, L.kb));
@@ -140,9 +143,12 @@ __device__ __forceinline__ void for_pairs_split(const Nbr& P, const PairList& L,
     const u32 cw = L.cnt[i];
     const u32 cf = cw & 0xFFFFu, cb = cw >> 16;
     if (cf <= L.kf && cb <= L.kb) {
+        const StageRanges R = L.plan[i / kTile];
         for (u32 k = g; k < cf; k += G) {
-            const uint2 e = ld_pair(L.f + pair_at(k, i, L.kf));
-            ff(e.x, __uint_as_float(e.y));
+            u32 j;
+            float c;
+            fluid_entry(L, R, k, i, j, c);
+            ff(j, R.total ? pair_c(p, xs[j], P.sp) : c);
         }
         for (u32 k = g; k < cb; k += G) {
             const uint2 e = ld_pair(L.b + pair_at(k, i, L.kb));

@@ -82,6 +82,15 @@ __global__ __launch_bounds__(NS) void k_gather_boundary(const u32* __restrict__ 
     xb[s] = make_float4(x[o], x[o + 1], x[o + 2], 0.0f);
 }
 
+// Staging plan of every 256-slot tile (one thread per tile), computed once
+// per grid build and read by the list builder and the iteration kernels.
+__global__ __launch_bounds__(NS) void k_stage_plan(Nbr P, StageRanges* __restrict__ plan) {
+    const u32 t = blockIdx.x * NS + threadIdx.x;
+    if (t >= P.nslots / kTile) return;
+    const u32 sc = P.tile_scene ? P.tile_scene[t] : 0u;
+    plan[t] = stage_ranges(P, P.seg[sc], t * kTile);
+}
+
 // Akinci boundary mass, in cell-sorted order (dfsph_kernels.cu:40-65).
 __global__ __launch_bounds__(NS) void k_boundary_mass(GridGeom g, const u32* __restrict__ cs,
                                                       const float4* __restrict__ xb, u32 nb,
@@ -132,6 +141,7 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
     float rho = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f, sq = 0.0f;
     u32 cf = 0, cb = 0;
     bool unlisted = false;  // an in-support fluid pair with a zero gradient
+    const StageRanges R = L.plan[i / kTile];
     for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
         const float4 q = xs[s];
         const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
@@ -147,7 +157,13 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
         sz += mz;
         sq += mx * mx + my * my + mz * mz;
         if (c != 0.0f) {
-            if (cf < L.kf) L.f[pair_at(cf, i, L.kf)] = make_uint2(s, __float_as_uint(c));
+            if (cf < L.kf) {
+                const u64 at = pair_at(cf, i, L.kf);
+                if (R.total)
+                    L.fi[at] = static_cast<uint16_t>(stage_index(R, s));
+                else
+                    L.f[at] = make_uint2(s, __float_as_uint(c));
+            }
             ++cf;
         } else if (s != i) {
             unlisted = true;
@@ -221,13 +237,15 @@ __global__ __launch_bounds__(NT) void k_xsph(Nbr P, PairList L, const float4* __
     float dx = 0.0f, dy = 0.0f, dz = 0.0f;
     const u32 cf = L.cnt[i] & 0xFFFFu;
     if (cf <= L.kf) {
+        const StageRanges R = L.plan[i / kTile];
+        float c_unused;
         u32 k = 0;
         for (; k + PU <= cf; k += PU) {
             u32 j[PU];
             float4 q[PU], vj[PU];
             float rj[PU];
 #pragma unroll
-            for (int u = 0; u < PU; ++u) j[u] = ld_pair(L.f + pair_at(k + u, i, L.kf)).x;
+            for (int u = 0; u < PU; ++u) fluid_entry(L, R, k + u, i, j[u], c_unused);
 #pragma unroll
             for (int u = 0; u < PU; ++u) {
                 q[u] = xs[j[u]];
@@ -238,7 +256,8 @@ __global__ __launch_bounds__(NT) void k_xsph(Nbr P, PairList L, const float4* __
             for (int u = 0; u < PU; ++u) xsph_term(dx, dy, dz, p, vi, rho_i, q[u], vj[u], rj[u], eps, P);
         }
         for (; k < cf; ++k) {
-            const u32 j = ld_pair(L.f + pair_at(k, i, L.kf)).x;
+            u32 j;
+            fluid_entry(L, R, k, i, j, c_unused);
             xsph_term(dx, dy, dz, p, vi, rho_i, xs[j], vin[j], rho[j], eps, P);
         }
         vout[i] = make_float4(vi.x + dx, vi.y + dy, vi.z + dz, 0.0f);
@@ -312,24 +331,30 @@ __global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* 
             const u32 cf = cw & 0xFFFFu, cb = cw >> 16;
             if (cf <= L.kf && cb <= L.kb) {
                 // PU pairs per trip: loads issued together, terms summed in order
+                const StageRanges R = L.plan[i / kTile];
                 u32 k = 0;
                 for (; k + PU <= cf; k += PU) {
-                    uint2 e[PU];
+                    u32 j[PU];
+                    float c[PU];
                     float4 q[PU], w[PU];
 #pragma unroll
-                    for (int u = 0; u < PU; ++u) e[u] = ld_pair(L.f + pair_at(k + u, i, L.kf));
+                    for (int u = 0; u < PU; ++u) fluid_entry(L, R, k + u, i, j[u], c[u]);
 #pragma unroll
                     for (int u = 0; u < PU; ++u) {
-                        q[u] = xs[e[u].x];
-                        w[u] = vs[e[u].x];
+                        q[u] = xs[j[u]];
+                        w[u] = vs[j[u]];
                     }
 #pragma unroll
                     for (int u = 0; u < PU; ++u)
-                        rho_dot_fluid(rd, p, vi, q[u], w[u], __uint_as_float(e[u].y), P.mass);
+                        rho_dot_fluid(rd, p, vi, q[u], w[u], R.total ? pair_c(p, q[u], P.sp) : c[u],
+                                      P.mass);
                 }
                 for (; k < cf; ++k) {
-                    const uint2 e = ld_pair(L.f + pair_at(k, i, L.kf));
-                    rho_dot_fluid(rd, p, vi, xs[e.x], vs[e.x], __uint_as_float(e.y), P.mass);
+                    u32 j;
+                    float c;
+                    fluid_entry(L, R, k, i, j, c);
+                    const float4 q = xs[j];
+                    rho_dot_fluid(rd, p, vi, q, vs[j], R.total ? pair_c(p, q, P.sp) : c, P.mass);
                 }
                 k = 0;
                 for (; k + PU <= cb; k += PU) {
@@ -393,6 +418,150 @@ __global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* 
     __syncthreads();
     if (lane == 0 && (w & 3) == 0 && active)
         part[i / kTile] = ((s_red[w] + s_red[w + 1]) + s_red[w + 2]) + s_red[w + 3];
+}
+
+// k_change with the tile's fluid neighbourhood staged in LDS (positions and
+// velocities); the same arithmetic in the same order as k_change.
+template <int MODE, int PU>
+__global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
+                                                     const float4* __restrict__ xs,
+                                                     const float4* __restrict__ vs,
+                                                     const float* __restrict__ rho,
+                                                     const float* __restrict__ alpha, float rho0,
+                                                     int pform, u32 iter, const DeviceState* st,
+                                                     double* __restrict__ part,
+                                                     float* __restrict__ rho_adv,
+                                                     float* __restrict__ kappa,
+                                                     float4* __restrict__ xk) {
+    __shared__ float4 s_x[kStageCap];
+    __shared__ float4 s_v[kStageCap];
+    __shared__ double s_red[kTile / 64];
+    const u32 tile = P.remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+    const u32 t0 = tile * kTile;
+    const u32 i = t0 + threadIdx.x;
+    const u32 sc = scene_of_slot(P, t0);
+    const u32 conv = MODE == 0 ? st[sc].conv_iter : st[sc].div_conv_iter;
+    if (conv < iter) return;  // the whole tile belongs to one scene
+    const SceneSeg& G = P.seg[sc];
+    const StageRanges R = L.plan[tile];
+    for (u32 e = threadIdx.x; e < R.total; e += kTile) {
+        const u32 src = stage_source(R, e);
+        s_x[e] = xs[src];
+        s_v[e] = vs[src];
+    }
+    __syncthreads();
+    const bool staged = R.total != 0;
+    double contrib = 0.0;
+    if (i - G.off < G.n) {
+        const u32* cs = P.csf + G.key_base;
+        const u32 shift = G.off - G.start;
+        const float4 p = xs[i];
+        const float4 vi = vs[i];
+        float rd = 0.0f;
+        const u32 cw = L.cnt[i];
+        const u32 cf = cw & 0xFFFFu, cb = cw >> 16;
+        if (cf <= L.kf && cb <= L.kb) {
+            u32 k = 0;
+            if (staged) {
+                // compact list: LDS entries only; c is recomputed from the staged
+                // positions with the expression that produced it
+                for (; k + PU <= cf; k += PU) {
+                    u32 a[PU];
+                    float4 q[PU], w[PU];
+#pragma unroll
+                    for (int u = 0; u < PU; ++u) a[u] = fluid_stage_entry(L, k + u, i);
+#pragma unroll
+                    for (int u = 0; u < PU; ++u) {
+                        q[u] = s_x[a[u]];
+                        w[u] = s_v[a[u]];
+                    }
+#pragma unroll
+                    for (int u = 0; u < PU; ++u) {
+                        rho_dot_fluid(rd, p, vi, q[u], w[u], pair_c(p, q[u], P.sp), P.mass);
+                    }
+                }
+                for (; k < cf; ++k) {
+                    const u32 a = fluid_stage_entry(L, k, i);
+                    const float4 q = s_x[a], w = s_v[a];
+                    rho_dot_fluid(rd, p, vi, q, w, pair_c(p, q, P.sp), P.mass);
+                }
+            } else {
+                for (; k + PU <= cf; k += PU) {
+                    uint2 e[PU];
+                    float4 q[PU], w[PU];
+#pragma unroll
+                    for (int u = 0; u < PU; ++u) e[u] = ld_pair(L.f + pair_at(k + u, i, L.kf));
+#pragma unroll
+                    for (int u = 0; u < PU; ++u) {
+                        q[u] = xs[e[u].x];
+                        w[u] = vs[e[u].x];
+                    }
+#pragma unroll
+                    for (int u = 0; u < PU; ++u)
+                        rho_dot_fluid(rd, p, vi, q[u], w[u], __uint_as_float(e[u].y), P.mass);
+                }
+                for (; k < cf; ++k) {
+                    const uint2 e = ld_pair(L.f + pair_at(k, i, L.kf));
+                    rho_dot_fluid(rd, p, vi, xs[e.x], vs[e.x], __uint_as_float(e.y), P.mass);
+                }
+            }
+            k = 0;
+            for (; k + PU <= cb; k += PU) {
+                uint2 e[PU];
+                float4 q[PU];
+#pragma unroll
+                for (int u = 0; u < PU; ++u) e[u] = ld_pair(L.b + pair_at(k + u, i, L.kb));
+#pragma unroll
+                for (int u = 0; u < PU; ++u) q[u] = P.xb[e[u].x];
+#pragma unroll
+                for (int u = 0; u < PU; ++u)
+                    rho_dot_boundary(rd, p, vi, q[u], __uint_as_float(e[u].y));
+            }
+            for (; k < cb; ++k) {
+                const uint2 e = ld_pair(L.b + pair_at(k, i, L.kb));
+                rho_dot_boundary(rd, p, vi, P.xb[e.x], __uint_as_float(e.y));
+            }
+        } else {
+            for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
+                if (s == i) return;
+                const float4 q = xs[s];
+                const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+                if (r2 >= P.sp.far2) return;
+                const float c = spline_gradc(sqrtf(r2), P.sp);
+                rho_dot_fluid(rd, p, vi, q, vs[s], c, P.mass);
+            });
+            if (P.nb) {
+                for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
+                    const float4 q = P.xb[s];
+                    const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+                    if (r2 >= P.sp.far2) return;
+                    const float c = spline_gradc(sqrtf(r2), P.sp);
+                    rho_dot_boundary(rd, p, vi, q, c);
+                });
+            }
+        }
+        const float rho_i = rho[i];
+        float kap, err;
+        if (MODE == 0) {
+            const float rho_pred = rho_i + G.dt * rd;
+            rho_adv[i] = rho_pred;
+            err = fmaxf(0.0f, rho_pred - rho0);
+            kap = err * alpha[i] / G.dt2;
+        } else {
+            rho_adv[i] = rd;
+            err = fmaxf(0.0f, rd);
+            kap = err * alpha[i] / G.dt;
+        }
+        kappa[i] = kap;
+        const float khat = pform ? kap / fmaxf(rho_i, 1e-9f) : kap / fmaxf(rho_i * rho_i, 1e-9f);
+        xk[i] = make_float4(p.x, p.y, p.z, khat);
+        contrib = static_cast<double>(err);
+    }
+    const double ws = wave_sum(contrib);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) s_red[w] = ws;
+    __syncthreads();
+    if (threadIdx.x == 0) part[tile] = ((s_red[0] + s_red[1]) + s_red[2]) + s_red[3];
 }
 
 // The first wave of each scene in k_apply(iter) sums the scene's tile
@@ -460,22 +629,146 @@ __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* _
     const u32 cw = L.cnt[i];
     const u32 cf = cw & 0xFFFFu, cb = cw >> 16;
     if (cf <= L.kf && cb <= L.kb) {
+        const StageRanges R = L.plan[i / kTile];
         u32 k = 0;
         for (; k + PU <= cf; k += PU) {
+            u32 j[PU];
+            float c[PU];
+            float4 q[PU];
+#pragma unroll
+            for (int u = 0; u < PU; ++u) fluid_entry(L, R, k + u, i, j[u], c[u]);
+#pragma unroll
+            for (int u = 0; u < PU; ++u) q[u] = xk[j[u]];
+#pragma unroll
+            for (int u = 0; u < PU; ++u)
+                kappa_term(dx, dy, dz, p, q[u], R.total ? pair_c(p, q[u], P.sp) : c[u],
+                           P.mass * (ki + q[u].w), dt);
+        }
+        for (; k < cf; ++k) {
+            u32 j;
+            float c;
+            fluid_entry(L, R, k, i, j, c);
+            const float4 q = xk[j];
+            kappa_term(dx, dy, dz, p, q, R.total ? pair_c(p, q, P.sp) : c, P.mass * (ki + q.w), dt);
+        }
+        k = 0;
+        for (; k + PU <= cb; k += PU) {
             uint2 e[PU];
             float4 q[PU];
 #pragma unroll
-            for (int u = 0; u < PU; ++u) e[u] = ld_pair(L.f + pair_at(k + u, i, L.kf));
+            for (int u = 0; u < PU; ++u) e[u] = ld_pair(L.b + pair_at(k + u, i, L.kb));
 #pragma unroll
-            for (int u = 0; u < PU; ++u) q[u] = xk[e[u].x];
+            for (int u = 0; u < PU; ++u) q[u] = P.xb[e[u].x];
 #pragma unroll
             for (int u = 0; u < PU; ++u)
-                kappa_term(dx, dy, dz, p, q[u], __uint_as_float(e[u].y), P.mass * (ki + q[u].w), dt);
+                kappa_term(dx, dy, dz, p, q[u], __uint_as_float(e[u].y), q[u].w * (ki + bf), dt);
         }
-        for (; k < cf; ++k) {
-            const uint2 e = ld_pair(L.f + pair_at(k, i, L.kf));
-            const float4 q = xk[e.x];
-            kappa_term(dx, dy, dz, p, q, __uint_as_float(e.y), P.mass * (ki + q.w), dt);
+        for (; k < cb; ++k) {
+            const uint2 e = ld_pair(L.b + pair_at(k, i, L.kb));
+            const float4 q = P.xb[e.x];
+            kappa_term(dx, dy, dz, p, q, __uint_as_float(e.y), q.w * (ki + bf), dt);
+        }
+    } else {
+        for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
+            if (s == i) return;
+            const float4 q = xk[s];
+            const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+            if (r2 >= P.sp.far2) return;
+            const float c = spline_gradc(sqrtf(r2), P.sp);
+            kappa_term(dx, dy, dz, p, q, c, P.mass * (ki + q.w), dt);
+        });
+        if (P.nb) {
+            for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
+                const float4 q = P.xb[s];
+                const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+                if (r2 >= P.sp.far2) return;
+                const float c = spline_gradc(sqrtf(r2), P.sp);
+                kappa_term(dx, dy, dz, p, q, c, q.w * (ki + bf), dt);
+            });
+        }
+    }
+    float4 v = vin[i];
+    v.x += dx;
+    v.y += dy;
+    v.z += dz;
+    vout[i] = v;
+}
+
+// k_apply with the tile's fluid (x, kappa-hat) neighbourhood staged in LDS.
+template <int PU>
+__global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
+                                                    const float4* __restrict__ xk,
+                                                    const float* __restrict__ kappa, float rho0,
+                                                    int pform, u32 iter, int mode, double thr,
+                                                    const double* __restrict__ part,
+                                                    DeviceState* st, const float4* vin,
+                                                    float4* vout) {
+    __shared__ float4 s_x[kStageCap];
+    const u32 tile = P.remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+    const u32 t0 = tile * kTile;
+    const u32 i = t0 + threadIdx.x;
+    const u32 sc = scene_of_slot(P, t0);
+    DeviceState* sst = st + sc;
+    const u32 conv = mode == 0 ? sst->conv_iter : sst->div_conv_iter;
+    if (conv < iter) return;  // tile-uniform
+    const SceneSeg& G = P.seg[sc];
+    const StageRanges R = L.plan[tile];
+    for (u32 e = threadIdx.x; e < R.total; e += kTile) s_x[e] = xk[stage_source(R, e)];
+    __syncthreads();
+    const bool staged = R.total != 0;
+    const u32 l = i - G.off;
+    if (l < 64) finish_error(mode, iter, thr, G, part, sst);
+    if (l >= G.n) return;
+    const u32* cs = P.csf + G.key_base;
+    const u32 shift = G.off - G.start;
+    const float dt = G.dt;
+    const float4 p = xk[i];
+    const float ki = p.w;
+    float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+    float bf = 0.0f;
+    if (P.nb) {
+        const float k_i = kappa[i];
+        bf = pform ? k_i / fmaxf(rho0, 1e-9f) : k_i / fmaxf(rho0 * rho0, 1e-9f);
+    }
+    const u32 cw = L.cnt[i];
+    const u32 cf = cw & 0xFFFFu, cb = cw >> 16;
+    if (cf <= L.kf && cb <= L.kb) {
+        u32 k = 0;
+        if (staged) {
+            for (; k + PU <= cf; k += PU) {
+                u32 a[PU];
+                float4 q[PU];
+#pragma unroll
+                for (int u = 0; u < PU; ++u) a[u] = fluid_stage_entry(L, k + u, i);
+#pragma unroll
+                for (int u = 0; u < PU; ++u) q[u] = s_x[a[u]];
+#pragma unroll
+                for (int u = 0; u < PU; ++u) {
+                    kappa_term(dx, dy, dz, p, q[u], pair_c(p, q[u], P.sp), P.mass * (ki + q[u].w),
+                               dt);
+                }
+            }
+            for (; k < cf; ++k) {
+                const float4 q = s_x[fluid_stage_entry(L, k, i)];
+                kappa_term(dx, dy, dz, p, q, pair_c(p, q, P.sp), P.mass * (ki + q.w), dt);
+            }
+        } else {
+            for (; k + PU <= cf; k += PU) {
+                uint2 e[PU];
+                float4 q[PU];
+#pragma unroll
+                for (int u = 0; u < PU; ++u) e[u] = ld_pair(L.f + pair_at(k + u, i, L.kf));
+#pragma unroll
+                for (int u = 0; u < PU; ++u) q[u] = xk[e[u].x];
+#pragma unroll
+                for (int u = 0; u < PU; ++u)
+                    kappa_term(dx, dy, dz, p, q[u], __uint_as_float(e[u].y), P.mass * (ki + q[u].w), dt);
+            }
+            for (; k < cf; ++k) {
+                const uint2 e = ld_pair(L.f + pair_at(k, i, L.kf));
+                const float4 q = xk[e.x];
+                kappa_term(dx, dy, dz, p, q, __uint_as_float(e.y), P.mass * (ki + q.w), dt);
+            }
         }
         k = 0;
         for (; k + PU <= cb; k += PU) {
@@ -730,9 +1023,27 @@ void change_pu(u32 pu, hipStream_t s, u32 nslots, const Nbr& P, const PairList& 
     }
 }
 
+template <int MODE, int PU>
+void change_staged(hipStream_t s, u32 nslots, const Nbr& P, const PairList& L, const ChangeArgs& a) {
+    k_change_st<MODE, PU><<<nslots / kTile, kTile, 0, s>>>(P, L, a.xs, a.vs, a.rho, a.alpha, a.rho0,
+                                                          a.pform, a.iter, a.st, a.part, a.rho_adv,
+                                                          a.kappa, a.xk);
+}
+
+// nt == 0 selects the LDS-staged kernel (one 256-slot tile per workgroup).
 template <int MODE>
 void launch_change(u32 nt, u32 pu, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
                    const ChangeArgs& a) {
+    if (nt == 0) {
+        switch (pu) {
+        case 2: change_staged<MODE, 2>(s, nslots, P, L, a); break;
+        case 4: change_staged<MODE, 4>(s, nslots, P, L, a); break;
+        case 8: change_staged<MODE, 8>(s, nslots, P, L, a); break;
+        default: fail(kInvalidArgument, "change pairs per trip");
+        }
+        WSIM_LAUNCH_CHECK();
+        return;
+    }
     switch (nt) {
     case 256: change_pu<MODE, 256>(pu, s, nslots, P, L, a); break;
     case 512: change_pu<MODE, 512>(pu, s, nslots, P, L, a); break;
@@ -774,8 +1085,25 @@ void apply_pu(u32 pu, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L
     }
 }
 
+template <int PU>
+void apply_staged(hipStream_t s, u32 nslots, const Nbr& P, const PairList& L, const ApplyArgs& a) {
+    k_apply_st<PU><<<nslots / kTile, kTile, 0, s>>>(P, L, a.xk, a.kappa, a.rho0, a.pform, a.iter,
+                                                    a.mode, a.thr, a.part, a.st, a.vin, a.vout);
+}
+
+// nt == 0 selects the LDS-staged kernel.
 void launch_apply(u32 nt, u32 pu, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
                   const ApplyArgs& a) {
+    if (nt == 0) {
+        switch (pu) {
+        case 2: apply_staged<2>(s, nslots, P, L, a); break;
+        case 4: apply_staged<4>(s, nslots, P, L, a); break;
+        case 8: apply_staged<8>(s, nslots, P, L, a); break;
+        default: fail(kInvalidArgument, "apply pairs per trip");
+        }
+        WSIM_LAUNCH_CHECK();
+        return;
+    }
     switch (nt) {
     case 64: apply_pu<64>(pu, s, nslots, P, L, a); break;
     case 128: apply_pu<128>(pu, s, nslots, P, L, a); break;
@@ -832,8 +1160,8 @@ bool valid_shapes(const KernelShapes& s) {
         return std::find(set.begin(), set.end(), v) != set.end();
     };
     return in(s.density_alpha_nt, {64, 128, 256, 512}) && in(s.xsph_nt, {64, 128, 256, 512}) &&
-           in(s.change_nt, {256, 512, 1024}) && in(s.change_pu, {2, 4, 8}) &&
-           in(s.apply_nt, {64, 128, 256, 512}) && in(s.apply_pu, {2, 4, 8}) &&
+           in(s.change_nt, {0, 256, 512, 1024}) && in(s.change_pu, {2, 4, 8}) &&
+           in(s.apply_nt, {0, 64, 128, 256, 512}) && in(s.apply_pu, {2, 4, 8}) &&
            in(s.cell_cpt, {4, 8, 16}) && in(s.remap, {0, 1});
 }
 
@@ -963,6 +1291,8 @@ void DFSPHSolver::init(const std::vector<u32>& counts, const std::vector<float>&
     pairs_f_ = DeviceBuffer<uint2>(static_cast<size_t>(kPairDepthFluid) * slots_cap);
     pairs_b_ = DeviceBuffer<uint2>(static_cast<size_t>(kPairDepthBoundary) * slots_cap);
     pair_cnt_ = DeviceBuffer<u32>(slots_cap);
+    pairs_fi_ = DeviceBuffer<uint16_t>(static_cast<size_t>(kPairDepthFluid) * slots_cap);
+    plan_ = DeviceBuffer<StageRanges>(slots_cap / kTile);
     part_d_ = DeviceBuffer<double>(slots_cap / kTile);
     part_f_ = DeviceBuffer<float>(slots_cap / 64);
     part_u_ = DeviceBuffer<u32>(slots_cap / 64);
@@ -1075,6 +1405,8 @@ PairList DFSPHSolver::pair_list() {
     L.f = pairs_f_.data();
     L.b = pairs_b_.data();
     L.cnt = pair_cnt_.data();
+    L.fi = pairs_fi_.data();
+    L.plan = plan_.data();
     L.kf = kPairDepthFluid;
     L.kb = nb_ ? kPairDepthBoundary : 0u;
     return L;
@@ -1163,6 +1495,8 @@ void DFSPHSolver::phase_prefix(bool gravity) {
                                                  gravity ? 1 : 0);
     WSIM_LAUNCH_CHECK();
     tend();
+    k_stage_plan<<<div_up(nslots_ / kTile, NS), NS, 0, stream_>>>(P, plan_.data());
+    WSIM_LAUNCH_CHECK();
     tbegin(kKDensityAlpha);
     launch_density_alpha(shapes_.density_alpha_nt, stream_, nslots_, P, pair_list(), xs_.data(),
                          rho_.data(), alpha_.data());
@@ -1515,14 +1849,14 @@ void DFSPHSolver::autotune(int reps) {
             } else {
                 b_xs = 0.0;
             }
-            for (u32 nt : {256u, 512u, 1024u})
+            for (u32 nt : {0u, 256u, 512u, 1024u})
                 for (u32 pu : {2u, 4u, 8u}) {
                     const double ms = time_ms(
                         [&] { launch_change<0>(nt, pu, stream_, nslots_, P, L, ca); });
                     tune_log_.push_back({kKDensityChange, nt, pu, remap, 0, ms});
                     if (ms < b_ch) b_ch = ms, cand.change_nt = nt, cand.change_pu = pu;
                 }
-            for (u32 nt : {64u, 128u, 256u, 512u})
+            for (u32 nt : {0u, 64u, 128u, 256u, 512u})
                 for (u32 pu : {2u, 4u, 8u}) {
                     const double ms =
                         time_ms([&] { launch_apply(nt, pu, stream_, nslots_, P, L, aa); });

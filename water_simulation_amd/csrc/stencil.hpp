@@ -26,6 +26,7 @@ struct Nbr {
     const SceneSeg* seg;
     const u32* tile_scene;  // scene of every tile; nullptr: one scene
     int remap;              // XCD-aware workgroup order
+    const u32* keys;        // fluid cell keys in sorted order (scene key base included)
 };
 
 // 27-cell stencil of ITER_NEIGHBORS (dfsph_kernels.cu:13-35): dz, dy, dx in
@@ -95,6 +96,97 @@ __device__ __forceinline__ bool finite3(float a, float b, float c) {
     return isfinite(a) && isfinite(b) && isfinite(c);
 }
 
+// ---- LDS-staged neighbourhoods ------------------------------------------
+// Keys grow fastest along z, so the stencil cells of a tile of consecutive
+// cell-sorted particles, whose keys span [kmin, kmax], lie for each x offset
+// dx in the key range [kmin + dx Dy Dz - Dz - 1, kmax + dx Dy Dz + Dz + 1]:
+// three contiguous slot ranges hold every fluid neighbour of the tile.  A
+// tile stages them in LDS when they are disjoint and fit; otherwise (sparse
+// tiles, particles outside the grid) it gathers from global memory.
+constexpr u32 kStageCap = 1536;  // staged fluid particles per 256-slot tile
+
+// (StageRanges is declared in wsim_internal.hpp; an empty range sits where
+// the previous one ends.)
+
+// Computed by one thread for the tile of working slots [t0, t0 + kTile).
+__device__ inline StageRanges stage_ranges(const Nbr& P, const SceneSeg& G, u32 t0) {
+    StageRanges R{0, 0, 0, 0, 0, 0};
+    const u32 l0 = t0 - G.off;
+    if (l0 >= G.n) return R;
+    const u32 l1 = (G.n - l0 < kTile ? G.n : l0 + kTile) - 1;
+    const u32 C = P.gf.C;
+    const u32 kmin = P.keys[G.start + l0] - G.key_base;
+    const u32 kmax = P.keys[G.start + l1] - G.key_base;
+    if (kmax >= C) return R;  // particles outside the grid: their stencils are elsewhere
+    const long long dz = P.gf.dz, dyz = static_cast<long long>(P.gf.dy) * P.gf.dz;
+    const u32* cs = P.csf + G.key_base;
+    const u32 shift = G.off - G.start;
+    u32 prev_end = 0;
+    bool overlap = false;
+    // slot range [s, e) of the stencil columns at x offset dx
+    auto range = [&](long long dx, u32& s, u32& e) {
+        long long klo = static_cast<long long>(kmin) + dx * dyz - dz - 1;
+        long long khi = static_cast<long long>(kmax) + dx * dyz + dz + 1;
+        if (klo < 0) klo = 0;
+        if (khi > static_cast<long long>(C) - 1) khi = static_cast<long long>(C) - 1;
+        s = e = prev_end;
+        if (klo <= khi) {
+            const u32 a = cs[klo] + shift, b = cs[khi + 1] + shift;
+            if (b > a) {
+                if (a < prev_end) overlap = true;
+                s = a;
+                e = b;
+                prev_end = b;
+            }
+        }
+    };
+    u32 e0, e1, e2;
+    range(-1, R.s0, e0);
+    range(0, R.s1, e1);
+    range(1, R.s2, e2);
+    if (overlap) return StageRanges{0, 0, 0, 0, 0, 0};
+    R.b1 = e0 - R.s0;
+    R.b2 = R.b1 + (e1 - R.s1);
+    const u32 total = R.b2 + (e2 - R.s2);
+    R.total = total <= kStageCap ? total : 0u;
+    return R;
+}
+
+// LDS entry of neighbour slot j (which lies in one of the staged ranges).
+__device__ __forceinline__ u32 stage_index(const StageRanges& R, u32 j) {
+    return j >= R.s2 ? R.b2 + (j - R.s2) : (j >= R.s1 ? R.b1 + (j - R.s1) : j - R.s0);
+}
+// Working slot of staged entry e.
+__device__ __forceinline__ u32 stage_source(const StageRanges& R, u32 e) {
+    return e >= R.b2 ? R.s2 + (e - R.b2) : (e >= R.b1 ? R.s1 + (e - R.b1) : R.s0 + e);
+}
+
+// Fluid list entry k of particle i.  A tile whose neighbourhood is staged
+// (plan total != 0) keeps only the 2-byte LDS entry of each neighbour, whose
+// working slot is stage_source(R, entry); other tiles keep (slot, bits of c).
+// Positions are frozen between the list build and advection, so c of a
+// staged entry is recomputed by pair_c with the expression that produced it
+// in the list build: the same binary32 value either way.  c is left 0 for a
+// staged entry.
+__device__ __forceinline__ void fluid_entry(const PairList& L, const StageRanges& R, u32 k, u32 i,
+                                            u32& j, float& c) {
+    const u64 at = pair_at(k, i, L.kf);
+    if (R.total) {
+        j = stage_source(R, __builtin_nontemporal_load(L.fi + at));
+        c = 0.0f;
+    } else {
+        const uint2 e = ld_pair(L.f + at);
+        j = e.x;
+        c = __uint_as_float(e.y);
+    }
+}
+__device__ __forceinline__ u32 fluid_stage_entry(const PairList& L, u32 k, u32 i) {
+    return __builtin_nontemporal_load(L.fi + pair_at(k, i, L.kf));
+}
+__device__ __forceinline__ float pair_c(const float4& p, const float4& q, const Spline& sp) {
+    return spline_gradc_near(sqrtf(dist2(p.x - q.x, p.y - q.y, p.z - q.z)), sp);
+}
+
 // Host-side assembly of the kernel inputs.
 inline Nbr make_nbr(const CellGrid& fg, const CellGrid* bg, const float4* xb, size_t nb,
                     const DFSPHConfig& cfg, float mass, u32 nslots, const SceneSeg* seg,
@@ -119,6 +211,7 @@ inline Nbr make_nbr(const CellGrid& fg, const CellGrid* bg, const float4* xb, si
     P.seg = seg;
     P.tile_scene = tile_scene;
     P.remap = static_cast<int>(remap);
+    P.keys = fg.sorted_keys();
     return P;
 }
 

@@ -227,12 +227,23 @@ private:
     int open_kind_ = 0, open_iter_ = 0;
 };
 
+// LDS staging plan of one 256-slot tile (see stencil.hpp): the tile's fluid
+// neighbours are the working slots [s0, s0 + b1), [s1, s1 + b2 - b1) and
+// [s2, s2 + total - b2), staged at LDS entries 0, b1 and b2.
+struct StageRanges {
+    u32 s0, s1, s2;  // first working slot of each range
+    u32 b1, b2;      // first LDS entry of ranges 1 and 2 (range 0 starts at 0)
+    u32 total;       // staged entries; 0: the tile gathers from global memory
+};
+
 // Per-particle pair lists of the pressure solves (see solver.hip).
 struct PairList {
-    uint2* f;   // fluid pairs (slot, bits of c), wave-blocked ELL
-    uint2* b;   // boundary pairs
-    u32* cnt;   // fluid count | boundary count << 16 (saturating)
-    u32 kf, kb; // list depths
+    uint2* f;            // fluid pairs (slot, bits of c), wave-blocked ELL
+    uint2* b;            // boundary pairs
+    u32* cnt;            // fluid count | boundary count << 16 (saturating)
+    uint16_t* fi;        // fluid pairs as LDS entries of the tile's staging plan (staged tiles)
+    const StageRanges* plan;  // one per 256-slot tile
+    u32 kf, kb;          // list depths
 };
 
 // Depth of the per-particle pair lists (pairs with a non-zero kernel
@@ -245,9 +256,9 @@ constexpr u32 kPairDepthBoundary = 48;
 struct KernelShapes {
     u32 density_alpha_nt = 256;  // workgroup size: 64, 128, 256, 512
     u32 xsph_nt = 256;           // 64, 128, 256, 512
-    u32 change_nt = 256;         // 256, 512, 1024
+    u32 change_nt = 0;           // 0 (LDS-staged neighbourhood, 256), 256, 512, 1024
     u32 change_pu = 4;           // pairs per trip of the list loops: 2, 4, 8
-    u32 apply_nt = 256;          // 64, 128, 256, 512
+    u32 apply_nt = 0;            // 0 (LDS-staged), 64, 128, 256, 512
     u32 apply_pu = 4;            // 2, 4, 8
     u32 cell_cpt = 16;           // cells per thread of the cell_start pass: 4, 8, 16
     u32 remap = 1;               // XCD-aware workgroup -> tile order
@@ -397,6 +408,8 @@ private:
     // cell-sorted working set (nslots_ entries, scenes tile-aligned)
     DeviceBuffer<float4> xs_, vs_, vpre_, xk_, xb_;
     DeviceBuffer<uint2> pairs_f_, pairs_b_;
+    DeviceBuffer<uint16_t> pairs_fi_;
+    DeviceBuffer<StageRanges> plan_;
     DeviceBuffer<u32> pair_cnt_;
     DeviceBuffer<float> rho_, alpha_, kappa_, rho_adv_;
     DeviceBuffer<double> part_d_;
