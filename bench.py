@@ -288,7 +288,8 @@ def adjoint_leg(substeps: int, every: int):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=500,
+                    help="timed substeps (the large dam break is quoted over 500)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n-side", type=int, default=100)
     ap.add_argument("--pressure-form", type=int, default=1,

@@ -813,37 +813,52 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
     vout[i] = v;
 }
 
-// Per-wave (max |v|, NaN count) partials; k_finish_vstats reduces them per scene.
-__device__ __forceinline__ void vstats_partial(u32 i, u32 nslots, float mag, u32 bad,
-                                               float* __restrict__ pmax, u32* __restrict__ pnan) {
-    const float m = wave_max(mag);
-    const u32 c = wave_sum_u32(bad);
-    if ((threadIdx.x & 63) == 0 && i < nslots) {
-        pmax[i >> 6] = m;
-        pnan[i >> 6] = c;
+// Advection of a scene runs once per substep, as soon as its density solve
+// has concluded: speculatively after an iteration chunk (force = 0: only
+// scenes whose error test passed) or unconditionally after the last chunk
+// (force = 1).  Scene-uniform.
+__device__ __forceinline__ bool advect_due(const DeviceState& S, int force) {
+    return !S.advected && (force || S.conv_iter != kNone);
+}
+
+// Per-tile (max |v|, NaN count) partials (one 256-slot tile per workgroup);
+// k_finish_vstats reduces them per scene.
+__device__ __forceinline__ void vstats_partial(float mag, u32 bad, float* __restrict__ pmax,
+                                               u32* __restrict__ pnan) {
+    __shared__ float s_f[NS / 64];
+    __shared__ u32 s_u[NS / 64];
+    const float m = block_max<NS>(mag, s_f);
+    const u32 c = block_sum_u32<NS>(bad, s_u);
+    if (threadIdx.x == 0) {
+        pmax[blockIdx.x] = m;
+        pnan[blockIdx.x] = c;
     }
 }
 
-// One workgroup per scene.
-__global__ __launch_bounds__(NS) void k_finish_vstats(const SceneSeg* __restrict__ seg,
+// One workgroup per scene.  guarded = 1: only scenes the advection just
+// processed (the advect_due test, evaluated before this kernel marks them).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_finish_vstats(const SceneSeg* __restrict__ seg,
                                                       const float* __restrict__ pmax,
                                                       const u32* __restrict__ pnan,
-                                                      DeviceState* st) {
-    __shared__ float s_f[NS / 64];
-    __shared__ u32 s_u[NS / 64];
+                                                      DeviceState* st, int guarded, int force) {
+    __shared__ float s_f[NT / 64];
+    __shared__ u32 s_u[NT / 64];
     const SceneSeg& G = seg[blockIdx.x];
-    const u32 w0 = G.off >> 6, nw = (G.n + 63u) >> 6;
+    DeviceState& S = st[blockIdx.x];
+    if (guarded && !advect_due(S, force)) return;  // block-uniform
     float m = 0.0f;
     u32 c = 0;
-    for (u32 w = threadIdx.x; w < nw; w += NS) {
-        m = fmaxf(m, pmax[w0 + w]);
-        c += pnan[w0 + w];
+    for (u32 t = threadIdx.x; t < G.ntiles; t += NT) {
+        m = fmaxf(m, pmax[G.tile0 + t]);
+        c += pnan[G.tile0 + t];
     }
-    m = block_max<NS>(m, s_f);
-    c = block_sum_u32<NS>(c, s_u);
+    m = block_max<NT>(m, s_f);
+    c = block_sum_u32<NT>(c, s_u);
     if (threadIdx.x == 0) {
-        st[blockIdx.x].max_v = m;
-        st[blockIdx.x].nan_count = c;
+        S.max_v = m;
+        S.nan_count = c;
+        if (guarded) S.advected = 1u;
     }
 }
 
@@ -867,13 +882,16 @@ __global__ __launch_bounds__(NS) void k_advect(Nbr P, const float4* __restrict__
                                                const float* __restrict__ alpha,
                                                const float* __restrict__ kappa,
                                                const float* __restrict__ rho_adv, int wb_solve,
+                                               const DeviceState* __restrict__ st, int force,
                                                float* __restrict__ pmax,
                                                u32* __restrict__ pnan) {
     const u32 i = blockIdx.x * NS + threadIdx.x;
     float mag = 0.0f;
     u32 bad = 0;
-    if (i < P.nslots) {
-        const SceneSeg& G = P.seg[scene_of_slot(P, i)];
+    {
+        const u32 sc = scene_of_slot(P, i);
+        if (!advect_due(st[sc], force)) return;  // the workgroup is one tile of one scene
+        const SceneSeg& G = P.seg[sc];
         const u32 l = i - G.off;
         if (l < G.n) {
             const float dt = G.dt;
@@ -911,7 +929,7 @@ __global__ __launch_bounds__(NS) void k_advect(Nbr P, const float4* __restrict__
             bad = finite3(px, py, pz) && finite3(v.x, v.y, v.z) ? 0u : 1u;
         }
     }
-    vstats_partial(i, P.nslots, mag, bad, pmax, pnan);
+    vstats_partial(mag, bad, pmax, pnan);
 }
 
 // End of the divergence solve: velocities and the recomputed attributes back
@@ -928,7 +946,7 @@ __global__ __launch_bounds__(NS) void k_div_writeback(Nbr P, const float4* __res
     const u32 i = blockIdx.x * NS + threadIdx.x;
     float mag = 0.0f;
     u32 bad = 0;
-    if (i < P.nslots) {
+    {
         const SceneSeg& G = P.seg[scene_of_slot(P, i)];
         const u32 l = i - G.off;
         if (l < G.n) {
@@ -949,7 +967,7 @@ __global__ __launch_bounds__(NS) void k_div_writeback(Nbr P, const float4* __res
             bad = finite3(x4.x, x4.y, x4.z) && finite3(v.x, v.y, v.z) ? 0u : 1u;
         }
     }
-    vstats_partial(i, P.nslots, mag, bad, pmax, pnan);
+    vstats_partial(mag, bad, pmax, pnan);
 }
 
 // Opens a solve: every non-empty scene unconverged, empty scenes done.
@@ -958,8 +976,12 @@ __global__ __launch_bounds__(NS) void k_reset_solve(const SceneSeg* __restrict__
     const u32 s = blockIdx.x * NS + threadIdx.x;
     if (s >= scenes) return;
     const u32 v = seg[s].n ? kNone : 0u;
-    if (which == 0) st[s].conv_iter = v;
-    else st[s].div_conv_iter = v;
+    if (which == 0) {
+        st[s].conv_iter = v;
+        st[s].advected = 0u;
+    } else {
+        st[s].div_conv_iter = v;
+    }
 }
 
 // ---- launch-shape dispatch ------------------------------------------------
@@ -1546,7 +1568,7 @@ static StoreOut store_out(ParticleStore& f, size_t hd, size_t ha, size_t hk, siz
     return out;
 }
 
-void DFSPHSolver::phase_advect() {
+void DFSPHSolver::phase_advect(bool force) {
     const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, nslots_,
                            dseg_.data(), batched_ ? tile_scene_.data() : nullptr, shapes_.remap);
     const StoreOut out = store_out(fluid_, h_density_, h_alpha_, h_kappa_, h_density_adv_);
@@ -1556,15 +1578,15 @@ void DFSPHSolver::phase_advect() {
         P, xs_.data(), vs_.data(), fluid_grid_.sorted_indices(), 5.0f, cfg_.domain_min[0] + eps,
         cfg_.domain_min[1] + eps, cfg_.domain_min[2] + eps, cfg_.domain_max[0] - eps,
         cfg_.domain_max[1] - eps, cfg_.domain_max[2] - eps, out, rho_.data(), alpha_.data(),
-        kappa_.data(), rho_adv_.data(), cfg_.max_density_iters > 0 ? 1 : 0, part_f_.data(),
-        part_u_.data());
+        kappa_.data(), rho_adv_.data(), cfg_.max_density_iters > 0 ? 1 : 0, dstate_.data(),
+        force ? 1 : 0, part_f_.data(), part_u_.data());
     WSIM_LAUNCH_CHECK();
     tend();
-    if (!cfg_.enable_divergence_solve) {
-        k_finish_vstats<<<static_cast<u32>(counts_.size()), NS, 0, stream_>>>(
-            dseg_.data(), part_f_.data(), part_u_.data(), dstate_.data());
-        WSIM_LAUNCH_CHECK();
-    }
+    // also marks the scenes advected (after the divergence solve, the
+    // writeback reports max |v| instead)
+    k_finish_vstats<1024><<<static_cast<u32>(counts_.size()), 1024, 0, stream_>>>(
+        dseg_.data(), part_f_.data(), part_u_.data(), dstate_.data(), 1, force ? 1 : 0);
+    WSIM_LAUNCH_CHECK();
 }
 
 void DFSPHSolver::phase_div_tail() {
@@ -1578,8 +1600,8 @@ void DFSPHSolver::phase_div_tail() {
         part_u_.data());
     WSIM_LAUNCH_CHECK();
     tend();
-    k_finish_vstats<<<static_cast<u32>(counts_.size()), NS, 0, stream_>>>(
-        dseg_.data(), part_f_.data(), part_u_.data(), dstate_.data());
+    k_finish_vstats<1024><<<static_cast<u32>(counts_.size()), 1024, 0, stream_>>>(
+        dseg_.data(), part_f_.data(), part_u_.data(), dstate_.data(), 0, 1);
     WSIM_LAUNCH_CHECK();
 }
 
@@ -1617,21 +1639,56 @@ void DFSPHSolver::solve(int mode) {
 
 void DFSPHSolver::run_substep() {
     run_phase(phase_key(kPhPrefix), [&] { phase_prefix(true); });
-    solve(0);
-    run_phase(phase_key(kPhAdvect), [&] { phase_advect(); });
     if (cfg_.enable_divergence_solve) {
+        solve(0);
+        run_phase(phase_key(kPhAdvect, 1), [&] { phase_advect(true); });
         run_phase(phase_key(kPhDivPrefix), [&] { phase_prefix(false); });
         solve(1);
         run_phase(phase_key(kPhDivTail), [&] { phase_div_tail(); });
+        read_state();
+        return;
     }
+    // Without the divergence solve the substep ends at advection, so each
+    // chunk is followed by a speculative advection of the scenes whose solve
+    // converged in it, and one state read both tests convergence and
+    // delivers the substep's statistics: one host round trip per substep
+    // when the first chunk suffices.
+    const u32 maxit = cfg_.max_density_iters;
+    u32 it = 0, chunk = 2;
+    while (chunk < last_iters_[0] && chunk < maxit) chunk <<= 1;
+    while (it < maxit) {
+        const u32 end = (maxit - it < chunk) ? maxit : it + chunk;
+        run_phase(phase_key(kPhDensityIters, it, end), [&] { phase_iterations(0, it, end); });
+        it = end;
+        if (it >= maxit) break;
+        run_phase(phase_key(kPhAdvect, 0), [&] { phase_advect(false); });
+        fetch_state();
+        bool all = true;
+        for (size_t s = 0; s < counts_.size(); ++s) all = all && hstate_[s].advected;
+        if (all) {
+            take_state();
+            return;
+        }
+        chunk = chunk * 2 > 32 ? 32 : chunk * 2;
+    }
+    run_phase(phase_key(kPhAdvect, 1), [&] { phase_advect(true); });
     read_state();
 }
 
 void DFSPHSolver::read_state() {
+    fetch_state();
+    take_state();
+}
+
+void DFSPHSolver::fetch_state() {
     const u32 scenes = static_cast<u32>(counts_.size());
     WSIM_HIP(hipMemcpyAsync(hstate_.data(), dstate_.data(), scenes * sizeof(DeviceState),
                             hipMemcpyDeviceToHost, stream_));
     WSIM_HIP(hipStreamSynchronize(stream_));
+}
+
+void DFSPHSolver::take_state() {
+    const u32 scenes = static_cast<u32>(counts_.size());
     u32 max_d = 0, max_v = 0;
     for (u32 s = 0; s < scenes; ++s) {
         const DeviceState& h = hstate_[s];
@@ -1750,7 +1807,7 @@ void DFSPHSolver::step_taped(float dt, SubstepTape& t) {
     if (k > 0)
         WSIM_HIP(hipMemcpyAsync(rho_adv_.data(), t.p.data() + (k - 1) * ns, ns * sizeof(float),
                                 hipMemcpyDeviceToDevice, stream_));
-    phase_advect();
+    phase_advect(true);
     read_state();
     t.dt = dt;
     t.iters = k;

@@ -182,6 +182,8 @@ struct DeviceState {
     float max_v;
     double err;         // mean error of the latest executed density iteration
     double div_err;     // same for the divergence solve
+    u32 advected;       // this substep's advection has run for the scene
+    u32 pad;
 };
 
 enum KernelKind : int {
@@ -365,7 +367,9 @@ private:
     void run_substep();
     void phase_prefix(bool gravity);
     void phase_iterations(int mode, u32 begin, u32 end);
-    void phase_advect();
+    void phase_advect(bool force);
+    void fetch_state();
+    void take_state();
     void phase_div_tail();
     void solve(int mode);
     bool all_converged(int mode);
