@@ -564,7 +564,7 @@ double DFSPHSolver::adjoint_step(const SubstepTape& t, const double* gx_out, con
     adj_.part.ensure(ns / kTile);
     if (adj_.hpart.size() < ns / kTile) adj_.hpart = PinnedBuffer<double>(ns / kTile);
 
-    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, nslots_,
+    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), bnear_.data(), nb_, cfg_, mass_, nslots_,
                            dseg_.data(), nullptr, shapes_.remap);
     const PairList L = pair_list();
     const u32 blocks = static_cast<u32>(ns / NA);

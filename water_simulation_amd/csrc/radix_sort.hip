@@ -23,9 +23,9 @@ namespace {
 
 constexpr int RS_THREADS = 256;
 constexpr int RS_WAVES = RS_THREADS / kWave;
-constexpr int RS_ITEMS = 16;
-constexpr int RS_WAVE_TILE = kWave * RS_ITEMS;      // 1024 keys per wave
-constexpr int RS_TILE = RS_THREADS * RS_ITEMS;      // 4096 keys per workgroup
+constexpr int RS_ITEMS = 4;
+constexpr int RS_WAVE_TILE = kWave * RS_ITEMS;      // 256 keys per wave
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;      // 1024 keys per workgroup
 constexpr int RS_ROW_PER = 4;  // tiles per thread and round of the row scan
 static_assert(RS_THREADS == 256, "one digit per thread in the digit scans");
 

@@ -84,6 +84,30 @@ __global__ __launch_bounds__(NS) void k_gather_boundary(const u32* __restrict__ 
 
 // Staging plan of every 256-slot tile (one thread per tile), computed once
 // per grid build and read by the list builder and the iteration kernels.
+// Boundary proximity mask: one bit per boundary-grid cell, set when a cell
+// of its 27-cell stencil is occupied.  Built once with the boundary grid.
+__global__ __launch_bounds__(NS) void k_boundary_near(GridGeom g, const u32* __restrict__ cs,
+                                                      u32* __restrict__ mask) {
+    const u32 k = blockIdx.x * NS + threadIdx.x;
+    bool near = false;
+    if (k < g.C) {
+        const u32 dyz = static_cast<u32>(g.dy) * static_cast<u32>(g.dz);
+        const int x = static_cast<int>(k / dyz);
+        const int y = static_cast<int>((k % dyz) / static_cast<u32>(g.dz));
+        const int z = static_cast<int>(k % static_cast<u32>(g.dz));
+        for (int a = -1; a <= 1; ++a)
+            for (int b = -1; b <= 1; ++b)
+                for (int c = -1; c <= 1; ++c) {
+                    const u32 n = g.key(x + a, y + b, z + c);
+                    if (n < g.C && cs[n + 1] > cs[n]) near = true;
+                }
+    }
+    const u64 bits = __ballot(near);
+    const u32 w = (blockIdx.x * NS + (threadIdx.x & ~63u)) >> 5;  // first word of this wave
+    if ((threadIdx.x & 63u) == 0) mask[w] = static_cast<u32>(bits);
+    if ((threadIdx.x & 63u) == 32) mask[w + 1] = static_cast<u32>(bits >> 32);
+}
+
 __global__ __launch_bounds__(NS) void k_stage_plan(Nbr P, StageRanges* __restrict__ plan) {
     const u32 t = blockIdx.x * NS + threadIdx.x;
     if (t >= P.nslots / kTile) return;
@@ -173,7 +197,7 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
     // a pair the list leaves out (coincident particles) sends this particle
     // to the stencil walk in every list kernel
     if (unlisted) cf = 0xFFFFu;
-    if (P.nb) {
+    if (P.nb && boundary_near(P.gb, P.bnear, p.x, p.y, p.z)) {
         for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
             const float4 q = P.xb[s];
             const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
@@ -1349,6 +1373,12 @@ void DFSPHSolver::init(const std::vector<u32>& counts, const std::vector<float>&
                 nb, boundary_grid_->sorted_indices(), mb.data(), xb_.data(),
                 static_cast<float*>(boundary_->attribute_data(h_bmass_)));
             WSIM_LAUNCH_CHECK();
+            const GridGeom bg = boundary_grid_->geom();
+            const u32 mblocks = div_up(bg.C, NS);
+            bnear_ = DeviceBuffer<u32>(static_cast<size_t>(mblocks) * (NS / 32));
+            k_boundary_near<<<mblocks, NS, 0, stream_>>>(bg, boundary_grid_->cell_start(),
+                                                         bnear_.data());
+            WSIM_LAUNCH_CHECK();
             WSIM_HIP(hipStreamSynchronize(stream_));
         }
     }
@@ -1507,7 +1537,7 @@ void DFSPHSolver::phase_prefix(bool gravity) {
     if (batched_) fluid_grid_.build_segmented(fluid_.positions(), total_, dstarts_.data(), scenes);
     else fluid_grid_.build(fluid_.positions(), total_);
     tend();
-    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, nslots_,
+    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), bnear_.data(), nb_, cfg_, mass_, nslots_,
                            dseg_.data(), batched_ ? tile_scene_.data() : nullptr, shapes_.remap);
     const bool xsph = gravity && any_xsph_;
     float4* vtarget = xsph ? vpre_.data() : vs_.data();
@@ -1535,7 +1565,7 @@ void DFSPHSolver::phase_prefix(bool gravity) {
 }
 
 void DFSPHSolver::phase_iterations(int mode, u32 begin, u32 end) {
-    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, nslots_,
+    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), bnear_.data(), nb_, cfg_, mass_, nslots_,
                            dseg_.data(), batched_ ? tile_scene_.data() : nullptr, shapes_.remap);
     const PairList L = pair_list();
     const double thr = mode == 0 ? static_cast<double>(cfg_.eta_density * cfg_.rest_density)
@@ -1569,7 +1599,7 @@ static StoreOut store_out(ParticleStore& f, size_t hd, size_t ha, size_t hk, siz
 }
 
 void DFSPHSolver::phase_advect(bool force) {
-    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, nslots_,
+    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), bnear_.data(), nb_, cfg_, mass_, nslots_,
                            dseg_.data(), batched_ ? tile_scene_.data() : nullptr, shapes_.remap);
     const StoreOut out = store_out(fluid_, h_density_, h_alpha_, h_kappa_, h_density_adv_);
     const float eps = 1e-3f;  // dfsph_kernels.cu:377
@@ -1590,7 +1620,7 @@ void DFSPHSolver::phase_advect(bool force) {
 }
 
 void DFSPHSolver::phase_div_tail() {
-    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, nslots_,
+    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), bnear_.data(), nb_, cfg_, mass_, nslots_,
                            dseg_.data(), batched_ ? tile_scene_.data() : nullptr, shapes_.remap);
     const StoreOut out = store_out(fluid_, h_density_, h_alpha_, h_kappa_v_, h_density_adv_);
     tbegin(kKDivWriteback);
@@ -1739,7 +1769,7 @@ float DFSPHSolver::attribute_at(int which, size_t i) const {
 // iteration reading its velocity from slab k of the tape and writing slab
 // k+1, and its (x, kappa-hat) and rho* into slab k.
 void DFSPHSolver::iterations_taped(u32 begin, u32 end, SubstepTape& t) {
-    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, nslots_,
+    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), bnear_.data(), nb_, cfg_, mass_, nslots_,
                            dseg_.data(), nullptr, shapes_.remap);
     const PairList L = pair_list();
     const double thr = static_cast<double>(cfg_.eta_density * cfg_.rest_density);
@@ -1860,7 +1890,7 @@ void DFSPHSolver::autotune(int reps) {
     const PairList L = pair_list();
     const u32* tsc = batched_ ? tile_scene_.data() : nullptr;
     auto nbr = [&](u32 remap) {
-        return make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), nb_, cfg_, mass_, nslots_,
+        return make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), bnear_.data(), nb_, cfg_, mass_, nslots_,
                         dseg_.data(), tsc, remap);
     };
     ChangeArgs ca{xs_.data(), vs_.data(), rho_.data(), alpha_.data(), cfg_.rest_density,

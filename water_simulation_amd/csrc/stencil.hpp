@@ -27,6 +27,7 @@ struct Nbr {
     const u32* tile_scene;  // scene of every tile; nullptr: one scene
     int remap;              // XCD-aware workgroup order
     const u32* keys;        // fluid cell keys in sorted order (scene key base included)
+    const u32* bnear;       // boundary proximity mask over the boundary grid, or nullptr
 };
 
 // 27-cell stencil of ITER_NEIGHBORS (dfsph_kernels.cu:13-35): dz, dy, dx in
@@ -54,6 +55,19 @@ __device__ __forceinline__ void for_stencil(const GridGeom& g, const u32* __rest
             }
         }
     }
+}
+
+// Boundary proximity: bit k of the mask is set when the 27-cell stencil of
+// boundary-grid cell k holds a boundary particle.  A fluid particle whose
+// home cell has a clear bit has no boundary candidates, so the boundary walk
+// can be skipped (it would add nothing).  Home cells outside the grid walk.
+__device__ __forceinline__ bool boundary_near(const GridGeom& g, const u32* __restrict__ mask,
+                                              float px, float py, float pz) {
+    if (!mask) return true;
+    const int x = g.cx(px), y = g.cy(py), z = g.cz(pz);
+    const u32 k = g.key(x, y, z);
+    if (k >= g.C) return true;
+    return (mask[k >> 5] >> (k & 31u)) & 1u;
 }
 
 template <int NT>
@@ -188,9 +202,9 @@ __device__ __forceinline__ float pair_c(const float4& p, const float4& q, const 
 }
 
 // Host-side assembly of the kernel inputs.
-inline Nbr make_nbr(const CellGrid& fg, const CellGrid* bg, const float4* xb, size_t nb,
-                    const DFSPHConfig& cfg, float mass, u32 nslots, const SceneSeg* seg,
-                    const u32* tile_scene, u32 remap) {
+inline Nbr make_nbr(const CellGrid& fg, const CellGrid* bg, const float4* xb, const u32* bnear,
+                    size_t nb, const DFSPHConfig& cfg, float mass, u32 nslots,
+                    const SceneSeg* seg, const u32* tile_scene, u32 remap) {
     Nbr P;
     P.gf = fg.geom();
     P.csf = fg.cell_start();
@@ -199,11 +213,13 @@ inline Nbr make_nbr(const CellGrid& fg, const CellGrid* bg, const float4* xb, si
         P.csb = bg->cell_start();
         P.xb = xb;
         P.nb = static_cast<u32>(nb);
+        P.bnear = bnear;
     } else {
         P.gb = fg.geom();
         P.csb = nullptr;
         P.xb = nullptr;
         P.nb = 0;
+        P.bnear = nullptr;
     }
     P.sp = Spline::make(cfg.smoothing_length);
     P.mass = mass;

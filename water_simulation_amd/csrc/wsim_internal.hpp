@@ -413,6 +413,7 @@ private:
     DeviceBuffer<float4> xs_, vs_, vpre_, xk_, xb_;
     DeviceBuffer<uint2> pairs_f_, pairs_b_;
     DeviceBuffer<uint16_t> pairs_fi_;
+    DeviceBuffer<u32> bnear_;  // boundary proximity mask (see stencil.hpp)
     DeviceBuffer<StageRanges> plan_;
     DeviceBuffer<u32> pair_cnt_;
     DeviceBuffer<float> rho_, alpha_, kappa_, rho_adv_;
