@@ -184,7 +184,7 @@ wsim_status wsim_solver_kernel_time(wsim_solver solver, int kind, uint64_t* laun
  * blocks, dfsph.cu:137-326).  Every shape computes the same bits. */
 typedef struct {
     uint32_t density_alpha_nt;   /* workgroup size: 64, 128, 256, 512 */
-    uint32_t xsph_nt;            /* 64, 128, 256, 512 */
+    uint32_t xsph_nt;            /* 0 (staged in LDS), 64, 128, 256, 512 */
     uint32_t change_nt;          /* 0 (neighbourhood staged in LDS, 256 threads), 256, 512, 1024 */
     uint32_t change_pu;          /* pairs per trip of the list loops: 2, 4, 8 */
     uint32_t apply_nt;           /* 0 (staged in LDS), 64, 128, 256, 512 */

@@ -305,6 +305,89 @@ __global__ __launch_bounds__(NT) void k_xsph(Nbr P, PairList L, const float4* __
     vout[i] = make_float4(vi.x + dx, vi.y + dy, vi.z + dz, 0.0f);
 }
 
+// k_xsph with the tile's fluid neighbourhood staged in LDS: positions, and
+// pre-XSPH velocities with the density in w.  The same terms in the same
+// order; tiles without a staging plan gather from global memory.
+template <int PU>
+__global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const float4* __restrict__ xs,
+                                                   const float4* __restrict__ vin,
+                                                   const float* __restrict__ rho,
+                                                   float4* __restrict__ vout) {
+    __shared__ float4 s_x[kStageCap];
+    __shared__ float4 s_v[kStageCap];
+    const u32 tile = P.remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+    const u32 t0 = tile * kTile;
+    const u32 i = t0 + threadIdx.x;
+    const SceneSeg& G = P.seg[scene_of_slot(P, t0)];
+    const u32 l = i - G.off;
+    const float eps = G.eps;
+    if (!(eps > 0.0f)) {  // scene-uniform
+        if (l < G.n) vout[i] = vin[i];
+        return;
+    }
+    const StageRanges R = L.plan[tile];
+    for (u32 e = threadIdx.x; e < R.total; e += kTile) {
+        const u32 src = stage_source(R, e);
+        s_x[e] = xs[src];
+        const float4 v = vin[src];
+        s_v[e] = make_float4(v.x, v.y, v.z, rho[src]);
+    }
+    __syncthreads();
+    if (l >= G.n) return;
+    const float4 vi = vin[i];
+    const float4 p = xs[i];
+    const float rho_i = rho[i];
+    float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+    const u32 cf = L.cnt[i] & 0xFFFFu;
+    if (cf <= L.kf) {
+        u32 k = 0;
+        if (R.total) {
+            for (; k + PU <= cf; k += PU) {
+                u32 a[PU];
+                float4 q[PU], w[PU];
+#pragma unroll
+                for (int u = 0; u < PU; ++u) a[u] = fluid_stage_entry(L, k + u, i);
+#pragma unroll
+                for (int u = 0; u < PU; ++u) {
+                    q[u] = s_x[a[u]];
+                    w[u] = s_v[a[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < PU; ++u)
+                    xsph_term(dx, dy, dz, p, vi, rho_i, q[u], w[u], w[u].w, eps, P);
+            }
+            for (; k < cf; ++k) {
+                const u32 a = fluid_stage_entry(L, k, i);
+                const float4 w = s_v[a];
+                xsph_term(dx, dy, dz, p, vi, rho_i, s_x[a], w, w.w, eps, P);
+            }
+        } else {
+            for (; k < cf; ++k) {
+                const u32 j = ld_pair(L.f + pair_at(k, i, L.kf)).x;
+                xsph_term(dx, dy, dz, p, vi, rho_i, xs[j], vin[j], rho[j], eps, P);
+            }
+        }
+        vout[i] = make_float4(vi.x + dx, vi.y + dy, vi.z + dz, 0.0f);
+        return;
+    }
+    const u32* cs = P.csf + G.key_base;
+    const u32 shift = G.off - G.start;
+    for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
+        if (s == i) return;
+        const float4 q = xs[s];
+        const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+        if (r2 >= P.sp.far2) return;  // W = 0
+        const float4 vj = vin[s];
+        const float w = spline_W_near(sqrtf(r2), P.sp);
+        const float rho_avg = 0.5f * (rho_i + rho[s]);
+        const float c = eps * (P.mass / fmaxf(rho_avg, 1e-9f)) * w;
+        dx += c * (vj.x - vi.x);
+        dy += c * (vj.y - vi.y);
+        dz += c * (vj.z - vi.z);
+    });
+    vout[i] = make_float4(vi.x + dx, vi.y + dy, vi.z + dz, 0.0f);
+}
+
 // rho_dot terms (dfsph_kernels.cu:210, 222) for one pair with gradient scalar c.
 __device__ __forceinline__ void rho_dot_fluid(float& rd, const float4& p, const float4& vi,
                                               const float4& q, const float4& vj, float c,
@@ -1027,6 +1110,7 @@ void launch_xsph(u32 nt, hipStream_t s, u32 nslots, const Nbr& P, const PairList
                  const float4* xs, const float4* vin, const float* rho, float4* vout) {
     constexpr int XPU = 4;
     switch (nt) {
+    case 0: k_xsph_st<XPU><<<nslots / kTile, kTile, 0, s>>>(P, L, xs, vin, rho, vout); break;
     case 64: k_xsph<64, XPU><<<grid_for(nslots, 64), 64, 0, s>>>(P, L, xs, vin, rho, vout); break;
     case 128: k_xsph<128, XPU><<<grid_for(nslots, 128), 128, 0, s>>>(P, L, xs, vin, rho, vout); break;
     case 256: k_xsph<256, XPU><<<grid_for(nslots, 256), 256, 0, s>>>(P, L, xs, vin, rho, vout); break;
@@ -1205,7 +1289,7 @@ bool valid_shapes(const KernelShapes& s) {
     auto in = [](u32 v, std::initializer_list<u32> set) {
         return std::find(set.begin(), set.end(), v) != set.end();
     };
-    return in(s.density_alpha_nt, {64, 128, 256, 512}) && in(s.xsph_nt, {64, 128, 256, 512}) &&
+    return in(s.density_alpha_nt, {64, 128, 256, 512}) && in(s.xsph_nt, {0, 64, 128, 256, 512}) &&
            in(s.change_nt, {0, 256, 512, 1024}) && in(s.change_pu, {2, 4, 8}) &&
            in(s.apply_nt, {0, 64, 128, 256, 512}) && in(s.apply_pu, {2, 4, 8}) &&
            in(s.cell_cpt, {4, 8, 16}) && in(s.remap, {0, 1});
@@ -1925,7 +2009,7 @@ void DFSPHSolver::autotune(int reps) {
                 if (ms < b_da) b_da = ms, cand.density_alpha_nt = nt;
             }
             if (any_xsph_) {
-                for (u32 nt : {64u, 128u, 256u, 512u}) {
+                for (u32 nt : {0u, 64u, 128u, 256u, 512u}) {
                     const double ms = time_ms([&] {
                         launch_xsph(nt, stream_, nslots_, P, L, xs_.data(), vpre_.data(),
                                     rho_.data(), vs_.data());

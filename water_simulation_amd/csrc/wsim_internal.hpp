@@ -257,7 +257,7 @@ constexpr u32 kPairDepthBoundary = 48;
 // the autotuner times them on the solver's own data and keeps the fastest.
 struct KernelShapes {
     u32 density_alpha_nt = 256;  // workgroup size: 64, 128, 256, 512
-    u32 xsph_nt = 256;           // 64, 128, 256, 512
+    u32 xsph_nt = 0;             // 0 (LDS-staged neighbourhood, 256), 64, 128, 256, 512
     u32 change_nt = 0;           // 0 (LDS-staged neighbourhood, 256), 256, 512, 1024
     u32 change_pu = 4;           // pairs per trip of the list loops: 2, 4, 8
     u32 apply_nt = 0;            // 0 (LDS-staged), 64, 128, 256, 512
