@@ -182,11 +182,10 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
         sq += mx * mx + my * my + mz * mz;
         if (c != 0.0f) {
             if (cf < L.kf) {
-                const u64 at = pair_at(cf, i, L.kf);
                 if (R.total)
-                    L.fi[at] = static_cast<uint16_t>(stage_index(R, s));
+                    L.fi[stage_entry_at(cf, i, L.kf)] = static_cast<uint16_t>(stage_index(R, s));
                 else
-                    L.f[at] = make_uint2(s, __float_as_uint(c));
+                    L.f[pair_at(cf, i, L.kf)] = make_uint2(s, __float_as_uint(c));
             }
             ++cf;
         } else if (s != i) {
@@ -227,13 +226,14 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
 __device__ __forceinline__ void xsph_term(float& dx, float& dy, float& dz, const float4& p,
                                           const float4& vi, float rho_i, const float4& q,
                                           const float4& vj, float rho_j, float eps,
-                                          const Nbr& P) {
+                                          const Nbr& P, bool keep = true) {
     const float w = spline_W_near(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
     const float rho_avg = 0.5f * (rho_i + rho_j);
     const float c = eps * (P.mass / fmaxf(rho_avg, 1e-9f)) * w;
-    dx += c * (vj.x - vi.x);
-    dy += c * (vj.y - vi.y);
-    dz += c * (vj.z - vi.z);
+    // a dropped term adds -0.0f, which leaves every value unchanged
+    dx += keep ? c * (vj.x - vi.x) : -0.0f;
+    dy += keep ? c * (vj.y - vi.y) : -0.0f;
+    dz += keep ? c * (vj.z - vi.z) : -0.0f;
 }
 
 // XSPH as a Jacobi sweep: every neighbour velocity is the pre-XSPH one.  A
@@ -326,11 +326,23 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
         return;
     }
     const StageRanges R = L.plan[tile];
-    for (u32 e = threadIdx.x; e < R.total; e += kTile) {
-        const u32 src = stage_source(R, e);
-        s_x[e] = xs[src];
-        const float4 v = vin[src];
-        s_v[e] = make_float4(v.x, v.y, v.z, rho[src]);
+    if (R.total) {
+        // every load of a thread is issued before its LDS stores
+        const StageLanes E(R);
+        const float4 x0 = xs[E.src[0]], x1 = xs[E.src[1]], x2 = xs[E.src[2]];
+        const float4 x3 = xs[E.src[3]], x4 = xs[E.src[4]], x5 = xs[E.src[5]];
+        const float4 v0 = vin[E.src[0]], v1 = vin[E.src[1]], v2 = vin[E.src[2]];
+        const float4 v3 = vin[E.src[3]], v4 = vin[E.src[4]], v5 = vin[E.src[5]];
+        const float r0 = rho[E.src[0]], r1 = rho[E.src[1]], r2 = rho[E.src[2]];
+        const float r3 = rho[E.src[3]], r4 = rho[E.src[4]], r5 = rho[E.src[5]];
+        E.put(s_x, 0, x0); E.put(s_x, 1, x1); E.put(s_x, 2, x2);
+        E.put(s_x, 3, x3); E.put(s_x, 4, x4); E.put(s_x, 5, x5);
+        E.put(s_v, 0, make_float4(v0.x, v0.y, v0.z, r0));
+        E.put(s_v, 1, make_float4(v1.x, v1.y, v1.z, r1));
+        E.put(s_v, 2, make_float4(v2.x, v2.y, v2.z, r2));
+        E.put(s_v, 3, make_float4(v3.x, v3.y, v3.z, r3));
+        E.put(s_v, 4, make_float4(v4.x, v4.y, v4.z, r4));
+        E.put(s_v, 5, make_float4(v5.x, v5.y, v5.z, r5));
     }
     __syncthreads();
     if (l >= G.n) return;
@@ -342,25 +354,14 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
     if (cf <= L.kf) {
         u32 k = 0;
         if (R.total) {
-            for (; k + PU <= cf; k += PU) {
-                u32 a[PU];
-                float4 q[PU], w[PU];
-#pragma unroll
-                for (int u = 0; u < PU; ++u) a[u] = fluid_stage_entry(L, k + u, i);
-#pragma unroll
-                for (int u = 0; u < PU; ++u) {
-                    q[u] = s_x[a[u]];
-                    w[u] = s_v[a[u]];
-                }
-#pragma unroll
-                for (int u = 0; u < PU; ++u)
-                    xsph_term(dx, dy, dz, p, vi, rho_i, q[u], w[u], w[u].w, eps, P);
-            }
-            for (; k < cf; ++k) {
-                const u32 a = fluid_stage_entry(L, k, i);
-                const float4 w = s_v[a];
-                xsph_term(dx, dy, dz, p, vi, rho_i, s_x[a], w, w.w, eps, P);
-            }
+            for_stage_quads(L, i, cf, [&](u32 a0, u32 a1, u32 a2, u32 a3, u32 m) {
+                const float4 q0 = s_x[a0], q1 = s_x[a1], q2 = s_x[a2], q3 = s_x[a3];
+                const float4 w0 = s_v[a0], w1 = s_v[a1], w2 = s_v[a2], w3 = s_v[a3];
+                xsph_term(dx, dy, dz, p, vi, rho_i, q0, w0, w0.w, eps, P);
+                xsph_term(dx, dy, dz, p, vi, rho_i, q1, w1, w1.w, eps, P, m > 1u);
+                xsph_term(dx, dy, dz, p, vi, rho_i, q2, w2, w2.w, eps, P, m > 2u);
+                xsph_term(dx, dy, dz, p, vi, rho_i, q3, w3, w3.w, eps, P, m > 3u);
+            });
         } else {
             for (; k < cf; ++k) {
                 const u32 j = ld_pair(L.f + pair_at(k, i, L.kf)).x;
@@ -391,10 +392,10 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
 // rho_dot terms (dfsph_kernels.cu:210, 222) for one pair with gradient scalar c.
 __device__ __forceinline__ void rho_dot_fluid(float& rd, const float4& p, const float4& vi,
                                               const float4& q, const float4& vj, float c,
-                                              float mass) {
+                                              float mass, bool keep = true) {
     const float gx = (p.x - q.x) * c, gy = (p.y - q.y) * c, gz = (p.z - q.z) * c;
     const float ux = vi.x - vj.x, uy = vi.y - vj.y, uz = vi.z - vj.z;
-    rd += mass * (ux * gx + uy * gy + uz * gz);
+    rd += keep ? mass * (ux * gx + uy * gy + uz * gz) : -0.0f;  // -0.0f: no change
 }
 __device__ __forceinline__ void rho_dot_boundary(float& rd, const float4& p, const float4& vi,
                                                  const float4& q, float c) {
@@ -551,10 +552,17 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
     if (conv < iter) return;  // the whole tile belongs to one scene
     const SceneSeg& G = P.seg[sc];
     const StageRanges R = L.plan[tile];
-    for (u32 e = threadIdx.x; e < R.total; e += kTile) {
-        const u32 src = stage_source(R, e);
-        s_x[e] = xs[src];
-        s_v[e] = vs[src];
+    if (R.total) {
+        // every load of a thread is issued before its LDS stores
+        const StageLanes E(R);
+        const float4 x0 = xs[E.src[0]], x1 = xs[E.src[1]], x2 = xs[E.src[2]];
+        const float4 x3 = xs[E.src[3]], x4 = xs[E.src[4]], x5 = xs[E.src[5]];
+        const float4 v0 = vs[E.src[0]], v1 = vs[E.src[1]], v2 = vs[E.src[2]];
+        const float4 v3 = vs[E.src[3]], v4 = vs[E.src[4]], v5 = vs[E.src[5]];
+        E.put(s_x, 0, x0); E.put(s_x, 1, x1); E.put(s_x, 2, x2);
+        E.put(s_x, 3, x3); E.put(s_x, 4, x4); E.put(s_x, 5, x5);
+        E.put(s_v, 0, v0); E.put(s_v, 1, v1); E.put(s_v, 2, v2);
+        E.put(s_v, 3, v3); E.put(s_v, 4, v4); E.put(s_v, 5, v5);
     }
     __syncthreads();
     const bool staged = R.total != 0;
@@ -572,26 +580,17 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
             if (staged) {
                 // compact list: LDS entries only; c is recomputed from the staged
                 // positions with the expression that produced it
-                for (; k + PU <= cf; k += PU) {
-                    u32 a[PU];
-                    float4 q[PU], w[PU];
-#pragma unroll
-                    for (int u = 0; u < PU; ++u) a[u] = fluid_stage_entry(L, k + u, i);
-#pragma unroll
-                    for (int u = 0; u < PU; ++u) {
-                        q[u] = s_x[a[u]];
-                        w[u] = s_v[a[u]];
-                    }
-#pragma unroll
-                    for (int u = 0; u < PU; ++u) {
-                        rho_dot_fluid(rd, p, vi, q[u], w[u], pair_c(p, q[u], P.sp), P.mass);
-                    }
-                }
-                for (; k < cf; ++k) {
-                    const u32 a = fluid_stage_entry(L, k, i);
-                    const float4 q = s_x[a], w = s_v[a];
-                    rho_dot_fluid(rd, p, vi, q, w, pair_c(p, q, P.sp), P.mass);
-                }
+                auto term = [&](const float4& q, const float4& w, bool keep) {
+                    rho_dot_fluid(rd, p, vi, q, w, pair_c(p, q, P.sp), P.mass, keep);
+                };
+                for_stage_quads(L, i, cf, [&](u32 a0, u32 a1, u32 a2, u32 a3, u32 m) {
+                    const float4 q0 = s_x[a0], q1 = s_x[a1], q2 = s_x[a2], q3 = s_x[a3];
+                    const float4 w0 = s_v[a0], w1 = s_v[a1], w2 = s_v[a2], w3 = s_v[a3];
+                    term(q0, w0, true);
+                    term(q1, w1, m > 1u);
+                    term(q2, w2, m > 2u);
+                    term(q3, w3, m > 3u);
+                });
             } else {
                 for (; k + PU <= cf; k += PU) {
                     uint2 e[PU];
@@ -696,11 +695,13 @@ __device__ __forceinline__ void finish_error(int mode, u32 iter, double thr, con
 
 // Pressure velocity terms (dfsph_kernels.cu:275-278, 291-294) for one pair.
 __device__ __forceinline__ void kappa_term(float& dx, float& dy, float& dz, const float4& p,
-                                           const float4& q, float c, float c2, float dt) {
+                                           const float4& q, float c, float c2, float dt,
+                                           bool keep = true) {
     const float gx = (p.x - q.x) * c, gy = (p.y - q.y) * c, gz = (p.z - q.z) * c;
-    dx -= dt * c2 * gx;
-    dy -= dt * c2 * gy;
-    dz -= dt * c2 * gz;
+    // a dropped term subtracts +0.0f, which leaves every value unchanged
+    dx -= keep ? dt * c2 * gx : 0.0f;
+    dy -= keep ? dt * c2 * gy : 0.0f;
+    dz -= keep ? dt * c2 * gz : 0.0f;
 }
 
 // Pressure velocity update (dfsph_kernels.cu:237-301); kappa-hat of every
@@ -820,7 +821,14 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
     if (conv < iter) return;  // tile-uniform
     const SceneSeg& G = P.seg[sc];
     const StageRanges R = L.plan[tile];
-    for (u32 e = threadIdx.x; e < R.total; e += kTile) s_x[e] = xk[stage_source(R, e)];
+    if (R.total) {
+        // every load of a thread is issued before its LDS stores
+        const StageLanes E(R);
+        const float4 x0 = xk[E.src[0]], x1 = xk[E.src[1]], x2 = xk[E.src[2]];
+        const float4 x3 = xk[E.src[3]], x4 = xk[E.src[4]], x5 = xk[E.src[5]];
+        E.put(s_x, 0, x0); E.put(s_x, 1, x1); E.put(s_x, 2, x2);
+        E.put(s_x, 3, x3); E.put(s_x, 4, x4); E.put(s_x, 5, x5);
+    }
     __syncthreads();
     const bool staged = R.total != 0;
     const u32 l = i - G.off;
@@ -842,23 +850,16 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
     if (cf <= L.kf && cb <= L.kb) {
         u32 k = 0;
         if (staged) {
-            for (; k + PU <= cf; k += PU) {
-                u32 a[PU];
-                float4 q[PU];
-#pragma unroll
-                for (int u = 0; u < PU; ++u) a[u] = fluid_stage_entry(L, k + u, i);
-#pragma unroll
-                for (int u = 0; u < PU; ++u) q[u] = s_x[a[u]];
-#pragma unroll
-                for (int u = 0; u < PU; ++u) {
-                    kappa_term(dx, dy, dz, p, q[u], pair_c(p, q[u], P.sp), P.mass * (ki + q[u].w),
-                               dt);
-                }
-            }
-            for (; k < cf; ++k) {
-                const float4 q = s_x[fluid_stage_entry(L, k, i)];
-                kappa_term(dx, dy, dz, p, q, pair_c(p, q, P.sp), P.mass * (ki + q.w), dt);
-            }
+            auto term = [&](const float4& q, bool keep) {
+                kappa_term(dx, dy, dz, p, q, pair_c(p, q, P.sp), P.mass * (ki + q.w), dt, keep);
+            };
+            for_stage_quads(L, i, cf, [&](u32 a0, u32 a1, u32 a2, u32 a3, u32 m) {
+                const float4 q0 = s_x[a0], q1 = s_x[a1], q2 = s_x[a2], q3 = s_x[a3];
+                term(q0, true);
+                term(q1, m > 1u);
+                term(q2, m > 2u);
+                term(q3, m > 3u);
+            });
         } else {
             for (; k + PU <= cf; k += PU) {
                 uint2 e[PU];

@@ -118,6 +118,7 @@ __device__ __forceinline__ bool finite3(float a, float b, float c) {
 // tile stages them in LDS when they are disjoint and fit; otherwise (sparse
 // tiles, particles outside the grid) it gathers from global memory.
 constexpr u32 kStageCap = 1536;  // staged fluid particles per 256-slot tile
+static_assert(kStageCap % kTile == 0, "whole staging rounds per thread");
 
 // (StageRanges is declared in wsim_internal.hpp; an empty range sits where
 // the previous one ends.)
@@ -180,26 +181,80 @@ __device__ __forceinline__ u32 stage_source(const StageRanges& R, u32 e) {
 // working slot is stage_source(R, entry); other tiles keep (slot, bits of c).
 // Positions are frozen between the list build and advection, so c of a
 // staged entry is recomputed by pair_c with the expression that produced it
-// in the list build: the same binary32 value either way.  c is left 0 for a
-// staged entry.
+// in the list build: the same binary32 value either way.
+//
+// The 2-byte entries are grouped by four: entries 4g..4g+3 of a lane form
+// one 8-byte word, and word g of the 64 lanes of a wave are contiguous, so a
+// lane reads four entries with one load.
+__device__ __forceinline__ u64 quad_at(u32 g, u32 i, u32 depth) {
+    return (static_cast<u64>(i >> 6) * (depth >> 2) + g) * 64u + (i & 63u);
+}
+__device__ __forceinline__ u64 stage_entry_at(u32 k, u32 i, u32 depth) {
+    return quad_at(k >> 2, i, depth) * 4u + (k & 3u);
+}
+__device__ __forceinline__ u64 ld_quad(const PairList& L, u32 g, u32 i) {
+    return __builtin_nontemporal_load(reinterpret_cast<const u64*>(L.fi) + quad_at(g, i, L.kf));
+}
+__device__ __forceinline__ u32 fluid_stage_entry(const PairList& L, u32 k, u32 i) {
+    return __builtin_nontemporal_load(L.fi + stage_entry_at(k, i, L.kf));
+}
+// c is left 0 for a staged entry.
 __device__ __forceinline__ void fluid_entry(const PairList& L, const StageRanges& R, u32 k, u32 i,
                                             u32& j, float& c) {
-    const u64 at = pair_at(k, i, L.kf);
     if (R.total) {
-        j = stage_source(R, __builtin_nontemporal_load(L.fi + at));
+        j = stage_source(R, fluid_stage_entry(L, k, i));
         c = 0.0f;
     } else {
-        const uint2 e = ld_pair(L.f + at);
+        const uint2 e = ld_pair(L.f + pair_at(k, i, L.kf));
         j = e.x;
         c = __uint_as_float(e.y);
     }
 }
-__device__ __forceinline__ u32 fluid_stage_entry(const PairList& L, u32 k, u32 i) {
-    return __builtin_nontemporal_load(L.fi + pair_at(k, i, L.kf));
+// Visits the cf staged entries of particle i in order, four per call:
+// f(e0, e1, e2, e3, m) with m in [1, 4] valid entries; the invalid ones of
+// the last word repeat e0, so every lane can load and compute all four and
+// drop the terms past m with a select (adding -0.0f, or subtracting +0.0f,
+// leaves every binary32 value unchanged).  One load per four entries; the
+// next word is requested before the current one is used.
+template <class F>
+__device__ __forceinline__ void for_stage_quads(const PairList& L, u32 i, u32 cf, F&& f) {
+    const u32 nq = (cf + 3u) >> 2;
+    u64 next = nq ? ld_quad(L, 0, i) : 0ull;
+    for (u32 g = 0; g < nq; ++g) {
+        const u64 cur = next;
+        if (g + 1 < nq) next = ld_quad(L, g + 1, i);
+        const u32 m = cf - 4u * g < 4u ? cf - 4u * g : 4u;
+        const u32 e0 = static_cast<u32>(cur & 0xFFFFu);
+        const u32 e1 = m > 1u ? static_cast<u32>((cur >> 16) & 0xFFFFu) : e0;
+        const u32 e2 = m > 2u ? static_cast<u32>((cur >> 32) & 0xFFFFu) : e0;
+        const u32 e3 = m > 3u ? static_cast<u32>(cur >> 48) : e0;
+        f(e0, e1, e2, e3, m);
+    }
 }
+
 __device__ __forceinline__ float pair_c(const float4& p, const float4& q, const Spline& sp) {
     return spline_gradc_near(sqrtf(dist2(p.x - q.x, p.y - q.y, p.z - q.z)), sp);
 }
+
+// The staging entries of one thread: tid, tid + 256, ..., tid + 1280.  src
+// holds their working slots (entries past the plan's end repeat its last
+// entry, so every load is in bounds and can be issued up front); put stores
+// only the entries that exist.
+constexpr int kStagePer = kStageCap / kTile;
+static_assert(kStagePer == 6, "the staging code unrolls six entries per thread");
+struct StageLanes {
+    u32 src[kStagePer];
+    u32 total;
+    __device__ __forceinline__ explicit StageLanes(const StageRanges& R) : total(R.total) {
+#pragma unroll
+        for (int u = 0; u < kStagePer; ++u)
+            src[u] = stage_source(R, min(threadIdx.x + static_cast<u32>(u) * kTile, R.total - 1u));
+    }
+    __device__ __forceinline__ void put(float4* dst, int u, const float4& v) const {
+        const u32 e = threadIdx.x + static_cast<u32>(u) * kTile;
+        if (e < total) dst[e] = v;
+    }
+};
 
 // Host-side assembly of the kernel inputs.
 inline Nbr make_nbr(const CellGrid& fg, const CellGrid* bg, const float4* xb, const u32* bnear,
