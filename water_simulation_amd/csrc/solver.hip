@@ -313,8 +313,8 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
                                                    const float4* __restrict__ vin,
                                                    const float* __restrict__ rho,
                                                    float4* __restrict__ vout) {
-    __shared__ float4 s_x[kStageCap];
-    __shared__ float4 s_v[kStageCap];
+    __shared__ float4 s_x[kStageCap];  // x, y, z, rho
+    __shared__ float4 s_v[kStageCap];  // pre-XSPH v
     const u32 tile = P.remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
     const u32 t0 = tile * kTile;
     const u32 i = t0 + threadIdx.x;
@@ -329,20 +329,20 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
     if (R.total) {
         // every load of a thread is issued before its LDS stores
         const StageLanes E(R);
-        const float4 x0 = xs[E.src[0]], x1 = xs[E.src[1]], x2 = xs[E.src[2]];
-        const float4 x3 = xs[E.src[3]], x4 = xs[E.src[4]], x5 = xs[E.src[5]];
-        const float4 v0 = vin[E.src[0]], v1 = vin[E.src[1]], v2 = vin[E.src[2]];
-        const float4 v3 = vin[E.src[3]], v4 = vin[E.src[4]], v5 = vin[E.src[5]];
-        const float r0 = rho[E.src[0]], r1 = rho[E.src[1]], r2 = rho[E.src[2]];
-        const float r3 = rho[E.src[3]], r4 = rho[E.src[4]], r5 = rho[E.src[5]];
-        E.put(s_x, 0, x0); E.put(s_x, 1, x1); E.put(s_x, 2, x2);
-        E.put(s_x, 3, x3); E.put(s_x, 4, x4); E.put(s_x, 5, x5);
-        E.put(s_v, 0, make_float4(v0.x, v0.y, v0.z, r0));
-        E.put(s_v, 1, make_float4(v1.x, v1.y, v1.z, r1));
-        E.put(s_v, 2, make_float4(v2.x, v2.y, v2.z, r2));
-        E.put(s_v, 3, make_float4(v3.x, v3.y, v3.z, r3));
-        E.put(s_v, 4, make_float4(v4.x, v4.y, v4.z, r4));
-        E.put(s_v, 5, make_float4(v5.x, v5.y, v5.z, r5));
+        f4v x[kStagePer], v[kStagePer];
+        float r[kStagePer];
+#pragma unroll
+        for (int u = 0; u < kStagePer; ++u) {
+            x[u] = ld4v(xs, E.src[u]);
+            v[u] = ld4v(vin, E.src[u]);
+            r[u] = rho[E.src[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < kStagePer; ++u)
+            if (E.has(u)) {
+                reinterpret_cast<f4v*>(s_x)[E.entry(u)] = f4v{x[u].x, x[u].y, x[u].z, r[u]};
+                reinterpret_cast<f4v*>(s_v)[E.entry(u)] = v[u];
+            }
     }
     __syncthreads();
     if (l >= G.n) return;
@@ -357,10 +357,10 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
             for_stage_quads(L, i, cf, [&](u32 a0, u32 a1, u32 a2, u32 a3, u32 m) {
                 const float4 q0 = s_x[a0], q1 = s_x[a1], q2 = s_x[a2], q3 = s_x[a3];
                 const float4 w0 = s_v[a0], w1 = s_v[a1], w2 = s_v[a2], w3 = s_v[a3];
-                xsph_term(dx, dy, dz, p, vi, rho_i, q0, w0, w0.w, eps, P);
-                xsph_term(dx, dy, dz, p, vi, rho_i, q1, w1, w1.w, eps, P, m > 1u);
-                xsph_term(dx, dy, dz, p, vi, rho_i, q2, w2, w2.w, eps, P, m > 2u);
-                xsph_term(dx, dy, dz, p, vi, rho_i, q3, w3, w3.w, eps, P, m > 3u);
+                xsph_term(dx, dy, dz, p, vi, rho_i, q0, w0, q0.w, eps, P);
+                xsph_term(dx, dy, dz, p, vi, rho_i, q1, w1, q1.w, eps, P, m > 1u);
+                xsph_term(dx, dy, dz, p, vi, rho_i, q2, w2, q2.w, eps, P, m > 2u);
+                xsph_term(dx, dy, dz, p, vi, rho_i, q3, w3, q3.w, eps, P, m > 3u);
             });
         } else {
             for (; k < cf; ++k) {
@@ -541,8 +541,8 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
                                                      float* __restrict__ rho_adv,
                                                      float* __restrict__ kappa,
                                                      float4* __restrict__ xk) {
-    __shared__ float4 s_x[kStageCap];
-    __shared__ float4 s_v[kStageCap];
+    __shared__ float4 s_a[kStageCap];  // x, y, z, vx
+    __shared__ float2 s_b[kStageCap];  // vy, vz
     __shared__ double s_red[kTile / 64];
     const u32 tile = P.remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
     const u32 t0 = tile * kTile;
@@ -555,14 +555,18 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
     if (R.total) {
         // every load of a thread is issued before its LDS stores
         const StageLanes E(R);
-        const float4 x0 = xs[E.src[0]], x1 = xs[E.src[1]], x2 = xs[E.src[2]];
-        const float4 x3 = xs[E.src[3]], x4 = xs[E.src[4]], x5 = xs[E.src[5]];
-        const float4 v0 = vs[E.src[0]], v1 = vs[E.src[1]], v2 = vs[E.src[2]];
-        const float4 v3 = vs[E.src[3]], v4 = vs[E.src[4]], v5 = vs[E.src[5]];
-        E.put(s_x, 0, x0); E.put(s_x, 1, x1); E.put(s_x, 2, x2);
-        E.put(s_x, 3, x3); E.put(s_x, 4, x4); E.put(s_x, 5, x5);
-        E.put(s_v, 0, v0); E.put(s_v, 1, v1); E.put(s_v, 2, v2);
-        E.put(s_v, 3, v3); E.put(s_v, 4, v4); E.put(s_v, 5, v5);
+        f4v x[kStagePer], v[kStagePer];
+#pragma unroll
+        for (int u = 0; u < kStagePer; ++u) {
+            x[u] = ld4v(xs, E.src[u]);
+            v[u] = ld4v(vs, E.src[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kStagePer; ++u)
+            if (E.has(u)) {
+                reinterpret_cast<f4v*>(s_a)[E.entry(u)] = f4v{x[u].x, x[u].y, x[u].z, v[u].x};
+                reinterpret_cast<f2v*>(s_b)[E.entry(u)] = f2v{v[u].y, v[u].z};
+            }
     }
     __syncthreads();
     const bool staged = R.total != 0;
@@ -583,9 +587,18 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
                 auto term = [&](const float4& q, const float4& w, bool keep) {
                     rho_dot_fluid(rd, p, vi, q, w, pair_c(p, q, P.sp), P.mass, keep);
                 };
+                auto at = [&](u32 a, float4& q, float4& w) {
+                    const float4 x = s_a[a];
+                    const float2 y = s_b[a];
+                    q = make_float4(x.x, x.y, x.z, 0.0f);
+                    w = make_float4(x.w, y.x, y.y, 0.0f);
+                };
                 for_stage_quads(L, i, cf, [&](u32 a0, u32 a1, u32 a2, u32 a3, u32 m) {
-                    const float4 q0 = s_x[a0], q1 = s_x[a1], q2 = s_x[a2], q3 = s_x[a3];
-                    const float4 w0 = s_v[a0], w1 = s_v[a1], w2 = s_v[a2], w3 = s_v[a3];
+                    float4 q0, q1, q2, q3, w0, w1, w2, w3;
+                    at(a0, q0, w0);
+                    at(a1, q1, w1);
+                    at(a2, q2, w2);
+                    at(a3, q3, w3);
                     term(q0, w0, true);
                     term(q1, w1, m > 1u);
                     term(q2, w2, m > 2u);
@@ -824,10 +837,12 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
     if (R.total) {
         // every load of a thread is issued before its LDS stores
         const StageLanes E(R);
-        const float4 x0 = xk[E.src[0]], x1 = xk[E.src[1]], x2 = xk[E.src[2]];
-        const float4 x3 = xk[E.src[3]], x4 = xk[E.src[4]], x5 = xk[E.src[5]];
-        E.put(s_x, 0, x0); E.put(s_x, 1, x1); E.put(s_x, 2, x2);
-        E.put(s_x, 3, x3); E.put(s_x, 4, x4); E.put(s_x, 5, x5);
+        f4v x[kStagePer];
+#pragma unroll
+        for (int u = 0; u < kStagePer; ++u) x[u] = ld4v(xk, E.src[u]);
+#pragma unroll
+        for (int u = 0; u < kStagePer; ++u)
+            if (E.has(u)) reinterpret_cast<f4v*>(s_x)[E.entry(u)] = x[u];
     }
     __syncthreads();
     const bool staged = R.total != 0;

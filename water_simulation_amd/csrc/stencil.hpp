@@ -236,25 +236,30 @@ __device__ __forceinline__ float pair_c(const float4& p, const float4& q, const 
     return spline_gradc_near(sqrtf(dist2(p.x - q.x, p.y - q.y, p.z - q.z)), sp);
 }
 
-// The staging entries of one thread: tid, tid + 256, ..., tid + 1280.  src
+// The staging entries of one thread: tid, tid + 256, tid + 512, ...  src
 // holds their working slots (entries past the plan's end repeat its last
-// entry, so every load is in bounds and can be issued up front); put stores
-// only the entries that exist.
+// entry, so every load is in bounds and all of them can be issued before
+// the first LDS store).  Register arrays of the staged data use clang vector
+// types, which stay in registers when the loops are unrolled.
 constexpr int kStagePer = kStageCap / kTile;
-static_assert(kStagePer == 6, "the staging code unrolls six entries per thread");
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
 struct StageLanes {
     u32 src[kStagePer];
     u32 total;
     __device__ __forceinline__ explicit StageLanes(const StageRanges& R) : total(R.total) {
 #pragma unroll
         for (int u = 0; u < kStagePer; ++u)
-            src[u] = stage_source(R, min(threadIdx.x + static_cast<u32>(u) * kTile, R.total - 1u));
+            src[u] = stage_source(R, min(entry(u), R.total - 1u));
     }
-    __device__ __forceinline__ void put(float4* dst, int u, const float4& v) const {
-        const u32 e = threadIdx.x + static_cast<u32>(u) * kTile;
-        if (e < total) dst[e] = v;
+    __device__ __forceinline__ static u32 entry(int u) {
+        return threadIdx.x + static_cast<u32>(u) * kTile;
     }
+    __device__ __forceinline__ bool has(int u) const { return entry(u) < total; }
 };
+__device__ __forceinline__ f4v ld4v(const float4* p, u32 i) {
+    return reinterpret_cast<const f4v*>(p)[i];
+}
 
 // Host-side assembly of the kernel inputs.
 inline Nbr make_nbr(const CellGrid& fg, const CellGrid* bg, const float4* xb, const u32* bnear,
