@@ -274,6 +274,18 @@ typedef struct {
 void wsim_timestep_config_default(wsim_timestep_config* cfg);
 float wsim_timestep_next_dt(const wsim_timestep_config* cfg, float max_velocity,
                             float particle_radius, float remaining_in_frame);
+/* The substep loop of sim_cli's frame (apps/sim_cli/main.cpp:171-178) in one
+ * call, for a plain (not batched) solver: while time remains (> 1e-6) and
+ * fewer than max_substeps ran, dt = next_dt(ts, max(max_velocity, v_floor),
+ * particle_radius, remaining), then step(dt).  Writes every dt to dts and
+ * every substep's statistics to stats (each NULL, or room for max_substeps)
+ * and the substep count to substeps.  Each substep computes the same bits as
+ * step(dt); without the divergence solve the grid phase of the next substep
+ * overlaps the host's per-substep state read.  No counterpart in the
+ * reference, whose loop lives in sim_cli. */
+wsim_status wsim_solver_advance(wsim_solver solver, const wsim_timestep_config* ts, float v_floor,
+                                float time, uint32_t max_substeps, float* dts,
+                                wsim_step_stats* stats, uint32_t* substeps);
 /* Lattice fluid block of sim_cli (apps/sim_cli/main.cpp:65-74). */
 wsim_status wsim_lattice_block(const float lo[3], const float hi[3], float spacing, float* out,
                                size_t capacity, size_t* count);

@@ -387,6 +387,29 @@ class DFSPHSolver:
     def step(self, dt: float) -> None:
         check(_L().wsim_solver_step(self._h, float(dt)))
 
+    def advance(self, time: float, max_substeps: int, stepper: Optional["TimeStepper"] = None,
+                v_floor: float = 0.0, with_stats: bool = False):
+        """sim_cli's substep loop (main.cpp:171-178) in one call: substeps of
+        dt = stepper.next_dt(max(max_velocity(), v_floor), r, remaining) until
+        the remaining time is at most 1e-6 or max_substeps ran.  Returns the
+        time steps taken (and, with_stats, each substep's statistics); each
+        substep computes the same bits as step(dt)."""
+        cfg = (stepper or TimeStepper()).cfg
+        c = _ffi.TimeStepConfigC(cfg.frame_dt, cfg.dt_max, cfg.cfl_lambda, cfg.max_substeps)
+        m = max(int(max_substeps), 1)
+        dts = np.zeros(m, np.float32)
+        recs = (_ffi.StepStatsC * m)() if with_stats else None
+        n = C.c_uint32()
+        check(_L().wsim_solver_advance(self._h, C.byref(c), float(v_floor), float(time),
+                                       int(max_substeps),
+                                       dts.ctypes.data_as(C.POINTER(C.c_float)), recs,
+                                       C.byref(n)))
+        dts = dts[: n.value].copy()
+        if not with_stats:
+            return dts
+        return dts, [{f[0]: getattr(recs[k], f[0]) for f in recs[k]._fields_}
+                     for k in range(n.value)]
+
     def max_velocity(self) -> float:
         return _L().wsim_solver_max_velocity(self._h)
 

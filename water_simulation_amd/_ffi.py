@@ -189,6 +189,9 @@ SIGNATURES = {
     "wsim_batch_scene_stats": (st, [vp, C.c_uint32, C.POINTER(StepStatsC)]),
     "wsim_sample_aabb_boundary": (st, [F3, F3, C.c_float, C.c_int, vp, sz, C.POINTER(sz)]),
     "wsim_timestep_config_default": (None, [C.POINTER(TimeStepConfigC)]),
+    "wsim_solver_advance": (st, [vp, C.POINTER(TimeStepConfigC), C.c_float, C.c_float,
+                                 C.c_uint32, C.POINTER(C.c_float), C.POINTER(StepStatsC),
+                                 C.POINTER(C.c_uint32)]),
     "wsim_timestep_next_dt": (C.c_float, [C.POINTER(TimeStepConfigC), C.c_float, C.c_float,
                                           C.c_float]),
     "wsim_lattice_block": (st, [F3, F3, C.c_float, vp, sz, C.POINTER(sz)]),

@@ -620,6 +620,25 @@ float wsim_timestep_next_dt(const wsim_timestep_config* c, float max_velocity,
     }
     return wsim::next_dt(d, max_velocity, particle_radius, remaining);
 }
+wsim_status wsim_solver_advance(wsim_solver s, const wsim_timestep_config* c, float v_floor,
+                                float time, uint32_t max_substeps, float* dts,
+                                wsim_step_stats* stats, uint32_t* substeps) {
+    return guarded([&] {
+        need(s, "solver");
+        wsim::TimeStepConfig d;
+        if (c) {
+            d.frame_dt = c->frame_dt;
+            d.dt_max = c->dt_max;
+            d.cfl_lambda = c->cfl_lambda;
+            d.max_substeps = c->max_substeps;
+        }
+        std::vector<wsim::StepStats> rec(stats ? max_substeps : 0u);
+        const uint32_t n = s->impl->advance(d, v_floor, time, max_substeps, dts,
+                                            stats ? rec.data() : nullptr);
+        for (uint32_t k = 0; stats && k < n; ++k) fill_stats(rec[k], stats + k);
+        if (substeps) *substeps = n;
+    });
+}
 wsim_status wsim_lattice_block(const float lo[3], const float hi[3], float spacing, float* out,
                                size_t capacity, size_t* count) {
     return guarded([&] {

@@ -52,18 +52,20 @@ namespace {
 
 using namespace dev;
 
+// parts: 1 positions, 2 velocities, 3 both.
 __global__ __launch_bounds__(NS) void k_reorder(Nbr P, const u32* __restrict__ perm,
                                                 const float* __restrict__ x,
                                                 const float* __restrict__ v,
                                                 float4* __restrict__ xs, float4* __restrict__ vs,
-                                                int gravity) {
+                                                int gravity, int parts) {
     const u32 i = blockIdx.x * NS + threadIdx.x;
     if (i >= P.nslots) return;
     const SceneSeg& G = P.seg[scene_of_slot(P, i)];
     const u32 l = i - G.off;
     if (l >= G.n) return;
     const u64 o = 3ull * perm[G.start + l];
-    xs[i] = make_float4(x[o], x[o + 1], x[o + 2], 0.0f);
+    if (parts & 1) xs[i] = make_float4(x[o], x[o + 1], x[o + 2], 0.0f);
+    if (!(parts & 2)) return;
     float a = v[o], b = v[o + 1], c = v[o + 2];
     if (gravity) {  // v * exp(-damping*dt) + g*dt (dfsph_kernels.cu:351-354)
         a = a * G.decay + G.gdx;
@@ -82,8 +84,6 @@ __global__ __launch_bounds__(NS) void k_gather_boundary(const u32* __restrict__ 
     xb[s] = make_float4(x[o], x[o + 1], x[o + 2], 0.0f);
 }
 
-// Staging plan of every 256-slot tile (one thread per tile), computed once
-// per grid build and read by the list builder and the iteration kernels.
 // Boundary proximity mask: one bit per boundary-grid cell, set when a cell
 // of its 27-cell stencil is occupied.  Built once with the boundary grid.
 __global__ __launch_bounds__(NS) void k_boundary_near(GridGeom g, const u32* __restrict__ cs,
@@ -108,6 +108,8 @@ __global__ __launch_bounds__(NS) void k_boundary_near(GridGeom g, const u32* __r
     if ((threadIdx.x & 63u) == 32) mask[w + 1] = static_cast<u32>(bits >> 32);
 }
 
+// Staging plan of every 256-slot tile (one thread per tile), computed once
+// per grid build and read by the list builder and the iteration kernels.
 __global__ __launch_bounds__(NS) void k_stage_plan(Nbr P, StageRanges* __restrict__ plan) {
     const u32 t = blockIdx.x * NS + threadIdx.x;
     if (t >= P.nslots / kTile) return;
@@ -1288,6 +1290,8 @@ enum PhaseKey : u64 {
     kPhDivTail = 4,
     kPhDensityIters = 5,
     kPhDivIters = 6,
+    kPhPrefixGrid = 7,
+    kPhPrefixSolve = 8,
 };
 u64 phase_key(u64 kind, u64 b = 0, u64 e = 0) { return (kind << 48) | (b << 24) | e; }
 
@@ -1383,6 +1387,7 @@ DFSPHSolver::DFSPHSolver(ParticleStore& fluid, ParticleStore* boundary, const DF
 DFSPHSolver::~DFSPHSolver() {
     (void)hipStreamSynchronize(stream_);
     drop_graphs();
+    if (state_ev_) (void)hipEventDestroy(state_ev_);
     if (own_stream_) (void)hipStreamDestroy(own_stream_);
 }
 
@@ -1588,6 +1593,89 @@ void DFSPHSolver::step(float dt) {
     step_scenes(d.data());
 }
 
+// Only the velocity update and what follows it depend on a substep's dt, so
+// with the divergence solve off the grid, density, alpha and pair lists of
+// substep k + 1 are enqueued before the host waits for substep k's state:
+// the wait, the time-step computation and the launches of the host overlap
+// the device's work on the next grid.
+u32 DFSPHSolver::advance(const TimeStepConfig& ts, float v_floor, float time, u32 max_substeps,
+                         float* dts, StepStats* stats) {
+    if (batched_) fail(kInvalidArgument, "advance: a batched solver steps with step_scenes");
+    const size_t nsz = fluid_.count();
+    const bool pipe = !cfg_.enable_divergence_solve && !timing_ && nsz > 0;
+    if (pipe && (counts_.empty() || counts_[0] != nsz)) layout({static_cast<u32>(nsz)});
+    float rem = time;
+    u32 sub = 0;
+    bool grid_ready = false;  // the next substep's grid phase is enqueued
+    while (rem > 1e-6f && sub < max_substeps) {
+        const float dt = next_dt(ts, std::max(max_velocity(), v_floor), cfg_.particle_radius, rem);
+        if (!pipe) {
+            step(dt);
+        } else {
+            set_substep(&dt);
+            if (!grid_ready) run_phase(phase_key(kPhPrefixGrid), [&] { phase_prefix_grid(); });
+            run_phase(phase_key(kPhPrefixSolve), [&] { phase_prefix_solve(true); });
+            const bool next = rem - dt > 1e-6f && sub + 1 < max_substeps;
+            grid_ready = substep_pipelined(next);
+        }
+        if (dts) dts[sub] = dt;
+        if (stats) stats[sub] = stats_[0];
+        rem -= dt;
+        ++sub;
+    }
+    return sub;
+}
+
+// The density solve and advection of a plain substep whose prefix is
+// enqueued.  With `next`, the grid phase of the following substep is
+// enqueued behind the state read; returns whether it stands.  When the first
+// iteration chunk does not finish the solve, that grid phase ran on the
+// store's unchanged positions and rebuilt exactly this substep's grid, lists,
+// density and alpha (functions of the positions only), so the solve goes on
+// and the next substep's grid is built after advection.
+bool DFSPHSolver::substep_pipelined(bool next) {
+    const u32 maxit = cfg_.max_density_iters;
+    u32 chunk = 2;
+    while (chunk < last_iters_[0] && chunk < maxit) chunk <<= 1;
+    const u32 end = std::min(chunk, maxit);
+    if (end > 0) run_phase(phase_key(kPhDensityIters, 0, end), [&] { phase_iterations(0, 0, end); });
+    const bool last = end >= maxit;
+    run_phase(phase_key(kPhAdvect, last ? 1 : 0), [&] { phase_advect(last); });
+    const u32 scenes = static_cast<u32>(counts_.size());
+    WSIM_HIP(hipMemcpyAsync(hstate_.data(), dstate_.data(), scenes * sizeof(DeviceState),
+                            hipMemcpyDeviceToHost, stream_));
+    if (!state_ev_) WSIM_HIP(hipEventCreateWithFlags(&state_ev_, hipEventDisableTiming));
+    WSIM_HIP(hipEventRecord(state_ev_, stream_));
+    if (next) run_phase(phase_key(kPhPrefixGrid), [&] { phase_prefix_grid(); });
+    WSIM_HIP(hipEventSynchronize(state_ev_));
+    bool all = true;
+    for (u32 s = 0; s < scenes; ++s) all = all && hstate_[s].advected;
+    if (all) {
+        take_state();
+        return next;
+    }
+    u32 it = end;
+    chunk = chunk * 2 > 32 ? 32 : chunk * 2;
+    while (it < maxit) {
+        const u32 e = (maxit - it < chunk) ? maxit : it + chunk;
+        run_phase(phase_key(kPhDensityIters, it, e), [&] { phase_iterations(0, it, e); });
+        it = e;
+        if (it >= maxit) break;
+        run_phase(phase_key(kPhAdvect, 0), [&] { phase_advect(false); });
+        fetch_state();
+        bool done = true;
+        for (u32 s = 0; s < scenes; ++s) done = done && hstate_[s].advected;
+        if (done) {
+            take_state();
+            return false;
+        }
+        chunk = chunk * 2 > 32 ? 32 : chunk * 2;
+    }
+    run_phase(phase_key(kPhAdvect, 1), [&] { phase_advect(true); });
+    read_state();
+    return false;
+}
+
 void DFSPHSolver::step_scenes(const float* dt) {
     if (!batched_) {
         const size_t nsz = fluid_.count();
@@ -1632,19 +1720,22 @@ void DFSPHSolver::run_phase(u64 key, const std::function<void()>& body) {
 }
 
 void DFSPHSolver::phase_prefix(bool gravity) {
+    phase_prefix_grid();
+    phase_prefix_solve(gravity);
+}
+
+// The part of a substep's prefix that does not depend on its time step:
+// grid, sorted positions, staging plans, density, alpha and pair lists.
+void DFSPHSolver::phase_prefix_grid() {
     const u32 scenes = static_cast<u32>(counts_.size());
     tbegin(kKGrid);
     if (batched_) fluid_grid_.build_segmented(fluid_.positions(), total_, dstarts_.data(), scenes);
     else fluid_grid_.build(fluid_.positions(), total_);
     tend();
-    const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), bnear_.data(), nb_, cfg_, mass_, nslots_,
-                           dseg_.data(), batched_ ? tile_scene_.data() : nullptr, shapes_.remap);
-    const bool xsph = gravity && any_xsph_;
-    float4* vtarget = xsph ? vpre_.data() : vs_.data();
+    const Nbr P = nbr();
     tbegin(kKReorder);
     k_reorder<<<nslots_ / NS, NS, 0, stream_>>>(P, fluid_grid_.sorted_indices(), fluid_.positions(),
-                                                 fluid_.velocities(), xs_.data(), vtarget,
-                                                 gravity ? 1 : 0);
+                                                 fluid_.velocities(), xs_.data(), nullptr, 0, 1);
     WSIM_LAUNCH_CHECK();
     tend();
     k_stage_plan<<<div_up(nslots_ / kTile, NS), NS, 0, stream_>>>(P, plan_.data());
@@ -1652,6 +1743,21 @@ void DFSPHSolver::phase_prefix(bool gravity) {
     tbegin(kKDensityAlpha);
     launch_density_alpha(shapes_.density_alpha_nt, stream_, nslots_, P, pair_list(), xs_.data(),
                          rho_.data(), alpha_.data());
+    tend();
+}
+
+// The rest: sorted velocities with gravity and damping (dfsph_kernels.cu:
+// 347-355), XSPH, and the opening of the solve.
+void DFSPHSolver::phase_prefix_solve(bool gravity) {
+    const u32 scenes = static_cast<u32>(counts_.size());
+    const Nbr P = nbr();
+    const bool xsph = gravity && any_xsph_;
+    float4* vtarget = xsph ? vpre_.data() : vs_.data();
+    tbegin(kKReorder);
+    k_reorder<<<nslots_ / NS, NS, 0, stream_>>>(P, fluid_grid_.sorted_indices(), fluid_.positions(),
+                                                 fluid_.velocities(), xs_.data(), vtarget,
+                                                 gravity ? 1 : 0, 2);
+    WSIM_LAUNCH_CHECK();
     tend();
     if (xsph) {
         tbegin(kKXsph);
@@ -1662,6 +1768,11 @@ void DFSPHSolver::phase_prefix(bool gravity) {
     k_reset_solve<<<div_up(scenes, NS), NS, 0, stream_>>>(dseg_.data(), dstate_.data(), scenes,
                                                           gravity ? 0 : 1);
     WSIM_LAUNCH_CHECK();
+}
+
+Nbr DFSPHSolver::nbr() const {
+    return make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), bnear_.data(), nb_, cfg_, mass_,
+                    nslots_, dseg_.data(), batched_ ? tile_scene_.data() : nullptr, shapes_.remap);
 }
 
 void DFSPHSolver::phase_iterations(int mode, u32 begin, u32 end) {

@@ -11,6 +11,7 @@
 
 #include "common.hpp"
 #include "device_math.hpp"
+#include "host_surface.hpp"
 
 namespace wsim {
 
@@ -55,6 +56,10 @@ private:
 // ---------------------------------------------------------------------------
 // CellGrid (cell_grid.h:16-50): cell keys, stable (key, id) sort, dense
 // cell_start[C+1] with cell_start[c] = #particles whose key < c.
+namespace dev {
+struct Nbr;  // kernel view of the grids and stores (stencil.hpp)
+}
+
 class CellGrid {
 public:
     CellGrid(const float origin[3], const int32_t dims[3], float cell_length, hipStream_t stream);
@@ -314,6 +319,14 @@ public:
 
     void step(float dt);               // every scene advances by dt
     void step_scenes(const float* dt);  // scene s advances by dt[s]
+    // The CFL substep loop of sim_cli (main.cpp:171-178) for a plain solver:
+    // before each substep dt = next_dt(ts, max(max_velocity(), v_floor), r,
+    // remaining) (timestep.h:23-28), until the remaining time is at most
+    // 1e-6 or max_substeps ran.  Writes each dt and each substep's statistics
+    // to dts and stats (where not null) and returns the substep count.  Every
+    // substep computes what step(dt) does.
+    u32 advance(const TimeStepConfig& ts, float v_floor, float time, u32 max_substeps, float* dts,
+                StepStats* stats);
 
     float max_velocity() const;  // over all scenes
     float particle_mass() const { return mass_; }
@@ -365,7 +378,11 @@ private:
     void layout(const std::vector<u32>& counts);
     void set_substep(const float* dt);
     void run_substep();
+    bool substep_pipelined(bool next);
     void phase_prefix(bool gravity);
+    void phase_prefix_grid();
+    void phase_prefix_solve(bool gravity);
+    dev::Nbr nbr() const;
     void phase_iterations(int mode, u32 begin, u32 end);
     void phase_advect(bool force);
     void fetch_state();
@@ -426,6 +443,7 @@ private:
     KernelShapes shapes_;
     std::vector<TuneRecord> tune_log_;
     u32 last_iters_[2] = {0, 0};  // iterations of the previous substep's solves
+    hipEvent_t state_ev_ = nullptr;  // state read of a pipelined substep
     bool use_graphs_ = false, capturing_ = false;
     std::map<u64, hipGraphExec_t> graphs_;
 
