@@ -286,6 +286,23 @@ float wsim_timestep_next_dt(const wsim_timestep_config* cfg, float max_velocity,
 wsim_status wsim_solver_advance(wsim_solver solver, const wsim_timestep_config* ts, float v_floor,
                                 float time, uint32_t max_substeps, float* dts,
                                 wsim_step_stats* stats, uint32_t* substeps);
+/* The solver's stream (the one it was created with, or its own). */
+void* wsim_solver_stream(wsim_solver solver);
+
+/* Asynchronous frame_%04d.bin output (replaces the synchronous copy and
+ * write of apps/sim_cli/main.cpp:163-166, 191-193).  push snapshots a
+ * store's positions (original order) in stream order on `stream`, copies
+ * them to one of two pinned host buffers on a copy stream, and a background
+ * thread writes dir/frame_<index>.bin; it blocks only while both buffers hold
+ * frames not yet written.  finish waits for every pushed frame and reports
+ * the first write error. */
+typedef struct wsim_frame_writer_t* wsim_frame_writer;
+wsim_status wsim_frame_writer_create(const char* dir, size_t max_points, wsim_frame_writer* out);
+wsim_status wsim_frame_writer_push(wsim_frame_writer writer, wsim_store store, int index,
+                                   void* stream);
+wsim_status wsim_frame_writer_finish(wsim_frame_writer writer);
+void wsim_frame_writer_destroy(wsim_frame_writer writer);
+
 /* Lattice fluid block of sim_cli (apps/sim_cli/main.cpp:65-74). */
 wsim_status wsim_lattice_block(const float lo[3], const float hi[3], float spacing, float* out,
                                size_t capacity, size_t* count);

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <filesystem>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -140,31 +141,26 @@ int main(int argc, char** argv) try {
     water::solvers::DFSPHSolver solver(store, &boundary, cfg);
     water::TimeStepper ts;
 
+    std::unique_ptr<water::FrameWriter> writer;
     if (args.record) {
         std::filesystem::create_directories(args.out_dir);
         write_frame(args.out_dir, args.frames_start, initial);
+        writer = std::make_unique<water::FrameWriter>(args.out_dir, store.count());
     }
 
+    // main.cpp:171-195: per frame, CFL substeps (velocity floor 0.5 m/s, at
+    // most 64) until the frame time is used up, then the frame.  The substep
+    // loop runs in the library (the next substep's grid overlaps the host's
+    // state read) and frames are written in the background while the next
+    // frame is simulated.
     const float frame_dt = 1.0f / scene.output.fps;
-    std::vector<water::Vec3f> host(store.count());
     for (int f = args.frames_start; f < args.frames_end; ++f) {
-        float t_remaining = frame_dt;
-        int sub = 0;
-        while (t_remaining > 1e-6f && sub < 64) {
-            const float dt = ts.next_dt(std::max(solver.max_velocity(), 0.5f), cfg.particle_radius,
-                                        t_remaining);
-            solver.step(dt);
-            t_remaining -= dt;
-            ++sub;
-        }
-        water::detail::check(wsim_device_synchronize());
-        std::printf("frame %4d: substeps=%2d, max_v=%.4f m/s\n", f + 1, sub, solver.max_velocity());
-        if (args.record) {
-            water::detail::check(wsim_memcpy_d2h(host.data(), store.positions(),
-                                                 sizeof(water::Vec3f) * host.size(), nullptr));
-            write_frame(args.out_dir, f + 1, host);
-        }
+        const water::u32 sub = solver.advance(ts, 0.5f, frame_dt, 64);
+        std::printf("frame %4d: substeps=%2u, max_v=%.4f m/s\n", f + 1, sub, solver.max_velocity());
+        if (writer) writer->push(store, f + 1, solver.stream());
     }
+    if (writer) writer->finish();
+    water::detail::check(wsim_device_synchronize());
     std::printf("=== sim_cli OK ===\n");
     return 0;
 } catch (const std::exception& e) {

@@ -41,7 +41,7 @@ HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math
 
 HIP_SOURCES = ["radix_sort.hip", "grid.hip", "morton.hip", "store.hip", "solver.hip", "adjoint.hip",
                "copy.hip"]
-HOST_SOURCES = ["host_surface.cpp", "capi.cpp"]
+HOST_SOURCES = ["host_surface.cpp", "capi.cpp", "frames.cpp"]
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/wsim.h"
+#include "frames.hpp"
 #include "host_surface.hpp"
 #include "wsim_internal.hpp"
 
@@ -639,6 +640,36 @@ wsim_status wsim_solver_advance(wsim_solver s, const wsim_timestep_config* c, fl
         if (substeps) *substeps = n;
     });
 }
+void* wsim_solver_stream(wsim_solver s) { return s ? static_cast<void*>(s->impl->stream()) : nullptr; }
+
+struct wsim_frame_writer_t {
+    std::unique_ptr<wsim::FrameWriter> impl;
+};
+wsim_status wsim_frame_writer_create(const char* dir, size_t max_points, wsim_frame_writer* out) {
+    return guarded([&] {
+        need(out, "out");
+        need(dir, "dir");
+        *out = nullptr;
+        auto h = std::make_unique<wsim_frame_writer_t>();
+        h->impl = std::make_unique<wsim::FrameWriter>(dir, max_points);
+        *out = h.release();
+    });
+}
+wsim_status wsim_frame_writer_push(wsim_frame_writer w, wsim_store store, int index, void* stream) {
+    return guarded([&] {
+        need(w, "writer");
+        need(store, "store");
+        w->impl->push(store->impl->positions(), store->impl->count(), index, as_stream(stream));
+    });
+}
+wsim_status wsim_frame_writer_finish(wsim_frame_writer w) {
+    return guarded([&] {
+        need(w, "writer");
+        w->impl->finish();
+    });
+}
+void wsim_frame_writer_destroy(wsim_frame_writer w) { delete w; }
+
 wsim_status wsim_lattice_block(const float lo[3], const float hi[3], float spacing, float* out,
                                size_t capacity, size_t* count) {
     return guarded([&] {

@@ -216,6 +216,25 @@ private:
     Config cfg_;
 };
 
+// Asynchronous frame_%04d.bin output (wsim_frame_writer_*); not in the
+// reference, whose sim_cli writes frames synchronously.
+class FrameWriter {
+public:
+    FrameWriter(const std::string& dir, std::size_t max_points) {
+        detail::check(wsim_frame_writer_create(dir.c_str(), max_points, &h_));
+    }
+    ~FrameWriter() { wsim_frame_writer_destroy(h_); }
+    FrameWriter(const FrameWriter&) = delete;
+    FrameWriter& operator=(const FrameWriter&) = delete;
+    void push(ParticleStore& store, int index, void* stream) {
+        detail::check(wsim_frame_writer_push(h_, store.native(), index, stream));
+    }
+    void finish() { detail::check(wsim_frame_writer_finish(h_)); }
+
+private:
+    wsim_frame_writer h_ = nullptr;
+};
+
 namespace solvers {
 
 class DFSPHSolver {
@@ -270,6 +289,18 @@ public:
     DFSPHSolver& operator=(const DFSPHSolver&) = delete;
 
     void step(f32 dt) { detail::check(wsim_solver_step(h_, dt)); }
+    // sim_cli's substep loop (main.cpp:171-178) in the library: substeps of
+    // dt = ts.next_dt(max(max_velocity(), v_floor), r, remaining) until `time`
+    // is used up (to 1e-6) or max_substeps ran; returns the substep count.
+    // Not in the reference's class.
+    u32 advance(const TimeStepper& ts, f32 v_floor, f32 time, u32 max_substeps) {
+        const TimeStepper::Config t = ts.config();
+        const wsim_timestep_config c{t.frame_dt, t.dt_max, t.cfl_lambda, t.max_substeps};
+        u32 n = 0;
+        detail::check(wsim_solver_advance(h_, &c, v_floor, time, max_substeps, nullptr, nullptr, &n));
+        return n;
+    }
+    void* stream() const noexcept { return wsim_solver_stream(h_); }
     f32 max_velocity() const noexcept { return wsim_solver_max_velocity(h_); }
     f32 particle_mass() const noexcept { return wsim_solver_particle_mass(h_); }
     u32 last_density_iters() const noexcept { return wsim_solver_last_density_iters(h_); }
