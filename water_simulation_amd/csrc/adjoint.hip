@@ -171,6 +171,13 @@ template <int G> __device__ __forceinline__ void group_sum3(D3& a) {
     a.z = group_sum<G>(a.z);
 }
 
+// acc[k] += part[k]: viscosity partials accumulated over substeps on device.
+__global__ __launch_bounds__(NS) void k_add_parts(const double* __restrict__ part, u32 n,
+                                                  double* __restrict__ acc) {
+    const u32 k = blockIdx.x * NS + threadIdx.x;
+    if (k < n) acc[k] += part[k];
+}
+
 // Advection (dfsph_kernels.cu:363-386) transposed with the recorded clamp
 // masks; gathers the output adjoints from original order and opens the
 // per-particle accumulators.
@@ -367,12 +374,14 @@ __global__ __launch_bounds__(NA) void k_adj_rhodot(Nbr P, PairList L, const floa
     xbar[i] = make_double4(x0.x + xa.x, x0.y + xa.y, x0.z + xa.z, 0.0);
 }
 
-// Transposed Jacobi XSPH (dfsph_kernels.cu:305-341): stencil walk of the
-// forward pass (pairs with W != 0 and a zero gradient included).  Writes
+// Transposed Jacobi XSPH (dfsph_kernels.cu:305-341) over the pairs of the
+// forward pass (every pair with W != 0: the fluid list, or the stencil walk
+// for particles whose list is incomplete), G lanes per particle.  Writes
 // v0_bar, adds the rho and x sensitivities of e_ij and one binary64
-// viscosity partial per tile.  With the viscosity <= 0 the forward pass
+// viscosity partial per workgroup.  With the viscosity <= 0 the forward pass
 // skipped XSPH (dfsph.cu:198): v0_bar = v1_bar and no viscosity derivative.
-__global__ __launch_bounds__(NA) void k_adj_xsph(Nbr P, const float4* __restrict__ xs,
+template <int G>
+__global__ __launch_bounds__(NA) void k_adj_xsph(Nbr P, PairList L, const float4* __restrict__ xs,
                                                  const float4* __restrict__ v0,
                                                  const double4* __restrict__ v1bar,
                                                  const float* __restrict__ rho, float eps,
@@ -381,24 +390,24 @@ __global__ __launch_bounds__(NA) void k_adj_xsph(Nbr P, const float4* __restrict
                                                  double4* __restrict__ xbar,
                                                  double* __restrict__ part) {
     __shared__ double s_red[NA / 64];
-    const u32 i = slot_index<NA>(P.remap);
+    const u32 t = slot_index<NA>(P.remap);
+    const u32 i = t / G, g = t % G;
     double ep = 0.0;
+    D3 acc = {0.0, 0.0, 0.0}, xa = {0.0, 0.0, 0.0};
+    double rb = 0.0;
+    bool valid = false;
+    D3 bi = {0.0, 0.0, 0.0};
     if (i < P.nslots) {
-        const SceneSeg& G = P.seg[scene_of_slot(P, i)];
-        if (i - G.off < G.n) {
+        const SceneSeg& S = P.seg[scene_of_slot(P, i)];
+        valid = i - S.off < S.n;
+        if (valid) {
             const float4 p = xs[i];
             const D3 vi = d3f(v0[i]);
-            const D3 bi = d3(v1bar[i]);
+            bi = d3(v1bar[i]);
             const float rho_i = rho[i];
             const bool on = eps > 0.0f;
             const double de = eps, m = P.mass;
-            D3 acc = {0.0, 0.0, 0.0}, xa = {0.0, 0.0, 0.0};
-            double rb = 0.0;
-            for_stencil(P.gf, P.csf + G.key_base, G.off - G.start, p.x, p.y, p.z, [&](u32 s) {
-                if (s == i) return;
-                const float4 q = xs[s];
-                const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
-                if (r2 >= P.sp.far2) return;
+            auto term = [&](u32 s, const float4& q, float r2) {
                 const float w = spline_W(sqrtf(r2), P.sp);
                 if (w == 0.0f) return;
                 const float avg = 0.5f * (rho_i + rho[s]);
@@ -412,18 +421,45 @@ __global__ __launch_bounds__(NA) void k_adj_xsph(Nbr P, const float4* __restrict
                     axpy(acc, e, sub(bj, bi));
                     const double sd = dot(sub(bi, bj), dv);
                     if (avg > 1e-9f) rb -= sd * e / (2.0 * static_cast<double>(am));
-                    const Geo g = pair_geo(p, q, P.sp);
-                    axpy(xa, sd * de * m / static_cast<double>(am), g.grad());
+                    const Geo gg = pair_geo(p, q, P.sp);
+                    axpy(xa, sd * de * m / static_cast<double>(am), gg.grad());
                 }
-            });
-            v0bar[i] = make_double4(bi.x + acc.x, bi.y + acc.y, bi.z + acc.z, 0.0);
-            rhobar[i] += rb;
-            const double4 x0 = xbar[i];
-            xbar[i] = make_double4(x0.x + xa.x, x0.y + xa.y, x0.z + xa.z, 0.0);
+            };
+            const u32 cf = L.cnt[i] & 0xFFFFu;
+            if (cf <= L.kf) {
+                // a complete fluid list holds every in-support pair but the self
+                // pair (the forward XSPH reads the same list)
+                const StageRanges R = L.plan[i / kTile];
+                for (u32 k = g; k < cf; k += G) {
+                    u32 j;
+                    float c;
+                    fluid_entry(L, R, k, i, j, c);
+                    const float4 q = xs[j];
+                    term(j, q, dist2(p.x - q.x, p.y - q.y, p.z - q.z));
+                }
+            } else if (g == 0) {
+                for_stencil(P.gf, P.csf + S.key_base, S.off - S.start, p.x, p.y, p.z, [&](u32 s) {
+                    if (s == i) return;
+                    const float4 q = xs[s];
+                    const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+                    if (r2 >= P.sp.far2) return;
+                    term(s, q, r2);
+                });
+            }
         }
     }
+    group_sum3<G>(acc);
+    group_sum3<G>(xa);
+    rb = group_sum<G>(rb);
+    if (valid && g == 0) {
+        v0bar[i] = make_double4(bi.x + acc.x, bi.y + acc.y, bi.z + acc.z, 0.0);
+        rhobar[i] += rb;
+        const double4 x0 = xbar[i];
+        xbar[i] = make_double4(x0.x + xa.x, x0.y + xa.y, x0.z + xa.z, 0.0);
+    }
+    // one viscosity partial per workgroup, summed in a fixed order
     const double bs = block_sum<NA>(ep, s_red);
-    if (threadIdx.x == 0 && (blockIdx.x + 1) * NA <= P.nslots) part[i / kTile] = bs;
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
 // alpha = rho / (|G|^2 + Q) (dfsph_kernels.cu:117-169): recomputes G and the
@@ -475,6 +511,7 @@ __global__ __launch_bounds__(NA) void k_adj_alpha(Nbr P, PairList L, const float
 // Position sensitivities of rho, G and Q (dfsph_kernels.cu:68-169), then the
 // input adjoints in original order: x_bar, and v_bar = decay v0_bar
 // (dfsph_kernels.cu:351-354).
+template <int G>
 __global__ __launch_bounds__(NA) void k_adj_geom(Nbr P, PairList L, const u32* __restrict__ perm,
                                                  const float4* __restrict__ xs,
                                                  const double* __restrict__ rhobar,
@@ -483,45 +520,55 @@ __global__ __launch_bounds__(NA) void k_adj_geom(Nbr P, PairList L, const u32* _
                                                  const double4* __restrict__ v0bar,
                                                  double* __restrict__ gx,
                                                  double* __restrict__ gv) {
-    const u32 i = slot_index<NA>(P.remap);
-    if (i >= P.nslots) return;
-    const SceneSeg& G = P.seg[scene_of_slot(P, i)];
-    const u32 l = i - G.off;
-    if (l >= G.n) return;
-    const float4 p = xs[i];
-    const double ri = rhobar[i];
-    const double4 gi4 = gbar[i];
-    const D3 gi = d3(gi4);
-    const double qi = gi4.w;
-    const double m = P.mass;
-    D3 xa = d3(xbar[i]);
-    for_pairs(
-        P, L, G, i, p, xs,
-        [&](u32 j, float) {
-            const Geo g = pair_geo(p, xs[j], P.sp);
-            const double4 gj4 = gbar[j];
-            const D3 gr = g.grad();
-            axpy(xa, m * (ri + rhobar[j]), gr);
-            const D3 hg = g.hess(sub(gi, d3(gj4)));
-            axpy(xa, m, hg);
-            axpy(xa, 2.0 * m * m * (qi + gj4.w), g.hess(gr));
-        },
-        [&](u32 b, float) {
-            const float4 q = P.xb[b];
-            const Geo g = pair_geo(p, q, P.sp);
-            const double mb = q.w;
-            const D3 gr = g.grad();
-            axpy(xa, mb * ri, gr);
-            axpy(xa, mb, g.hess(gi));
-            axpy(xa, 2.0 * mb * mb * qi, g.hess(gr));
-        });
-    const u32 id = perm[G.start + l];
+    const u32 t = slot_index<NA>(P.remap);
+    const u32 i = t / G, g = t % G;
+    bool valid = false;
+    const SceneSeg* Sp = nullptr;
+    D3 xa = {0.0, 0.0, 0.0};
+    if (i < P.nslots) {
+        Sp = &P.seg[scene_of_slot(P, i)];
+        valid = i - Sp->off < Sp->n;
+    }
+    if (valid) {
+        const SceneSeg& S = *Sp;
+        const float4 p = xs[i];
+        const double ri = rhobar[i];
+        const double4 gi4 = gbar[i];
+        const D3 gi = d3(gi4);
+        const double qi = gi4.w;
+        const double m = P.mass;
+        for_pairs_split<G>(
+            P, L, S, i, g, p, xs,
+            [&](u32 j, float) {
+                const Geo ge = pair_geo(p, xs[j], P.sp);
+                const double4 gj4 = gbar[j];
+                const D3 gr = ge.grad();
+                axpy(xa, m * (ri + rhobar[j]), gr);
+                const D3 hg = ge.hess(sub(gi, d3(gj4)));
+                axpy(xa, m, hg);
+                axpy(xa, 2.0 * m * m * (qi + gj4.w), ge.hess(gr));
+            },
+            [&](u32 b, float) {
+                const float4 q = P.xb[b];
+                const Geo ge = pair_geo(p, q, P.sp);
+                const double mb = q.w;
+                const D3 gr = ge.grad();
+                axpy(xa, mb * ri, gr);
+                axpy(xa, mb, ge.hess(gi));
+                axpy(xa, 2.0 * mb * mb * qi, ge.hess(gr));
+            });
+    }
+    group_sum3<G>(xa);
+    if (!valid || g != 0) return;
+    const SceneSeg& S = *Sp;
+    const D3 x0 = d3(xbar[i]);
+    const u32 id = perm[S.start + (i - S.off)];
     const u64 o = 3ull * id;
-    gx[o] = xa.x;
-    gx[o + 1] = xa.y;
-    gx[o + 2] = xa.z;
+    gx[o] = x0.x + xa.x;
+    gx[o + 1] = x0.y + xa.y;
+    gx[o + 2] = x0.z + xa.z;
     const double4 vb = v0bar[i];
-    const double dec = G.decay;
+    const double dec = S.decay;
     gv[o] = dec * vb.x;
     gv[o + 1] = dec * vb.y;
     gv[o + 2] = dec * vb.z;
@@ -538,7 +585,7 @@ void DFSPHSolver::set_viscosity(float eps) {
 }
 
 double DFSPHSolver::adjoint_step(const SubstepTape& t, const double* gx_out, const double* gv_out,
-                                 double* gx_in, double* gv_in) {
+                                 double* gx_in, double* gv_in, double* visc_acc) {
     if (batched_) fail(kInvalidArgument, "adjoint_step: plain solver only");
     const size_t nsz = fluid_.count();
     if (nsz == 0 || t.n != nsz) fail(kInvalidArgument, "adjoint_step: tape and store sizes differ");
@@ -561,8 +608,9 @@ double DFSPHSolver::adjoint_step(const SubstepTape& t, const double* gx_out, con
     adj_.rdbar.ensure(ns);
     adj_.rhobar.ensure(ns);
     adj_.alphabar.ensure(ns);
-    adj_.part.ensure(ns / kTile);
-    if (adj_.hpart.size() < ns / kTile) adj_.hpart = PinnedBuffer<double>(ns / kTile);
+    const size_t nparts = ns * kAdjointLanes / NA;  // viscosity partials (one per workgroup)
+    adj_.part.ensure(nparts);
+    if (adj_.hpart.size() < nparts) adj_.hpart = PinnedBuffer<double>(nparts);
 
     const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), bnear_.data(), nb_, cfg_, mass_, nslots_,
                            dseg_.data(), nullptr, shapes_.remap);
@@ -592,25 +640,33 @@ double DFSPHSolver::adjoint_step(const SubstepTape& t, const double* gx_out, con
         WSIM_LAUNCH_CHECK();
     }
     const float4* v0 = any_xsph_ ? vpre_.data() : vs_.data();
-    k_adj_xsph<<<blocks, NA, 0, stream_>>>(P, xs_.data(), v0, adj_.wbar.data(), rho_.data(),
-                                            viscosity_[0], adj_.v0bar.data(), adj_.rhobar.data(),
-                                            adj_.xbar.data(), adj_.part.data());
+    k_adj_xsph<GL><<<gblocks, NA, 0, stream_>>>(P, L, xs_.data(), v0, adj_.wbar.data(),
+                                                 rho_.data(), viscosity_[0], adj_.v0bar.data(),
+                                                 adj_.rhobar.data(), adj_.xbar.data(),
+                                                 adj_.part.data());
     WSIM_LAUNCH_CHECK();
     k_adj_alpha<<<blocks, NA, 0, stream_>>>(P, L, xs_.data(), rho_.data(), adj_.alphabar.data(),
                                              adj_.rhobar.data(), adj_.gbar.data());
     WSIM_LAUNCH_CHECK();
-    k_adj_geom<<<blocks, NA, 0, stream_>>>(P, L, fluid_grid_.sorted_indices(), xs_.data(),
-                                            adj_.rhobar.data(), adj_.gbar.data(), adj_.xbar.data(),
-                                            adj_.v0bar.data(), gx_in, gv_in);
+    k_adj_geom<GL><<<gblocks, NA, 0, stream_>>>(P, L, fluid_grid_.sorted_indices(), xs_.data(),
+                                                 adj_.rhobar.data(), adj_.gbar.data(),
+                                                 adj_.xbar.data(), adj_.v0bar.data(), gx_in, gv_in);
     WSIM_LAUNCH_CHECK();
-    const size_t ntiles = ns / kTile;
-    WSIM_HIP(hipMemcpyAsync(adj_.hpart.data(), adj_.part.data(), ntiles * sizeof(double),
+    if (visc_acc) {
+        // the caller sums the partials once, after many substeps
+        k_add_parts<<<div_up(gblocks, NS), NS, 0, stream_>>>(adj_.part.data(), gblocks, visc_acc);
+        WSIM_LAUNCH_CHECK();
+        return 0.0;
+    }
+    WSIM_HIP(hipMemcpyAsync(adj_.hpart.data(), adj_.part.data(), gblocks * sizeof(double),
                             hipMemcpyDeviceToHost, stream_));
     WSIM_HIP(hipStreamSynchronize(stream_));
     double g = 0.0;
-    for (size_t k = 0; k < ntiles; ++k) g += adj_.hpart[k];
+    for (u32 k = 0; k < gblocks; ++k) g += adj_.hpart[k];
     return g;
 }
+
+size_t DFSPHSolver::adjoint_partials() const { return nslots_ * kAdjointLanes / NA; }
 
 // ---------------------------------------------------------------------------
 Rollout::Rollout(DFSPHSolver& solver, ParticleStore& fluid, u32 every)
@@ -659,6 +715,9 @@ double Rollout::backward(const double* gx, const double* gv, double* gx0, double
     int cur = 0;
     double gvisc = 0.0;
     if (tapes_.size() < std::min<u32>(every_, T)) tapes_.resize(std::min<u32>(every_, T));
+    const size_t nparts = solver_.adjoint_partials();
+    visc_acc_.ensure(nparts);
+    WSIM_HIP(hipMemsetAsync(visc_acc_.data(), 0, nparts * sizeof(double), s));
     const u32 nck = (T + every_ - 1) / every_;
     for (u32 seg = nck; seg-- > 0;) {
         const u32 start = seg * every_, end = std::min(T, start + every_);
@@ -667,14 +726,18 @@ double Rollout::backward(const double* gx, const double* gv, double* gx0, double
         for (u32 k = start; k < end; ++k) solver_.step_taped(dts_[k], tapes_[k - start]);
         for (u32 k = end; k-- > start;) {
             const int nxt = cur ^ 2;
-            gvisc += solver_.adjoint_step(tapes_[k - start], g_[cur].data(), g_[cur + 1].data(),
-                                          g_[nxt].data(), g_[nxt + 1].data());
+            solver_.adjoint_step(tapes_[k - start], g_[cur].data(), g_[cur + 1].data(),
+                                 g_[nxt].data(), g_[nxt + 1].data(), visc_acc_.data());
             cur = nxt;
         }
     }
     WSIM_HIP(hipMemcpyAsync(gx0, g_[cur].data(), db, hipMemcpyDeviceToDevice, s));
     WSIM_HIP(hipMemcpyAsync(gv0, g_[cur + 1].data(), db, hipMemcpyDeviceToDevice, s));
+    std::vector<double> acc(nparts);
+    WSIM_HIP(hipMemcpyAsync(acc.data(), visc_acc_.data(), nparts * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
     WSIM_HIP(hipStreamSynchronize(s));
+    for (double a : acc) gvisc += a;
     return gvisc;
 }
 

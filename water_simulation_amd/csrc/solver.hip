@@ -2112,7 +2112,9 @@ void DFSPHSolver::autotune(int reps) {
                  dstate_.data(), vs_.data(), vs_.data()};
     try {
         // cell_start pass
-        const u32 ncs = scenes * (fluid_grid_.total_cells() + 1u) + 1u;
+        // cell_start entries: C + 1 for one scene, scenes * (C + 1) + 1 segmented
+        const u32 ncs = batched_ ? scenes * (fluid_grid_.total_cells() + 1u) + 1u
+                                 : fluid_grid_.total_cells() + 1u;
         double bt = 1e300;
         for (u32 cpt : {4u, 8u, 16u}) {
             const double ms = time_ms([&] {

@@ -369,8 +369,12 @@ public:
     // d(loss)/d(input positions, velocities), all binary64 device arrays of
     // 3n in original order; returns this substep's d(loss)/d(viscosity).
     // Leaves the store at the substep's input state.
+    // With visc_acc, the viscosity partials are added to visc_acc on the
+    // device (adjoint_partials() entries) and 0 is returned; otherwise the
+    // step waits for them and returns their sum.
     double adjoint_step(const SubstepTape& tape, const double* gx_out, const double* gv_out,
-                        double* gx_in, double* gv_in);
+                        double* gx_in, double* gv_in, double* visc_acc = nullptr);
+    size_t adjoint_partials() const;
 
 private:
     void iterations_taped(u32 begin, u32 end, SubstepTape& t);
@@ -475,6 +479,7 @@ private:
     std::vector<DeviceBuffer<float>> ckx_, ckv_;
     std::vector<SubstepTape> tapes_;
     DeviceBuffer<double> g_[4];
+    DeviceBuffer<double> visc_acc_;  // viscosity partials summed over the backward pass
 };
 
 }  // namespace wsim
