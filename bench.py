@@ -292,6 +292,27 @@ def adjoint_leg(substeps: int, every: int):
     }
 
 
+def lbvh_leg(x, reps: int) -> dict:
+    """SpatialAccel::build on the large dam break's 1M positions: Morton codes,
+    their 30-bit stable sort, the Karras internal nodes and the node boxes
+    (spatial_accel.cu:111-199, internal boxes merged bottom-up)."""
+    import water_simulation_amd as ws
+    lo, hi = x.min(0), x.max(0)
+    acc = ws.SpatialAccel()
+    buf = ws.DeviceBuffer.from_array(x)
+    acc.build(buf.ptr, len(x), lo, hi)  # warm-up
+    ws.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        acc.build(buf.ptr, len(x), lo, hi)
+    ws.device_synchronize()
+    el = (time.perf_counter() - t0) / reps
+    _, _, parent = acc.read_internal_nodes()
+    return {"workload": f"SpatialAccel build over the {len(x)} dam-break positions",
+            "points": len(x), "builds": reps, "ms_per_build": el * 1e3,
+            "points_per_s": len(x) / el, "root_parent_ok": bool(parent[0] == 0xFFFFFFFF)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -314,6 +335,9 @@ def main() -> int:
     ap.add_argument("--adjoint-substeps", type=int, default=200,
                     help="substeps of the adjoint leg (0: skip it)")
     ap.add_argument("--only-adjoint", action="store_true", help="run only the adjoint leg")
+    ap.add_argument("--lbvh-reps", type=int, default=10,
+                    help="SpatialAccel builds timed on the 1M positions (0: skip)")
+    ap.add_argument("--only-lbvh", action="store_true", help="run only the SpatialAccel leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run libwsim's streaming copy kernel 5 times over BYTES bytes "
                          "(the known-size workload for converting HBM byte counters)")
@@ -343,6 +367,10 @@ def main() -> int:
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
 
+    if args.only_lbvh:
+        x, _, _ = make_workload(args.n_side)
+        print(json.dumps({"lbvh": lbvh_leg(x, max(args.lbvh_reps, 1))}))
+        return 0
     if args.only_adjoint:
         print(json.dumps({"adjoint": adjoint_leg(args.adjoint_substeps, 20)}))
         return 0
@@ -454,6 +482,7 @@ def main() -> int:
         "reference_pressure_form": reference_form,
         "batched": batched,
         "adjoint": adjoint,
+        "lbvh": lbvh_leg(x, args.lbvh_reps) if args.lbvh_reps > 0 else None,
     }
     print(json.dumps(out))
     return 0

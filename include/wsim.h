@@ -104,18 +104,32 @@ float wsim_grid_cell_length(wsim_grid grid);
 uint32_t wsim_grid_total_cells(wsim_grid grid);
 size_t wsim_grid_count(wsim_grid grid);
 
-/* ---- SpatialAccel, Morton part (core/include/water/core/spatial_accel.h:19-53) ---- */
+/* ---- SpatialAccel (core/include/water/core/spatial_accel.h:19-53) ---- */
 typedef struct wsim_accel_t* wsim_accel;
 wsim_status wsim_accel_create(wsim_accel* out);
 void wsim_accel_destroy(wsim_accel accel);
-/* build(positions, count, scene_bounds, stream) (spatial_accel.cu:111-179):
- * 30-bit Morton codes and their stable sort. */
+/* build(positions, count, scene_bounds, stream) (spatial_accel.cu:111-199):
+ * 30-bit Morton codes, their stable sort, the Karras internal nodes and the
+ * node boxes (leaves: the point; internal: the union of the children, which
+ * the reference leaves for later). */
 wsim_status wsim_accel_build(wsim_accel accel, const float* positions_xyz, size_t count,
                              const float bounds_min[3], const float bounds_max[3], void* stream);
 size_t wsim_accel_leaf_count(wsim_accel accel);
 const uint32_t* wsim_accel_sorted_indices(wsim_accel accel);
 /* Sorted Morton codes (the reference keeps them in morton_codes_). */
 const uint32_t* wsim_accel_morton_codes(wsim_accel accel);
+/* internal_left() / internal_right() (spatial_accel.h:36-37): children of the
+ * leaf_count - 1 internal nodes; a child >= leaf_count - 1 is the leaf
+ * child - (leaf_count - 1), i.e. the sorted particle at that slot.  NULL for
+ * fewer than 2 leaves. */
+const uint32_t* wsim_accel_internal_left(wsim_accel accel);
+const uint32_t* wsim_accel_internal_right(wsim_accel accel);
+/* Parent of every node (2 leaf_count - 1 entries, internal nodes first); the
+ * root's is 0xFFFFFFFF (the reference keeps internal_parent_ private). */
+const uint32_t* wsim_accel_internal_parent(wsim_accel accel);
+/* node_aabbs() (spatial_accel.h:38): per node 6 floats (min x, y, z, max x,
+ * y, z), the leaf_count - 1 internal nodes first, then the leaves. */
+const float* wsim_accel_node_aabbs(wsim_accel accel);
 
 /* ---- DFSPHSolver (solvers/dfsph/include/water/solvers/dfsph.h:17-84) ---- */
 typedef struct {

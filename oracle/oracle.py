@@ -77,6 +77,7 @@ _SIGS = {
     "orc_make_grid": (None, [F3, F3, C.c_float, F3, I3]),
     "orc_neighbor_counts": (None, [vp, sz, vp, sz, F3, I3, C.c_float, C.c_float, vp, vp]),
     "orc_morton": (None, [vp, sz, F3, F3, vp, vp, vp]),
+    "orc_lbvh": (None, [vp, vp, vp, sz, vp, vp, vp, vp]),
     "orc_sample_aabb_boundary": (sz, [F3, F3, C.c_float, C.c_int, vp, sz]),
     "orc_lattice_block": (sz, [F3, F3, C.c_float, vp, sz]),
     "orc_next_dt": (C.c_float, [C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_float,
@@ -194,6 +195,22 @@ def morton(x, bmin, bmax):
     lib().orc_morton(x.ctypes.data, n, _f3(bmin), _f3(bmax), codes.ctypes.data, sc.ctypes.data,
                      perm.ctypes.data)
     return codes, sc, perm
+
+
+def lbvh(x, sorted_codes, perm):
+    """Karras radix tree and node boxes over sorted Morton codes
+    (spatial_accel.cu:44-113): -> (left, right, parent, boxes (2n-1, 2, 3))."""
+    x = _xyz(x)
+    n = len(sorted_codes)
+    ni = max(n - 1, 0)
+    left, right = np.zeros(ni, np.uint32), np.zeros(ni, np.uint32)
+    parent = np.zeros(n + ni, np.uint32)
+    boxes = np.zeros((n + ni, 2, 3), np.float32)
+    sc = np.ascontiguousarray(sorted_codes, np.uint32)
+    pm = np.ascontiguousarray(perm, np.uint32)
+    lib().orc_lbvh(sc.ctypes.data, x.ctypes.data, pm.ctypes.data, n, left.ctypes.data,
+                   right.ctypes.data, parent.ctypes.data, boxes.ctypes.data)
+    return left, right, parent, boxes
 
 
 def sample_aabb_boundary(lo, hi, spacing: float, n_layers: int = 3, use_reference=False):

@@ -855,6 +855,71 @@ void orc_morton(const float* xyz, size_t n, const float bmin[3], const float bma
     }
 }
 
+// Karras 2012 radix tree over the sorted codes (spatial_accel.cu:44-100):
+// common prefix of leaves i, j in the total order (code, index); direction,
+// range end and split of every internal node.  Then the boxes: leaves are
+// their points (spatial_accel.cu:102-113, radius 0) and internal nodes the
+// union of their children, filled children-first by an explicit post-order.
+static int lbvh_prefix(const uint32_t* c, long long n, long long i, long long j) {
+    if (j < 0 || j >= n) return -1;
+    if (c[i] == c[j]) {
+        const uint32_t x = static_cast<uint32_t>(i ^ j);
+        return 32 + (x ? __builtin_clz(x) : 32);
+    }
+    return __builtin_clz(c[i] ^ c[j]);
+}
+
+void orc_lbvh(const uint32_t* codes, const float* xyz, const uint32_t* perm, size_t count,
+              uint32_t* left, uint32_t* right, uint32_t* parent, float* boxes) {
+    const long long n = static_cast<long long>(count), ni = n - 1;
+    if (n == 0) return;
+    parent[0] = 0xFFFFFFFFu;
+    for (long long i = 0; i < ni; ++i) {
+        const long long d = lbvh_prefix(codes, n, i, i + 1) - lbvh_prefix(codes, n, i, i - 1) > 0 ? 1 : -1;
+        const int dmin = lbvh_prefix(codes, n, i, i - d);
+        long long lmax = 2;
+        while (lbvh_prefix(codes, n, i, i + lmax * d) > dmin) lmax *= 2;
+        long long l = 0;
+        for (long long t = lmax / 2; t > 0; t /= 2)
+            if (lbvh_prefix(codes, n, i, i + (l + t) * d) > dmin) l += t;
+        const long long j = i + l * d;
+        const int dnode = lbvh_prefix(codes, n, i, j);
+        long long s = 0;
+        for (long long t = (l + 1) / 2; t > 0; t = (t == 1) ? 0 : (t + 1) / 2)
+            if (lbvh_prefix(codes, n, i, i + (s + t) * d) > dnode) s += t;
+        const long long g = i + s * d + (d < 0 ? -1 : 0);
+        const long long lo = std::min(i, j), hi = std::max(i, j);
+        left[i] = static_cast<uint32_t>(lo == g ? ni + g : g);
+        right[i] = static_cast<uint32_t>(hi == g + 1 ? ni + g + 1 : g + 1);
+        parent[left[i]] = static_cast<uint32_t>(i);
+        parent[right[i]] = static_cast<uint32_t>(i);
+    }
+    for (long long k = 0; k < n; ++k) {
+        float* b = boxes + 6 * (ni + k);
+        for (int a = 0; a < 3; ++a) b[a] = b[a + 3] = xyz[3 * size_t(perm[k]) + a];
+    }
+    if (ni == 0) return;
+    std::vector<std::pair<uint32_t, bool>> stack{{0u, false}};
+    while (!stack.empty()) {
+        auto [node, done] = stack.back();
+        stack.pop_back();
+        if (node >= ni) continue;  // a leaf
+        if (!done) {
+            stack.push_back({node, true});
+            stack.push_back({left[node], false});
+            stack.push_back({right[node], false});
+            continue;
+        }
+        const float* a = boxes + 6 * size_t(left[node]);
+        const float* b = boxes + 6 * size_t(right[node]);
+        float* c = boxes + 6 * size_t(node);
+        for (int k = 0; k < 3; ++k) {
+            c[k] = std::fmin(a[k], b[k]);
+            c[k + 3] = std::fmax(a[k + 3], b[k + 3]);
+        }
+    }
+}
+
 // boundary_sampler.cpp:6-32: six faces (-X,+X,-Y,+Y,-Z,+Z), n_layers inward
 // layers each, binary32 accumulating loops over the two in-face axes.
 size_t orc_sample_aabb_boundary(const float lo[3], const float hi[3], float spacing,

@@ -107,6 +107,8 @@ void orc_neighbor_counts(const float* xq, size_t nq, const float* xs, size_t ns,
 /* --- Morton path of SpatialAccel (core/src/spatial_accel.cu:11-40,139-179) --- */
 void orc_morton(const float* xyz, size_t n, const float bmin[3], const float bmax[3],
                 uint32_t* codes, uint32_t* sorted_codes, uint32_t* perm);
+void orc_lbvh(const uint32_t* sorted_codes, const float* xyz, const uint32_t* perm, size_t n,
+              uint32_t* left, uint32_t* right, uint32_t* parent, float* boxes);
 
 /* --- host-side surface --- */
 /* solvers/dfsph/src/boundary_sampler.cpp:6-32; returns count, writes if out != NULL */

@@ -101,3 +101,41 @@ class Pair:
         if rs["density_iters"]:
             assert abs(gs["mean_density_error"] - rs["mean_density_error"]) <= \
                 1e-12 * max(1.0, abs(rs["mean_density_error"])), (tag, gs, rs)
+
+
+def radix_tree_top_down(sorted_codes, points):
+    """Binary radix tree over sorted Morton codes built top-down from its
+    definition: a node's range [lo, hi] splits after the last leaf that shares
+    the range's longest common prefix in the order (code, index); Karras's
+    numbering puts an internal node at the end of its range that touches its
+    parent's split (the root is node 0).  Independent of the per-node search
+    of spatial_accel.cu:57-100.  -> (left, right, parent, boxes (2n-1, 2, 3))."""
+    n = len(sorted_codes)
+    ni = n - 1
+    key = [(int(c) << 32) | k for k, c in enumerate(sorted_codes)]
+    left = np.zeros(max(ni, 0), np.uint32)
+    right = np.zeros(max(ni, 0), np.uint32)
+    parent = np.zeros(n + max(ni, 0), np.uint32)
+    parent[0] = 0xFFFFFFFF
+    boxes = np.zeros((n + max(ni, 0), 2, 3), np.float32)
+    pts = np.asarray(points, np.float32)
+    boxes[ni:, 0] = pts
+    boxes[ni:, 1] = pts
+    stack = [(0, 0, n - 1)] if n > 1 else []
+    while stack:
+        node, lo, hi = stack.pop()
+        top = (key[lo] ^ key[hi]).bit_length()  # bits below the common prefix
+        g = lo
+        while g + 1 <= hi and (key[lo] ^ key[g + 1]).bit_length() < top:
+            g += 1
+        a = ni + g if g == lo else g
+        b = ni + g + 1 if g + 1 == hi else g + 1
+        left[node], right[node] = a, b
+        parent[a] = parent[b] = node
+        boxes[node, 0] = pts[lo:hi + 1].min(0)
+        boxes[node, 1] = pts[lo:hi + 1].max(0)
+        if a < ni:
+            stack.append((a, lo, g))
+        if b < ni:
+            stack.append((b, g + 1, hi))
+    return left, right, parent, boxes

@@ -116,3 +116,44 @@ def test_morton_single_particle():
     ws.device_synchronize()
     assert acc.leaf_count() == 1
     assert acc.read_sorted_indices().tolist() == [0]
+
+
+@pytest.mark.parametrize("case,n", [("random", 8), ("random", 5000), ("random", 300_000),
+                                    ("duplicates", 20_000), ("identical", 513), ("two", 2)])
+def test_lbvh_nodes_and_boxes_match_oracle(case, n):
+    # spatial_accel.cu:44-113 (Karras internal nodes, leaf boxes) plus the
+    # bottom-up internal boxes; integers and floats bit for bit
+    rng = np.random.default_rng(n)
+    if case == "random":
+        x = rng.uniform(-0.05, 1.05, (n, 3)).astype(np.float32)
+    elif case == "duplicates":
+        x = (rng.integers(0, 6, (n, 3)) / 6 + 0.02).astype(np.float32)
+    elif case == "identical":
+        x = np.full((n, 3), 0.3, np.float32)
+    else:
+        x = np.array([[0.2, 0.2, 0.2], [0.8, 0.1, 0.3]], np.float32)
+    acc = ws.SpatialAccel()
+    buf = ws.DeviceBuffer.from_array(x)
+    acc.build(buf.ptr, n, (0, 0, 0), (1, 1, 1))
+    ws.device_synchronize()
+    codes, sc, perm = O.morton(x, (0, 0, 0), (1, 1, 1))
+    wl, wr, wp, wb = O.lbvh(x, sc, perm)
+    gl, gr, gp = acc.read_internal_nodes()
+    assert_bitexact(gl, wl, "left")
+    assert_bitexact(gr, wr, "right")
+    assert_bitexact(gp, wp, "parent")
+    assert_bitexact(acc.read_node_aabbs(), wb, "boxes")
+    # rebuilding into the same object clears the arrival counters
+    acc.build(buf.ptr, n, (0, 0, 0), (1, 1, 1))
+    ws.device_synchronize()
+    assert_bitexact(acc.read_node_aabbs(), wb, "boxes, second build")
+
+
+def test_lbvh_single_leaf():
+    acc = ws.SpatialAccel()
+    buf = ws.DeviceBuffer.from_array(np.array([[0.5, 0.25, 0.75]], np.float32))
+    acc.build(buf.ptr, 1, (0, 0, 0), (1, 1, 1))
+    ws.device_synchronize()
+    l, r, p = acc.read_internal_nodes()
+    assert len(l) == 0 and len(r) == 0 and p.tolist() == [0xFFFFFFFF]
+    assert acc.read_node_aabbs().tolist() == [[[0.5, 0.25, 0.75], [0.5, 0.25, 0.75]]]

@@ -239,3 +239,29 @@ def test_lattice_counts_match_reference():
     p = O.lattice_block((0.1, 0.2, 0.3), (0.2, 0.3, 0.4), 0.05)
     assert p[0].tolist() == pytest.approx([0.1, 0.2, 0.3])      # x-major ordering
     assert p[1].tolist() == pytest.approx([0.1, 0.2, 0.35])
+
+
+@pytest.mark.parametrize("case", ["random", "duplicates", "identical", "two"])
+def test_lbvh_against_top_down_radix_tree(case):
+    # the oracle's per-node Karras search (spatial_accel.cu:44-100) against the
+    # tree built top-down from its definition, and the node boxes against
+    # brute-force unions over each node's leaf range
+    from helpers import radix_tree_top_down
+    rng = np.random.default_rng(9)
+    if case == "random":
+        x = rng.uniform(0, 1, (700, 3)).astype(np.float32)
+    elif case == "duplicates":  # many equal codes: the index breaks the ties
+        x = rng.integers(0, 4, (500, 3)).astype(np.float32) / 4 + 0.01
+    elif case == "identical":
+        x = np.full((37, 3), 0.25, np.float32)
+    else:
+        x = np.array([[0.2, 0.2, 0.2], [0.8, 0.1, 0.3]], np.float32)
+    codes, sc, perm = O.morton(x, (0, 0, 0), (1, 1, 1))
+    l, r, p, b = O.lbvh(x, sc, perm)
+    tl, tr, tp, tb = radix_tree_top_down(sc, x[perm])
+    assert np.array_equal(l, tl) and np.array_equal(r, tr) and np.array_equal(p, tp)
+    assert np.array_equal(b.view(np.uint32), tb.view(np.uint32))
+    # every internal node and every leaf but the root has exactly one parent
+    n = len(x)
+    kids = np.concatenate([l, r])
+    assert sorted(kids.tolist()) == list(range(1, 2 * n - 1))

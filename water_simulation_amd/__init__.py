@@ -288,7 +288,8 @@ class CellGrid:
 
 # --------------------------------------------------------------------------
 class SpatialAccel:
-    """Morton codes + their stable sort (spatial_accel.h:19-53, spatial_accel.cu:111-179)."""
+    """Morton codes, their stable sort and the linear BVH over them
+    (spatial_accel.h:19-53, spatial_accel.cu:44-199)."""
 
     def __init__(self):
         h = C.c_void_p()
@@ -318,6 +319,25 @@ class SpatialAccel:
 
     def read_morton_codes(self) -> np.ndarray:
         return _read_u32(_L().wsim_accel_morton_codes(self._h) or 0, self.leaf_count())
+
+    def read_internal_nodes(self):
+        """(left, right, parent): children of the n - 1 internal nodes (a child
+        >= n - 1 is leaf child - (n - 1)) and the parent of every node (root:
+        0xFFFFFFFF)."""
+        n = self.leaf_count()
+        ni = max(n - 1, 0)
+        return (_read_u32(_L().wsim_accel_internal_left(self._h) or 0, ni),
+                _read_u32(_L().wsim_accel_internal_right(self._h) or 0, ni),
+                _read_u32(_L().wsim_accel_internal_parent(self._h) or 0, n + ni))
+
+    def read_node_aabbs(self) -> np.ndarray:
+        """(2n - 1, 2, 3) float32: min and max of every node, internal nodes first."""
+        n = self.leaf_count()
+        m = 2 * n - 1 if n else 0
+        out = np.empty((m, 2, 3), np.float32)
+        if m:
+            memcpy_d2h(out, _L().wsim_accel_node_aabbs(self._h))
+        return out
 
 
 # --------------------------------------------------------------------------

@@ -151,6 +151,10 @@ SIGNATURES = {
     "wsim_accel_leaf_count": (sz, [vp]),
     "wsim_accel_sorted_indices": (vp, [vp]),
     "wsim_accel_morton_codes": (vp, [vp]),
+    "wsim_accel_internal_left": (vp, [vp]),
+    "wsim_accel_internal_right": (vp, [vp]),
+    "wsim_accel_internal_parent": (vp, [vp]),
+    "wsim_accel_node_aabbs": (vp, [vp]),
     "wsim_dfsph_config_default": (None, [C.POINTER(DFSPHConfigC)]),
     "wsim_solver_create": (st, [vp, vp, C.POINTER(DFSPHConfigC), vp, C.POINTER(vp)]),
     "wsim_solver_destroy": (None, [vp]),

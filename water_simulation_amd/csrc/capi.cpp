@@ -296,6 +296,14 @@ const uint32_t* wsim_accel_sorted_indices(wsim_accel a) {
     return a ? a->impl.sorted_indices() : nullptr;
 }
 const uint32_t* wsim_accel_morton_codes(wsim_accel a) { return a ? a->impl.morton_codes() : nullptr; }
+const uint32_t* wsim_accel_internal_left(wsim_accel a) { return a ? a->impl.internal_left() : nullptr; }
+const uint32_t* wsim_accel_internal_right(wsim_accel a) {
+    return a ? a->impl.internal_right() : nullptr;
+}
+const uint32_t* wsim_accel_internal_parent(wsim_accel a) {
+    return a ? a->impl.internal_parent() : nullptr;
+}
+const float* wsim_accel_node_aabbs(wsim_accel a) { return a ? a->impl.node_aabbs() : nullptr; }
 
 // ---- DFSPHSolver ----
 void wsim_dfsph_config_default(wsim_dfsph_config* c) {

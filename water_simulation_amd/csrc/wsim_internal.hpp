@@ -117,10 +117,23 @@ public:
     size_t leaf_count() const { return leaf_count_; }
     const u32* sorted_indices() const { return leaf_count_ ? v_[cur_].data() : nullptr; }
     const u32* morton_codes() const { return leaf_count_ ? k_[cur_].data() : nullptr; }
+    // Linear BVH (spatial_accel.h:34-53): internal node i in [0, n-1) has
+    // children left/right (>= n - 1: leaf n-1+k, the k-th sorted particle);
+    // parent of every node (root: ~0u); boxes are 6 floats (min, max) per
+    // node, internal nodes first, then leaves.
+    const u32* internal_left() const { return leaf_count_ > 1 ? left_.data() : nullptr; }
+    const u32* internal_right() const { return leaf_count_ > 1 ? right_.data() : nullptr; }
+    const u32* internal_parent() const { return leaf_count_ ? parent_.data() : nullptr; }
+    const float* node_aabbs() const { return leaf_count_ ? boxes_.data() : nullptr; }
 
 private:
     size_t leaf_count_ = 0;
     DeviceBuffer<u32> k_[2], v_[2], hist_;
+    DeviceBuffer<u32> left_, right_, parent_;
+    DeviceBuffer<uint2> range_;      // leaf range [lo, hi] of every internal node
+    DeviceBuffer<float> boxes_, seg_;  // node boxes; segment tree over the leaf boxes
+    DeviceBuffer<u64> seg_off_;
+    std::vector<u64> seg_off_host_;
     int cur_ = 0;
 };
 

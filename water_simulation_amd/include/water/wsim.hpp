@@ -191,6 +191,11 @@ public:
     std::size_t leaf_count() const noexcept { return wsim_accel_leaf_count(h_); }
     const u32* sorted_indices() const noexcept { return wsim_accel_sorted_indices(h_); }
     const u32* morton_codes() const noexcept { return wsim_accel_morton_codes(h_); }
+    const u32* internal_left() const noexcept { return wsim_accel_internal_left(h_); }
+    const u32* internal_right() const noexcept { return wsim_accel_internal_right(h_); }
+    const AABB* node_aabbs() const noexcept {
+        return reinterpret_cast<const AABB*>(wsim_accel_node_aabbs(h_));
+    }
 
 private:
     wsim_accel h_ = nullptr;
