@@ -90,6 +90,8 @@ def cpu_baseline(x, v, xb, cfg, dts, budget_s: float):
         "sample": f"{done} substeps of the timed trajectory from its first state "
                   f"({len(x)} fluid + {len(xb)} boundary particles, {iters} density iterations, "
                   f"{el:.1f} s, binary32 oracle, OpenMP)",
+        "ms_per_substep": el / max(done, 1) * 1e3,
+        "mean_density_iters": iters / max(done, 1),
     }, done, iters
 
 

@@ -138,7 +138,7 @@ typedef struct {
     float smoothing_length;
     float viscosity;             /* XSPH epsilon */
     float damping;
-    float surface_tension;       /* accepted, unused (as in the reference) */
+    float surface_tension;       /* gamma of the surface-tension force when enabled below */
     float gravity[3];
     uint32_t max_density_iters;
     uint32_t max_divergence_iters;
@@ -148,6 +148,9 @@ typedef struct {
     float domain_max[3];
     uint32_t enable_divergence_solve; /* extension: 0 = the reference's step() */
     uint32_t pressure_form;           /* extension: 0 = reference kappa/rho^2, 1 = kappa/rho */
+    uint32_t enable_surface_tension;  /* extension: Akinci 2013 cohesion + curvature with
+                                         coefficient surface_tension (the reference parses the
+                                         coefficient and applies no force) */
 } wsim_dfsph_config;
 
 typedef struct {

@@ -41,6 +41,7 @@ class OrcConfig(C.Structure):
         ("domain_max", C.c_float * 3),
         ("enable_divergence_solve", C.c_uint32),
         ("pressure_form", C.c_uint32),
+        ("enable_surface_tension", C.c_uint32),
     ]
 
 
@@ -87,6 +88,7 @@ _SIGS = {
     "orc_solver_destroy": (None, [vp]),
     "orc_solver_step": (None, [vp, C.c_float]),
     "orc_solver_density_alpha": (None, [vp]),
+    "orc_solver_surface_tension": (None, [vp, C.c_float]),
     "orc_solver_set_state": (None, [vp, vp, vp]),
     "orc_solver_get": (None, [vp, C.c_int, vp]),
     "orc_solver_get_f32": (None, [vp, C.c_int, vp]),
@@ -247,7 +249,7 @@ def config(**kw) -> OrcConfig:
         if k in ("gravity", "domain_min", "domain_max"):
             setattr(c, k, (C.c_float * 3)(*[float(x) for x in v]))
         elif k in ("max_density_iters", "max_divergence_iters", "pressure_form",
-                   "enable_divergence_solve"):
+                   "enable_divergence_solve", "enable_surface_tension"):
             setattr(c, k, int(v))
         else:
             setattr(c, k, float(v))
@@ -282,6 +284,10 @@ class Solver:
 
     def density_alpha(self) -> None:
         lib().orc_solver_density_alpha(self._h)
+
+    def surface_tension(self, dt: float) -> None:
+        """The surface-tension phase alone, on the state after density_alpha()."""
+        lib().orc_solver_surface_tension(self._h, float(dt))
 
     def set_state(self, x=None, v=None) -> None:
         xa = None if x is None else _xyz(x)

@@ -471,6 +471,76 @@ template <class R> struct Solver {
         v.swap(vtmp);
     }
 
+    // Akinci et al. 2013 surface tension (cohesion + curvature), with the
+    // reference's cohesion kernel C(r, L) (sph_kernels.cuh:44-51) on the
+    // kernel support L = 2l; the reference parses the coefficient
+    // (loader.cpp:50, main.cpp:132) but applies no force.  After XSPH:
+    //   n_i = L sum_{j != i} (m / rho_j) grad W_ij
+    //   a_i = sum_{j != i, |r| < L} K_ij (s_ij r_ij - gamma (n_i - n_j)),
+    //   K_ij = 2 rho0 / (rho_i + rho_j),  s_ij = -gamma m C(|r|) / |r| (0 at r = 0)
+    //   v_i += dt a_i
+    // powf(L, 9) and powf(L, 6) are evaluated once on the host.
+    void surface_tension(float dt) {
+        if (!cfg.enable_surface_tension) return;
+        const float Lf = 2.0f * cfg.smoothing_length;
+        R Lr, sig, c6;
+        if constexpr (F32Values<R>::value) {
+            Lr = R(Lf);
+            sig = R(32.0f / (3.14159265358979323846f * std::pow(Lf, 9.0f)));
+            c6 = R(std::pow(Lf, 6.0f) / 64.0f);
+        } else {
+            Lr = R(2) * h;
+            sig = R(32) / (kpi<R>() * std::pow(Lr, R(9)));
+            c6 = std::pow(Lr, R(6)) / R(64);
+        }
+        const R gam = R(cfg.surface_tension), rho0 = R(cfg.rest_density), dtr = R(dt);
+        std::vector<V3<R>> nrm(n);
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)n; ++i) {
+            const V3<R> pi = x[i];
+            V3<R> sn{0, 0, 0};
+            fg.for_neighbors(pi, [&](u32 j) {
+                if (j == u32(i)) return;
+                const V3<R> pj = x[j];
+                const V3<R> g = gradW(V3<R>{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z}, h);
+                const R f = mass / rho[j];
+                sn.x += f * g.x;
+                sn.y += f * g.y;
+                sn.z += f * g.z;
+            }, hc(i));
+            nrm[i] = {Lr * sn.x, Lr * sn.y, Lr * sn.z};
+        }
+        auto cohesion = [&](R r) -> R {
+            if (r <= R(0) || r > Lr) return R(0);
+            const R lm = Lr - r;
+            const R t = lm * lm * lm * r * r * r;
+            if (R(2) * r > Lr) return sig * t;
+            return sig * (R(2) * t - c6);
+        };
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)n; ++i) {
+            const V3<R> pi = x[i];
+            const V3<R> ni = nrm[i];
+            const R rho_i = rho[i];
+            V3<R> a{0, 0, 0};
+            fg.for_neighbors(pi, [&](u32 j) {
+                if (j == u32(i)) return;
+                const V3<R> pj = x[j];
+                const V3<R> r{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
+                const R d = o_sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+                if (!(d / h < R(2))) return;
+                const R K = R(2) * rho0 / (rho_i + rho[j]);
+                const R sc = d > R(0) ? -gam * mass * cohesion(d) / d : R(0);
+                const V3<R> nj = nrm[j];
+                a.x += K * (sc * r.x + -gam * (ni.x - nj.x));
+                a.y += K * (sc * r.y + -gam * (ni.y - nj.y));
+                a.z += K * (sc * r.z + -gam * (ni.z - nj.z));
+            }, hc(i));
+            vtmp[i] = {v[i].x + dtr * a.x, v[i].y + dtr * a.y, v[i].z + dtr * a.z};
+        }
+        v.swap(vtmp);
+    }
+
     // rho_dot_i (dfsph_kernels.cu:196-224)
     R rho_dot(long long i) const {
         const V3<R> pi = x[i];
@@ -711,6 +781,7 @@ template <class R> struct Solver {
         density_alpha();
         gravity(dt);
         xsph();
+        surface_tension(dt);
         density_solve(dt);
         advect(dt);
         if (cfg.enable_divergence_solve) {
@@ -993,6 +1064,7 @@ void orc_config_default(orc_config* c) {
     c->domain_max[0] = c->domain_max[1] = c->domain_max[2] = 1.0f;
     c->enable_divergence_solve = 0;
     c->pressure_form = 0;
+    c->enable_surface_tension = 0;
 }
 
 void* orc_solver_create(const orc_config* cfg, int precision, const float* x, const float* v,
@@ -1081,6 +1153,7 @@ void orc_solver_set_tape(void* s, void* tape, int mode) {
 
 void orc_solver_step(void* s, float dt) { ORC_DISPATCH(s, S->step(dt)); }
 void orc_solver_density_alpha(void* s) { ORC_DISPATCH(s, S->density_alpha()); }
+void orc_solver_surface_tension(void* s, float dt) { ORC_DISPATCH(s, S->surface_tension(dt)); }
 
 void orc_solver_set_state(void* s, const float* x, const float* v) {
     ORC_DISPATCH(s, {

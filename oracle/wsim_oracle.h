@@ -53,6 +53,7 @@ typedef struct {
     float    domain_max[3];
     uint32_t enable_divergence_solve;
     uint32_t pressure_form; /* 0 = reference kappa/rho^2, 1 = kappa/rho (Bender-Koschier) */
+    uint32_t enable_surface_tension; /* Akinci 2013 cohesion + curvature, coefficient surface_tension */
 } orc_config;
 
 /* Fields readable from an oracle solver (values converted to double). */
@@ -129,6 +130,8 @@ void  orc_solver_destroy(void* s);
 void  orc_solver_step(void* s, float dt);
 /* Grid + density + alpha only, on the current positions (dfsph.cu:124-126). */
 void  orc_solver_density_alpha(void* s);
+/* the surface-tension phase alone on the current state (after density_alpha) */
+void  orc_solver_surface_tension(void* s, float dt);
 void  orc_solver_set_state(void* s, const float* x, const float* v);
 /* Same in binary64 (rounded to binary32 for F32 and DUAL solvers). */
 void  orc_solver_set_state_f64(void* s, const double* x, const double* v);

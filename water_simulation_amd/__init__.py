@@ -359,6 +359,7 @@ class DFSPHConfig:
     domain_max: tuple = (1.0, 1.0, 1.0)
     enable_divergence_solve: bool = False
     pressure_form: int = 0  # 0: reference kappa/rho^2, 1: kappa/rho
+    enable_surface_tension: bool = False  # Akinci 2013, coefficient surface_tension
 
     def to_c(self) -> _ffi.DFSPHConfigC:
         c = _ffi.DFSPHConfigC()
@@ -368,7 +369,7 @@ class DFSPHConfig:
                 setattr(c, f.name, (C.c_float * 3)(*[float(x) for x in v]))
             elif f.name in ("max_density_iters", "max_divergence_iters", "pressure_form"):
                 setattr(c, f.name, int(v))
-            elif f.name == "enable_divergence_solve":
+            elif f.name in ("enable_divergence_solve", "enable_surface_tension"):
                 setattr(c, f.name, 1 if v else 0)
             else:
                 setattr(c, f.name, float(v))

@@ -37,6 +37,7 @@ class DFSPHConfigC(C.Structure):
         ("domain_max", C.c_float * 3),
         ("enable_divergence_solve", C.c_uint32),
         ("pressure_form", C.c_uint32),
+        ("enable_surface_tension", C.c_uint32),
     ]
 
 

@@ -673,6 +673,9 @@ Rollout::Rollout(DFSPHSolver& solver, ParticleStore& fluid, u32 every)
     : solver_(solver), fluid_(fluid), every_(every) {
     if (every == 0) fail(kInvalidArgument, "Rollout: checkpoint interval must be positive");
     if (solver.batched()) fail(kInvalidArgument, "Rollout: plain solver only");
+    if (solver.config().enable_divergence_solve || solver.config().enable_surface_tension)
+        fail(kInvalidArgument,
+             "Rollout: the adjoint covers the density solve without surface tension");
 }
 
 void Rollout::forward(const float* x0, const float* v0, const float* dts, u32 substeps) {

@@ -326,6 +326,7 @@ void wsim_dfsph_config_default(wsim_dfsph_config* c) {
     c->eta_divergence = d.eta_divergence;
     c->enable_divergence_solve = d.enable_divergence_solve;
     c->pressure_form = d.pressure_form;
+    c->enable_surface_tension = d.enable_surface_tension;
 }
 
 static wsim::DFSPHConfig to_config(const wsim_dfsph_config* c) {
@@ -348,6 +349,7 @@ static wsim::DFSPHConfig to_config(const wsim_dfsph_config* c) {
     d.eta_divergence = c->eta_divergence;
     d.enable_divergence_solve = c->enable_divergence_solve;
     d.pressure_form = c->pressure_form;
+    d.enable_surface_tension = c->enable_surface_tension;
     return d;
 }
 

@@ -391,6 +391,114 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
     vout[i] = make_float4(vi.x + dx, vi.y + dy, vi.z + dz, 0.0f);
 }
 
+// ---- surface tension ------------------------------------------------------
+// Akinci et al. 2013 cohesion + curvature after XSPH, with the reference's
+// cohesion kernel C(r, L) (sph_kernels.cuh:44-51) on the kernel support
+// L = 2l; the reference parses the coefficient (loader.cpp:50,
+// main.cpp:132) but applies no force.  The powers of L are evaluated once
+// on the host.
+//   n_i = L sum_{j != i} (m / rho_j) grad W_ij
+//   a_i = sum_{j != i, |r| < L} K_ij (s_ij r_ij - gamma (n_i - n_j)),
+//   K_ij = 2 rho0 / (rho_i + rho_j),  s_ij = -gamma m C(|r|) / |r| (0 at r = 0)
+//   v_i += dt a_i
+
+__device__ __forceinline__ float cohesion_C(float r, const TensionArgs& T) {
+    if (r <= 0.0f || r > T.L) return 0.0f;
+    const float lm = T.L - r;
+    const float t = lm * lm * lm * r * r * r;
+    if (2.0f * r > T.L) return T.sig * t;
+    return T.sig * (2.0f * t - T.c6);
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_tension_normals(Nbr P, PairList L,
+                                                        const float4* __restrict__ xs,
+                                                        const float* __restrict__ rho,
+                                                        TensionArgs T, float4* __restrict__ nrm) {
+    const u32 i = slot_index<NT>(P.remap);
+    if (i >= P.nslots) return;
+    const SceneSeg& G = P.seg[scene_of_slot(P, i)];
+    if (i - G.off >= G.n) return;
+    const float4 p = xs[i];
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    // zero-gradient pairs add +-0 to sums that never hold -0: left out
+    auto add = [&](const float4& q, float c, float rj) {
+        const float gx = (p.x - q.x) * c, gy = (p.y - q.y) * c, gz = (p.z - q.z) * c;
+        const float f = P.mass / rj;
+        sx += f * gx;
+        sy += f * gy;
+        sz += f * gz;
+    };
+    const u32 cf = L.cnt[i] & 0xFFFFu;
+    if (cf <= L.kf) {
+        const StageRanges R = L.plan[i / kTile];
+        for (u32 k = 0; k < cf; ++k) {
+            u32 j;
+            float c;
+            fluid_entry(L, R, k, i, j, c);
+            const float4 q = xs[j];
+            add(q, R.total ? pair_c(p, q, P.sp) : c, rho[j]);
+        }
+    } else {
+        for_stencil(P.gf, P.csf + G.key_base, G.off - G.start, p.x, p.y, p.z, [&](u32 s) {
+            if (s == i) return;
+            const float4 q = xs[s];
+            const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+            if (r2 >= P.sp.far2) return;
+            const float c = spline_gradc(sqrtf(r2), P.sp);
+            if (c != 0.0f) add(q, c, rho[s]);
+        });
+    }
+    nrm[i] = make_float4(T.L * sx, T.L * sy, T.L * sz, 0.0f);
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_tension_force(Nbr P, PairList L,
+                                                      const float4* __restrict__ xs,
+                                                      const float* __restrict__ rho,
+                                                      const float4* __restrict__ nrm,
+                                                      TensionArgs T, float4* __restrict__ vs) {
+    const u32 i = slot_index<NT>(P.remap);
+    if (i >= P.nslots) return;
+    const SceneSeg& G = P.seg[scene_of_slot(P, i)];
+    if (i - G.off >= G.n) return;
+    const float4 p = xs[i];
+    const float4 ni = nrm[i];
+    const float rho_i = rho[i];
+    const float mg = -T.gamma * P.mass;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    auto add = [&](const float4& q, float rj, const float4& nj) {
+        const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+        const float d = sqrtf(dist2(rx, ry, rz));
+        const float K = 2.0f * T.rho0 / (rho_i + rj);
+        const float sc = d > 0.0f ? mg * cohesion_C(d, T) / d : 0.0f;
+        ax += K * (sc * rx + -T.gamma * (ni.x - nj.x));
+        ay += K * (sc * ry + -T.gamma * (ni.y - nj.y));
+        az += K * (sc * rz + -T.gamma * (ni.z - nj.z));
+    };
+    const u32 cf = L.cnt[i] & 0xFFFFu;
+    if (cf <= L.kf) {
+        // a complete list holds every in-support pair but the self pair
+        const StageRanges R = L.plan[i / kTile];
+        for (u32 k = 0; k < cf; ++k) {
+            u32 j;
+            float c;
+            fluid_entry(L, R, k, i, j, c);
+            add(xs[j], rho[j], nrm[j]);
+        }
+    } else {
+        for_stencil(P.gf, P.csf + G.key_base, G.off - G.start, p.x, p.y, p.z, [&](u32 s) {
+            if (s == i) return;
+            const float4 q = xs[s];
+            if (dist2(p.x - q.x, p.y - q.y, p.z - q.z) >= P.sp.far2) return;
+            add(q, rho[s], nrm[s]);
+        });
+    }
+    const float4 v = vs[i];
+    const float dt = G.dt;
+    vs[i] = make_float4(v.x + dt * ax, v.y + dt * ay, v.z + dt * az, 0.0f);
+}
+
 // rho_dot terms (dfsph_kernels.cu:210, 222) for one pair with gradient scalar c.
 __device__ __forceinline__ void rho_dot_fluid(float& rd, const float4& p, const float4& vi,
                                               const float4& q, const float4& vj, float c,
@@ -1301,7 +1409,7 @@ const char* kernel_kind_name(int k) {
     static const char* names[kKNumKinds] = {"grid_build",     "reorder",     "density_alpha",
                                             "xsph",           "density_change", "apply_kappa",
                                             "advect",         "div_change",  "div_apply",
-                                            "div_writeback"};
+                                            "div_writeback",  "surface_tension"};
     return (k >= 0 && k < kKNumKinds) ? names[k] : "?";
 }
 
@@ -1443,6 +1551,7 @@ void DFSPHSolver::init(const std::vector<u32>& counts, const std::vector<float>&
     pairs_b_ = DeviceBuffer<uint2>(static_cast<size_t>(kPairDepthBoundary) * slots_cap);
     pair_cnt_ = DeviceBuffer<u32>(slots_cap);
     pairs_fi_ = DeviceBuffer<uint16_t>(static_cast<size_t>(kPairDepthFluid) * slots_cap);
+    if (cfg_.enable_surface_tension) nrm_ = DeviceBuffer<float4>(slots_cap);
     plan_ = DeviceBuffer<StageRanges>(slots_cap / kTile);
     part_d_ = DeviceBuffer<double>(slots_cap / kTile);
     part_f_ = DeviceBuffer<float>(slots_cap / 64);
@@ -1765,6 +1874,20 @@ void DFSPHSolver::phase_prefix_solve(bool gravity) {
                     rho_.data(), vs_.data());
         tend();
     }
+    if (gravity && cfg_.enable_surface_tension) {
+        const float Lsup = 2.0f * cfg_.smoothing_length;
+        const TensionArgs T{Lsup, 32.0f / (3.14159265358979323846f * std::pow(Lsup, 9.0f)),
+                            std::pow(Lsup, 6.0f) / 64.0f, cfg_.surface_tension, cfg_.rest_density};
+        tbegin(kKTension);
+        k_tension_normals<NS><<<nslots_ / NS, NS, 0, stream_>>>(P, pair_list(), xs_.data(),
+                                                                rho_.data(), T, nrm_.data());
+        WSIM_LAUNCH_CHECK();
+        k_tension_force<NS><<<nslots_ / NS, NS, 0, stream_>>>(P, pair_list(), xs_.data(),
+                                                              rho_.data(), nrm_.data(), T,
+                                                              vs_.data());
+        WSIM_LAUNCH_CHECK();
+        tend();
+    }
     k_reset_solve<<<div_up(scenes, NS), NS, 0, stream_>>>(dseg_.data(), dstate_.data(), scenes,
                                                           gravity ? 0 : 1);
     WSIM_LAUNCH_CHECK();
@@ -2009,8 +2132,9 @@ void DFSPHSolver::iterations_taped(u32 begin, u32 end, SubstepTape& t) {
 
 void DFSPHSolver::step_taped(float dt, SubstepTape& t) {
     if (batched_) fail(kInvalidArgument, "step_taped: plain solver only");
-    if (cfg_.enable_divergence_solve)
-        fail(kInvalidArgument, "step_taped: the adjoint covers the density solve only");
+    if (cfg_.enable_divergence_solve || cfg_.enable_surface_tension)
+        fail(kInvalidArgument,
+             "step_taped: the adjoint covers the density solve without surface tension");
     const size_t nsz = fluid_.count();
     if (nsz == 0) fail(kInvalidArgument, "step_taped: empty store");
     if (counts_.empty() || counts_[0] != nsz) layout({static_cast<u32>(nsz)});

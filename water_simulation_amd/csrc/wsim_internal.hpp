@@ -156,6 +156,7 @@ struct DFSPHConfig {
     float domain_max[3] = {1.0f, 1.0f, 1.0f};
     u32 enable_divergence_solve = 0;
     u32 pressure_form = 0;  // 0: kappa/rho^2 (reference), 1: kappa/rho
+    u32 enable_surface_tension = 0;  // Akinci 2013 cohesion + curvature, gamma = surface_tension
 };
 
 struct StepStats {
@@ -191,6 +192,12 @@ struct SceneSeg {
 };
 static_assert(sizeof(SceneSeg) == 64, "SceneSeg layout");
 
+// Surface-tension constants: support L = 2l, cohesion normalisation
+// 32 / (pi L^9), L^6 / 64, coefficient gamma, rest density.
+struct TensionArgs {
+    float L, sig, c6, gamma, rho0;
+};
+
 // Device-resident solve state of one scene; kernels keep it current, the
 // host reads it once per substep (and between iteration chunks).
 struct DeviceState {
@@ -215,6 +222,7 @@ enum KernelKind : int {
     kKDivChange,
     kKDivApply,
     kKDivWriteback,
+    kKTension,
     kKNumKinds
 };
 const char* kernel_kind_name(int k);
@@ -447,6 +455,7 @@ private:
     DeviceBuffer<float4> xs_, vs_, vpre_, xk_, xb_;
     DeviceBuffer<uint2> pairs_f_, pairs_b_;
     DeviceBuffer<uint16_t> pairs_fi_;
+    DeviceBuffer<float4> nrm_;  // surface-tension normals (when enabled)
     DeviceBuffer<u32> bnear_;  // boundary proximity mask (see stencil.hpp)
     DeviceBuffer<StageRanges> plan_;
     DeviceBuffer<u32> pair_cnt_;

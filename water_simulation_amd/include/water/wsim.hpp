@@ -260,6 +260,7 @@ public:
         Vec3f domain_max{1.0f, 1.0f, 1.0f};
         bool enable_divergence_solve = false;  // extension
         u32 pressure_form = 0;                 // extension: 0 kappa/rho^2, 1 kappa/rho
+        bool enable_surface_tension = false;   // extension: Akinci 2013 with surface_tension
     };
 
     DFSPHSolver(ParticleStore& fluid, ParticleStore* boundary, Config cfg, Stream stream = nullptr) {
@@ -286,6 +287,7 @@ public:
         c.domain_max[2] = cfg.domain_max.z;
         c.enable_divergence_solve = cfg.enable_divergence_solve ? 1u : 0u;
         c.pressure_form = cfg.pressure_form;
+        c.enable_surface_tension = cfg.enable_surface_tension ? 1u : 0u;
         detail::check(wsim_solver_create(fluid.native(), boundary ? boundary->native() : nullptr,
                                          &c, stream, &h_));
     }
