@@ -50,6 +50,16 @@ wsim_status wsim_memset(void* dst_device, int value, size_t bytes, void* stream)
 /* Device-to-device copy by libwsim's own streaming kernel (no reference
  * counterpart): the known-size workload used to calibrate HBM byte counters. */
 wsim_status wsim_device_copy(void* dst_device, const void* src_device, size_t bytes, void* stream);
+/* Diagnostic (no reference counterpart).  The solver kernels evaluate the
+ * pair distance sqrt, q = d / l and the kernel-gradient quotient with the
+ * hardware approximations and their fma correction steps instead of the
+ * compiler's range-scaled IEEE sequences, when l is in the range where the
+ * two agree (out[0]: 1 if so).  This runs every binary32 operand the
+ * kernels can produce for smoothing length l through both on the current
+ * device and returns the counts that differ in any bit: out[1] sqrt,
+ * out[2] d / l, out[3] gradient quotient (all 0 when the solver is
+ * bit-exact to the IEEE evaluation).  Synchronous; takes about a second. */
+wsim_status wsim_pair_ops_check(float smoothing_length, uint64_t out[4]);
 wsim_status wsim_stream_create(void** out_stream);
 wsim_status wsim_stream_destroy(void* stream);
 wsim_status wsim_stream_synchronize(void* stream);

@@ -31,7 +31,7 @@ __all__ = [
     "BatchSolver", "TimeStepper", "TimeStepperConfig", "Scene", "load_scene", "sample_aabb_boundary",
     "lattice_block", "write_frame", "read_frame", "DeviceBuffer", "memcpy_h2d", "memcpy_d2h",
     "device_count", "device_synchronize", "WsimError", "OutOfRange", "InvalidArgument",
-    "DeviceError", "library_path",
+    "DeviceError", "library_path", "pair_ops_check",
 ]
 
 
@@ -61,6 +61,17 @@ def device_count() -> int:
 
 def device_synchronize() -> None:
     check(_L().wsim_device_synchronize())
+
+
+def pair_ops_check(smoothing_length: float) -> dict:
+    """Run every operand the solver kernels can produce for this smoothing
+    length through the pair-path sqrt / quotients and the IEEE operations on
+    the device; returns whether the kernels use the pair path and the counts
+    of results that differ in any bit."""
+    out = (C.c_uint64 * 4)()
+    check(_L().wsim_pair_ops_check(float(smoothing_length), out))
+    return {"fast": bool(out[0]), "sqrt": int(out[1]), "quotient": int(out[2]),
+            "gradient": int(out[3])}
 
 
 def memcpy_h2d(dst: int, array: np.ndarray, stream=None) -> None:

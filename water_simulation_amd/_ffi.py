@@ -117,6 +117,7 @@ SIGNATURES = {
     "wsim_memcpy_d2d": (st, [vp, vp, sz, vp]),
     "wsim_memset": (st, [vp, C.c_int, sz, vp]),
     "wsim_device_copy": (st, [vp, vp, sz, vp]),
+    "wsim_pair_ops_check": (st, [C.c_float, C.POINTER(C.c_uint64)]),
     "wsim_stream_create": (st, [C.POINTER(vp)]),
     "wsim_stream_destroy": (st, [vp]),
     "wsim_stream_synchronize": (st, [vp]),

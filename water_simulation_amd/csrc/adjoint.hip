@@ -120,7 +120,7 @@ __device__ __forceinline__ void for_pairs(const Nbr& P, const PairList& L, const
         const float4 q = xs[s];
         const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
         if (r2 >= P.sp.far2) return;
-        const float c = spline_gradc(sqrtf(r2), P.sp);
+        const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
         if (c != 0.0f) ff(s, c);
     });
     if (P.nb) {
@@ -128,7 +128,7 @@ __device__ __forceinline__ void for_pairs(const Nbr& P, const PairList& L, const
             const float4 q = P.xb[s];
             const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
             if (r2 >= P.sp.far2) return;
-            const float c = spline_gradc(sqrtf(r2), P.sp);
+            const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
             if (c != 0.0f) fb(s, c);
         });
     }
@@ -408,7 +408,7 @@ __global__ __launch_bounds__(NA) void k_adj_xsph(Nbr P, PairList L, const float4
             const bool on = eps > 0.0f;
             const double de = eps, m = P.mass;
             auto term = [&](u32 s, const float4& q, float r2) {
-                const float w = spline_W(sqrtf(r2), P.sp);
+                const float w = spline_W(pair_sqrt(r2, P.sp), P.sp);
                 if (w == 0.0f) return;
                 const float avg = 0.5f * (rho_i + rho[s]);
                 const float am = fmaxf(avg, 1e-9f);

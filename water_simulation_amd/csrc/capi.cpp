@@ -124,6 +124,16 @@ wsim_status wsim_device_copy(void* dst, const void* src, size_t bytes, void* str
         wsim::device_copy(dst, src, bytes, as_stream(stream));
     });
 }
+wsim_status wsim_pair_ops_check(float l, uint64_t out[4]) {
+    return guarded([&] {
+        need(out, "out");
+        const wsim::PairOpsReport r = wsim::pair_ops_check(l);
+        out[0] = static_cast<uint64_t>(r.fast);
+        out[1] = r.sqrt_mismatches;
+        out[2] = r.quotient_mismatches;
+        out[3] = r.gradient_mismatches;
+    });
+}
 wsim_status wsim_memset(void* dst, int value, size_t bytes, void* stream) {
     return guarded([&] {
         if (!bytes) return;

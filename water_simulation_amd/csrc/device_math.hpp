@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -67,23 +68,99 @@ struct Spline {
     // square root and divisions.  Their terms would add +-0 to sums that
     // never hold -0, so skipping them changes no bit.
     float far2;
-    static Spline make(float l) {
-        Spline s;
-        s.l = l;
-        s.sigma = 1.0f / (3.14159265358979323846f * l * l * l);
-        s.sigma_m075 = s.sigma * (-0.75f);
-        s.sigma_025 = s.sigma * 0.25f;
-        // start a little below (2l)^2 and walk up ulp by ulp
-        float x = (2.0f * l) * (2.0f * l) * 0.99999f;
-        for (int k = 0; k < 4096 && !(sqrtf(x) / l >= 2.0f); ++k) x = nextafterf(x, 3.0e38f);
-        s.far2 = (sqrtf(x) / l >= 2.0f) ? x : 3.0e38f;
-        return s;
-    }
+    float inv_l;  // RN(1 / l)
+    int fast;     // the pair-path sqrt and quotients below apply (pair_ops_in_range)
+    static Spline make(float l);
 };
+
+// ---- correctly rounded sqrt and quotients on the pair path ----------------
+// The compiler's IEEE sqrt and divide wrap the hardware approximations
+// (v_sqrt_f32, v_rcp_f32) in range scaling and special-case fix-ups.  The
+// pair path keeps the rounding steps and drops what its operands never need:
+//
+//  * pair_sqrt: v_sqrt_f32 and the same two fma residual corrections, i.e.
+//    the compiler's sqrtf sequence without its scaling for x < 2^-96 and its
+//    +-0 / +inf pass-through, so it equals sqrtf for +0, NaN, +inf and every
+//    x >= 2^-96.  Below 2^-96 (d < 2^-48) it returns an approximation, and
+//    the callers' results do not depend on d there: W = sigma * 1 exactly
+//    once q < 2^-13 (l >= 2^-30), and the gradient factor is 0 for d < 1e-12.
+//  * pair_q: d / l as q0 = d * RN(1/l), r = fma(-q0, l, d) (exact),
+//    fma(r, RN(1/l), q0).  RN(1/l) is within half an ulp of 1/l and q0
+//    within one ulp of d/l, so the corrected quotient is the correctly
+//    rounded one (Markstein's theorem); scaling d by a power of two scales
+//    every step exactly while no step is subnormal (d >= 2^-48 here).
+//  * pair_div: a / b as the compiler's Newton-Raphson and residual steps
+//    without v_div_scale / v_div_fmas scaling / v_div_fixup, which are
+//    identities when a, b, 1/b and a/b are normal and the exponents of a and
+//    b differ by less than 96.  It is used for the gradient factor only,
+//    whose operands pair_ops_in_range bounds from l: |dW/dq| in
+//    [0.75 sigma min(q_min, 2^-46), sigma] with q_min = 1e-12 / l, and
+//    l * d in [1e-12 l, 2 l^2].
+//
+// Spline::fast selects them when pair_ops_in_range(l) holds; otherwise every
+// kernel uses the IEEE operations.  wsim_pair_ops_check compares all three
+// with the IEEE operations on the device for every operand a given l can
+// produce.
+__device__ __forceinline__ float pair_sqrt_fast(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float dn = __uint_as_float(__float_as_uint(s) - 1u);
+    const float up = __uint_as_float(__float_as_uint(s) + 1u);
+    const float t = __builtin_fmaf(-dn, s, x) <= 0.0f ? dn : s;
+    return __builtin_fmaf(-up, s, x) > 0.0f ? up : t;
+}
+__device__ __forceinline__ float pair_q_fast(float d, float l, float inv_l) {
+    const float q0 = d * inv_l;
+    return __builtin_fmaf(__builtin_fmaf(-q0, l, d), inv_l, q0);
+}
+__device__ __forceinline__ float pair_div_fast(float a, float b) {
+    float y = __builtin_amdgcn_rcpf(b);
+    y = __builtin_fmaf(__builtin_fmaf(-b, y, 1.0f), y, y);
+    float q = a * y;
+    q = __builtin_fmaf(__builtin_fmaf(-b, q, a), y, q);
+    return __builtin_fmaf(__builtin_fmaf(-b, q, a), y, q);
+}
+__device__ __forceinline__ float pair_sqrt(float x, const Spline& sp) {
+    if (sp.fast) return pair_sqrt_fast(x);
+    return sqrtf(x);
+}
+__device__ __forceinline__ float pair_q(float d, const Spline& sp) {
+    if (sp.fast) return pair_q_fast(d, sp.l, sp.inv_l);
+    return d / sp.l;
+}
+__device__ __forceinline__ float pair_div(float a, float b, const Spline& sp) {
+    if (sp.fast) return pair_div_fast(a, b);
+    return a / b;
+}
+
+// Host: the operand bounds above, with margins, in binary64.
+inline bool pair_ops_in_range(float l) {
+    if (!(l >= 0x1p-30f && l <= 0x1p30f)) return false;
+    const double L = l, sigma = 1.0 / (3.14159265358979323846 * L * L * L);
+    const double a_lo = 0.75 * sigma * std::min(1e-12 / L, std::ldexp(1.0, -46)) * 0.999;
+    const double a_hi = sigma * 1.001, b_lo = 1e-12 * L * 0.999, b_hi = 2.0 * L * L * 1.001;
+    const double lo = std::ldexp(1.0, -100), hi = std::ldexp(1.0, 100);
+    return a_lo > lo && a_hi < hi && b_lo > lo && b_hi < hi && a_hi / b_lo < std::ldexp(1.0, 90) &&
+           a_lo / b_hi > lo;
+}
+
+inline Spline Spline::make(float l) {
+    Spline s;
+    s.l = l;
+    s.sigma = 1.0f / (3.14159265358979323846f * l * l * l);
+    s.sigma_m075 = s.sigma * (-0.75f);
+    s.sigma_025 = s.sigma * 0.25f;
+    // start a little below (2l)^2 and walk up ulp by ulp
+    float x = (2.0f * l) * (2.0f * l) * 0.99999f;
+    for (int k = 0; k < 4096 && !(sqrtf(x) / l >= 2.0f); ++k) x = nextafterf(x, 3.0e38f);
+    s.far2 = (sqrtf(x) / l >= 2.0f) ? x : 3.0e38f;
+    s.inv_l = 1.0f / l;
+    s.fast = pair_ops_in_range(l) ? 1 : 0;
+    return s;
+}
 
 // W(d, l) for d = sqrtf(r.r) (sph_kernels.cuh:12-22).
 __device__ inline float spline_W(float d, const Spline& sp) {
-    const float q = d / sp.l;
+    const float q = pair_q(d, sp);
     if (q >= 2.0f) return 0.0f;
     if (q < 1.0f) return sp.sigma * (1.0f - 1.5f * q * q + 0.75f * q * q * q);
     const float t = 2.0f - q;
@@ -94,7 +171,7 @@ __device__ inline float spline_W(float d, const Spline& sp) {
 // returns 0 where the reference returns the zero vector.
 __device__ inline float spline_gradc(float d, const Spline& sp) {
     if (d < 1e-12f) return 0.0f;
-    const float q = d / sp.l;
+    const float q = pair_q(d, sp);
     if (q >= 2.0f) return 0.0f;
     float dWdq;
     if (q < 1.0f) {
@@ -103,7 +180,7 @@ __device__ inline float spline_gradc(float d, const Spline& sp) {
         const float t = 2.0f - q;
         dWdq = sp.sigma_m075 * t * t;
     }
-    return dWdq / (sp.l * d);
+    return pair_div(dWdq, sp.l * d, sp);
 }
 
 // W and the gradient factor for a candidate known to lie inside the support
@@ -111,18 +188,18 @@ __device__ inline float spline_gradc(float d, const Spline& sp) {
 // is selected, which gives the same bits as spline_W / spline_gradc without
 // their branches.
 __device__ inline float spline_W_near(float d, const Spline& sp) {
-    const float q = d / sp.l;
+    const float q = pair_q(d, sp);
     const float inner = sp.sigma * (1.0f - 1.5f * q * q + 0.75f * q * q * q);
     const float t = 2.0f - q;
     const float outer = sp.sigma_025 * t * t * t;
     return q < 1.0f ? inner : outer;
 }
 __device__ inline float spline_gradc_near(float d, const Spline& sp) {
-    const float q = d / sp.l;
+    const float q = pair_q(d, sp);
     const float inner = sp.sigma * (-3.0f * q + 2.25f * q * q);
     const float t = 2.0f - q;
     const float outer = sp.sigma_m075 * t * t;
-    const float c = (q < 1.0f ? inner : outer) / (sp.l * d);
+    const float c = pair_div(q < 1.0f ? inner : outer, sp.l * d, sp);
     return d < 1e-12f ? 0.0f : c;
 }
 

@@ -130,7 +130,7 @@ __global__ __launch_bounds__(NS) void k_boundary_mass(GridGeom g, const u32* __r
         const float4 q = xb[s];
         const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
         if (r2 >= sp.far2) return;  // W = 0
-        sum_w += spline_W_near(sqrtf(r2), sp);
+        sum_w += spline_W_near(pair_sqrt(r2, sp), sp);
     });
     mb[i] = rho0 / fmaxf(sum_w, 1e-9f);
 }
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
         const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
         const float r2 = dist2(rx, ry, rz);
         if (r2 >= P.sp.far2) return;  // W = 0 and c = 0
-        const float d = sqrtf(r2);
+        const float d = pair_sqrt(r2, P.sp);
         rho += P.mass * spline_W_near(d, P.sp);
         // the self pair (d = 0) has c = 0: its terms add +0, and it is not listed
         const float c = spline_gradc_near(d, P.sp);
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
             const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
             const float r2 = dist2(rx, ry, rz);
             if (r2 >= P.sp.far2) return;
-            const float d = sqrtf(r2);
+            const float d = pair_sqrt(r2, P.sp);
             rho += q.w * spline_W_near(d, P.sp);
             const float c = spline_gradc_near(d, P.sp);
             const float mx = q.w * (rx * c), my = q.w * (ry * c), mz = q.w * (rz * c);
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(NT) void k_xsph(Nbr P, PairList L, const float4* __
         const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
         if (r2 >= P.sp.far2) return;  // W = 0
         const float4 vj = vin[s];
-        const float w = spline_W_near(sqrtf(r2), P.sp);
+        const float w = spline_W_near(pair_sqrt(r2, P.sp), P.sp);
         const float rho_avg = 0.5f * (rho_i + rho[s]);
         const float c = eps * (P.mass / fmaxf(rho_avg, 1e-9f)) * w;
         dx += c * (vj.x - vi.x);
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
         const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
         if (r2 >= P.sp.far2) return;  // W = 0
         const float4 vj = vin[s];
-        const float w = spline_W_near(sqrtf(r2), P.sp);
+        const float w = spline_W_near(pair_sqrt(r2, P.sp), P.sp);
         const float rho_avg = 0.5f * (rho_i + rho[s]);
         const float c = eps * (P.mass / fmaxf(rho_avg, 1e-9f)) * w;
         dx += c * (vj.x - vi.x);
@@ -445,7 +445,7 @@ __global__ __launch_bounds__(NT) void k_tension_normals(Nbr P, PairList L,
             const float4 q = xs[s];
             const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
             if (r2 >= P.sp.far2) return;
-            const float c = spline_gradc(sqrtf(r2), P.sp);
+            const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
             if (c != 0.0f) add(q, c, rho[s]);
         });
     }
@@ -596,7 +596,7 @@ __global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* 
                     const float4 q = xs[s];
                     const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
                     if (r2 >= P.sp.far2) return;
-                    const float c = spline_gradc(sqrtf(r2), P.sp);
+                    const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
                     rho_dot_fluid(rd, p, vi, q, vs[s], c, P.mass);
                 });
                 if (P.nb) {
@@ -604,7 +604,7 @@ __global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* 
                         const float4 q = P.xb[s];
                         const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
                         if (r2 >= P.sp.far2) return;
-                        const float c = spline_gradc(sqrtf(r2), P.sp);
+                        const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
                         rho_dot_boundary(rd, p, vi, q, c);
                     });
                 }
@@ -756,7 +756,7 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
                 const float4 q = xs[s];
                 const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
                 if (r2 >= P.sp.far2) return;
-                const float c = spline_gradc(sqrtf(r2), P.sp);
+                const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
                 rho_dot_fluid(rd, p, vi, q, vs[s], c, P.mass);
             });
             if (P.nb) {
@@ -764,7 +764,7 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
                     const float4 q = P.xb[s];
                     const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
                     if (r2 >= P.sp.far2) return;
-                    const float c = spline_gradc(sqrtf(r2), P.sp);
+                    const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
                     rho_dot_boundary(rd, p, vi, q, c);
                 });
             }
@@ -905,7 +905,7 @@ __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* _
             const float4 q = xk[s];
             const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
             if (r2 >= P.sp.far2) return;
-            const float c = spline_gradc(sqrtf(r2), P.sp);
+            const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
             kappa_term(dx, dy, dz, p, q, c, P.mass * (ki + q.w), dt);
         });
         if (P.nb) {
@@ -913,7 +913,7 @@ __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* _
                 const float4 q = P.xb[s];
                 const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
                 if (r2 >= P.sp.far2) return;
-                const float c = spline_gradc(sqrtf(r2), P.sp);
+                const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
                 kappa_term(dx, dy, dz, p, q, c, q.w * (ki + bf), dt);
             });
         }
@@ -1026,7 +1026,7 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
             const float4 q = xk[s];
             const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
             if (r2 >= P.sp.far2) return;
-            const float c = spline_gradc(sqrtf(r2), P.sp);
+            const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
             kappa_term(dx, dy, dz, p, q, c, P.mass * (ki + q.w), dt);
         });
         if (P.nb) {
@@ -1034,7 +1034,7 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
                 const float4 q = P.xb[s];
                 const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
                 if (r2 >= P.sp.far2) return;
-                const float c = spline_gradc(sqrtf(r2), P.sp);
+                const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
                 kappa_term(dx, dy, dz, p, q, c, q.w * (ki + bf), dt);
             });
         }

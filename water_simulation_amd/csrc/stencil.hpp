@@ -12,6 +12,11 @@ namespace dev {
 constexpr u32 kNone = 0xFFFFFFFFu;
 constexpr int NS = 256;  // workgroup size of the streaming kernels
 
+// Register arrays of staged or batched loads use clang vector types, which
+// stay in registers when the loops over them are unrolled.
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+
 // Neighbour-search and scene inputs shared by the per-particle kernels.
 struct Nbr {
     GridGeom gf;
@@ -233,7 +238,7 @@ __device__ __forceinline__ void for_stage_quads(const PairList& L, u32 i, u32 cf
 }
 
 __device__ __forceinline__ float pair_c(const float4& p, const float4& q, const Spline& sp) {
-    return spline_gradc_near(sqrtf(dist2(p.x - q.x, p.y - q.y, p.z - q.z)), sp);
+    return spline_gradc_near(pair_sqrt(dist2(p.x - q.x, p.y - q.y, p.z - q.z), sp), sp);
 }
 
 // The staging entries of one thread: tid, tid + 256, tid + 512, ...  src
@@ -242,8 +247,6 @@ __device__ __forceinline__ float pair_c(const float4& p, const float4& q, const 
 // the first LDS store).  Register arrays of the staged data use clang vector
 // types, which stay in registers when the loops are unrolled.
 constexpr int kStagePer = kStageCap / kTile;
-typedef float f4v __attribute__((ext_vector_type(4)));
-typedef float f2v __attribute__((ext_vector_type(2)));
 struct StageLanes {
     u32 src[kStagePer];
     u32 total;

@@ -103,6 +103,16 @@ private:
 // Plain 16-byte streaming copy kernel (copy.hip).
 void device_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
 
+// Exhaustive comparison of the pair-path sqrt and quotients with the IEEE
+// operations for one smoothing length (pair_ops.hip); bit ranges are the
+// binary32 patterns covered ([lo, hi)).
+struct PairOpsReport {
+    int fast;  // Spline::make(l).fast: the solver kernels use the pair-path operations
+    u64 sqrt_mismatches, quotient_mismatches, gradient_mismatches;
+    u32 x_lo, x_hi, d_lo, d_hi, c_lo;
+};
+PairOpsReport pair_ops_check(float l);
+
 // Cell-start table from sorted keys (one pass over the C+1 cells).
 void build_cell_start(const u32* sorted_keys, u32 n, u32 num_cells_plus_one, u32* cell_start,
                       hipStream_t stream, u32 cells_per_thread = 16);
