@@ -315,6 +315,44 @@ def lbvh_leg(x, reps: int) -> dict:
             "points_per_s": len(x) / el, "root_parent_ok": bool(parent[0] == 0xFFFFFFFF)}
 
 
+def mesh_leg(x, r: float, reps: int, cpu: bool) -> dict:
+    """The splat surface mesher (tools/particles_to_mesh.py:77-138, position
+    pre-smoothing and Laplacian smoothing off) on the dam-break positions:
+    voxel r/2, blur sigma 2.5 voxels, iso 0.18 max.  The CPU baseline runs the
+    reference's own numpy splat and scipy blur on the 50^3 corner of the block."""
+    import water_simulation_amd as ws
+    from water_simulation_amd import mesh as WM
+    buf = ws.DeviceBuffer.from_array(x)
+    m = WM.SurfaceMesher()
+    info = m.run((buf.ptr, len(x)), radius=r)  # warm-up
+    ws.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        info = m.run((buf.ptr, len(x)), radius=r)
+    el = (time.perf_counter() - t0) / reps
+    voxels = int(np.prod(info["dims"]))
+    out = {"workload": f"surface mesh of the {len(x)} dam-break positions, radius {r}",
+           "points": len(x), "grid": list(info["dims"]), "voxels": voxels,
+           "vertices": info["n_vertices"], "triangles": info["n_triangles"], "runs": reps,
+           "ms_per_mesh": el * 1e3, "voxels_per_s": voxels / el, "cpu_baseline": None}
+    if cpu:
+        import scipy.ndimage
+        from oracle import mesh_oracle as MO
+        lo = x.min(0)
+        sub = x[np.all(x < lo + 50 * 2 * r, axis=1)].astype(np.float64)[:, [0, 2, 1]]
+        t0 = time.perf_counter()
+        l2, e2, r2 = MO.grid_setup(sub, r, 0.5)
+        g = MO.splat(sub, l2, e2, r2)
+        scipy.ndimage.gaussian_filter(g, sigma=2.5)
+        ec = time.perf_counter() - t0
+        out["cpu_baseline"] = {"ms": ec * 1e3, "voxels": int(np.prod(r2)),
+                               "voxels_per_s": float(np.prod(r2)) / ec, "cores": 1,
+                               "kind": "reference",
+                               "sample": f"numpy splat + scipy gaussian_filter of {len(sub)} "
+                                         "points (the reference's own calls; no isosurface)"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -340,6 +378,9 @@ def main() -> int:
     ap.add_argument("--lbvh-reps", type=int, default=10,
                     help="SpatialAccel builds timed on the 1M positions (0: skip)")
     ap.add_argument("--only-lbvh", action="store_true", help="run only the SpatialAccel leg")
+    ap.add_argument("--mesh-reps", type=int, default=3,
+                    help="surface meshes of the dam-break positions (0: skip)")
+    ap.add_argument("--only-mesh", action="store_true", help="run only the surface-mesh leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run libwsim's streaming copy kernel 5 times over BYTES bytes "
                          "(the known-size workload for converting HBM byte counters)")
@@ -369,6 +410,11 @@ def main() -> int:
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
 
+    if args.only_mesh:
+        x, _, cfg = make_workload(args.n_side)
+        print(json.dumps({"mesh": mesh_leg(x, cfg["particle_radius"], max(args.mesh_reps, 1),
+                                           not args.no_cpu_baseline)}))
+        return 0
     if args.only_lbvh:
         x, _, _ = make_workload(args.n_side)
         print(json.dumps({"lbvh": lbvh_leg(x, max(args.lbvh_reps, 1))}))
@@ -485,6 +531,8 @@ def main() -> int:
         "batched": batched,
         "adjoint": adjoint,
         "lbvh": lbvh_leg(x, args.lbvh_reps) if args.lbvh_reps > 0 else None,
+        "mesh": (mesh_leg(x, cfg["particle_radius"], args.mesh_reps, not args.no_cpu_baseline)
+                 if args.mesh_reps > 0 else None),
     }
     print(json.dumps(out))
     return 0

@@ -330,6 +330,49 @@ wsim_status wsim_frame_writer_push(wsim_frame_writer writer, wsim_store store, i
 wsim_status wsim_frame_writer_finish(wsim_frame_writer writer);
 void wsim_frame_writer_destroy(wsim_frame_writer writer);
 
+/* ---- fluid surface mesh: the splat method of tools/particles_to_mesh.py
+ * (reconstruct_splat :77-138, load_frame :46-58) on the device ----
+ * Bounds padded by 4 r, +1 per point in voxel ((p - lo) / extent * res)
+ * with res = max(ceil(extent / (factor r)), 8), the Gaussian blur of
+ * scipy.ndimage.gaussian_filter (truncate 4, 'reflect'), iso = max *
+ * iso_factor, then marching cubes with one shared vertex per crossing grid
+ * edge.  Its triangle table resolves ambiguous faces by separating the
+ * inside corners (the reference calls skimage's marching_cubes).  The
+ * reference's optional k-nearest-neighbour position smoothing and Laplacian
+ * mesh smoothing are not applied.  weights, when not NULL, are the 2R + 1
+ * blur weights (R = (int)(4 sigma + 0.5)) to use instead of exp(). */
+typedef struct wsim_mesher_t* wsim_mesher;
+typedef struct {
+    double particle_radius;   /* --radius (0.012) */
+    double grid_voxel_factor; /* 0.5 */
+    double smooth_sigma;      /* blur sigma in voxels (2.5; 0 = none) */
+    double iso_factor;        /* 0.18 */
+    uint32_t swap_yz;         /* 1: (x, y, z) -> (x, z, y) as load_frame */
+    uint32_t reserved;
+    const double* weights;
+} wsim_mesh_params;
+typedef struct {
+    int32_t dims[3];
+    int32_t reserved;
+    double lo[3], extent[3], spacing[3];
+    float max_density, iso;
+    uint64_t n_vertices, n_triangles;
+} wsim_mesh_info;
+void wsim_mesh_params_default(wsim_mesh_params* out);
+wsim_status wsim_mesher_create(void* stream, wsim_mesher* out);
+void wsim_mesher_destroy(wsim_mesher mesher);
+/* positions_device: count x 3 binary32 on the device (e.g. wsim_store_positions).
+ * Synchronous; the outputs stay valid until the next run. */
+wsim_status wsim_mesher_run(wsim_mesher mesher, const float* positions_device, size_t count,
+                            const wsim_mesh_params* params);
+wsim_status wsim_mesher_info(wsim_mesher mesher, wsim_mesh_info* out);
+wsim_status wsim_mesher_grid(wsim_mesher mesher, const float** out_device);
+wsim_status wsim_mesher_vertices(wsim_mesher mesher, const float** out_device);
+wsim_status wsim_mesher_triangles(wsim_mesher mesher, const uint32_t** out_device);
+/* The triangle table: counts[256], edges[256 * 24] (edge a * 4 + b along
+ * axis a from the cube corner with bit a clear and other bits b). */
+wsim_status wsim_mc_table(uint8_t* counts, uint8_t* edges);
+
 /* Lattice fluid block of sim_cli (apps/sim_cli/main.cpp:65-74). */
 wsim_status wsim_lattice_block(const float lo[3], const float hi[3], float spacing, float* out,
                                size_t capacity, size_t* count);

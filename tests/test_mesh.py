@@ -1,0 +1,195 @@
+"""Surface meshes from particle frames (the splat method of
+tools/particles_to_mesh.py:77-138 on the GPU).
+
+CPU: the oracle is pinned -- its blur restatement equals
+scipy.ndimage.gaussian_filter bit for bit (the GPU computes that
+restatement), the marching-cubes table closes every surface, orients it
+outwards, and gives a sphere's area and volume and the Euler characteristic
+of a sphere and a torus; the library's table equals the oracle's.
+GPU: grid set-up, splat, blur, iso level, vertices and triangles of the HIP
+path equal the oracle bit for bit (and the blurred grid equals scipy's).
+"""
+from __future__ import annotations
+
+from collections import Counter
+
+import numpy as np
+import pytest
+import scipy.ndimage
+
+from oracle import mesh_oracle as M
+
+
+def _mesh_checks(v, t):
+    """(closed and consistently oriented, Euler characteristic, area, volume)."""
+    d = Counter()
+    for a, b, c in t:
+        for x, y in ((a, b), (b, c), (c, a)):
+            d[(int(x), int(y))] += 1
+    closed = all(d[(y, x)] == n for (x, y), n in d.items())
+    E = len({tuple(sorted(k)) for k in d})
+    V = len(np.unique(t))
+    p = v.astype(np.float64)
+    a, b, c = p[t[:, 0]], p[t[:, 1]], p[t[:, 2]]
+    area = 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1).sum()
+    vol = np.einsum("ij,ij->i", a, np.cross(b, c)).sum() / 6.0
+    return closed, V - E + len(t), area, vol
+
+
+def _centered(n):
+    x = np.arange(n) - (n - 1) / 2
+    return np.meshgrid(x, x, x, indexing="ij")
+
+
+def test_blur_restatement_is_scipy():
+    rng = np.random.default_rng(0)
+    for shape, sigma in (((8, 9, 10), 2.5), ((20, 13, 31), 2.5), ((11, 12, 9), 1.3)):
+        g = rng.integers(0, 5, shape).astype(np.float32)
+        got = M.blur_restated(g, sigma)
+        want = scipy.ndimage.gaussian_filter(g, sigma=sigma)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (shape, sigma)
+
+
+def test_sphere_closed_outward_area_volume():
+    n, R = 48, 15.3
+    X, Y, Z = _centered(n)
+    g = (R - np.sqrt(X ** 2 + Y ** 2 + Z ** 2)).astype(np.float32)
+    v, t = M.marching_cubes(g, np.float32(0.0), -np.full(3, (n - 1) / 2), np.ones(3))
+    closed, chi, area, vol = _mesh_checks(v, t)
+    assert closed and chi == 2
+    assert abs(area / (4 * np.pi * R ** 2) - 1) < 0.01
+    assert abs(vol / (4 / 3 * np.pi * R ** 3) - 1) < 0.01  # positive: normals point outwards
+
+
+def test_torus_euler_characteristic():
+    n = 40
+    X, Y, Z = _centered(n)
+    q = np.sqrt(X ** 2 + Y ** 2) - 11.0
+    g = (4.5 - np.sqrt(q ** 2 + Z ** 2)).astype(np.float32)
+    v, t = M.marching_cubes(g, np.float32(0.0), np.zeros(3), np.ones(3))
+    closed, chi, _, vol = _mesh_checks(v, t)
+    assert closed and chi == 0 and vol > 0
+
+
+def test_random_fields_closed():
+    for seed in range(3):
+        g = np.random.default_rng(seed).random((12, 11, 13)).astype(np.float32)
+        g[[0, -1]] = 0
+        g[:, [0, -1]] = 0
+        g[:, :, [0, -1]] = 0
+        v, t = M.marching_cubes(g, np.float32(0.5), np.zeros(3), np.ones(3))
+        closed, _, _, vol = _mesh_checks(v, t)
+        assert closed and vol > 0, seed
+
+
+def test_complementary_configurations_cut_the_same_edges():
+    counts, tris = M.table()
+    for cfg in range(256):
+        a = set(tris[cfg, :3 * counts[cfg]].tolist())
+        b = set(tris[255 - cfg, :3 * counts[255 - cfg]].tolist())
+        assert a == b, cfg
+        # the edges cut are exactly those between an inside and an outside corner
+        cut = set()
+        for c0 in range(8):
+            for ax in range(3):
+                c1 = c0 | (1 << ax)
+                if c1 != c0 and ((cfg >> c0) & 1) != ((cfg >> c1) & 1):
+                    cut.add(M._edge(c0, c1))
+        assert a == cut, cfg
+
+
+def test_library_table_is_the_oracles():
+    from water_simulation_amd import mesh as WM
+
+    c, e = WM.mc_table()
+    oc, oe = M.table()
+    assert np.array_equal(c, oc) and np.array_equal(e, oe)
+
+
+# ---- GPU ---------------------------------------------------------------------
+
+def _cloud(seed=0, n_side=24, r=0.012):
+    rng = np.random.default_rng(seed)
+    i = np.arange(n_side, dtype=np.float32)
+    g = np.stack(np.meshgrid(i, i, i, indexing="ij"), -1).reshape(-1, 3)
+    x = np.float32(0.1) + np.float32(2 * r) * g
+    x += rng.uniform(-0.3 * r, 0.3 * r, x.shape).astype(np.float32)
+    # a splash above the block
+    s = rng.normal([0.4, 0.8, 0.35], 0.03, (300, 3)).astype(np.float32)
+    return np.concatenate([x, s]).astype(np.float32)
+
+
+def _compare(mesher, want, tag):
+    info = mesher.info()
+    assert tuple(info["dims"]) == tuple(int(k) for k in want["res"]), tag
+    assert np.array_equal(info["lo"], want["lo"]) and np.array_equal(info["extent"],
+                                                                      want["extent"]), tag
+    g = mesher.grid()
+    assert np.array_equal(g.view(np.uint32), want["grid"].view(np.uint32)), tag
+    assert info["iso"].view(np.uint32) == np.float32(want["iso"]).view(np.uint32), tag
+    v, t = mesher.vertices(), mesher.triangles()
+    assert v.shape == want["vertices"].shape and t.shape == want["triangles"].shape, tag
+    assert np.array_equal(v.view(np.uint32), want["vertices"].view(np.uint32)), tag
+    assert np.array_equal(t, want["triangles"]), tag
+    return v, t
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("swap", [True, False])
+def test_gpu_mesh_matches_oracle(swap):
+    from water_simulation_amd import mesh as WM
+
+    x = _cloud(1)
+    m = WM.SurfaceMesher()
+    m.run(x, radius=0.012, swap_yz=swap)
+    want = M.reconstruct_splat(x, radius=0.012, swap_yz=swap)
+    v, t = _compare(m, want, f"swap {swap}")
+    closed, _, _, vol = _mesh_checks(v, t)
+    assert closed and vol > 0
+
+
+@pytest.mark.gpu
+def test_gpu_mesh_options_and_store_input():
+    import water_simulation_amd as ws
+    from water_simulation_amd import mesh as WM
+
+    x = _cloud(2, n_side=16, r=0.01)
+    st = ws.ParticleStore(len(x))
+    st.resize(len(x))
+    st.set_positions(x)
+    m = WM.SurfaceMesher()
+    for kw in (dict(smooth_sigma=0.0), dict(grid_voxel_factor=0.8, smooth_sigma=1.5,
+                                           iso_factor=0.3)):
+        m.run(st, radius=0.01, **kw)
+        want = M.reconstruct_splat(x, radius=0.01, voxel_factor=kw.get("grid_voxel_factor", 0.5),
+                                   sigma=kw["smooth_sigma"], iso_factor=kw.get("iso_factor", 0.18))
+        _compare(m, want, str(kw))
+
+
+@pytest.mark.gpu
+def test_gpu_mesh_cli_writes_ply(tmp_path):
+    import water_simulation_amd as ws
+    from water_simulation_amd import mesh as WM
+
+    x = _cloud(3, n_side=12)
+    ws.write_frame(str(tmp_path / "frame_0001.bin"), x)
+    ws.write_frame(str(tmp_path / "frame_0002.bin"), x[::-1].copy())
+    assert WM.main(["--seq", str(tmp_path), "--out-dir", str(tmp_path / "mesh")]) == 0
+    for k in (1, 2):
+        data = (tmp_path / "mesh" / f"frame_000{k}.ply").read_bytes()
+        head, _, body = data.partition(b"end_header\n")
+        nv = int(head.split(b"element vertex ")[1].split(b"\n")[0])
+        nf = int(head.split(b"element face ")[1].split(b"\n")[0])
+        assert len(body) == 12 * nv + 13 * nf and nf > 0
+    with pytest.raises(SystemExit):
+        WM.main(["--frame", str(tmp_path / "frame_0001.bin"), "--out", str(tmp_path / "a.ply"),
+                 "--smooth", "2"])
+
+
+@pytest.mark.gpu
+def test_gpu_mesh_rejects_empty():
+    from water_simulation_amd import mesh as WM
+    from water_simulation_amd._ffi import InvalidArgument
+
+    with pytest.raises(InvalidArgument):
+        WM.SurfaceMesher().run(np.zeros((0, 3), np.float32))

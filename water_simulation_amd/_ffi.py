@@ -103,6 +103,32 @@ class SceneC(C.Structure):
 vp = C.c_void_p
 st = C.c_int
 sz = C.c_size_t
+
+
+class MeshParamsC(C.Structure):
+    _fields_ = [
+        ("particle_radius", C.c_double),
+        ("grid_voxel_factor", C.c_double),
+        ("smooth_sigma", C.c_double),
+        ("iso_factor", C.c_double),
+        ("swap_yz", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("weights", C.c_void_p),
+    ]
+
+
+class MeshInfoC(C.Structure):
+    _fields_ = [
+        ("dims", C.c_int32 * 3),
+        ("reserved", C.c_int32),
+        ("lo", C.c_double * 3),
+        ("extent", C.c_double * 3),
+        ("spacing", C.c_double * 3),
+        ("max_density", C.c_float),
+        ("iso", C.c_float),
+        ("n_vertices", C.c_uint64),
+        ("n_triangles", C.c_uint64),
+    ]
 u32p = C.POINTER(C.c_uint32)
 
 # name -> (restype, argtypes); every symbol here is declared in include/wsim.h
@@ -118,6 +144,15 @@ SIGNATURES = {
     "wsim_memset": (st, [vp, C.c_int, sz, vp]),
     "wsim_device_copy": (st, [vp, vp, sz, vp]),
     "wsim_pair_ops_check": (st, [C.c_float, C.POINTER(C.c_uint64)]),
+    "wsim_mesh_params_default": (None, [C.POINTER(MeshParamsC)]),
+    "wsim_mesher_create": (st, [vp, C.POINTER(vp)]),
+    "wsim_mesher_destroy": (None, [vp]),
+    "wsim_mesher_run": (st, [vp, vp, sz, C.POINTER(MeshParamsC)]),
+    "wsim_mesher_info": (st, [vp, C.POINTER(MeshInfoC)]),
+    "wsim_mesher_grid": (st, [vp, C.POINTER(vp)]),
+    "wsim_mesher_vertices": (st, [vp, C.POINTER(vp)]),
+    "wsim_mesher_triangles": (st, [vp, C.POINTER(vp)]),
+    "wsim_mc_table": (st, [vp, vp]),
     "wsim_stream_create": (st, [C.POINTER(vp)]),
     "wsim_stream_destroy": (st, [vp]),
     "wsim_stream_synchronize": (st, [vp]),

@@ -32,6 +32,9 @@ struct wsim_solver_t {
 struct wsim_rollout_t {
     std::unique_ptr<wsim::Rollout> impl;
 };
+struct wsim_mesher_t {
+    std::unique_ptr<wsim::SurfaceMesher> impl;
+};
 
 namespace {
 
@@ -665,6 +668,91 @@ void* wsim_solver_stream(wsim_solver s) { return s ? static_cast<void*>(s->impl-
 struct wsim_frame_writer_t {
     std::unique_ptr<wsim::FrameWriter> impl;
 };
+// ---- surface mesh ----
+void wsim_mesh_params_default(wsim_mesh_params* out) {
+    if (!out) return;
+    const wsim::MeshParams d;
+    out->particle_radius = d.particle_radius;
+    out->grid_voxel_factor = d.grid_voxel_factor;
+    out->smooth_sigma = d.smooth_sigma;
+    out->iso_factor = d.iso_factor;
+    out->swap_yz = d.swap_yz ? 1u : 0u;
+    out->reserved = 0;
+    out->weights = nullptr;
+}
+wsim_status wsim_mesher_create(void* stream, wsim_mesher* out) {
+    return guarded([&] {
+        need(out, "out");
+        *out = nullptr;
+        auto h = std::make_unique<wsim_mesher_t>();
+        h->impl = std::make_unique<wsim::SurfaceMesher>(as_stream(stream));
+        *out = h.release();
+    });
+}
+void wsim_mesher_destroy(wsim_mesher m) { delete m; }
+wsim_status wsim_mesher_run(wsim_mesher m, const float* xyz, size_t count,
+                            const wsim_mesh_params* p) {
+    return guarded([&] {
+        need(m, "mesher");
+        need(p, "params");
+        wsim::MeshParams q;
+        q.particle_radius = p->particle_radius;
+        q.grid_voxel_factor = p->grid_voxel_factor;
+        q.smooth_sigma = p->smooth_sigma;
+        q.iso_factor = p->iso_factor;
+        q.swap_yz = p->swap_yz != 0;
+        q.weights = p->weights;
+        m->impl->run(xyz, count, q);
+    });
+}
+wsim_status wsim_mesher_info(wsim_mesher m, wsim_mesh_info* out) {
+    return guarded([&] {
+        need(m, "mesher");
+        need(out, "out");
+        const wsim::MeshInfo& i = m->impl->info();
+        std::memset(out, 0, sizeof(*out));
+        for (int a = 0; a < 3; ++a) {
+            out->dims[a] = i.dims[a];
+            out->lo[a] = i.lo[a];
+            out->extent[a] = i.extent[a];
+            out->spacing[a] = i.spacing[a];
+        }
+        out->max_density = i.max_density;
+        out->iso = i.iso;
+        out->n_vertices = i.n_vertices;
+        out->n_triangles = i.n_triangles;
+    });
+}
+wsim_status wsim_mesher_grid(wsim_mesher m, const float** out) {
+    return guarded([&] {
+        need(m, "mesher");
+        need(out, "out");
+        *out = m->impl->grid();
+    });
+}
+wsim_status wsim_mesher_vertices(wsim_mesher m, const float** out) {
+    return guarded([&] {
+        need(m, "mesher");
+        need(out, "out");
+        *out = m->impl->vertices();
+    });
+}
+wsim_status wsim_mesher_triangles(wsim_mesher m, const uint32_t** out) {
+    return guarded([&] {
+        need(m, "mesher");
+        need(out, "out");
+        *out = m->impl->triangles();
+    });
+}
+wsim_status wsim_mc_table(uint8_t* counts, uint8_t* edges) {
+    return guarded([&] {
+        need(counts, "counts");
+        need(edges, "edges");
+        std::memcpy(counts, wsim::mc_table_counts(), 256);
+        std::memcpy(edges, wsim::mc_table_triangles(), 256 * 24);
+    });
+}
+
 wsim_status wsim_frame_writer_create(const char* dir, size_t max_points, wsim_frame_writer* out) {
     return guarded([&] {
         need(out, "out");

@@ -1,0 +1,546 @@
+// surface.hip -- fluid surface mesh from particle positions on the GPU.
+//
+// Replaces the splat method of tools/particles_to_mesh.py
+// (reconstruct_splat, :77-138 with load_frame :46-58), the consumer of the
+// frame format downstream of sim_cli:
+//   1. bounds of the points (optionally with y and z swapped, the
+//      reference's sim Y-up -> Blender Z-up), padded by 4 r;
+//   2. splat: +1 in the voxel (p - lo) / extent * res, truncated and
+//      clipped, res = max(ceil(extent / (factor r)), 8) per axis;
+//   3. Gaussian blur of the voxel counts, the three 1-D passes of
+//      scipy.ndimage.gaussian_filter (truncate 4, mode 'reflect', binary64
+//      sums in its order for symmetric weights, binary32 between passes);
+//   4. iso = max * iso_factor in binary32 (numpy 2 scalar rules);
+//   5. marching cubes with shared vertices: one vertex per grid edge whose
+//      end values straddle the iso level (linear interpolation), triangles
+//      from a table built from the cube faces (mc_table).
+// The reference's optional position pre-smoothing (k nearest neighbours)
+// and post Laplacian smoothing are not part of this path.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+#include <vector>
+
+#include "common.hpp"
+#include "device_math.hpp"
+#include "wsim_internal.hpp"
+
+namespace wsim {
+namespace {
+
+constexpr int kMeshThreads = 256;
+constexpr int kMaxTri = 8;  // triangles per cube configuration (the table has at most 5)
+
+// ---- marching-cubes table ---------------------------------------------------
+// Corner c of a cube sits at offset (c & 1, c >> 1 & 1, c >> 2 & 1) along the
+// grid axes (0, 1, 2).  Edge a * 4 + b runs along axis a from the corner
+// with bit a clear; b holds that corner's other two bits (lower axis first).
+// A corner is inside when its value exceeds the iso level.  On every face,
+// each run of consecutive inside corners (going round the face; diagonal
+// inside corners are separate runs) is bounded by two crossing edges, which
+// a surface segment joins.  The segments close into loops around the
+// edge-connected inside corners, and each loop becomes a triangle fan.
+// Neighbouring cubes build the same segments on their shared face, so the
+// surface is closed; segments are oriented so that triangle normals point
+// from the inside corners outwards.
+struct McTable {
+    uint8_t ntri[256];
+    uint8_t tri[256][kMaxTri * 3];
+};
+
+int edge_between(int c0, int c1) {
+    const int a = __builtin_ctz(static_cast<unsigned>(c0 ^ c1));
+    const int base = c0 & ~(1 << a);
+    int b = 0, k = 0;
+    for (int ax = 0; ax < 3; ++ax) {
+        if (ax == a) continue;
+        b |= ((base >> ax) & 1) << k;
+        ++k;
+    }
+    return a * 4 + b;
+}
+
+McTable build_mc_table() {
+    McTable t{};
+    for (int cfg = 0; cfg < 256; ++cfg) {
+        auto inside = [&](int c) { return ((cfg >> c) & 1) != 0; };
+        int next[12];
+        for (int& e : next) e = -1;
+        for (int a = 0; a < 3; ++a) {
+            const int u = (a + 1) % 3, v = (a + 2) % 3;  // (u, v, a) right-handed
+            for (int s = 0; s < 2; ++s) {
+                int cyc[4];
+                const int du[4] = {0, 1, 1, 0}, dv[4] = {0, 0, 1, 1};
+                for (int m = 0; m < 4; ++m) cyc[m] = (s << a) | (du[m] << u) | (dv[m] << v);
+                if (s == 0) std::swap(cyc[1], cyc[3]);  // counter-clockwise seen from outside
+                for (int m = 0; m < 4; ++m) {
+                    if (!inside(cyc[m]) || inside(cyc[(m + 3) % 4])) continue;  // run start
+                    int k = m;
+                    while (inside(cyc[(k + 1) % 4])) k = (k + 1) % 4;
+                    const int e_in = edge_between(cyc[(m + 3) % 4], cyc[m]);
+                    const int e_out = edge_between(cyc[k], cyc[(k + 1) % 4]);
+                    next[e_in] = e_out;
+                }
+            }
+        }
+        bool seen[12] = {};
+        int nt = 0;
+        for (int e0 = 0; e0 < 12; ++e0) {
+            if (next[e0] < 0 || seen[e0]) continue;
+            int loop[12], len = 0;
+            for (int e = e0; !seen[e]; e = next[e]) {
+                seen[e] = true;
+                loop[len++] = e;
+            }
+            for (int k = 1; k + 1 < len; ++k) {
+                if (nt >= kMaxTri) fail(kRuntimeError, "marching-cubes table overflow");
+                t.tri[cfg][3 * nt + 0] = static_cast<uint8_t>(loop[0]);
+                t.tri[cfg][3 * nt + 1] = static_cast<uint8_t>(loop[k]);
+                t.tri[cfg][3 * nt + 2] = static_cast<uint8_t>(loop[k + 1]);
+                ++nt;
+            }
+        }
+        t.ntri[cfg] = static_cast<uint8_t>(nt);
+    }
+    return t;
+}
+
+const McTable& mc_table() {
+    static const McTable t = build_mc_table();
+    return t;
+}
+
+// ---- kernels ----------------------------------------------------------------
+struct MeshGeom {
+    int r0, r1, r2;
+    double lo0, lo1, lo2;
+    double ext0, ext1, ext2;
+    double sp0, sp1, sp2;
+};
+
+__device__ __forceinline__ float3 mesh_point(const float* xyz, u64 i, int swap) {
+    const float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+    return swap ? make_float3(x, z, y) : make_float3(x, y, z);
+}
+
+__global__ __launch_bounds__(kMeshThreads) void k_mesh_bounds(const float* __restrict__ xyz, u64 n,
+                                                              int swap, float* __restrict__ part) {
+    __shared__ float s[6][kMeshThreads];
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    const u64 stride = static_cast<u64>(gridDim.x) * kMeshThreads;
+    for (u64 i = static_cast<u64>(blockIdx.x) * kMeshThreads + threadIdx.x; i < n; i += stride) {
+        const float3 p = mesh_point(xyz, i, swap);
+        mn[0] = fminf(mn[0], p.x);
+        mn[1] = fminf(mn[1], p.y);
+        mn[2] = fminf(mn[2], p.z);
+        mx[0] = fmaxf(mx[0], p.x);
+        mx[1] = fmaxf(mx[1], p.y);
+        mx[2] = fmaxf(mx[2], p.z);
+    }
+    for (int a = 0; a < 3; ++a) {
+        s[a][threadIdx.x] = mn[a];
+        s[3 + a][threadIdx.x] = mx[a];
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        float r = s[threadIdx.x][0];
+        for (int k = 1; k < kMeshThreads; ++k)
+            r = threadIdx.x < 3 ? fminf(r, s[threadIdx.x][k]) : fmaxf(r, s[threadIdx.x][k]);
+        part[6 * blockIdx.x + threadIdx.x] = r;
+    }
+}
+
+// numpy's ((p - lo) / extent * res).astype(int), then clip to [0, res - 1]
+__device__ __forceinline__ int voxel_of(float p, double lo, double ext, int res) {
+    const double f = (static_cast<double>(p) - lo) / ext * static_cast<double>(res);
+    long long k = f == f ? static_cast<long long>(f) : 0;
+    if (k < 0) k = 0;
+    if (k > res - 1) k = res - 1;
+    return static_cast<int>(k);
+}
+
+__global__ __launch_bounds__(kMeshThreads) void k_mesh_splat(const float* __restrict__ xyz, u64 n,
+                                                             int swap, MeshGeom G,
+                                                             u32* __restrict__ count) {
+    const u64 i = static_cast<u64>(blockIdx.x) * kMeshThreads + threadIdx.x;
+    if (i >= n) return;
+    const float3 p = mesh_point(xyz, i, swap);
+    const int a = voxel_of(p.x, G.lo0, G.ext0, G.r0);
+    const int b = voxel_of(p.y, G.lo1, G.ext1, G.r1);
+    const int c = voxel_of(p.z, G.lo2, G.ext2, G.r2);
+    atomicAdd(count + (static_cast<u64>(a) * G.r1 + b) * G.r2 + c, 1u);
+}
+
+__global__ __launch_bounds__(kMeshThreads) void k_mesh_to_float(const u32* __restrict__ count,
+                                                                float* __restrict__ grid, u64 nv) {
+    const u64 i = static_cast<u64>(blockIdx.x) * kMeshThreads + threadIdx.x;
+    if (i < nv) grid[i] = static_cast<float>(count[i]);
+}
+
+// scipy.ndimage 'reflect' extension: (d c b a | a b c d | d c b a), repeated
+__device__ __forceinline__ int reflect_index(int k, int n) {
+    const int p = 2 * n;
+    k %= p;
+    if (k < 0) k += p;
+    return k >= n ? p - 1 - k : k;
+}
+
+// One 1-D pass along `axis` with 2R+1 symmetric weights w (w[R] the centre):
+// acc = x[i] w[R] + sum_{j=1..R} (x[i-j] + x[i+j]) w[R+j] in binary64.
+__global__ __launch_bounds__(kMeshThreads) void k_mesh_blur(const float* __restrict__ in,
+                                                            float* __restrict__ out, int r0, int r1,
+                                                            int r2, int axis,
+                                                            const double* __restrict__ w, int R) {
+    const u64 nv = static_cast<u64>(r0) * r1 * r2;
+    const u64 i = static_cast<u64>(blockIdx.x) * kMeshThreads + threadIdx.x;
+    if (i >= nv) return;
+    const int c = static_cast<int>(i % r2);
+    const int b = static_cast<int>((i / r2) % r1);
+    const int a = static_cast<int>(i / (static_cast<u64>(r2) * r1));
+    int pos, len;
+    u64 step;
+    if (axis == 0) {
+        pos = a, len = r0, step = static_cast<u64>(r1) * r2;
+    } else if (axis == 1) {
+        pos = b, len = r1, step = static_cast<u64>(r2);
+    } else {
+        pos = c, len = r2, step = 1;
+    }
+    const u64 base = i - static_cast<u64>(pos) * step;
+    double acc = static_cast<double>(in[i]) * w[R];
+    if (pos >= R && pos + R < len) {  // no reflection: consecutive taps
+        for (int j = 1; j <= R; ++j) {
+            const double lo = in[i - static_cast<u64>(j) * step];
+            const double hi = in[i + static_cast<u64>(j) * step];
+            acc += (lo + hi) * w[R + j];
+        }
+    } else {
+        for (int j = 1; j <= R; ++j) {
+            const double lo = in[base + static_cast<u64>(reflect_index(pos - j, len)) * step];
+            const double hi = in[base + static_cast<u64>(reflect_index(pos + j, len)) * step];
+            acc += (lo + hi) * w[R + j];
+        }
+    }
+    out[i] = static_cast<float>(acc);
+}
+
+__global__ __launch_bounds__(kMeshThreads) void k_mesh_max(const float* __restrict__ g, u64 nv,
+                                                           float* __restrict__ part) {
+    __shared__ float s[kMeshThreads];
+    float m = -INFINITY;
+    const u64 stride = static_cast<u64>(gridDim.x) * kMeshThreads;
+    for (u64 i = static_cast<u64>(blockIdx.x) * kMeshThreads + threadIdx.x; i < nv; i += stride)
+        m = fmaxf(m, g[i]);
+    s[threadIdx.x] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kMeshThreads; ++k) m = fmaxf(m, s[k]);
+        part[blockIdx.x] = m;
+    }
+}
+
+struct McDev {
+    uint8_t ntri[256];
+    uint8_t tri[256][kMaxTri * 3];
+};
+
+// Per grid point: the crossing edges it owns (+axis from the point) and,
+// where a cube starts here, that cube's triangle count.
+__global__ __launch_bounds__(kMeshThreads) void k_mc_count(const float* __restrict__ g, int r0,
+                                                           int r1, int r2, float iso,
+                                                           const McDev* __restrict__ T,
+                                                           uint8_t* __restrict__ emask,
+                                                           u32* __restrict__ ecount,
+                                                           u32* __restrict__ tcount) {
+    const u64 nv = static_cast<u64>(r0) * r1 * r2;
+    const u64 i = static_cast<u64>(blockIdx.x) * kMeshThreads + threadIdx.x;
+    if (i >= nv) return;
+    const int c = static_cast<int>(i % r2);
+    const int b = static_cast<int>((i / r2) % r1);
+    const int a = static_cast<int>(i / (static_cast<u64>(r2) * r1));
+    const u64 st[3] = {static_cast<u64>(r1) * r2, static_cast<u64>(r2), 1};
+    const bool in0 = g[i] > iso;
+    u32 m = 0;
+    if (a + 1 < r0 && ((g[i + st[0]] > iso) != in0)) m |= 1u;
+    if (b + 1 < r1 && ((g[i + st[1]] > iso) != in0)) m |= 2u;
+    if (c + 1 < r2 && ((g[i + st[2]] > iso) != in0)) m |= 4u;
+    emask[i] = static_cast<uint8_t>(m);
+    ecount[i] = static_cast<u32>(__popc(m));
+    u32 nt = 0;
+    if (a + 1 < r0 && b + 1 < r1 && c + 1 < r2) {
+        int cfg = 0;
+        for (int k = 0; k < 8; ++k) {
+            const u64 j = i + (k & 1) * st[0] + ((k >> 1) & 1) * st[1] + ((k >> 2) & 1) * st[2];
+            cfg |= (g[j] > iso ? 1 : 0) << k;
+        }
+        nt = T->ntri[cfg];
+    }
+    tcount[i] = nt;
+}
+
+__global__ __launch_bounds__(kMeshThreads) void k_mc_emit(
+    const float* __restrict__ g, MeshGeom G, float iso, const McDev* __restrict__ T,
+    const uint8_t* __restrict__ emask, const u32* __restrict__ eoff, const u32* __restrict__ toff,
+    const u32* __restrict__ tcount, float* __restrict__ verts, u32* __restrict__ tris) {
+    const int r0 = G.r0, r1 = G.r1, r2 = G.r2;
+    const u64 nv = static_cast<u64>(r0) * r1 * r2;
+    const u64 i = static_cast<u64>(blockIdx.x) * kMeshThreads + threadIdx.x;
+    if (i >= nv) return;
+    const int c = static_cast<int>(i % r2);
+    const int b = static_cast<int>((i / r2) % r1);
+    const int a = static_cast<int>(i / (static_cast<u64>(r2) * r1));
+    const u64 st[3] = {static_cast<u64>(r1) * r2, static_cast<u64>(r2), 1};
+    const int idx[3] = {a, b, c};
+    const double lo[3] = {G.lo0, G.lo1, G.lo2}, sp[3] = {G.sp0, G.sp1, G.sp2};
+    const u32 m = emask[i];
+    u32 v = eoff[i];
+    const double v0 = static_cast<double>(g[i]);
+    for (int ax = 0; ax < 3; ++ax) {
+        if (!((m >> ax) & 1u)) continue;
+        const double v1 = static_cast<double>(g[i + st[ax]]);
+        const double t = (static_cast<double>(iso) - v0) / (v1 - v0);
+        for (int k = 0; k < 3; ++k) {
+            const double f = k == ax ? static_cast<double>(idx[k]) + t : static_cast<double>(idx[k]);
+            verts[3 * static_cast<u64>(v) + k] = static_cast<float>(f * sp[k] + lo[k]);
+        }
+        ++v;
+    }
+    const u32 nt = tcount[i];
+    if (!nt) return;
+    int cfg = 0;
+    for (int k = 0; k < 8; ++k) {
+        const u64 j = i + (k & 1) * st[0] + ((k >> 1) & 1) * st[1] + ((k >> 2) & 1) * st[2];
+        cfg |= (g[j] > iso ? 1 : 0) << k;
+    }
+    const u32 t0 = toff[i];
+    for (u32 t = 0; t < nt; ++t) {
+        for (int q = 0; q < 3; ++q) {
+            const int e = T->tri[cfg][3 * t + q];
+            const int ax = e >> 2, bb = e & 3;
+            // owner corner: bit ax clear, the other two bits from bb (lower axis first)
+            int off[3] = {0, 0, 0}, k = 0;
+            for (int x = 0; x < 3; ++x) {
+                if (x == ax) continue;
+                off[x] = (bb >> k) & 1;
+                ++k;
+            }
+            const u64 o = i + off[0] * st[0] + off[1] * st[1] + off[2] * st[2];
+            const u32 rank = static_cast<u32>(__popc(emask[o] & ((1u << ax) - 1u)));
+            tris[3 * static_cast<u64>(t0 + t) + q] = eoff[o] + rank;
+        }
+    }
+}
+
+// ---- exclusive scan of u32 counts (1024 per block) ----------------------------
+constexpr int kScanPer = 4;
+constexpr int kScanBlock = kMeshThreads * kScanPer;
+
+__global__ __launch_bounds__(kMeshThreads) void k_scan_local(const u32* __restrict__ in,
+                                                             u32* __restrict__ out, u64 n,
+                                                             u32* __restrict__ sums) {
+    __shared__ u32 s[kMeshThreads];
+    const u64 base = static_cast<u64>(blockIdx.x) * kScanBlock + threadIdx.x * kScanPer;
+    u32 v[kScanPer], tot = 0;
+    for (int k = 0; k < kScanPer; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        tot += v[k];
+    }
+    s[threadIdx.x] = tot;
+    __syncthreads();
+    for (int o = 1; o < kMeshThreads; o <<= 1) {
+        const u32 add = threadIdx.x >= static_cast<u32>(o) ? s[threadIdx.x - o] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += add;
+        __syncthreads();
+    }
+    u32 run = s[threadIdx.x] - tot;
+    for (int k = 0; k < kScanPer; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == kMeshThreads - 1) sums[blockIdx.x] = s[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kMeshThreads) void k_scan_add(u32* __restrict__ out, u64 n,
+                                                           const u32* __restrict__ offs) {
+    const u64 base = static_cast<u64>(blockIdx.x) * kScanBlock + threadIdx.x * kScanPer;
+    const u32 add = offs[blockIdx.x];
+    for (int k = 0; k < kScanPer; ++k)
+        if (base + k < n) out[base + k] += add;
+}
+
+}  // namespace
+
+// Exclusive scan of n counts into out; returns the total (synchronous).
+static u64 scan_u32(const u32* in, u32* out, u64 n, hipStream_t s) {
+    if (n == 0) return 0;
+    const u64 nb = (n + kScanBlock - 1) / kScanBlock;
+    DeviceBuffer<u32> sums(nb), offs(nb);
+    k_scan_local<<<static_cast<u32>(nb), kMeshThreads, 0, s>>>(in, out, n, sums.data());
+    WSIM_LAUNCH_CHECK();
+    std::vector<u32> h(nb);
+    WSIM_HIP(hipMemcpyAsync(h.data(), sums.data(), nb * sizeof(u32), hipMemcpyDeviceToHost, s));
+    WSIM_HIP(hipStreamSynchronize(s));
+    u64 run = 0;
+    for (u64 k = 0; k < nb; ++k) {
+        const u64 t = h[k];
+        h[k] = static_cast<u32>(run);
+        run += t;
+        if (run > 0xFFFFFFFFull) fail(kOutOfRange, "SurfaceMesher: more than 2^32 - 1 elements");
+    }
+    WSIM_HIP(hipMemcpyAsync(offs.data(), h.data(), nb * sizeof(u32), hipMemcpyHostToDevice, s));
+    k_scan_add<<<static_cast<u32>(nb), kMeshThreads, 0, s>>>(out, n, offs.data());
+    WSIM_LAUNCH_CHECK();
+    WSIM_HIP(hipStreamSynchronize(s));
+    return run;
+}
+
+std::vector<double> gaussian_weights(double sigma, int* radius) {
+    // scipy.ndimage._gaussian_kernel1d, order 0, truncate 4.0
+    const int R = static_cast<int>(4.0 * sigma + 0.5);
+    std::vector<double> w(2 * R + 1);
+    const double s2 = sigma * sigma;
+    double sum = 0.0;
+    for (int k = -R; k <= R; ++k) {
+        w[k + R] = std::exp(-0.5 / s2 * static_cast<double>(k * k));
+    }
+    for (double x : w) sum += x;
+    for (double& x : w) x = x / sum;
+    *radius = R;
+    return w;
+}
+
+const uint8_t* mc_table_counts() { return mc_table().ntri; }
+const uint8_t* mc_table_triangles() { return &mc_table().tri[0][0]; }
+
+SurfaceMesher::SurfaceMesher(hipStream_t stream) : stream_(stream) {}
+
+void SurfaceMesher::run(const float* xyz, size_t n, const MeshParams& P) {
+    if (!xyz && n) fail(kInvalidArgument, "SurfaceMesher: NULL positions");
+    if (n == 0) fail(kInvalidArgument, "SurfaceMesher: no points");
+    if (!(P.particle_radius > 0.0) || !(P.grid_voxel_factor > 0.0) || !(P.smooth_sigma >= 0.0))
+        fail(kInvalidArgument, "SurfaceMesher: radius, voxel factor and sigma must be positive");
+    hipStream_t s = stream_;
+    // 1. bounds
+    const u32 nbb = static_cast<u32>(std::min<u64>((n + kMeshThreads - 1) / kMeshThreads, 1024));
+    DeviceBuffer<float> part(6 * static_cast<size_t>(nbb));
+    k_mesh_bounds<<<nbb, kMeshThreads, 0, s>>>(xyz, n, P.swap_yz ? 1 : 0, part.data());
+    WSIM_LAUNCH_CHECK();
+    std::vector<float> hp(6 * static_cast<size_t>(nbb));
+    WSIM_HIP(hipMemcpyAsync(hp.data(), part.data(), hp.size() * sizeof(float), hipMemcpyDeviceToHost,
+                            s));
+    WSIM_HIP(hipStreamSynchronize(s));
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (u32 k = 0; k < nbb; ++k)
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = std::fmin(mn[a], hp[6 * k + a]);
+            mx[a] = std::fmax(mx[a], hp[6 * k + 3 + a]);
+        }
+    // 2. grid (tools/particles_to_mesh.py:104-109), binary64 as numpy
+    const double pad = P.particle_radius * 4.0;
+    const double voxel = P.particle_radius * P.grid_voxel_factor;
+    MeshGeom G{};
+    int res[3];
+    double lo[3], ext[3];
+    for (int a = 0; a < 3; ++a) {
+        if (!std::isfinite(mn[a]) || !std::isfinite(mx[a]))
+            fail(kInvalidArgument, "SurfaceMesher: non-finite positions");
+        lo[a] = static_cast<double>(mn[a]) - pad;
+        const double hi = static_cast<double>(mx[a]) + pad;
+        ext[a] = hi - lo[a];
+        const double r = std::ceil(ext[a] / voxel);
+        if (!(r < 65536.0)) fail(kOutOfRange, "SurfaceMesher: grid too large");
+        res[a] = std::max(static_cast<int>(r), 8);
+    }
+    G.r0 = res[0], G.r1 = res[1], G.r2 = res[2];
+    G.lo0 = lo[0], G.lo1 = lo[1], G.lo2 = lo[2];
+    G.ext0 = ext[0], G.ext1 = ext[1], G.ext2 = ext[2];
+    G.sp0 = ext[0] / res[0], G.sp1 = ext[1] / res[1], G.sp2 = ext[2] / res[2];
+    const u64 nv = static_cast<u64>(res[0]) * res[1] * res[2];
+    if (nv > (1ull << 31)) fail(kOutOfRange, "SurfaceMesher: grid too large");
+    info_ = MeshInfo{};
+    for (int a = 0; a < 3; ++a) {
+        info_.dims[a] = res[a];
+        info_.lo[a] = lo[a];
+        info_.extent[a] = ext[a];
+    }
+    info_.spacing[0] = G.sp0, info_.spacing[1] = G.sp1, info_.spacing[2] = G.sp2;
+    const u32 gv = static_cast<u32>((nv + kMeshThreads - 1) / kMeshThreads);
+    // grid-sized work buffers are kept between runs and grow as needed
+    auto fit = [](auto& b, u64 want) {
+        if (b.size() < want) b = std::remove_reference_t<decltype(b)>(want);
+    };
+    fit(grid_, nv);
+    fit(tmp_, nv);
+    fit(count_, nv);
+    fit(ecount_, nv);
+    fit(tcount_, nv);
+    fit(eoff_, nv);
+    fit(toff_, nv);
+    fit(emask_, nv);
+    // 3. splat
+    u32* count = count_.data();
+    WSIM_HIP(hipMemsetAsync(count, 0, nv * sizeof(u32), s));
+    k_mesh_splat<<<static_cast<u32>((n + kMeshThreads - 1) / kMeshThreads), kMeshThreads, 0, s>>>(
+        xyz, n, P.swap_yz ? 1 : 0, G, count);
+    WSIM_LAUNCH_CHECK();
+    k_mesh_to_float<<<gv, kMeshThreads, 0, s>>>(count, grid_.data(), nv);
+    WSIM_LAUNCH_CHECK();
+    // 4. blur
+    if (P.smooth_sigma > 0.0) {
+        int R = 0;
+        std::vector<double> w = gaussian_weights(P.smooth_sigma, &R);
+        if (P.weights) std::memcpy(w.data(), P.weights, w.size() * sizeof(double));
+        DeviceBuffer<double> dw(w.size());
+        WSIM_HIP(hipMemcpyAsync(dw.data(), w.data(), w.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+        float* src = grid_.data();
+        float* dst = tmp_.data();
+        for (int axis = 0; axis < 3; ++axis) {
+            k_mesh_blur<<<gv, kMeshThreads, 0, s>>>(src, dst, res[0], res[1], res[2], axis,
+                                                    dw.data(), R);
+            WSIM_LAUNCH_CHECK();
+            std::swap(src, dst);
+        }
+        if (src != grid_.data()) std::swap(grid_, tmp_);
+        WSIM_HIP(hipStreamSynchronize(s));
+    }
+    // 5. iso level
+    const u32 nbm = std::min<u32>(gv, 1024);
+    DeviceBuffer<float> mpart(nbm);
+    k_mesh_max<<<nbm, kMeshThreads, 0, s>>>(grid_.data(), nv, mpart.data());
+    WSIM_LAUNCH_CHECK();
+    std::vector<float> hm(nbm);
+    WSIM_HIP(hipMemcpyAsync(hm.data(), mpart.data(), nbm * sizeof(float), hipMemcpyDeviceToHost, s));
+    WSIM_HIP(hipStreamSynchronize(s));
+    float gmax = -INFINITY;
+    for (float v : hm) gmax = std::fmax(gmax, v);
+    info_.max_density = gmax;
+    // numpy 2: float32 scalar * Python float stays float32
+    const float iso = gmax * static_cast<float>(P.iso_factor);
+    info_.iso = iso;
+    info_.n_vertices = info_.n_triangles = 0;
+    if (!(gmax > 0.0f)) return;
+    // 6. marching cubes
+    const McTable& T = mc_table();
+    static_assert(sizeof(McDev) == sizeof(McTable), "table layout");
+    DeviceBuffer<McDev> dT(1);
+    WSIM_HIP(hipMemcpyAsync(dT.data(), &T, sizeof(McTable), hipMemcpyHostToDevice, s));
+    k_mc_count<<<gv, kMeshThreads, 0, s>>>(grid_.data(), res[0], res[1], res[2], iso, dT.data(),
+                                           emask_.data(), ecount_.data(), tcount_.data());
+    WSIM_LAUNCH_CHECK();
+    const u64 nvert = scan_u32(ecount_.data(), eoff_.data(), nv, s);
+    const u64 ntri = scan_u32(tcount_.data(), toff_.data(), nv, s);
+    fit(verts_, 3 * std::max<u64>(nvert, 1));
+    fit(tris_, 3 * std::max<u64>(ntri, 1));
+    k_mc_emit<<<gv, kMeshThreads, 0, s>>>(grid_.data(), G, iso, dT.data(), emask_.data(),
+                                          eoff_.data(), toff_.data(), tcount_.data(), verts_.data(),
+                                          tris_.data());
+    WSIM_LAUNCH_CHECK();
+    WSIM_HIP(hipStreamSynchronize(s));
+    info_.n_vertices = nvert;
+    info_.n_triangles = ntri;
+}
+
+}  // namespace wsim

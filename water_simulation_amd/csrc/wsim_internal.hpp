@@ -103,6 +103,43 @@ private:
 // Plain 16-byte streaming copy kernel (copy.hip).
 void device_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
 
+// Fluid surface mesh from particle positions (surface.hip): splat, Gaussian
+// blur, iso level, marching cubes (tools/particles_to_mesh.py:77-138).
+struct MeshParams {
+    double particle_radius = 0.012;   // --radius
+    double grid_voxel_factor = 0.5;   // voxel = factor * radius
+    double smooth_sigma = 2.5;        // blur sigma in voxels (0: none)
+    double iso_factor = 0.18;         // iso = factor * max density
+    bool swap_yz = true;              // sim Y-up -> Blender Z-up (load_frame)
+    const double* weights = nullptr;  // the 2R+1 blur weights, R = int(4 sigma + 0.5); NULL: exp()
+};
+struct MeshInfo {
+    int dims[3];
+    double lo[3], extent[3], spacing[3];
+    float max_density, iso;
+    u64 n_vertices, n_triangles;
+};
+class SurfaceMesher {
+public:
+    explicit SurfaceMesher(hipStream_t stream);
+    // positions: device binary32 xyz, n points
+    void run(const float* xyz, size_t n, const MeshParams& p);
+    const MeshInfo& info() const { return info_; }
+    const float* grid() const { return grid_.data(); }      // blurred density, dims[0..2] C order
+    const float* vertices() const { return verts_.data(); }  // n_vertices x 3
+    const u32* triangles() const { return tris_.data(); }    // n_triangles x 3 vertex indices
+
+private:
+    hipStream_t stream_;
+    MeshInfo info_{};
+    DeviceBuffer<float> grid_, tmp_, verts_;
+    DeviceBuffer<u32> tris_, count_, ecount_, tcount_, eoff_, toff_;
+    DeviceBuffer<uint8_t> emask_;
+};
+// The marching-cubes table: triangle counts [256] and edge ids [256][24].
+const uint8_t* mc_table_counts();
+const uint8_t* mc_table_triangles();
+
 // Exhaustive comparison of the pair-path sqrt and quotients with the IEEE
 // operations for one smoothing length (pair_ops.hip); bit ranges are the
 // binary32 patterns covered ([lo, hi)).
