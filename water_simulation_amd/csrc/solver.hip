@@ -229,7 +229,7 @@ __device__ __forceinline__ void xsph_term(float& dx, float& dy, float& dz, const
                                           const float4& vi, float rho_i, const float4& q,
                                           const float4& vj, float rho_j, float eps,
                                           const Nbr& P, bool keep = true) {
-    const float w = spline_W_near(norm3(p.x - q.x, p.y - q.y, p.z - q.z), P.sp);
+    const float w = spline_W_near(pair_sqrt(dist2(p.x - q.x, p.y - q.y, p.z - q.z), P.sp), P.sp);
     const float rho_avg = 0.5f * (rho_i + rho_j);
     const float c = eps * (P.mass / fmaxf(rho_avg, 1e-9f)) * w;
     // a dropped term adds -0.0f, which leaves every value unchanged
@@ -315,8 +315,10 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
                                                    const float4* __restrict__ vin,
                                                    const float* __restrict__ rho,
                                                    float4* __restrict__ vout) {
-    __shared__ float4 s_x[kStageCap];  // x, y, z, rho
-    __shared__ float4 s_v[kStageCap];  // pre-XSPH v
+    // 24 B per staged neighbour (as k_change_st): rho_j is gathered from
+    // global memory, which keeps four workgroups per CU instead of three
+    __shared__ float4 s_a[kStageCap];  // x, y, z, pre-XSPH vx
+    __shared__ float2 s_b[kStageCap];  // pre-XSPH vy, vz
     const u32 tile = P.remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
     const u32 t0 = tile * kTile;
     const u32 i = t0 + threadIdx.x;
@@ -332,18 +334,16 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
         // every load of a thread is issued before its LDS stores
         const StageLanes E(R);
         f4v x[kStagePer], v[kStagePer];
-        float r[kStagePer];
 #pragma unroll
         for (int u = 0; u < kStagePer; ++u) {
             x[u] = ld4v(xs, E.src[u]);
             v[u] = ld4v(vin, E.src[u]);
-            r[u] = rho[E.src[u]];
         }
 #pragma unroll
         for (int u = 0; u < kStagePer; ++u)
             if (E.has(u)) {
-                reinterpret_cast<f4v*>(s_x)[E.entry(u)] = f4v{x[u].x, x[u].y, x[u].z, r[u]};
-                reinterpret_cast<f4v*>(s_v)[E.entry(u)] = v[u];
+                reinterpret_cast<f4v*>(s_a)[E.entry(u)] = f4v{x[u].x, x[u].y, x[u].z, v[u].x};
+                reinterpret_cast<f2v*>(s_b)[E.entry(u)] = f2v{v[u].y, v[u].z};
             }
     }
     __syncthreads();
@@ -357,12 +357,18 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
         u32 k = 0;
         if (R.total) {
             for_stage_quads(L, i, cf, [&](u32 a0, u32 a1, u32 a2, u32 a3, u32 m) {
-                const float4 q0 = s_x[a0], q1 = s_x[a1], q2 = s_x[a2], q3 = s_x[a3];
-                const float4 w0 = s_v[a0], w1 = s_v[a1], w2 = s_v[a2], w3 = s_v[a3];
-                xsph_term(dx, dy, dz, p, vi, rho_i, q0, w0, q0.w, eps, P);
-                xsph_term(dx, dy, dz, p, vi, rho_i, q1, w1, q1.w, eps, P, m > 1u);
-                xsph_term(dx, dy, dz, p, vi, rho_i, q2, w2, q2.w, eps, P, m > 2u);
-                xsph_term(dx, dy, dz, p, vi, rho_i, q3, w3, q3.w, eps, P, m > 3u);
+                const float r0 = rho[stage_source(R, a0)], r1 = rho[stage_source(R, a1)];
+                const float r2 = rho[stage_source(R, a2)], r3 = rho[stage_source(R, a3)];
+                const float4 q0 = s_a[a0], q1 = s_a[a1], q2 = s_a[a2], q3 = s_a[a3];
+                const float2 b0 = s_b[a0], b1 = s_b[a1], b2 = s_b[a2], b3 = s_b[a3];
+                const float4 w0 = make_float4(q0.w, b0.x, b0.y, 0.0f);
+                const float4 w1 = make_float4(q1.w, b1.x, b1.y, 0.0f);
+                const float4 w2 = make_float4(q2.w, b2.x, b2.y, 0.0f);
+                const float4 w3 = make_float4(q3.w, b3.x, b3.y, 0.0f);
+                xsph_term(dx, dy, dz, p, vi, rho_i, q0, w0, r0, eps, P);
+                xsph_term(dx, dy, dz, p, vi, rho_i, q1, w1, r1, eps, P, m > 1u);
+                xsph_term(dx, dy, dz, p, vi, rho_i, q2, w2, r2, eps, P, m > 2u);
+                xsph_term(dx, dy, dz, p, vi, rho_i, q3, w3, r3, eps, P, m > 3u);
             });
         } else {
             for (; k < cf; ++k) {
