@@ -20,7 +20,10 @@
 //   k_advect           clamp, integrate, domain clamp, scatter position,
 //                      velocity and the solver attributes back to the caller's
 //                      store in original order, max |v| and NaN count
-//                      (dfsph_kernels.cu:363-386, 396-400; dfsph.cu:303-326)
+//                      (dfsph_kernels.cu:363-386, 396-400; dfsph.cu:303-326);
+//                      a single scene uses k_advect_gather, one thread per
+//                      original index reading through the inverse permutation
+//                      that k_reorder records
 //   [divergence solve] grid, reorder, density/alpha, k_change<1> / k_apply,
 //                      k_div_writeback (phase-2 plan text :1858-2003)
 //
@@ -53,20 +56,27 @@ namespace {
 using namespace dev;
 
 // parts: 1 positions, 2 velocities, 3 both.
+// inv (optional, with parts & 1): inv[original index] = working slot.
 __global__ __launch_bounds__(NS) void k_reorder(Nbr P, const u32* __restrict__ perm,
                                                 const float* __restrict__ x,
                                                 const float* __restrict__ v,
                                                 float4* __restrict__ xs, float4* __restrict__ vs,
-                                                int gravity, int parts) {
+                                                int gravity, int parts, u32* __restrict__ inv) {
     const u32 i = blockIdx.x * NS + threadIdx.x;
     if (i >= P.nslots) return;
     const SceneSeg& G = P.seg[scene_of_slot(P, i)];
     const u32 l = i - G.off;
     if (l >= G.n) return;
-    const u64 o = 3ull * perm[G.start + l];
-    if (parts & 1) xs[i] = make_float4(x[o], x[o + 1], x[o + 2], 0.0f);
+    const u32 id = perm[G.start + l];
+    const u64 o = 3ull * id;
+    if (parts & 1) {
+        const float3 p = *reinterpret_cast<const float3*>(x + o);
+        xs[i] = make_float4(p.x, p.y, p.z, 0.0f);
+        if (inv) inv[id] = i;
+    }
     if (!(parts & 2)) return;
-    float a = v[o], b = v[o + 1], c = v[o + 2];
+    const float3 w = *reinterpret_cast<const float3*>(v + o);
+    float a = w.x, b = w.y, c = w.z;
     if (gravity) {  // v * exp(-damping*dt) + g*dt (dfsph_kernels.cu:351-354)
         a = a * G.decay + G.gdx;
         b = b * G.decay + G.gdy;
@@ -1171,6 +1181,59 @@ __global__ __launch_bounds__(NS) void k_advect(Nbr P, const float4* __restrict__
     vstats_partial(mag, bad, pmax, pnan);
 }
 
+// k_advect for a single scene, one thread per original index: the sorted
+// state is gathered through inv (working slot of each original index) and
+// the store is written in order, which keeps the writes coalesced where
+// the scatter of k_advect is not.  The same arithmetic per particle; the
+// max |v| and NaN partials are per block of original indices, which
+// k_finish_vstats reduces over the same block range.
+__global__ __launch_bounds__(NS) void k_advect_gather(
+    Nbr P, const float4* __restrict__ xs, const float4* __restrict__ vs,
+    const u32* __restrict__ inv, float vmax, float lx, float ly, float lz, float hx, float hy,
+    float hz, StoreOut out, const float* __restrict__ rho, const float* __restrict__ alpha,
+    const float* __restrict__ kappa, const float* __restrict__ rho_adv, int wb_solve,
+    const DeviceState* __restrict__ st, int force, float* __restrict__ pmax,
+    u32* __restrict__ pnan) {
+    const u32 id = blockIdx.x * NS + threadIdx.x;
+    float mag = 0.0f;
+    u32 bad = 0;
+    if (!advect_due(st[0], force)) return;  // grid-uniform
+    const SceneSeg& G = P.seg[0];
+    if (id < G.n) {
+        const float dt = G.dt;
+        const u32 i = inv[id];
+        const float4 x4 = xs[i];
+        float4 v = vs[i];
+        const float sp = norm3(v.x, v.y, v.z);
+        if (sp > vmax) {
+            const float f = vmax / sp;
+            v.x *= f;
+            v.y *= f;
+            v.z *= f;
+        }
+        float px = x4.x + v.x * dt, py = x4.y + v.y * dt, pz = x4.z + v.z * dt;
+        if (px < lx) { px = lx; if (v.x < 0) v.x = 0; }
+        if (py < ly) { py = ly; if (v.y < 0) v.y = 0; }
+        if (pz < lz) { pz = lz; if (v.z < 0) v.z = 0; }
+        if (px > hx) { px = hx; if (v.x > 0) v.x = 0; }
+        if (py > hy) { py = hy; if (v.y > 0) v.y = 0; }
+        if (pz > hz) { pz = hz; if (v.z > 0) v.z = 0; }
+        const u64 o = 3ull * id;
+        // one 12-byte store per vector: a wave writes 768 contiguous bytes
+        *reinterpret_cast<float3*>(out.x + o) = make_float3(px, py, pz);
+        *reinterpret_cast<float3*>(out.v + o) = make_float3(v.x, v.y, v.z);
+        out.rho[id] = rho[i];
+        out.alpha[id] = alpha[i];
+        if (wb_solve) {
+            out.kappa[id] = kappa[i];
+            out.rho_adv[id] = rho_adv[i];
+        }
+        mag = norm3(v.x, v.y, v.z);
+        bad = finite3(px, py, pz) && finite3(v.x, v.y, v.z) ? 0u : 1u;
+    }
+    vstats_partial(mag, bad, pmax, pnan);
+}
+
 // End of the divergence solve: velocities and the recomputed attributes back
 // to original order, then max |v|.
 __global__ __launch_bounds__(NS) void k_div_writeback(Nbr P, const float4* __restrict__ xs,
@@ -1558,6 +1621,7 @@ void DFSPHSolver::init(const std::vector<u32>& counts, const std::vector<float>&
     pair_cnt_ = DeviceBuffer<u32>(slots_cap);
     pairs_fi_ = DeviceBuffer<uint16_t>(static_cast<size_t>(kPairDepthFluid) * slots_cap);
     if (cfg_.enable_surface_tension) nrm_ = DeviceBuffer<float4>(slots_cap);
+    inv_ = DeviceBuffer<u32>(slots_cap);
     plan_ = DeviceBuffer<StageRanges>(slots_cap / kTile);
     part_d_ = DeviceBuffer<double>(slots_cap / kTile);
     part_f_ = DeviceBuffer<float>(slots_cap / 64);
@@ -1850,7 +1914,8 @@ void DFSPHSolver::phase_prefix_grid() {
     const Nbr P = nbr();
     tbegin(kKReorder);
     k_reorder<<<nslots_ / NS, NS, 0, stream_>>>(P, fluid_grid_.sorted_indices(), fluid_.positions(),
-                                                 fluid_.velocities(), xs_.data(), nullptr, 0, 1);
+                                                 fluid_.velocities(), xs_.data(), nullptr, 0, 1,
+                                                 batched_ ? nullptr : inv_.data());
     WSIM_LAUNCH_CHECK();
     tend();
     k_stage_plan<<<div_up(nslots_ / kTile, NS), NS, 0, stream_>>>(P, plan_.data());
@@ -1871,7 +1936,7 @@ void DFSPHSolver::phase_prefix_solve(bool gravity) {
     tbegin(kKReorder);
     k_reorder<<<nslots_ / NS, NS, 0, stream_>>>(P, fluid_grid_.sorted_indices(), fluid_.positions(),
                                                  fluid_.velocities(), xs_.data(), vtarget,
-                                                 gravity ? 1 : 0, 2);
+                                                 gravity ? 1 : 0, 2, nullptr);
     WSIM_LAUNCH_CHECK();
     tend();
     if (xsph) {
@@ -1944,12 +2009,21 @@ void DFSPHSolver::phase_advect(bool force) {
     const StoreOut out = store_out(fluid_, h_density_, h_alpha_, h_kappa_, h_density_adv_);
     const float eps = 1e-3f;  // dfsph_kernels.cu:377
     tbegin(kKAdvect);
-    k_advect<<<nslots_ / NS, NS, 0, stream_>>>(
-        P, xs_.data(), vs_.data(), fluid_grid_.sorted_indices(), 5.0f, cfg_.domain_min[0] + eps,
-        cfg_.domain_min[1] + eps, cfg_.domain_min[2] + eps, cfg_.domain_max[0] - eps,
-        cfg_.domain_max[1] - eps, cfg_.domain_max[2] - eps, out, rho_.data(), alpha_.data(),
-        kappa_.data(), rho_adv_.data(), cfg_.max_density_iters > 0 ? 1 : 0, dstate_.data(),
-        force ? 1 : 0, part_f_.data(), part_u_.data());
+    if (!batched_) {
+        k_advect_gather<<<nslots_ / NS, NS, 0, stream_>>>(
+            P, xs_.data(), vs_.data(), inv_.data(), 5.0f, cfg_.domain_min[0] + eps,
+            cfg_.domain_min[1] + eps, cfg_.domain_min[2] + eps, cfg_.domain_max[0] - eps,
+            cfg_.domain_max[1] - eps, cfg_.domain_max[2] - eps, out, rho_.data(), alpha_.data(),
+            kappa_.data(), rho_adv_.data(), cfg_.max_density_iters > 0 ? 1 : 0, dstate_.data(),
+            force ? 1 : 0, part_f_.data(), part_u_.data());
+    } else {
+        k_advect<<<nslots_ / NS, NS, 0, stream_>>>(
+            P, xs_.data(), vs_.data(), fluid_grid_.sorted_indices(), 5.0f, cfg_.domain_min[0] + eps,
+            cfg_.domain_min[1] + eps, cfg_.domain_min[2] + eps, cfg_.domain_max[0] - eps,
+            cfg_.domain_max[1] - eps, cfg_.domain_max[2] - eps, out, rho_.data(), alpha_.data(),
+            kappa_.data(), rho_adv_.data(), cfg_.max_density_iters > 0 ? 1 : 0, dstate_.data(),
+            force ? 1 : 0, part_f_.data(), part_u_.data());
+    }
     WSIM_LAUNCH_CHECK();
     tend();
     // also marks the scenes advected (after the divergence solve, the

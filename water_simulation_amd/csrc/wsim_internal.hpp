@@ -503,6 +503,7 @@ private:
     DeviceBuffer<uint2> pairs_f_, pairs_b_;
     DeviceBuffer<uint16_t> pairs_fi_;
     DeviceBuffer<float4> nrm_;  // surface-tension normals (when enabled)
+    DeviceBuffer<u32> inv_;     // working slot of each original index (plain solver)
     DeviceBuffer<u32> bnear_;  // boundary proximity mask (see stencil.hpp)
     DeviceBuffer<StageRanges> plan_;
     DeviceBuffer<u32> pair_cnt_;
