@@ -193,3 +193,29 @@ def test_gpu_mesh_rejects_empty():
 
     with pytest.raises(InvalidArgument):
         WM.SurfaceMesher().run(np.zeros((0, 3), np.float32))
+
+
+@pytest.mark.gpu
+def test_gpu_mesh_of_a_solver_store_after_steps():
+    """The consumer downstream of the solver: mesh the store's device
+    positions after a few substeps, against the oracle on the same frame."""
+    import water_simulation_amd as ws
+    from water_simulation_amd import mesh as WM
+    from helpers import random_box
+
+    x, v, xb, cfg = random_box(n=3000, seed=11)
+    f = ws.ParticleStore(len(x))
+    f.resize(len(x))
+    f.set_positions(x)
+    f.set_velocities(v)
+    b = ws.ParticleStore(len(xb))
+    b.resize(len(xb))
+    b.set_positions(xb)
+    s = ws.DFSPHSolver(f, b, ws.DFSPHConfig(**cfg))
+    for _ in range(3):
+        s.step(1e-3)
+    m = WM.SurfaceMesher()
+    m.run(f, radius=cfg["particle_radius"])
+    want = M.reconstruct_splat(f.get_positions(), radius=cfg["particle_radius"])
+    v_, t_ = _compare(m, want, "solver store")
+    assert len(t_) > 0 and _mesh_checks(v_, t_)[0]
