@@ -178,7 +178,7 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
     u32 cf = 0, cb = 0;
     bool unlisted = false;  // an in-support fluid pair with a zero gradient
     const StageRanges R = L.plan[i / kTile];
-    for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
+    for_stencil_rows(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
         const float4 q = xs[s];
         const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
         const float r2 = dist2(rx, ry, rz);
