@@ -209,7 +209,7 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
     // to the stencil walk in every list kernel
     if (unlisted) cf = 0xFFFFu;
     if (P.nb && boundary_near(P.gb, P.bnear, p.x, p.y, p.z)) {
-        for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
+        for_stencil_rows(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
             const float4 q = P.xb[s];
             const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
             const float r2 = dist2(rx, ry, rz);
