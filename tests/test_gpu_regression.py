@@ -98,7 +98,8 @@ def test_settled_puddle_literal_depth_check():
 
 def test_settled_puddle_bender_koschier_meets_eta():
     # kappa/rho pressure scaling plus the divergence solve: the density error
-    # reaches eta, which the reference's kappa/rho^2 scaling cannot (survey F5)
+    # reaches eta, which the reference's kappa/rho^2 scaling cannot: each of its
+    # sweeps removes about 1/rho of the density excess
     x, _, xb, cfg = puddle_ref()
     r = cfg["particle_radius"]
     cfg.update(pressure_form=1, enable_divergence_solve=True, max_divergence_iters=10,

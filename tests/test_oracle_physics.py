@@ -5,7 +5,7 @@ tests/unit/test_dfsph_density.cu:16-36), interior density and alpha of an
 ideal lattice, Akinci boundary mass of a 3-layer wall, free fall under
 gravity with damping.  Invariants: momentum conservation of XSPH and of the
 pressure solve, uniform-velocity and epsilon = 0 no-ops, the kappa/rho^2
-versus kappa/rho contraction ratio (survey A18), advection clamps,
+versus kappa/rho contraction ratio, advection clamps,
 divergence-free fields, and the binary64 error reduction.  The binary32
 oracle's error against the binary64 oracle is stated for every field.
 """
@@ -177,7 +177,7 @@ def test_pressure_solve_conserves_momentum():
 
 
 def test_kappa_scaling_contraction_ratio():
-    # survey A18: with alpha = rho/D, one sweep of the reference kappa/rho^2 form
+    # With alpha = rho/D, one sweep of the reference kappa/rho^2 form
     # removes 1/rho of what a kappa/rho sweep removes.  Cell-aligned lattice
     # (identical interior rho), compressed through the mass (r > h/2), with a
     # converging velocity field so that kappa varies between particles.

@@ -210,7 +210,7 @@ def test_sampler_reference_unit_cases():
     near = ((p2 <= mi) | (p2 >= 1 - mi)).any(axis=1)
     assert near.all()
     assert 1.5 * len(p1) < len(p2) < 2.5 * len(p1)
-    # survey counts of the benchmark tanks
+    # wall-particle counts of the benchmark tanks (0.024 m and 0.005 m spacing, 3 layers)
     assert len(O.sample_aabb_boundary((0, 0, 0), (1, 1, 1), 0.024, 3)) == 33282
     assert len(O.sample_aabb_boundary((0, 0, 0), (2, 1, 1), 0.005, 3)) == 1209618
 
