@@ -579,7 +579,7 @@ template <class R> struct Solver {
         close_masks();
     }
 
-    // divergence_change_kernel, phase-2 plan text (plan :1862-1907); boundary
+    // divergence_change_kernel (docs/superpowers/plans/2026-04-24-water-simulation-v2-phase2-dfsph-solver.md:1862-1907); boundary
     // term as in the shipped density_change_kernel (dfsph_kernels.cu:213-224).
     void divergence_change(float dt) {
         const R dtr = R(dt);
@@ -671,7 +671,7 @@ template <class R> struct Solver {
         record_iters(st.density_iters);
     }
 
-    // divergence_solve, plan text :1967-1990
+    // divergence_solve (docs/superpowers/plans/2026-04-24-water-simulation-v2-phase2-dfsph-solver.md:1967-1990)
     void divergence_solve(float dt) {
         const double thr = double(cfg.eta_divergence * cfg.rest_density);
         const u32 fixed = replay_iters();
@@ -775,7 +775,7 @@ template <class R> struct Solver {
         alpha_factor();
     }
 
-    // dfsph.cu:120-133 (+ divergence solve after advect, plan text :1992-2003)
+    // dfsph.cu:120-133 (+ divergence solve after advect, docs/superpowers/plans/2026-04-24-water-simulation-v2-phase2-dfsph-solver.md:1992-2003)
     void step(float dt) {
         if (n == 0) return;
         density_alpha();

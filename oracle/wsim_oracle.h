@@ -19,11 +19,11 @@
  *   boundary mass, density, alpha        -- lattice closed forms, reference KAT m*W(0)
  *   gravity                              -- geometric-series closed form
  *   XSPH, density change, apply kappa    -- momentum invariants, uniform-field invariants,
- *                                           A18 one-sweep contraction factor
+ *                                           the kappa/rho^2 vs kappa/rho one-sweep contraction factor (rho)
  *   advect, max |v|, density error       -- closed-form cases, numpy reductions
  *   boundary sampler, time stepper       -- bit-exact against the compiled reference (oracle/_ref)
  *   divergence solve                     -- invariants only (the reference has no
- *                                           implementation; plan text only)
+ *                                           implementation; docs/superpowers/plans/2026-04-24-water-simulation-v2-phase2-dfsph-solver.md only)
  */
 #ifndef WSIM_ORACLE_H
 #define WSIM_ORACLE_H

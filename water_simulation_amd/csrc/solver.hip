@@ -25,7 +25,7 @@
 //                      original index reading through the inverse permutation
 //                      that k_reorder records
 //   [divergence solve] grid, reorder, density/alpha, k_change<1> / k_apply,
-//                      k_div_writeback (phase-2 plan text :1858-2003)
+//                      k_div_writeback (docs/superpowers/plans/2026-04-24-water-simulation-v2-phase2-dfsph-solver.md:1858-2003)
 //
 // Scenes.  Scene s owns store indices [start_s, start_s + n_s) and working
 // slots [off_s, off_s + n_s), off_s a multiple of kTile, so every tile and
@@ -530,7 +530,7 @@ __device__ __forceinline__ void rho_dot_boundary(float& rd, const float4& p, con
 }
 
 // One predictor sweep.  MODE 0: density change + kappa (dfsph_kernels.cu:
-// 176-232).  MODE 1: divergence change + kappa_v (plan text :1862-1907, with
+// 176-232).  MODE 1: divergence change + kappa_v (docs/superpowers/plans/2026-04-24-water-simulation-v2-phase2-dfsph-solver.md:1862-1907, with
 // the boundary term of the density change).  Writes (x, kappa-hat) for the
 // following k_apply and one binary64 error partial per tile: the four waves
 // of a tile are summed in a fixed order, so the partials do not depend on
