@@ -240,6 +240,55 @@ private:
     wsim_frame_writer h_ = nullptr;
 };
 
+// Surface mesh of device-resident positions: the splat method of
+// tools/particles_to_mesh.py (reconstruct_splat :77-138) on the device.
+class SurfaceMesher {
+public:
+    struct Params {
+        double particle_radius = 0.012;
+        double grid_voxel_factor = 0.5;
+        double smooth_sigma = 2.5;
+        double iso_factor = 0.18;
+        bool swap_yz = true;               // sim Y-up -> Blender Z-up (load_frame :46-58)
+        const double* weights = nullptr;   // 2R + 1 blur weights, R = int(4 sigma + 0.5)
+    };
+    explicit SurfaceMesher(void* stream = nullptr) { detail::check(wsim_mesher_create(stream, &h_)); }
+    ~SurfaceMesher() { wsim_mesher_destroy(h_); }
+    SurfaceMesher(const SurfaceMesher&) = delete;
+    SurfaceMesher& operator=(const SurfaceMesher&) = delete;
+    // Synchronous; vertices() / triangles() stay valid until the next run.
+    wsim_mesh_info run(const float* positions_device, std::size_t count, const Params& p) {
+        wsim_mesh_params c;
+        wsim_mesh_params_default(&c);
+        c.particle_radius = p.particle_radius;
+        c.grid_voxel_factor = p.grid_voxel_factor;
+        c.smooth_sigma = p.smooth_sigma;
+        c.iso_factor = p.iso_factor;
+        c.swap_yz = p.swap_yz ? 1u : 0u;
+        c.weights = p.weights;
+        detail::check(wsim_mesher_run(h_, positions_device, count, &c));
+        wsim_mesh_info info;
+        detail::check(wsim_mesher_info(h_, &info));
+        return info;
+    }
+    wsim_mesh_info run(ParticleStore& store, const Params& p) {
+        return run(reinterpret_cast<const float*>(store.positions()), store.count(), p);
+    }
+    const float* vertices() const {  // device, n_vertices x 3
+        const float* v = nullptr;
+        detail::check(wsim_mesher_vertices(h_, &v));
+        return v;
+    }
+    const u32* triangles() const {  // device, n_triangles x 3
+        const uint32_t* t = nullptr;
+        detail::check(wsim_mesher_triangles(h_, &t));
+        return t;
+    }
+
+private:
+    wsim_mesher h_ = nullptr;
+};
+
 namespace solvers {
 
 class DFSPHSolver {
