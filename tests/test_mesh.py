@@ -219,3 +219,30 @@ def test_gpu_mesh_of_a_solver_store_after_steps():
     want = M.reconstruct_splat(f.get_positions(), radius=cfg["particle_radius"])
     v_, t_ = _compare(m, want, "solver store")
     assert len(t_) > 0 and _mesh_checks(v_, t_)[0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["one point", "identical points", "wide blur", "no surface"])
+def test_gpu_mesh_edge_cases(case):
+    from water_simulation_amd import mesh as WM
+
+    kw = dict(radius=0.01)
+    if case == "one point":
+        x = np.array([[0.3, 0.2, 0.1]], np.float32)
+    elif case == "identical points":
+        x = np.tile(np.array([[0.5, 0.5, 0.5]], np.float32), (50, 1))
+    elif case == "wide blur":  # 41 taps on an 8-voxel minimum grid: reflections wrap
+        x = np.random.default_rng(4).uniform(0.0, 0.01, (40, 3)).astype(np.float32)
+        kw.update(smooth_sigma=5.0, grid_voxel_factor=4.0)
+    else:  # the iso level above the maximum: no crossing edge
+        x = _cloud(5, n_side=6)
+        kw.update(iso_factor=1.5)
+    m = WM.SurfaceMesher()
+    m.run(x, **kw)
+    want = M.reconstruct_splat(x, radius=kw["radius"], voxel_factor=kw.get("grid_voxel_factor", 0.5),
+                               sigma=kw.get("smooth_sigma", 2.5), iso_factor=kw.get("iso_factor", 0.18))
+    v, t = _compare(m, want, case)
+    if case == "no surface":
+        assert len(v) == 0 and len(t) == 0
+    else:
+        assert _mesh_checks(v, t)[0]
