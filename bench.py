@@ -143,7 +143,7 @@ def dam_break(ws, x, xb, cfg, steps: int, warmup: int, timing: bool) -> dict:
     out = {"value": n * steps / elapsed, "elapsed": elapsed, "stats": stats,
            "eta_met_fraction": float(np.mean(np.array(errs) < eta)),
            "kernel_times": kernel_times, "iters": iters, "dts": dts, "x0": x0, "v0": v0,
-           "cfg": cfg}
+           "cfg": cfg, "x_end": fluid.get_positions()}
     solver.close()
     return out
 
@@ -315,11 +315,12 @@ def lbvh_leg(x, reps: int) -> dict:
             "points_per_s": len(x) / el, "root_parent_ok": bool(parent[0] == 0xFFFFFFFF)}
 
 
-def mesh_leg(x, r: float, reps: int, cpu: bool) -> dict:
-    """The splat surface mesher (tools/particles_to_mesh.py:77-138, position
-    pre-smoothing and Laplacian smoothing off) on the dam-break positions:
-    voxel r/2, blur sigma 2.5 voxels, iso 0.18 max.  The CPU baseline runs the
-    reference's own numpy splat and scipy blur on the 50^3 corner of the block."""
+def mesh_leg(x, r: float, reps: int, cpu: bool, what: str = "dam-break positions") -> dict:
+    """The splat surface mesher with the tool's defaults
+    (tools/particles_to_mesh.py:61-138: two 15-nearest-neighbour smoothing
+    passes, voxel r/2, blur sigma 2.5 voxels, iso 0.18 max; no Laplacian
+    smoothing) on a dam-break frame.  The CPU baseline runs the reference's
+    own numpy splat and scipy blur on the 50^3 corner of the block."""
     import water_simulation_amd as ws
     from water_simulation_amd import mesh as WM
     buf = ws.DeviceBuffer.from_array(x)
@@ -331,7 +332,8 @@ def mesh_leg(x, r: float, reps: int, cpu: bool) -> dict:
         info = m.run((buf.ptr, len(x)), radius=r)
     el = (time.perf_counter() - t0) / reps
     voxels = int(np.prod(info["dims"]))
-    out = {"workload": f"surface mesh of the {len(x)} dam-break positions, radius {r}",
+    out = {"workload": f"surface mesh of the {len(x)} {what}, radius {r}, "
+                       "2 x 15-NN position smoothing",
            "points": len(x), "grid": list(info["dims"]), "voxels": voxels,
            "vertices": info["n_vertices"], "triangles": info["n_triangles"], "runs": reps,
            "ms_per_mesh": el * 1e3, "voxels_per_s": voxels / el, "cpu_baseline": None}
@@ -413,7 +415,8 @@ def main() -> int:
     if args.only_mesh:
         x, _, cfg = make_workload(args.n_side)
         print(json.dumps({"mesh": mesh_leg(x, cfg["particle_radius"], max(args.mesh_reps, 1),
-                                           not args.no_cpu_baseline)}))
+                                           not args.no_cpu_baseline,
+                                           what="initial dam-break lattice positions")}))
         return 0
     if args.only_lbvh:
         x, _, _ = make_workload(args.n_side)
@@ -531,7 +534,9 @@ def main() -> int:
         "batched": batched,
         "adjoint": adjoint,
         "lbvh": lbvh_leg(x, args.lbvh_reps) if args.lbvh_reps > 0 else None,
-        "mesh": (mesh_leg(x, cfg["particle_radius"], args.mesh_reps, not args.no_cpu_baseline)
+        "mesh": (mesh_leg(run["x_end"], cfg["particle_radius"], args.mesh_reps,
+                          not args.no_cpu_baseline,
+                          what=f"dam-break positions after the {args.steps} timed substeps")
                  if args.mesh_reps > 0 else None),
     }
     print(json.dumps(out))

@@ -337,10 +337,12 @@ void wsim_frame_writer_destroy(wsim_frame_writer writer);
  * scipy.ndimage.gaussian_filter (truncate 4, 'reflect'), iso = max *
  * iso_factor, then marching cubes with one shared vertex per crossing grid
  * edge.  Its triangle table resolves ambiguous faces by separating the
- * inside corners (the reference calls skimage's marching_cubes).  The
- * reference's optional k-nearest-neighbour position smoothing and Laplacian
- * mesh smoothing are not applied.  weights, when not NULL, are the 2R + 1
- * blur weights (R = (int)(4 sigma + 0.5)) to use instead of exp(). */
+ * inside corners (the reference calls skimage's marching_cubes).  Before
+ * the bounds, smooth_pre_iters passes replace every point by the mean of its
+ * k nearest points (smooth_positions_knn :61-74, exact kNN, ties by index);
+ * the reference's Laplacian mesh smoothing is not applied.  weights, when
+ * not NULL, are the 2R + 1 blur weights (R = (int)(4 sigma + 0.5)) to use
+ * instead of exp(). */
 typedef struct wsim_mesher_t* wsim_mesher;
 typedef struct {
     double particle_radius;   /* --radius (0.012) */
@@ -350,6 +352,8 @@ typedef struct {
     uint32_t swap_yz;         /* 1: (x, y, z) -> (x, z, y) as load_frame */
     uint32_t reserved;
     const double* weights;
+    int32_t smooth_pre_iters; /* k-NN position smoothing passes (--smooth-pre, 2) */
+    int32_t k_neighbors;      /* --k-neighbors: 8, 10, 15 (default), 20 or 30 */
 } wsim_mesh_params;
 typedef struct {
     int32_t dims[3];

@@ -1,8 +1,9 @@
 """CPU oracle of the surface mesher (test infrastructure only).
 
 Follows the splat method of tools/particles_to_mesh.py: load_frame (:46-58,
-the y/z swap), reconstruct_splat (:77-138) with the position pre-smoothing
-and the Laplacian post-smoothing off: the grid set-up and the splat
+the y/z swap), the k-nearest-neighbour position smoothing with scipy's
+cKDTree as the tool calls it (:61-74), reconstruct_splat (:77-138) with the
+Laplacian post-smoothing off: the grid set-up and the splat
 (:104-115) in numpy exactly as written there, the blur by
 scipy.ndimage.gaussian_filter itself (:118), iso = max * iso_factor
 (:120-123, numpy 2 scalar rules keep it binary32).  The isosurface step
@@ -17,8 +18,11 @@ Pinned by tests/test_mesh.py: blur_restated equals scipy bit for bit;
 every mesh of a field that stays inside the grid is closed and consistently
 oriented; a sphere's area and enclosed volume converge to 4 pi r^2 and
 4/3 pi r^3 with the outward orientation, Euler characteristic 2 (a torus 0);
-complementary corner configurations cut the same edges.  Parity with
-skimage's Lewiner tables is unpinned (not available).
+complementary corner configurations cut the same edges; cKDTree's
+neighbours equal a brute-force search ordered by (distance, index) on
+inputs without distance ties (the GPU's order).  Parity with skimage's
+Lewiner tables is unpinned (not available), and so is cKDTree's order
+among exactly equal distances.
 """
 from __future__ import annotations
 
@@ -55,6 +59,28 @@ def blur_restated(grid: np.ndarray, sigma: float) -> np.ndarray:
             acc = acc + (a[..., refl(idx - j)] + a[..., refl(idx + j)]) * w[R + j]
         out = np.moveaxis(acc.astype(np.float32), -1, axis)
     return out
+
+
+def smooth_positions_knn(pts: np.ndarray, k: int = 15, iters: int = 2) -> np.ndarray:
+    """tools/particles_to_mesh.py:61-74: every point becomes the mean of its
+    k nearest points (itself included) found by scipy's cKDTree, twice."""
+    from scipy.spatial import cKDTree
+
+    smoothed = pts.copy()
+    for _ in range(iters):
+        tree = cKDTree(smoothed)
+        _, idx = tree.query(smoothed, k=k)
+        smoothed = smoothed[idx].mean(axis=1)
+    return smoothed
+
+
+def knn_brute(pts: np.ndarray, k: int) -> np.ndarray:
+    """k nearest of every point by brute force, ordered by (squared distance
+    with ((dx^2 + dy^2) + dz^2), index)."""
+    d = pts[None, :, :] - pts[:, None, :]
+    d2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+    order = np.lexsort((np.broadcast_to(np.arange(len(pts)), d2.shape), d2), axis=1)
+    return order[:, :k]
 
 
 def grid_setup(pts: np.ndarray, radius: float, voxel_factor: float):
@@ -199,11 +225,13 @@ def marching_cubes(grid: np.ndarray, iso, lo, spacing):
 
 
 def reconstruct_splat(points, radius=0.012, voxel_factor=0.5, sigma=2.5, iso_factor=0.18,
-                      swap_yz=True):
+                      swap_yz=True, smooth_pre=2, k=15):
     """The whole path on the CPU -> dict(lo, extent, res, grid, iso, vertices, triangles)."""
     pts = np.asarray(points, np.float32).astype(np.float64)
     if swap_yz:
         pts = pts[:, [0, 2, 1]]
+    if smooth_pre > 0:
+        pts = smooth_positions_knn(pts, k=k, iters=smooth_pre)
     lo, extent, res = grid_setup(pts, radius, voxel_factor)
     grid = splat(pts, lo, extent, res)
     if sigma > 0:

@@ -98,6 +98,32 @@ def test_complementary_configurations_cut_the_same_edges():
         assert a == cut, cfg
 
 
+def test_ckdtree_neighbours_are_brute_force_order():
+    """On points without distance ties, cKDTree.query's rows are the k
+    nearest in ascending distance, self first: the order the GPU sums in."""
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(7)
+    x = rng.uniform(0, 1, (400, 3))
+    _, idx = cKDTree(x).query(x, k=15)
+    assert np.array_equal(idx, M.knn_brute(x, 15))
+    assert np.array_equal(idx[:, 0], np.arange(len(x)))
+
+
+def test_knn_smoothing_pulls_points_together():
+    """Each pass replaces a point by a mean of neighbours: the cloud's spread
+    shrinks, its mean stays within the hull, a lattice interior is a fixed
+    point of one pass up to rounding."""
+    rng = np.random.default_rng(8)
+    x = rng.normal(0, 1, (300, 3))
+    y = M.smooth_positions_knn(x, k=10, iters=1)
+    assert y.std(0).max() < x.std(0).max()
+    g = np.stack(np.meshgrid(*[np.arange(9.0)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    s = M.smooth_positions_knn(g, k=7, iters=1)
+    inner = np.all((g >= 2) & (g <= 6), axis=1)
+    assert np.abs(s[inner] - g[inner]).max() < 1e-12  # self + 6 face neighbours
+
+
 def test_library_table_is_the_oracles():
     from water_simulation_amd import mesh as WM
 
@@ -227,10 +253,12 @@ def test_gpu_mesh_edge_cases(case):
     from water_simulation_amd import mesh as WM
 
     kw = dict(radius=0.01)
-    if case == "one point":
+    if case == "one point":  # fewer points than k: no position smoothing
         x = np.array([[0.3, 0.2, 0.1]], np.float32)
-    elif case == "identical points":
+        kw.update(smooth_pre=0)
+    elif case == "identical points":  # all distances tie: no position smoothing
         x = np.tile(np.array([[0.5, 0.5, 0.5]], np.float32), (50, 1))
+        kw.update(smooth_pre=0)
     elif case == "wide blur":  # 41 taps on an 8-voxel minimum grid: reflections wrap
         x = np.random.default_rng(4).uniform(0.0, 0.01, (40, 3)).astype(np.float32)
         kw.update(smooth_sigma=5.0, grid_voxel_factor=4.0)
@@ -240,9 +268,33 @@ def test_gpu_mesh_edge_cases(case):
     m = WM.SurfaceMesher()
     m.run(x, **kw)
     want = M.reconstruct_splat(x, radius=kw["radius"], voxel_factor=kw.get("grid_voxel_factor", 0.5),
-                               sigma=kw.get("smooth_sigma", 2.5), iso_factor=kw.get("iso_factor", 0.18))
+                               sigma=kw.get("smooth_sigma", 2.5), iso_factor=kw.get("iso_factor", 0.18),
+                               smooth_pre=kw.get("smooth_pre", 2))
     v, t = _compare(m, want, case)
     if case == "no surface":
         assert len(v) == 0 and len(t) == 0
     else:
         assert _mesh_checks(v, t)[0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k,iters", [(8, 1), (15, 3), (30, 2)])
+def test_gpu_knn_smoothing_matches_ckdtree(k, iters):
+    from water_simulation_amd import mesh as WM
+
+    x = _cloud(6, n_side=14)
+    m = WM.SurfaceMesher()
+    m.run(x, radius=0.012, smooth_pre=iters, k_neighbors=k)
+    _compare(m, M.reconstruct_splat(x, radius=0.012, smooth_pre=iters, k=k), f"k={k} x{iters}")
+
+
+@pytest.mark.gpu
+def test_gpu_knn_smoothing_rejects_bad_k():
+    from water_simulation_amd import mesh as WM
+    from water_simulation_amd._ffi import InvalidArgument
+
+    m = WM.SurfaceMesher()
+    with pytest.raises(InvalidArgument):
+        m.run(_cloud(1, n_side=4), k_neighbors=7)
+    with pytest.raises(InvalidArgument):
+        m.run(np.zeros((10, 3), np.float32), k_neighbors=15)

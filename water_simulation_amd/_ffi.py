@@ -114,6 +114,8 @@ class MeshParamsC(C.Structure):
         ("swap_yz", C.c_uint32),
         ("reserved", C.c_uint32),
         ("weights", C.c_void_p),
+        ("smooth_pre_iters", C.c_int32),
+        ("k_neighbors", C.c_int32),
     ]
 
 

@@ -679,6 +679,8 @@ void wsim_mesh_params_default(wsim_mesh_params* out) {
     out->swap_yz = d.swap_yz ? 1u : 0u;
     out->reserved = 0;
     out->weights = nullptr;
+    out->smooth_pre_iters = d.smooth_pre_iters;
+    out->k_neighbors = d.k_neighbors;
 }
 wsim_status wsim_mesher_create(void* stream, wsim_mesher* out) {
     return guarded([&] {
@@ -702,6 +704,8 @@ wsim_status wsim_mesher_run(wsim_mesher m, const float* xyz, size_t count,
         q.iso_factor = p->iso_factor;
         q.swap_yz = p->swap_yz != 0;
         q.weights = p->weights;
+        q.smooth_pre_iters = p->smooth_pre_iters;
+        q.k_neighbors = p->k_neighbors;
         m->impl->run(xyz, count, q);
     });
 }

@@ -14,8 +14,9 @@
 //   5. marching cubes with shared vertices: one vertex per grid edge whose
 //      end values straddle the iso level (linear interpolation), triangles
 //      from a table built from the cube faces (mc_table).
-// The reference's optional position pre-smoothing (k nearest neighbours)
-// and post Laplacian smoothing are not part of this path.
+// Before the bounds, the reference's k-nearest-neighbour position smoothing
+// (smooth_positions_knn :61-74) runs smooth_pre_iters times; its Laplacian
+// mesh post-smoothing is not part of this path.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -24,6 +25,7 @@
 
 #include "common.hpp"
 #include "device_math.hpp"
+#include "radix_sort.hpp"
 #include "wsim_internal.hpp"
 
 namespace wsim {
@@ -119,24 +121,28 @@ struct MeshGeom {
     double sp0, sp1, sp2;
 };
 
-__device__ __forceinline__ float3 mesh_point(const float* xyz, u64 i, int swap) {
-    const float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
-    return swap ? make_float3(x, z, y) : make_float3(x, y, z);
+// binary64 points as load_frame makes them (:46-58): astype(float64), then
+// (x, y, z) -> (x, z, y) when swap is set
+__global__ __launch_bounds__(kMeshThreads) void k_mesh_points(const float* __restrict__ xyz, u64 n,
+                                                              int swap, double* __restrict__ P) {
+    const u64 i = static_cast<u64>(blockIdx.x) * kMeshThreads + threadIdx.x;
+    if (i >= n) return;
+    const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+    P[3 * i] = x;
+    P[3 * i + 1] = swap ? z : y;
+    P[3 * i + 2] = swap ? y : z;
 }
 
-__global__ __launch_bounds__(kMeshThreads) void k_mesh_bounds(const float* __restrict__ xyz, u64 n,
-                                                              int swap, float* __restrict__ part) {
-    __shared__ float s[6][kMeshThreads];
-    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+__global__ __launch_bounds__(kMeshThreads) void k_mesh_bounds(const double* __restrict__ P, u64 n,
+                                                              double* __restrict__ part) {
+    __shared__ double s[6][kMeshThreads];
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     const u64 stride = static_cast<u64>(gridDim.x) * kMeshThreads;
     for (u64 i = static_cast<u64>(blockIdx.x) * kMeshThreads + threadIdx.x; i < n; i += stride) {
-        const float3 p = mesh_point(xyz, i, swap);
-        mn[0] = fminf(mn[0], p.x);
-        mn[1] = fminf(mn[1], p.y);
-        mn[2] = fminf(mn[2], p.z);
-        mx[0] = fmaxf(mx[0], p.x);
-        mx[1] = fmaxf(mx[1], p.y);
-        mx[2] = fmaxf(mx[2], p.z);
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = fmin(mn[a], P[3 * i + a]);
+            mx[a] = fmax(mx[a], P[3 * i + a]);
+        }
     }
     for (int a = 0; a < 3; ++a) {
         s[a][threadIdx.x] = mn[a];
@@ -144,32 +150,145 @@ __global__ __launch_bounds__(kMeshThreads) void k_mesh_bounds(const float* __res
     }
     __syncthreads();
     if (threadIdx.x < 6) {
-        float r = s[threadIdx.x][0];
+        double r = s[threadIdx.x][0];
         for (int k = 1; k < kMeshThreads; ++k)
-            r = threadIdx.x < 3 ? fminf(r, s[threadIdx.x][k]) : fmaxf(r, s[threadIdx.x][k]);
+            r = threadIdx.x < 3 ? fmin(r, s[threadIdx.x][k]) : fmax(r, s[threadIdx.x][k]);
         part[6 * blockIdx.x + threadIdx.x] = r;
     }
 }
 
 // numpy's ((p - lo) / extent * res).astype(int), then clip to [0, res - 1]
-__device__ __forceinline__ int voxel_of(float p, double lo, double ext, int res) {
-    const double f = (static_cast<double>(p) - lo) / ext * static_cast<double>(res);
+__device__ __forceinline__ int voxel_of(double p, double lo, double ext, int res) {
+    const double f = (p - lo) / ext * static_cast<double>(res);
     long long k = f == f ? static_cast<long long>(f) : 0;
     if (k < 0) k = 0;
     if (k > res - 1) k = res - 1;
     return static_cast<int>(k);
 }
 
-__global__ __launch_bounds__(kMeshThreads) void k_mesh_splat(const float* __restrict__ xyz, u64 n,
-                                                             int swap, MeshGeom G,
-                                                             u32* __restrict__ count) {
+__global__ __launch_bounds__(kMeshThreads) void k_mesh_splat(const double* __restrict__ P, u64 n,
+                                                             MeshGeom G, u32* __restrict__ count) {
     const u64 i = static_cast<u64>(blockIdx.x) * kMeshThreads + threadIdx.x;
     if (i >= n) return;
-    const float3 p = mesh_point(xyz, i, swap);
-    const int a = voxel_of(p.x, G.lo0, G.ext0, G.r0);
-    const int b = voxel_of(p.y, G.lo1, G.ext1, G.r1);
-    const int c = voxel_of(p.z, G.lo2, G.ext2, G.r2);
+    const int a = voxel_of(P[3 * i], G.lo0, G.ext0, G.r0);
+    const int b = voxel_of(P[3 * i + 1], G.lo1, G.ext1, G.r1);
+    const int c = voxel_of(P[3 * i + 2], G.lo2, G.ext2, G.r2);
     atomicAdd(count + (static_cast<u64>(a) * G.r1 + b) * G.r2 + c, 1u);
+}
+
+// ---- k-nearest-neighbour position smoothing (smooth_positions_knn, :61-74)
+// Each point becomes the mean of its k nearest points (itself included), in
+// binary64, summed in ascending (distance^2, index) order and divided by k,
+// as numpy's mean over cKDTree.query's rows.  Exact kNN on a uniform grid:
+// rings of cells around the home cell are scanned until the k-th distance is
+// below the ring's reach (points outside rings 0..R lie at least R cells away).
+constexpr int kKnnMax = 32;
+
+struct KnnGrid {
+    double lo0, lo1, lo2, cell;
+    int d0, d1, d2;
+};
+
+__device__ __forceinline__ int knn_cell(double p, double lo, double cell, int d) {
+    const double f = (p - lo) / cell;
+    int k = f >= 0.0 ? static_cast<int>(f) : 0;
+    return k > d - 1 ? d - 1 : k;
+}
+
+__global__ __launch_bounds__(kMeshThreads) void k_knn_keys(const double* __restrict__ P, u32 n,
+                                                           KnnGrid g, u32* __restrict__ keys,
+                                                           u32* __restrict__ vals) {
+    const u32 i = blockIdx.x * kMeshThreads + threadIdx.x;
+    if (i >= n) return;
+    const int a = knn_cell(P[3ull * i], g.lo0, g.cell, g.d0);
+    const int b = knn_cell(P[3ull * i + 1], g.lo1, g.cell, g.d1);
+    const int c = knn_cell(P[3ull * i + 2], g.lo2, g.cell, g.d2);
+    keys[i] = (static_cast<u32>(a) * g.d1 + b) * g.d2 + c;
+    vals[i] = i;
+}
+
+// Points in cell order, so that a cell's candidates are read contiguously.
+__global__ __launch_bounds__(kMeshThreads) void k_knn_gather(const double* __restrict__ P, u32 n,
+                                                             const u32* __restrict__ sorted,
+                                                             double* __restrict__ Ps) {
+    const u32 s = blockIdx.x * kMeshThreads + threadIdx.x;
+    if (s >= n) return;
+    const u32 j = sorted[s];
+    Ps[3ull * s] = P[3ull * j];
+    Ps[3ull * s + 1] = P[3ull * j + 1];
+    Ps[3ull * s + 2] = P[3ull * j + 2];
+}
+
+template <int K>
+__global__ __launch_bounds__(kMeshThreads) void k_knn_smooth(const double* __restrict__ P, u32 n,
+                                                             KnnGrid g,
+                                                             const u32* __restrict__ sorted,
+                                                             const double* __restrict__ Ps,
+                                                             const u32* __restrict__ cs,
+                                                             double* __restrict__ out) {
+    // one thread per point in cell order: a wave's points share cells
+    const u32 q = blockIdx.x * kMeshThreads + threadIdx.x;
+    if (q >= n) return;
+    const u32 i = sorted[q];
+    const double px = Ps[3ull * q], py = Ps[3ull * q + 1], pz = Ps[3ull * q + 2];
+    const int h0 = knn_cell(px, g.lo0, g.cell, g.d0);
+    const int h1 = knn_cell(py, g.lo1, g.cell, g.d1);
+    const int h2 = knn_cell(pz, g.lo2, g.cell, g.d2);
+    double bd[K];
+    u32 bi[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        bd[k] = INFINITY;
+        bi[k] = 0xFFFFFFFFu;
+    }
+    const int rmax = max(g.d0, max(g.d1, g.d2));
+    for (int R = 0; R <= rmax; ++R) {
+        for (int a = h0 - R; a <= h0 + R; ++a) {
+            if (a < 0 || a >= g.d0) continue;
+            for (int b = h1 - R; b <= h1 + R; ++b) {
+                if (b < 0 || b >= g.d1) continue;
+                const bool face = a == h0 - R || a == h0 + R || b == h1 - R || b == h1 + R;
+                // interior of the shell in z: only its two ends unless on a face
+                for (int c = h2 - R; c <= h2 + R; c += (face || R == 0) ? 1 : 2 * R) {
+                    if (c < 0 || c >= g.d2) continue;
+                    const u32 key = (static_cast<u32>(a) * g.d1 + b) * g.d2 + c;
+                    for (u32 s = cs[key]; s < cs[key + 1]; ++s) {
+                        const u32 j = sorted[s];
+                        const double dx = Ps[3ull * s] - px, dy = Ps[3ull * s + 1] - py,
+                                     dz = Ps[3ull * s + 2] - pz;
+                        double d = (dx * dx + dy * dy) + dz * dz;
+                        u32 id = j;
+                        // most candidates do not beat the current k-th: one test
+                        if (!(d < bd[K - 1] || (d == bd[K - 1] && id < bi[K - 1]))) continue;
+                        // insert (d, id) into the ascending list (compile-time indices)
+#pragma unroll
+                        for (int k = 0; k < K; ++k) {
+                            const bool less = d < bd[k] || (d == bd[k] && id < bi[k]);
+                            const double td = bd[k];
+                            const u32 ti = bi[k];
+                            bd[k] = less ? d : td;
+                            bi[k] = less ? id : ti;
+                            d = less ? td : d;
+                            id = less ? ti : id;
+                        }
+                    }
+                }
+            }
+        }
+        const double reach = static_cast<double>(R) * g.cell * (1.0 - 1e-9);
+        if (bi[K - 1] != 0xFFFFFFFFu && bd[K - 1] < reach * reach) break;
+    }
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const u32 j = bi[k];
+        sx = sx + P[3ull * j];
+        sy = sy + P[3ull * j + 1];
+        sz = sz + P[3ull * j + 2];
+    }
+    out[3ull * i] = sx / static_cast<double>(K);
+    out[3ull * i + 1] = sy / static_cast<double>(K);
+    out[3ull * i + 2] = sz / static_cast<double>(K);
 }
 
 __global__ __launch_bounds__(kMeshThreads) void k_mesh_to_float(const u32* __restrict__ count,
@@ -421,22 +540,96 @@ void SurfaceMesher::run(const float* xyz, size_t n, const MeshParams& P) {
     if (n == 0) fail(kInvalidArgument, "SurfaceMesher: no points");
     if (!(P.particle_radius > 0.0) || !(P.grid_voxel_factor > 0.0) || !(P.smooth_sigma >= 0.0))
         fail(kInvalidArgument, "SurfaceMesher: radius, voxel factor and sigma must be positive");
+    if (P.smooth_pre_iters < 0) fail(kInvalidArgument, "SurfaceMesher: negative smoothing iterations");
+    if (P.smooth_pre_iters > 0) {
+        const int k = P.k_neighbors;
+        if (k != 8 && k != 10 && k != 15 && k != 20 && k != 30)
+            fail(kInvalidArgument, "SurfaceMesher: k_neighbors must be 8, 10, 15, 20 or 30");
+        if (n < static_cast<size_t>(k))
+            fail(kInvalidArgument, "SurfaceMesher: fewer points than k_neighbors");
+    }
+    if (n > 0xFFFFF000ull) fail(kInvalidArgument, "SurfaceMesher: too many points");
     hipStream_t s = stream_;
-    // 1. bounds
-    const u32 nbb = static_cast<u32>(std::min<u64>((n + kMeshThreads - 1) / kMeshThreads, 1024));
-    DeviceBuffer<float> part(6 * static_cast<size_t>(nbb));
-    k_mesh_bounds<<<nbb, kMeshThreads, 0, s>>>(xyz, n, P.swap_yz ? 1 : 0, part.data());
+    const u32 nn = static_cast<u32>(n);
+    const u32 gp = div_up(nn, static_cast<u32>(kMeshThreads));
+    if (pts_[0].size() < 3 * n) {
+        pts_[0] = DeviceBuffer<double>(3 * n);
+        pts_[1] = DeviceBuffer<double>(3 * n);
+    }
+    k_mesh_points<<<gp, kMeshThreads, 0, s>>>(xyz, n, P.swap_yz ? 1 : 0, pts_[0].data());
     WSIM_LAUNCH_CHECK();
-    std::vector<float> hp(6 * static_cast<size_t>(nbb));
-    WSIM_HIP(hipMemcpyAsync(hp.data(), part.data(), hp.size() * sizeof(float), hipMemcpyDeviceToHost,
-                            s));
-    WSIM_HIP(hipStreamSynchronize(s));
-    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (u32 k = 0; k < nbb; ++k)
+    const u32 nbb = std::min<u32>(gp, 1024);
+    DeviceBuffer<double> part(6 * static_cast<size_t>(nbb));
+    auto bounds = [&](const double* pts, double* mn, double* mx) {
+        k_mesh_bounds<<<nbb, kMeshThreads, 0, s>>>(pts, n, part.data());
+        WSIM_LAUNCH_CHECK();
+        std::vector<double> hp(6 * static_cast<size_t>(nbb));
+        WSIM_HIP(hipMemcpyAsync(hp.data(), part.data(), hp.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+        WSIM_HIP(hipStreamSynchronize(s));
         for (int a = 0; a < 3; ++a) {
-            mn[a] = std::fmin(mn[a], hp[6 * k + a]);
-            mx[a] = std::fmax(mx[a], hp[6 * k + 3 + a]);
+            mn[a] = INFINITY;
+            mx[a] = -INFINITY;
         }
+        for (u32 k = 0; k < nbb; ++k)
+            for (int a = 0; a < 3; ++a) {
+                mn[a] = std::fmin(mn[a], hp[6 * k + a]);
+                mx[a] = std::fmax(mx[a], hp[6 * k + 3 + a]);
+            }
+        for (int a = 0; a < 3; ++a)
+            if (!std::isfinite(mn[a]) || !std::isfinite(mx[a]))
+                fail(kInvalidArgument, "SurfaceMesher: non-finite positions");
+    };
+    // 1. k-nearest-neighbour position smoothing (:61-74)
+    int cur = 0;
+    double mn[3], mx[3];
+    for (int it = 0; it < P.smooth_pre_iters; ++it) {
+        bounds(pts_[cur].data(), mn, mx);
+        KnnGrid kg{};
+        kg.lo0 = mn[0], kg.lo1 = mn[1], kg.lo2 = mn[2];
+        kg.cell = 4.0 * P.particle_radius;
+        u64 cells = 0;
+        for (;;) {
+            kg.d0 = static_cast<int>((mx[0] - mn[0]) / kg.cell) + 1;
+            kg.d1 = static_cast<int>((mx[1] - mn[1]) / kg.cell) + 1;
+            kg.d2 = static_cast<int>((mx[2] - mn[2]) / kg.cell) + 1;
+            cells = static_cast<u64>(kg.d0) * kg.d1 * kg.d2;
+            if (cells <= (1ull << 26)) break;
+            kg.cell *= 2.0;
+        }
+        if (knn_keys_[0].size() < n) {
+            for (auto& b : knn_keys_) b = DeviceBuffer<u32>(n);
+            for (auto& b : knn_vals_) b = DeviceBuffer<u32>(n);
+            knn_hist_ = DeviceBuffer<u32>(radix_hist_elems(n));
+        }
+        if (knn_cs_.size() < cells + 1) knn_cs_ = DeviceBuffer<u32>(cells + 1);
+        k_knn_keys<<<gp, kMeshThreads, 0, s>>>(pts_[cur].data(), nn, kg, knn_keys_[0].data(),
+                                               knn_vals_[0].data());
+        WSIM_LAUNCH_CHECK();
+        const int w = radix_sort_pairs(knn_keys_[0].data(), knn_vals_[0].data(), knn_keys_[1].data(),
+                                       knn_vals_[1].data(), n, key_bits(static_cast<u32>(cells - 1)),
+                                       knn_hist_.data(), s);
+        build_cell_start(knn_keys_[w].data(), nn, static_cast<u32>(cells + 1), knn_cs_.data(), s);
+        const u32* sorted = knn_vals_[w].data();
+        double* out = pts_[cur ^ 1].data();
+        if (knn_ps_.size() < 3 * n) knn_ps_ = DeviceBuffer<double>(3 * n);
+        k_knn_gather<<<gp, kMeshThreads, 0, s>>>(pts_[cur].data(), nn, sorted, knn_ps_.data());
+        WSIM_LAUNCH_CHECK();
+        const double* in = pts_[cur].data();
+        const double* ps = knn_ps_.data();
+        const u32* cs = knn_cs_.data();
+        switch (P.k_neighbors) {
+        case 8: k_knn_smooth<8><<<gp, kMeshThreads, 0, s>>>(in, nn, kg, sorted, ps, cs, out); break;
+        case 10: k_knn_smooth<10><<<gp, kMeshThreads, 0, s>>>(in, nn, kg, sorted, ps, cs, out); break;
+        case 15: k_knn_smooth<15><<<gp, kMeshThreads, 0, s>>>(in, nn, kg, sorted, ps, cs, out); break;
+        case 20: k_knn_smooth<20><<<gp, kMeshThreads, 0, s>>>(in, nn, kg, sorted, ps, cs, out); break;
+        default: k_knn_smooth<30><<<gp, kMeshThreads, 0, s>>>(in, nn, kg, sorted, ps, cs, out); break;
+        }
+        WSIM_LAUNCH_CHECK();
+        cur ^= 1;
+    }
+    const double* pts = pts_[cur].data();
+    bounds(pts, mn, mx);
     // 2. grid (tools/particles_to_mesh.py:104-109), binary64 as numpy
     const double pad = P.particle_radius * 4.0;
     const double voxel = P.particle_radius * P.grid_voxel_factor;
@@ -444,10 +637,8 @@ void SurfaceMesher::run(const float* xyz, size_t n, const MeshParams& P) {
     int res[3];
     double lo[3], ext[3];
     for (int a = 0; a < 3; ++a) {
-        if (!std::isfinite(mn[a]) || !std::isfinite(mx[a]))
-            fail(kInvalidArgument, "SurfaceMesher: non-finite positions");
-        lo[a] = static_cast<double>(mn[a]) - pad;
-        const double hi = static_cast<double>(mx[a]) + pad;
+        lo[a] = mn[a] - pad;
+        const double hi = mx[a] + pad;
         ext[a] = hi - lo[a];
         const double r = std::ceil(ext[a] / voxel);
         if (!(r < 65536.0)) fail(kOutOfRange, "SurfaceMesher: grid too large");
@@ -482,8 +673,7 @@ void SurfaceMesher::run(const float* xyz, size_t n, const MeshParams& P) {
     // 3. splat
     u32* count = count_.data();
     WSIM_HIP(hipMemsetAsync(count, 0, nv * sizeof(u32), s));
-    k_mesh_splat<<<static_cast<u32>((n + kMeshThreads - 1) / kMeshThreads), kMeshThreads, 0, s>>>(
-        xyz, n, P.swap_yz ? 1 : 0, G, count);
+    k_mesh_splat<<<gp, kMeshThreads, 0, s>>>(pts, n, G, count);
     WSIM_LAUNCH_CHECK();
     k_mesh_to_float<<<gv, kMeshThreads, 0, s>>>(count, grid_.data(), nv);
     WSIM_LAUNCH_CHECK();

@@ -112,6 +112,8 @@ struct MeshParams {
     double iso_factor = 0.18;         // iso = factor * max density
     bool swap_yz = true;              // sim Y-up -> Blender Z-up (load_frame)
     const double* weights = nullptr;  // the 2R+1 blur weights, R = int(4 sigma + 0.5); NULL: exp()
+    int smooth_pre_iters = 2;         // k-NN position smoothing passes (--smooth-pre)
+    int k_neighbors = 15;             // --k-neighbors: 8, 10, 15, 20 or 30
 };
 struct MeshInfo {
     int dims[3];
@@ -135,6 +137,9 @@ private:
     DeviceBuffer<float> grid_, tmp_, verts_;
     DeviceBuffer<u32> tris_, count_, ecount_, tcount_, eoff_, toff_;
     DeviceBuffer<uint8_t> emask_;
+    DeviceBuffer<double> pts_[2];  // binary64 points, smoothed in place
+    DeviceBuffer<u32> knn_keys_[2], knn_vals_[2], knn_hist_, knn_cs_;
+    DeviceBuffer<double> knn_ps_;  // points in cell order
 };
 // The marching-cubes table: triangle counts [256] and edge ids [256][24].
 const uint8_t* mc_table_counts();

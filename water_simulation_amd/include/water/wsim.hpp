@@ -251,6 +251,8 @@ public:
         double iso_factor = 0.18;
         bool swap_yz = true;               // sim Y-up -> Blender Z-up (load_frame :46-58)
         const double* weights = nullptr;   // 2R + 1 blur weights, R = int(4 sigma + 0.5)
+        int smooth_pre_iters = 2;          // k-NN position smoothing passes
+        int k_neighbors = 15;              // 8, 10, 15, 20 or 30
     };
     explicit SurfaceMesher(void* stream = nullptr) { detail::check(wsim_mesher_create(stream, &h_)); }
     ~SurfaceMesher() { wsim_mesher_destroy(h_); }
@@ -266,6 +268,8 @@ public:
         c.iso_factor = p.iso_factor;
         c.swap_yz = p.swap_yz ? 1u : 0u;
         c.weights = p.weights;
+        c.smooth_pre_iters = p.smooth_pre_iters;
+        c.k_neighbors = p.k_neighbors;
         detail::check(wsim_mesher_run(h_, positions_device, count, &c));
         wsim_mesh_info info;
         detail::check(wsim_mesher_info(h_, &info));

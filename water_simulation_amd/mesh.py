@@ -3,9 +3,9 @@
 The splat method of tools/particles_to_mesh.py (reconstruct_splat :77-138,
 load_frame :46-58, the command line :194-264) on libwsim's SurfaceMesher:
 splat, Gaussian blur (the weights scipy.ndimage uses), iso level, marching
-cubes.  The reference's k-nearest-neighbour position pre-smoothing and
-Laplacian post-smoothing are not provided (--smooth-pre and --smooth must
-be 0), nor its Poisson method.
+cubes, after the k-nearest-neighbour position smoothing.  The reference's
+Laplacian mesh post-smoothing (--smooth must be 0) and its Poisson method
+are not provided.
 
     python -m water_simulation_amd.mesh --frame out/frame_0030.bin --out mesh.ply --radius 0.012
     python -m water_simulation_amd.mesh --seq out/dam_break --out-dir out/mesh --stride 2
@@ -60,7 +60,8 @@ class SurfaceMesher:
             pass
 
     def run(self, points, radius: float = 0.012, grid_voxel_factor: float = 0.5,
-            smooth_sigma: float = 2.5, iso_factor: float = 0.18, swap_yz: bool = True) -> dict:
+            smooth_sigma: float = 2.5, iso_factor: float = 0.18, swap_yz: bool = True,
+            smooth_pre: int = 2, k_neighbors: int = 15) -> dict:
         """points: (N, 3) binary32 array, a ParticleStore, or (device pointer, N)."""
         if isinstance(points, ParticleStore):
             ptr, n = points.positions(), points.count()
@@ -77,6 +78,8 @@ class SurfaceMesher:
         p.smooth_sigma = float(smooth_sigma)
         p.iso_factor = float(iso_factor)
         p.swap_yz = 1 if swap_yz else 0
+        p.smooth_pre_iters = int(smooth_pre)
+        p.k_neighbors = int(k_neighbors)
         w = gaussian_weights(smooth_sigma) if smooth_sigma > 0 else np.zeros(1)
         p.weights = w.ctypes.data
         check(_L().wsim_mesher_run(self._h, ptr, n, C.byref(p)))
@@ -147,22 +150,24 @@ def main(argv=None) -> int:
     p.add_argument("--grid-voxel-factor", type=float, default=0.5)
     p.add_argument("--smooth-sigma", type=float, default=2.5)
     p.add_argument("--iso-factor", type=float, default=0.18)
-    p.add_argument("--smooth-pre", type=int, default=0,
-                   help="k-NN position smoothing iterations (not provided: must be 0)")
-    p.add_argument("--k-neighbors", type=int, default=15)
+    p.add_argument("--smooth-pre", type=int, default=2,
+                   help="k-NN position smoothing iterations (Zhu-Bridson)")
+    p.add_argument("--k-neighbors", type=int, default=15,
+                   help="K-NN count for position smoothing (8, 10, 15, 20 or 30)")
     p.add_argument("--smooth", type=int, default=0,
                    help="Laplacian mesh smoothing iterations (not provided: must be 0)")
     p.add_argument("--stride", type=int, default=1)
     args = p.parse_args(argv)
-    if args.smooth_pre or args.smooth:
-        p.error("--smooth-pre and --smooth are not provided by this mesher; pass 0")
+    if args.smooth:
+        p.error("--smooth (Laplacian mesh smoothing) is not provided by this mesher; pass 0")
     mesher = SurfaceMesher()
 
     def one(frame, out):
         pts = read_frame(str(frame))
         t0 = time.time()
         info = mesher.run(pts, args.radius, args.grid_voxel_factor, args.smooth_sigma,
-                          args.iso_factor)
+                          args.iso_factor, smooth_pre=args.smooth_pre,
+                          k_neighbors=args.k_neighbors)
         v, t = mesher.vertices(), mesher.triangles()
         dt = time.time() - t0
         Path(out).parent.mkdir(parents=True, exist_ok=True)
