@@ -248,7 +248,8 @@ def test_gpu_mesh_of_a_solver_store_after_steps():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["one point", "identical points", "wide blur", "no surface"])
+@pytest.mark.parametrize("case", ["one point", "identical points", "wide blur", "widest blur",
+                                  "no surface"])
 def test_gpu_mesh_edge_cases(case):
     from water_simulation_amd import mesh as WM
 
@@ -262,6 +263,9 @@ def test_gpu_mesh_edge_cases(case):
     elif case == "wide blur":  # 41 taps on an 8-voxel minimum grid: reflections wrap
         x = np.random.default_rng(4).uniform(0.0, 0.01, (40, 3)).astype(np.float32)
         kw.update(smooth_sigma=5.0, grid_voxel_factor=4.0)
+    elif case == "widest blur":  # 137 taps: more than the tiled passes stage, one load per tap
+        x = np.random.default_rng(5).uniform(0.0, 0.01, (40, 3)).astype(np.float32)
+        kw.update(smooth_sigma=17.0, grid_voxel_factor=4.0)
     else:  # the iso level above the maximum: no crossing edge
         x = _cloud(5, n_side=6)
         kw.update(iso_factor=1.5)

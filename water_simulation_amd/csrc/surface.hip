@@ -182,7 +182,6 @@ __global__ __launch_bounds__(kMeshThreads) void k_mesh_splat(const double* __res
 // as numpy's mean over cKDTree.query's rows.  Exact kNN on a uniform grid:
 // rings of cells around the home cell are scanned until the k-th distance is
 // below the ring's reach (points outside rings 0..R lie at least R cells away).
-constexpr int kKnnMax = 32;
 
 struct KnnGrid {
     double lo0, lo1, lo2, cell;
@@ -342,6 +341,93 @@ __global__ __launch_bounds__(kMeshThreads) void k_mesh_blur(const float* __restr
         }
     }
     out[i] = static_cast<float>(acc);
+}
+
+// Tiled blur passes.  A strided pass (axis 0 or 1) stages, per workgroup,
+// 64 consecutive c values x (kBlurTile + 2R) positions along the axis in LDS,
+// with the 'reflect' ends resolved while loading, so every input is read from
+// HBM about (kBlurTile + 2R) / kBlurTile times instead of 2R + 1 times; each
+// of the 4 wave rows then forms kBlurTile / 4 outputs, taps outermost so each
+// weight is loaded once.  Sums and their order are k_mesh_blur's.
+constexpr int kBlurTile = 64;
+constexpr int kBlurMaxR = 64;  // LDS (kBlurTile + 2R) x 64 floats, 48 KiB at most
+
+__device__ __forceinline__ float blur_in(const float* p, u64 i) { return p[i]; }
+__device__ __forceinline__ float blur_in(const u32* p, u64 i) { return static_cast<float>(p[i]); }
+
+template <typename In>
+__global__ __launch_bounds__(256) void k_blur_strided(const In* __restrict__ in,
+                                                      float* __restrict__ out, int r0, int r1,
+                                                      int r2, int axis,
+                                                      const double* __restrict__ w, int R) {
+    extern __shared__ float sb[];  // [(kBlurTile + 2R) * 64]
+    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + lx;
+    const int p0 = blockIdx.y * kBlurTile;
+    const int len = axis == 0 ? r0 : r1;
+    const u64 step = axis == 0 ? static_cast<u64>(r1) * r2 : static_cast<u64>(r2);
+    // position 0 of this column: (0, b, c) for axis 0, (a, 0, c) for axis 1
+    const u64 base = axis == 0 ? static_cast<u64>(blockIdx.z) * r2 + c
+                               : static_cast<u64>(blockIdx.z) * r1 * r2 + c;
+    const bool cin = c < r2;
+    const int rows = kBlurTile + 2 * R;
+    for (int k = ly; k < rows; k += 4)
+        sb[k * 64 + lx] =
+            cin ? blur_in(in, base + static_cast<u64>(reflect_index(p0 - R + k, len)) * step)
+                : 0.0f;
+    __syncthreads();
+    constexpr int M = kBlurTile / 4;
+    double acc[M];
+    const double w0 = w[R];
+#pragma unroll
+    for (int m = 0; m < M; ++m) acc[m] = static_cast<double>(sb[(ly + 4 * m + R) * 64 + lx]) * w0;
+    for (int j = 1; j <= R; ++j) {
+        const double wj = w[R + j];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int k = ly + 4 * m + R;
+            acc[m] += (static_cast<double>(sb[(k - j) * 64 + lx]) +
+                       static_cast<double>(sb[(k + j) * 64 + lx])) * wj;
+        }
+    }
+    if (!cin) return;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int p = p0 + ly + 4 * m;
+        if (p < len) out[base + static_cast<u64>(p) * step] = static_cast<float>(acc[m]);
+    }
+}
+
+// The contiguous pass (axis 2): 256 consecutive c of one (a, b) row plus 2R
+// halo in LDS, one output per thread; each workgroup also writes the maximum
+// of its outputs to part[] (blurred densities are >= 0, so 0 is neutral).
+__global__ __launch_bounds__(256) void k_blur_rows(const float* __restrict__ in,
+                                                   float* __restrict__ out, int r1, int r2,
+                                                   const double* __restrict__ w, int R,
+                                                   float* __restrict__ part) {
+    extern __shared__ float sb[];  // [256 + 2R]
+    __shared__ float wmax[4];
+    const int c0 = blockIdx.x * 256;
+    const u64 row = (static_cast<u64>(blockIdx.z) * r1 + blockIdx.y) * r2;
+    for (int k = threadIdx.x; k < 256 + 2 * R; k += 256)
+        sb[k] = in[row + reflect_index(c0 - R + k, r2)];
+    __syncthreads();
+    const int k = threadIdx.x + R;
+    double acc = static_cast<double>(sb[k]) * w[R];
+    for (int j = 1; j <= R; ++j)
+        acc += (static_cast<double>(sb[k - j]) + static_cast<double>(sb[k + j])) * w[R + j];
+    const int c = c0 + threadIdx.x;
+    float v = static_cast<float>(acc);
+    if (c < r2)
+        out[row + c] = v;
+    else
+        v = 0.0f;
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        part[(static_cast<u64>(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] =
+            fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
 }
 
 __global__ __launch_bounds__(kMeshThreads) void k_mesh_max(const float* __restrict__ g, u64 nv,
@@ -675,37 +761,67 @@ void SurfaceMesher::run(const float* xyz, size_t n, const MeshParams& P) {
     WSIM_HIP(hipMemsetAsync(count, 0, nv * sizeof(u32), s));
     k_mesh_splat<<<gp, kMeshThreads, 0, s>>>(pts, n, G, count);
     WSIM_LAUNCH_CHECK();
-    k_mesh_to_float<<<gv, kMeshThreads, 0, s>>>(count, grid_.data(), nv);
-    WSIM_LAUNCH_CHECK();
-    // 4. blur
+    // 4. blur and 5. the maximum for the iso level
+    int R = 0;
+    std::vector<double> w;
+    DeviceBuffer<double> dw;
     if (P.smooth_sigma > 0.0) {
-        int R = 0;
-        std::vector<double> w = gaussian_weights(P.smooth_sigma, &R);
+        w = gaussian_weights(P.smooth_sigma, &R);
         if (P.weights) std::memcpy(w.data(), P.weights, w.size() * sizeof(double));
-        DeviceBuffer<double> dw(w.size());
+        dw = DeviceBuffer<double>(w.size());
         WSIM_HIP(hipMemcpyAsync(dw.data(), w.data(), w.size() * sizeof(double),
                                 hipMemcpyHostToDevice, s));
-        float* src = grid_.data();
-        float* dst = tmp_.data();
-        for (int axis = 0; axis < 3; ++axis) {
-            k_mesh_blur<<<gv, kMeshThreads, 0, s>>>(src, dst, res[0], res[1], res[2], axis,
-                                                    dw.data(), R);
-            WSIM_LAUNCH_CHECK();
-            std::swap(src, dst);
-        }
-        if (src != grid_.data()) std::swap(grid_, tmp_);
-        WSIM_HIP(hipStreamSynchronize(s));
     }
-    // 5. iso level
-    const u32 nbm = std::min<u32>(gv, 1024);
-    DeviceBuffer<float> mpart(nbm);
-    k_mesh_max<<<nbm, kMeshThreads, 0, s>>>(grid_.data(), nv, mpart.data());
-    WSIM_LAUNCH_CHECK();
-    std::vector<float> hm(nbm);
-    WSIM_HIP(hipMemcpyAsync(hm.data(), mpart.data(), nbm * sizeof(float), hipMemcpyDeviceToHost, s));
-    WSIM_HIP(hipStreamSynchronize(s));
-    float gmax = -INFINITY;
-    for (float v : hm) gmax = std::fmax(gmax, v);
+    auto max_of = [&](const float* a, u64 na) {
+        const u32 nbm = static_cast<u32>(std::min<u64>((na + kMeshThreads - 1) / kMeshThreads, 1024));
+        DeviceBuffer<float> mpart(nbm);
+        k_mesh_max<<<nbm, kMeshThreads, 0, s>>>(a, na, mpart.data());
+        WSIM_LAUNCH_CHECK();
+        std::vector<float> hm(nbm);
+        WSIM_HIP(hipMemcpyAsync(hm.data(), mpart.data(), nbm * sizeof(float),
+                                hipMemcpyDeviceToHost, s));
+        WSIM_HIP(hipStreamSynchronize(s));
+        float m = -INFINITY;
+        for (float v : hm) m = std::fmax(m, v);
+        return m;
+    };
+    float gmax;
+    if (P.smooth_sigma > 0.0 && R <= kBlurMaxR) {
+        // counts -> grid_ (axis 0) -> tmp_ (axis 1) -> grid_ (axis 2)
+        const size_t lds = static_cast<size_t>(kBlurTile + 2 * R) * 64 * sizeof(float);
+        const u32 tx = static_cast<u32>((res[2] + 63) / 64);
+        const dim3 g0(tx, static_cast<u32>((res[0] + kBlurTile - 1) / kBlurTile), static_cast<u32>(res[1]));
+        k_blur_strided<u32><<<g0, 256, lds, s>>>(count, grid_.data(), res[0], res[1], res[2], 0,
+                                                 dw.data(), R);
+        WSIM_LAUNCH_CHECK();
+        const dim3 g1(tx, static_cast<u32>((res[1] + kBlurTile - 1) / kBlurTile), static_cast<u32>(res[0]));
+        k_blur_strided<float><<<g1, 256, lds, s>>>(grid_.data(), tmp_.data(), res[0], res[1],
+                                                   res[2], 1, dw.data(), R);
+        WSIM_LAUNCH_CHECK();
+        const dim3 g2(static_cast<u32>((res[2] + 255) / 256), static_cast<u32>(res[1]),
+                      static_cast<u32>(res[0]));
+        const u64 nparts = static_cast<u64>(g2.x) * g2.y * g2.z;
+        fit(part_, nparts);
+        k_blur_rows<<<g2, 256, (256 + 2 * R) * sizeof(float), s>>>(
+            tmp_.data(), grid_.data(), res[1], res[2], dw.data(), R, part_.data());
+        WSIM_LAUNCH_CHECK();
+        gmax = max_of(part_.data(), nparts);
+    } else {
+        k_mesh_to_float<<<gv, kMeshThreads, 0, s>>>(count, grid_.data(), nv);
+        WSIM_LAUNCH_CHECK();
+        if (P.smooth_sigma > 0.0) {  // taps beyond the tiles' LDS: one load per tap
+            float* src = grid_.data();
+            float* dst = tmp_.data();
+            for (int axis = 0; axis < 3; ++axis) {
+                k_mesh_blur<<<gv, kMeshThreads, 0, s>>>(src, dst, res[0], res[1], res[2], axis,
+                                                        dw.data(), R);
+                WSIM_LAUNCH_CHECK();
+                std::swap(src, dst);
+            }
+            if (src != grid_.data()) std::swap(grid_, tmp_);
+        }
+        gmax = max_of(grid_.data(), nv);
+    }
     info_.max_density = gmax;
     // numpy 2: float32 scalar * Python float stays float32
     const float iso = gmax * static_cast<float>(P.iso_factor);

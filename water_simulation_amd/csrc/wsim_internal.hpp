@@ -134,7 +134,7 @@ public:
 private:
     hipStream_t stream_;
     MeshInfo info_{};
-    DeviceBuffer<float> grid_, tmp_, verts_;
+    DeviceBuffer<float> grid_, tmp_, verts_, part_;
     DeviceBuffer<u32> tris_, count_, ecount_, tcount_, eoff_, toff_;
     DeviceBuffer<uint8_t> emask_;
     DeviceBuffer<double> pts_[2];  // binary64 points, smoothed in place
