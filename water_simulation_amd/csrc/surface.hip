@@ -450,57 +450,124 @@ struct McDev {
     uint8_t tri[256][kMaxTri * 3];
 };
 
-// Per grid point: the crossing edges it owns (+axis from the point) and,
-// where a cube starts here, that cube's triangle count.
-__global__ __launch_bounds__(kMeshThreads) void k_mc_count(const float* __restrict__ g, int r0,
-                                                           int r1, int r2, float iso,
-                                                           const McDev* __restrict__ T,
-                                                           uint8_t* __restrict__ emask,
-                                                           u32* __restrict__ ecount,
-                                                           u32* __restrict__ tcount) {
-    const u64 nv = static_cast<u64>(r0) * r1 * r2;
-    const u64 i = static_cast<u64>(blockIdx.x) * kMeshThreads + threadIdx.x;
-    if (i >= nv) return;
-    const int c = static_cast<int>(i % r2);
-    const int b = static_cast<int>((i / r2) % r1);
-    const int a = static_cast<int>(i / (static_cast<u64>(r2) * r1));
-    const u64 st[3] = {static_cast<u64>(r1) * r2, static_cast<u64>(r2), 1};
-    const bool in0 = g[i] > iso;
-    u32 m = 0;
-    if (a + 1 < r0 && ((g[i + st[0]] > iso) != in0)) m |= 1u;
-    if (b + 1 < r1 && ((g[i + st[1]] > iso) != in0)) m |= 2u;
-    if (c + 1 < r2 && ((g[i + st[2]] > iso) != in0)) m |= 4u;
-    emask[i] = static_cast<uint8_t>(m);
-    ecount[i] = static_cast<u32>(__popc(m));
-    u32 nt = 0;
-    if (a + 1 < r0 && b + 1 < r1 && c + 1 < r2) {
-        int cfg = 0;
-        for (int k = 0; k < 8; ++k) {
-            const u64 j = i + (k & 1) * st[0] + ((k >> 1) & 1) * st[1] + ((k >> 2) & 1) * st[2];
-            cfg |= (g[j] > iso ? 1 : 0) << k;
-        }
-        nt = T->ntri[cfg];
-    }
-    tcount[i] = nt;
+// Marching cubes over wave segments.  A segment is 64 consecutive c of one
+// (a, b) row, handled by one wave; segments in linear order cover the grid
+// in grid-point order, so an exclusive scan of per-segment counts plus a
+// prefix across the wave numbers every vertex by (grid point, axis) and
+// every triangle by (cube, table order).  A grid point owns the crossing
+// edges from it along +a, +b, +c (edge mask bits 0, 1, 2); a cube starts at
+// its lowest corner.
+constexpr int kMcPlanes = 16;  // a planes walked by each k_mc_segments thread
+
+// exclusive prefix over the wave of small per-lane counts (< 2^bits)
+__device__ __forceinline__ u32 wave_prefix(u32 v, int bits) {
+    const unsigned long long below = (1ull << (threadIdx.x & 63u)) - 1ull;
+    u32 r = 0;
+    for (int k = 0; k < bits; ++k)
+        r += static_cast<u32>(__popcll(__ballot((v >> k) & 1u) & below)) << k;
+    return r;
 }
 
-__global__ __launch_bounds__(kMeshThreads) void k_mc_emit(
-    const float* __restrict__ g, MeshGeom G, float iso, const McDev* __restrict__ T,
-    const uint8_t* __restrict__ emask, const u32* __restrict__ eoff, const u32* __restrict__ toff,
-    const u32* __restrict__ tcount, float* __restrict__ verts, u32* __restrict__ tris) {
+// corner k of a cube sits at (a + (k & 1), b + (k >> 1 & 1), c + (k >> 2 & 1))
+__device__ __forceinline__ int mc_config(const float* __restrict__ g, u64 i, u64 sa, u64 sb,
+                                         float iso) {
+    int cfg = 0;
+    for (int k = 0; k < 8; ++k) {
+        const u64 j = i + (k & 1) * sa + ((k >> 1) & 1) * sb + ((k >> 2) & 1);
+        cfg |= (g[j] > iso ? 1 : 0) << k;
+    }
+    return cfg;
+}
+
+// Vertex and triangle counts per segment.  Each thread walks kMcPlanes
+// planes of a at one (b, c), keeping the inside bits of the (b, c), (b+1, c),
+// (b, c+1), (b+1, c+1) points of the current plane (bits 0..3), so it loads
+// four grid values per grid point.
+__global__ __launch_bounds__(256) void k_mc_segments(const float* __restrict__ g, int r0, int r1,
+                                                     int r2, float iso,
+                                                     const McDev* __restrict__ T, u32 ntx,
+                                                     u32* __restrict__ segE,
+                                                     u32* __restrict__ segT) {
+    const int lane = threadIdx.x & 63;
+    const int b = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (b >= r1) return;  // whole wave
+    const int c = blockIdx.x * 64 + lane;
+    const int a0 = blockIdx.z * kMcPlanes;
+    const int a1 = min(a0 + kMcPlanes, r0);
+    const bool cv = c < r2, bn = b + 1 < r1, cn = c + 1 < r2;
+    auto plane = [&](int a) -> u32 {
+        if (!cv) return 0u;
+        const u64 i = (static_cast<u64>(a) * r1 + b) * r2 + c;
+        u32 q = g[i] > iso ? 1u : 0u;
+        if (bn && g[i + r2] > iso) q |= 2u;
+        if (cn && g[i + 1] > iso) q |= 4u;
+        if (bn && cn && g[i + r2 + 1] > iso) q |= 8u;
+        return q;
+    };
+    u32 cur = plane(a0);
+    for (int a = a0; a < a1; ++a) {
+        const bool an = a + 1 < r0;
+        const u32 nxt = an ? plane(a + 1) : 0u;
+        u32 e = 0, t = 0;
+        if (cv) {
+            const u32 in0 = cur & 1u;
+            u32 m = 0;
+            if (an && (nxt & 1u) != in0) m |= 1u;
+            if (bn && ((cur >> 1) & 1u) != in0) m |= 2u;
+            if (cn && ((cur >> 2) & 1u) != in0) m |= 4u;
+            e = static_cast<u32>(__popc(m));
+            if (an && bn && cn) {
+                // plane bit j = db + 2 dc is corner 2j (this plane) and 2j + 1 (next)
+                u32 cfg = 0;
+                for (int j = 0; j < 4; ++j)
+                    cfg |= (((cur >> j) & 1u) << (2 * j)) | (((nxt >> j) & 1u) << (2 * j + 1));
+                t = T->ntri[cfg];
+            }
+        }
+        u32 p = e | (t << 16);  // at most 3 x 64 and 8 x 64 per wave
+        for (int o = 32; o > 0; o >>= 1) p += __shfl_xor(p, o, 64);
+        if (lane == 0) {
+            const u64 seg = (static_cast<u64>(a) * r1 + b) * ntx + blockIdx.x;
+            segE[seg] = p & 0xFFFFu;
+            segT[seg] = p >> 16;
+        }
+        cur = nxt;
+    }
+}
+
+// Vertices of the segments that have any; each grid point with crossing
+// edges also records its edge mask and first vertex for k_mc_triangles.
+__global__ __launch_bounds__(256) void k_mc_vertices(const float* __restrict__ g, MeshGeom G,
+                                                     float iso, u32 ntx,
+                                                     const u32* __restrict__ segE,
+                                                     const u32* __restrict__ segEoff,
+                                                     uint8_t* __restrict__ emask,
+                                                     u32* __restrict__ eoff,
+                                                     float* __restrict__ verts) {
     const int r0 = G.r0, r1 = G.r1, r2 = G.r2;
-    const u64 nv = static_cast<u64>(r0) * r1 * r2;
-    const u64 i = static_cast<u64>(blockIdx.x) * kMeshThreads + threadIdx.x;
-    if (i >= nv) return;
-    const int c = static_cast<int>(i % r2);
-    const int b = static_cast<int>((i / r2) % r1);
-    const int a = static_cast<int>(i / (static_cast<u64>(r2) * r1));
+    const int a = blockIdx.z, b = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (b >= r1) return;
+    const u64 seg = (static_cast<u64>(a) * r1 + b) * ntx + blockIdx.x;
+    if (segE[seg] == 0) return;  // whole wave
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
     const u64 st[3] = {static_cast<u64>(r1) * r2, static_cast<u64>(r2), 1};
+    const u64 i = (static_cast<u64>(a) * r1 + b) * r2 + c;
+    u32 m = 0;
+    float g0 = 0.0f;
+    if (c < r2) {
+        g0 = g[i];
+        const bool in0 = g0 > iso;
+        if (a + 1 < r0 && ((g[i + st[0]] > iso) != in0)) m |= 1u;
+        if (b + 1 < r1 && ((g[i + st[1]] > iso) != in0)) m |= 2u;
+        if (c + 1 < r2 && ((g[i + st[2]] > iso) != in0)) m |= 4u;
+    }
+    u32 v = segEoff[seg] + wave_prefix(static_cast<u32>(__popc(m)), 2);
+    if (!m) return;
+    emask[i] = static_cast<uint8_t>(m);
+    eoff[i] = v;
     const int idx[3] = {a, b, c};
     const double lo[3] = {G.lo0, G.lo1, G.lo2}, sp[3] = {G.sp0, G.sp1, G.sp2};
-    const u32 m = emask[i];
-    u32 v = eoff[i];
-    const double v0 = static_cast<double>(g[i]);
+    const double v0 = static_cast<double>(g0);
     for (int ax = 0; ax < 3; ++ax) {
         if (!((m >> ax) & 1u)) continue;
         const double v1 = static_cast<double>(g[i + st[ax]]);
@@ -511,14 +578,32 @@ __global__ __launch_bounds__(kMeshThreads) void k_mc_emit(
         }
         ++v;
     }
-    const u32 nt = tcount[i];
-    if (!nt) return;
+}
+
+// Triangles of the segments that have any: each vertex is the owner grid
+// point's first vertex plus the crossing edges it owns on lower axes.
+__global__ __launch_bounds__(256) void k_mc_triangles(const float* __restrict__ g, int r0, int r1,
+                                                      int r2, float iso,
+                                                      const McDev* __restrict__ T, u32 ntx,
+                                                      const u32* __restrict__ segT,
+                                                      const u32* __restrict__ segToff,
+                                                      const uint8_t* __restrict__ emask,
+                                                      const u32* __restrict__ eoff,
+                                                      u32* __restrict__ tris) {
+    const int a = blockIdx.z, b = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (b >= r1) return;
+    const u64 seg = (static_cast<u64>(a) * r1 + b) * ntx + blockIdx.x;
+    if (segT[seg] == 0) return;  // whole wave
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const u64 st[3] = {static_cast<u64>(r1) * r2, static_cast<u64>(r2), 1};
+    const u64 i = (static_cast<u64>(a) * r1 + b) * r2 + c;
+    u32 nt = 0;
     int cfg = 0;
-    for (int k = 0; k < 8; ++k) {
-        const u64 j = i + (k & 1) * st[0] + ((k >> 1) & 1) * st[1] + ((k >> 2) & 1) * st[2];
-        cfg |= (g[j] > iso ? 1 : 0) << k;
+    if (c + 1 < r2 && a + 1 < r0 && b + 1 < r1) {
+        cfg = mc_config(g, i, st[0], st[1], iso);
+        nt = T->ntri[cfg];
     }
-    const u32 t0 = toff[i];
+    const u32 t0 = segToff[seg] + wave_prefix(nt, 4);
     for (u32 t = 0; t < nt; ++t) {
         for (int q = 0; q < 3; ++q) {
             const int e = T->tri[cfg][3 * t + q];
@@ -751,10 +836,7 @@ void SurfaceMesher::run(const float* xyz, size_t n, const MeshParams& P) {
     fit(grid_, nv);
     fit(tmp_, nv);
     fit(count_, nv);
-    fit(ecount_, nv);
-    fit(tcount_, nv);
     fit(eoff_, nv);
-    fit(toff_, nv);
     fit(emask_, nv);
     // 3. splat
     u32* count = count_.data();
@@ -833,16 +915,28 @@ void SurfaceMesher::run(const float* xyz, size_t n, const MeshParams& P) {
     static_assert(sizeof(McDev) == sizeof(McTable), "table layout");
     DeviceBuffer<McDev> dT(1);
     WSIM_HIP(hipMemcpyAsync(dT.data(), &T, sizeof(McTable), hipMemcpyHostToDevice, s));
-    k_mc_count<<<gv, kMeshThreads, 0, s>>>(grid_.data(), res[0], res[1], res[2], iso, dT.data(),
-                                           emask_.data(), ecount_.data(), tcount_.data());
+    const u32 ntx = static_cast<u32>((res[2] + 63) / 64);
+    const u64 nseg = static_cast<u64>(res[0]) * res[1] * ntx;
+    fit(segE_, nseg);
+    fit(segT_, nseg);
+    fit(segEoff_, nseg);
+    fit(segToff_, nseg);
+    const u32 by = static_cast<u32>((res[1] + 3) / 4);
+    const dim3 gs(ntx, by, static_cast<u32>((res[0] + kMcPlanes - 1) / kMcPlanes));
+    k_mc_segments<<<gs, 256, 0, s>>>(grid_.data(), res[0], res[1], res[2], iso, dT.data(), ntx,
+                                     segE_.data(), segT_.data());
     WSIM_LAUNCH_CHECK();
-    const u64 nvert = scan_u32(ecount_.data(), eoff_.data(), nv, s);
-    const u64 ntri = scan_u32(tcount_.data(), toff_.data(), nv, s);
+    const u64 nvert = scan_u32(segE_.data(), segEoff_.data(), nseg, s);
+    const u64 ntri = scan_u32(segT_.data(), segToff_.data(), nseg, s);
     fit(verts_, 3 * std::max<u64>(nvert, 1));
     fit(tris_, 3 * std::max<u64>(ntri, 1));
-    k_mc_emit<<<gv, kMeshThreads, 0, s>>>(grid_.data(), G, iso, dT.data(), emask_.data(),
-                                          eoff_.data(), toff_.data(), tcount_.data(), verts_.data(),
-                                          tris_.data());
+    const dim3 ge(ntx, by, static_cast<u32>(res[0]));
+    k_mc_vertices<<<ge, 256, 0, s>>>(grid_.data(), G, iso, ntx, segE_.data(), segEoff_.data(),
+                                     emask_.data(), eoff_.data(), verts_.data());
+    WSIM_LAUNCH_CHECK();
+    k_mc_triangles<<<ge, 256, 0, s>>>(grid_.data(), res[0], res[1], res[2], iso, dT.data(), ntx,
+                                      segT_.data(), segToff_.data(), emask_.data(), eoff_.data(),
+                                      tris_.data());
     WSIM_LAUNCH_CHECK();
     WSIM_HIP(hipStreamSynchronize(s));
     info_.n_vertices = nvert;

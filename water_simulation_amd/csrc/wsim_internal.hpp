@@ -135,7 +135,8 @@ private:
     hipStream_t stream_;
     MeshInfo info_{};
     DeviceBuffer<float> grid_, tmp_, verts_, part_;
-    DeviceBuffer<u32> tris_, count_, ecount_, tcount_, eoff_, toff_;
+    DeviceBuffer<u32> tris_, count_, eoff_;
+    DeviceBuffer<u32> segE_, segT_, segEoff_, segToff_;  // per 64-point row segment
     DeviceBuffer<uint8_t> emask_;
     DeviceBuffer<double> pts_[2];  // binary64 points, smoothed in place
     DeviceBuffer<u32> knn_keys_[2], knn_vals_[2], knn_hist_, knn_cs_;
