@@ -240,37 +240,47 @@ __global__ __launch_bounds__(kMeshThreads) void k_knn_smooth(const double* __res
         bd[k] = INFINITY;
         bi[k] = 0xFFFFFFFFu;
     }
+    // candidates s in [lo, hi) of the cell order, the next one loaded while
+    // the current one is compared
+    auto scan = [&](u32 lo, u32 hi) {
+        if (lo >= hi) return;
+        double nx = Ps[3ull * lo], ny = Ps[3ull * lo + 1], nz = Ps[3ull * lo + 2];
+        u32 nj = sorted[lo];
+        for (u32 s = lo; s < hi; ++s) {
+            const double dx = nx - px, dy = ny - py, dz = nz - pz;
+            u32 id = nj;
+            if (s + 1 < hi) {
+                nx = Ps[3ull * s + 3], ny = Ps[3ull * s + 4], nz = Ps[3ull * s + 5];
+                nj = sorted[s + 1];
+            }
+            double d = (dx * dx + dy * dy) + dz * dz;
+            // most candidates do not beat the current k-th: one test
+            if (!(d < bd[K - 1] || (d == bd[K - 1] && id < bi[K - 1]))) continue;
+            // insert (d, id) into the ascending list (compile-time indices)
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const bool less = d < bd[k] || (d == bd[k] && id < bi[k]);
+                const double td = bd[k];
+                const u32 ti = bi[k];
+                bd[k] = less ? d : td;
+                bi[k] = less ? id : ti;
+                d = less ? td : d;
+                id = less ? ti : id;
+            }
+        }
+    };
     const int rmax = max(g.d0, max(g.d1, g.d2));
     for (int R = 0; R <= rmax; ++R) {
-        for (int a = h0 - R; a <= h0 + R; ++a) {
-            if (a < 0 || a >= g.d0) continue;
-            for (int b = h1 - R; b <= h1 + R; ++b) {
-                if (b < 0 || b >= g.d1) continue;
-                const bool face = a == h0 - R || a == h0 + R || b == h1 - R || b == h1 + R;
-                // interior of the shell in z: only its two ends unless on a face
-                for (int c = h2 - R; c <= h2 + R; c += (face || R == 0) ? 1 : 2 * R) {
-                    if (c < 0 || c >= g.d2) continue;
-                    const u32 key = (static_cast<u32>(a) * g.d1 + b) * g.d2 + c;
-                    for (u32 s = cs[key]; s < cs[key + 1]; ++s) {
-                        const u32 j = sorted[s];
-                        const double dx = Ps[3ull * s] - px, dy = Ps[3ull * s + 1] - py,
-                                     dz = Ps[3ull * s + 2] - pz;
-                        double d = (dx * dx + dy * dy) + dz * dz;
-                        u32 id = j;
-                        // most candidates do not beat the current k-th: one test
-                        if (!(d < bd[K - 1] || (d == bd[K - 1] && id < bi[K - 1]))) continue;
-                        // insert (d, id) into the ascending list (compile-time indices)
-#pragma unroll
-                        for (int k = 0; k < K; ++k) {
-                            const bool less = d < bd[k] || (d == bd[k] && id < bi[k]);
-                            const double td = bd[k];
-                            const u32 ti = bi[k];
-                            bd[k] = less ? d : td;
-                            bi[k] = less ? id : ti;
-                            d = less ? td : d;
-                            id = less ? ti : id;
-                        }
-                    }
+        const int c0 = max(h2 - R, 0), c1 = min(h2 + R, g.d2 - 1);
+        for (int a = max(h0 - R, 0); a <= min(h0 + R, g.d0 - 1); ++a) {
+            for (int b = max(h1 - R, 0); b <= min(h1 + R, g.d1 - 1); ++b) {
+                const u32 row = (static_cast<u32>(a) * g.d1 + b) * g.d2;
+                if (R == 0 || a == h0 - R || a == h0 + R || b == h1 - R || b == h1 + R) {
+                    // a face of the shell: the whole row, one contiguous range
+                    scan(cs[row + c0], cs[row + c1 + 1]);
+                } else {  // inside the shell's faces: its two ends in c
+                    if (h2 - R >= 0) scan(cs[row + h2 - R], cs[row + h2 - R + 1]);
+                    if (h2 + R < g.d2) scan(cs[row + h2 + R], cs[row + h2 + R + 1]);
                 }
             }
         }
