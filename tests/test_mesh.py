@@ -293,6 +293,22 @@ def test_gpu_knn_smoothing_matches_ckdtree(k, iters):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("k,iters", [(15, 2), (30, 1)])
+def test_gpu_knn_smoothing_isolated_points(k, iters):
+    """Points far from the rest (tens of 4r cells away) are searched by the
+    wave-per-point pass after the per-point ring walk gives up."""
+    from water_simulation_amd import mesh as WM
+
+    rng = np.random.default_rng(11)
+    far = (np.float32(0.1) + rng.uniform(0.5, 1.5, (12, 3))).astype(np.float32)
+    x = np.concatenate([_cloud(7, n_side=10), far]).astype(np.float32)
+    m = WM.SurfaceMesher()
+    m.run(x, radius=0.012, smooth_pre=iters, k_neighbors=k)
+    _compare(m, M.reconstruct_splat(x, radius=0.012, smooth_pre=iters, k=k),
+             f"isolated k={k} x{iters}")
+
+
+@pytest.mark.gpu
 def test_gpu_knn_smoothing_rejects_bad_k():
     from water_simulation_amd import mesh as WM
     from water_simulation_amd._ffi import InvalidArgument

@@ -218,13 +218,54 @@ __global__ __launch_bounds__(kMeshThreads) void k_knn_gather(const double* __res
     Ps[3ull * s + 2] = P[3ull * j + 2];
 }
 
+// Candidates s in [lo, hi) of the cell order into the ascending (d^2, id)
+// list of the K nearest so far, the next candidate loaded while the current
+// one is compared.
+template <int K>
+__device__ __forceinline__ void knn_scan(u32 lo, u32 hi, double px, double py, double pz,
+                                         const double* __restrict__ Ps,
+                                         const u32* __restrict__ sorted, double (&bd)[K],
+                                         u32 (&bi)[K]) {
+    if (lo >= hi) return;
+    double nx = Ps[3ull * lo], ny = Ps[3ull * lo + 1], nz = Ps[3ull * lo + 2];
+    u32 nj = sorted[lo];
+    for (u32 s = lo; s < hi; ++s) {
+        const double dx = nx - px, dy = ny - py, dz = nz - pz;
+        u32 id = nj;
+        if (s + 1 < hi) {
+            nx = Ps[3ull * s + 3], ny = Ps[3ull * s + 4], nz = Ps[3ull * s + 5];
+            nj = sorted[s + 1];
+        }
+        double d = (dx * dx + dy * dy) + dz * dz;
+        // most candidates do not beat the current k-th: one test
+        if (!(d < bd[K - 1] || (d == bd[K - 1] && id < bi[K - 1]))) continue;
+        // insert (d, id) into the ascending list (compile-time indices)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const bool less = d < bd[k] || (d == bd[k] && id < bi[k]);
+            const double td = bd[k];
+            const u32 ti = bi[k];
+            bd[k] = less ? d : td;
+            bi[k] = less ? id : ti;
+            d = less ? td : d;
+            id = less ? ti : id;
+        }
+    }
+}
+
+// Rings a lane walks on its own; points whose k-th neighbour lies further
+// out go to a list that k_knn_far searches a wave per point.
+constexpr int kKnnLaneRings = 3;
+
 template <int K>
 __global__ __launch_bounds__(kMeshThreads) void k_knn_smooth(const double* __restrict__ P, u32 n,
                                                              KnnGrid g,
                                                              const u32* __restrict__ sorted,
                                                              const double* __restrict__ Ps,
                                                              const u32* __restrict__ cs,
-                                                             double* __restrict__ out) {
+                                                             double* __restrict__ out,
+                                                             u32* __restrict__ far_count,
+                                                             u32* __restrict__ far) {
     // one thread per point in cell order: a wave's points share cells
     const u32 q = blockIdx.x * kMeshThreads + threadIdx.x;
     if (q >= n) return;
@@ -240,37 +281,11 @@ __global__ __launch_bounds__(kMeshThreads) void k_knn_smooth(const double* __res
         bd[k] = INFINITY;
         bi[k] = 0xFFFFFFFFu;
     }
-    // candidates s in [lo, hi) of the cell order, the next one loaded while
-    // the current one is compared
-    auto scan = [&](u32 lo, u32 hi) {
-        if (lo >= hi) return;
-        double nx = Ps[3ull * lo], ny = Ps[3ull * lo + 1], nz = Ps[3ull * lo + 2];
-        u32 nj = sorted[lo];
-        for (u32 s = lo; s < hi; ++s) {
-            const double dx = nx - px, dy = ny - py, dz = nz - pz;
-            u32 id = nj;
-            if (s + 1 < hi) {
-                nx = Ps[3ull * s + 3], ny = Ps[3ull * s + 4], nz = Ps[3ull * s + 5];
-                nj = sorted[s + 1];
-            }
-            double d = (dx * dx + dy * dy) + dz * dz;
-            // most candidates do not beat the current k-th: one test
-            if (!(d < bd[K - 1] || (d == bd[K - 1] && id < bi[K - 1]))) continue;
-            // insert (d, id) into the ascending list (compile-time indices)
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const bool less = d < bd[k] || (d == bd[k] && id < bi[k]);
-                const double td = bd[k];
-                const u32 ti = bi[k];
-                bd[k] = less ? d : td;
-                bi[k] = less ? id : ti;
-                d = less ? td : d;
-                id = less ? ti : id;
-            }
-        }
-    };
+    auto scan = [&](u32 lo, u32 hi) { knn_scan<K>(lo, hi, px, py, pz, Ps, sorted, bd, bi); };
     const int rmax = max(g.d0, max(g.d1, g.d2));
-    for (int R = 0; R <= rmax; ++R) {
+    const int rlim = min(rmax, kKnnLaneRings);
+    bool done = false;
+    for (int R = 0; R <= rlim; ++R) {
         const int c0 = max(h2 - R, 0), c1 = min(h2 + R, g.d2 - 1);
         for (int a = max(h0 - R, 0); a <= min(h0 + R, g.d0 - 1); ++a) {
             for (int b = max(h1 - R, 0); b <= min(h1 + R, g.d1 - 1); ++b) {
@@ -285,7 +300,14 @@ __global__ __launch_bounds__(kMeshThreads) void k_knn_smooth(const double* __res
             }
         }
         const double reach = static_cast<double>(R) * g.cell * (1.0 - 1e-9);
-        if (bi[K - 1] != 0xFFFFFFFFu && bd[K - 1] < reach * reach) break;
+        if (bi[K - 1] != 0xFFFFFFFFu && bd[K - 1] < reach * reach) {
+            done = true;
+            break;
+        }
+    }
+    if (!done && rlim < rmax) {  // rings beyond rlim hold cells: leave it to k_knn_far
+        far[atomicAdd(far_count, 1u)] = q;
+        return;
     }
     double sx = 0.0, sy = 0.0, sz = 0.0;
 #pragma unroll
@@ -298,6 +320,97 @@ __global__ __launch_bounds__(kMeshThreads) void k_knn_smooth(const double* __res
     out[3ull * i] = sx / static_cast<double>(K);
     out[3ull * i + 1] = sy / static_cast<double>(K);
     out[3ull * i + 2] = sz / static_cast<double>(K);
+}
+
+// The listed points, one wave each: the lanes take the (a, b) rows of every
+// shell in turn, each keeping the K nearest of its own rows; the shells stop
+// once K candidates of the union lie inside the reach (then the union's K-th
+// does), and K rounds of the wave minimum by (d^2, id) give the K nearest in
+// ascending order for the sum.
+template <int K>
+__global__ __launch_bounds__(kMeshThreads) void k_knn_far(const double* __restrict__ P,
+                                                          KnnGrid g,
+                                                          const u32* __restrict__ sorted,
+                                                          const double* __restrict__ Ps,
+                                                          const u32* __restrict__ cs,
+                                                          const u32* __restrict__ far_count,
+                                                          const u32* __restrict__ far,
+                                                          double* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const u32 wave = (blockIdx.x * kMeshThreads + threadIdx.x) >> 6;
+    const u32 nwaves = (gridDim.x * kMeshThreads) >> 6;
+    const u32 nfar = *far_count;
+    const int rmax = max(g.d0, max(g.d1, g.d2));
+    for (u32 f = wave; f < nfar; f += nwaves) {
+        const u32 q = far[f];
+        const double px = Ps[3ull * q], py = Ps[3ull * q + 1], pz = Ps[3ull * q + 2];
+        const int h0 = knn_cell(px, g.lo0, g.cell, g.d0);
+        const int h1 = knn_cell(py, g.lo1, g.cell, g.d1);
+        const int h2 = knn_cell(pz, g.lo2, g.cell, g.d2);
+        double bd[K];
+        u32 bi[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            bd[k] = INFINITY;
+            bi[k] = 0xFFFFFFFFu;
+        }
+        for (int R = 0; R <= rmax; ++R) {
+            const int side = 2 * R + 1;
+            const int c0 = max(h2 - R, 0), c1 = min(h2 + R, g.d2 - 1);
+            for (int t = lane; t < side * side; t += 64) {
+                const int a = h0 - R + t / side, b = h1 - R + t % side;
+                if (a < 0 || a >= g.d0 || b < 0 || b >= g.d1) continue;
+                const u32 row = (static_cast<u32>(a) * g.d1 + b) * g.d2;
+                if (R == 0 || a == h0 - R || a == h0 + R || b == h1 - R || b == h1 + R) {
+                    knn_scan<K>(cs[row + c0], cs[row + c1 + 1], px, py, pz, Ps, sorted, bd, bi);
+                } else {
+                    if (h2 - R >= 0)
+                        knn_scan<K>(cs[row + h2 - R], cs[row + h2 - R + 1], px, py, pz, Ps,
+                                    sorted, bd, bi);
+                    if (h2 + R < g.d2)
+                        knn_scan<K>(cs[row + h2 + R], cs[row + h2 + R + 1], px, py, pz, Ps,
+                                    sorted, bd, bi);
+                }
+            }
+            const double reach = static_cast<double>(R) * g.cell * (1.0 - 1e-9);
+            u32 inside = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) inside += bd[k] < reach * reach ? 1u : 0u;
+            for (int o = 32; o > 0; o >>= 1) inside += __shfl_xor(inside, o, 64);
+            if (inside >= static_cast<u32>(K)) break;  // the same on every lane
+        }
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int r = 0; r < K; ++r) {
+            double d = bd[0];
+            u32 id = bi[0];
+            for (int o = 32; o > 0; o >>= 1) {
+                const double od = __shfl_xor(d, o, 64);
+                const u32 oi = __shfl_xor(id, o, 64);
+                if (od < d || (od == d && oi < id)) {
+                    d = od;
+                    id = oi;
+                }
+            }
+            // every candidate sits in one lane's list: its owner pops it
+            const bool mine = bi[0] == id;
+#pragma unroll
+            for (int k = 0; k + 1 < K; ++k) {
+                bd[k] = mine ? bd[k + 1] : bd[k];
+                bi[k] = mine ? bi[k + 1] : bi[k];
+            }
+            bd[K - 1] = mine ? INFINITY : bd[K - 1];
+            bi[K - 1] = mine ? 0xFFFFFFFFu : bi[K - 1];
+            sx = sx + P[3ull * id];
+            sy = sy + P[3ull * id + 1];
+            sz = sz + P[3ull * id + 2];
+        }
+        if (lane == 0) {
+            const u32 i = sorted[q];
+            out[3ull * i] = sx / static_cast<double>(K);
+            out[3ull * i + 1] = sy / static_cast<double>(K);
+            out[3ull * i + 2] = sz / static_cast<double>(K);
+        }
+    }
 }
 
 __global__ __launch_bounds__(kMeshThreads) void k_mesh_to_float(const u32* __restrict__ count,
@@ -365,6 +478,11 @@ constexpr int kBlurMaxR = 64;  // LDS (kBlurTile + 2R) x 64 floats, 48 KiB at mo
 __device__ __forceinline__ float blur_in(const float* p, u64 i) { return p[i]; }
 __device__ __forceinline__ float blur_in(const u32* p, u64 i) { return static_cast<float>(p[i]); }
 
+// position k of an axis of length n, 'reflect' beyond the ends
+__device__ __forceinline__ int reflect_fast(int k, int n) {
+    return k >= 0 && k < n ? k : reflect_index(k, n);
+}
+
 template <typename In>
 __global__ __launch_bounds__(256) void k_blur_strided(const In* __restrict__ in,
                                                       float* __restrict__ out, int r0, int r1,
@@ -383,7 +501,7 @@ __global__ __launch_bounds__(256) void k_blur_strided(const In* __restrict__ in,
     const int rows = kBlurTile + 2 * R;
     for (int k = ly; k < rows; k += 4)
         sb[k * 64 + lx] =
-            cin ? blur_in(in, base + static_cast<u64>(reflect_index(p0 - R + k, len)) * step)
+            cin ? blur_in(in, base + static_cast<u64>(reflect_fast(p0 - R + k, len)) * step)
                 : 0.0f;
     __syncthreads();
     constexpr int M = kBlurTile / 4;
@@ -411,19 +529,24 @@ __global__ __launch_bounds__(256) void k_blur_strided(const In* __restrict__ in,
 // The contiguous pass (axis 2): 256 consecutive c of one (a, b) row plus 2R
 // halo in LDS, one output per thread; each workgroup also writes the maximum
 // of its outputs to part[] (blurred densities are >= 0, so 0 is neutral).
+// RT > 0 fixes the tap radius at compile time (RT == R); RT == 0 takes R.
+template <int RT>
 __global__ __launch_bounds__(256) void k_blur_rows(const float* __restrict__ in,
                                                    float* __restrict__ out, int r1, int r2,
-                                                   const double* __restrict__ w, int R,
+                                                   const double* __restrict__ w, int Rrt,
                                                    float* __restrict__ part) {
+    const int R = RT > 0 ? RT : Rrt;
+    constexpr int kTapUnroll = RT > 0 ? RT : 1;
     extern __shared__ float sb[];  // [256 + 2R]
     __shared__ float wmax[4];
     const int c0 = blockIdx.x * 256;
     const u64 row = (static_cast<u64>(blockIdx.z) * r1 + blockIdx.y) * r2;
     for (int k = threadIdx.x; k < 256 + 2 * R; k += 256)
-        sb[k] = in[row + reflect_index(c0 - R + k, r2)];
+        sb[k] = in[row + reflect_fast(c0 - R + k, r2)];
     __syncthreads();
     const int k = threadIdx.x + R;
     double acc = static_cast<double>(sb[k]) * w[R];
+#pragma unroll kTapUnroll
     for (int j = 1; j <= R; ++j)
         acc += (static_cast<double>(sb[k - j]) + static_cast<double>(sb[k + j])) * w[R + j];
     const int c = c0 + threadIdx.x;
@@ -467,7 +590,8 @@ struct McDev {
 // every triangle by (cube, table order).  A grid point owns the crossing
 // edges from it along +a, +b, +c (edge mask bits 0, 1, 2); a cube starts at
 // its lowest corner.
-constexpr int kMcPlanes = 16;  // a planes walked by each k_mc_segments thread
+constexpr int kMcPlanes = 16;     // a planes walked by each k_mc_segments thread
+constexpr int kMcEmitPlanes = 8;  // a planes (segments) per wave of the emit passes
 
 // exclusive prefix over the wave of small per-lane counts (< 2^bits)
 __device__ __forceinline__ u32 wave_prefix(u32 v, int bits) {
@@ -555,38 +679,40 @@ __global__ __launch_bounds__(256) void k_mc_vertices(const float* __restrict__ g
                                                      u32* __restrict__ eoff,
                                                      float* __restrict__ verts) {
     const int r0 = G.r0, r1 = G.r1, r2 = G.r2;
-    const int a = blockIdx.z, b = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int b = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (b >= r1) return;
-    const u64 seg = (static_cast<u64>(a) * r1 + b) * ntx + blockIdx.x;
-    if (segE[seg] == 0) return;  // whole wave
     const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    const u64 st[3] = {static_cast<u64>(r1) * r2, static_cast<u64>(r2), 1};
-    const u64 i = (static_cast<u64>(a) * r1 + b) * r2 + c;
-    u32 m = 0;
-    float g0 = 0.0f;
-    if (c < r2) {
-        g0 = g[i];
-        const bool in0 = g0 > iso;
-        if (a + 1 < r0 && ((g[i + st[0]] > iso) != in0)) m |= 1u;
-        if (b + 1 < r1 && ((g[i + st[1]] > iso) != in0)) m |= 2u;
-        if (c + 1 < r2 && ((g[i + st[2]] > iso) != in0)) m |= 4u;
-    }
-    u32 v = segEoff[seg] + wave_prefix(static_cast<u32>(__popc(m)), 2);
-    if (!m) return;
-    emask[i] = static_cast<uint8_t>(m);
-    eoff[i] = v;
-    const int idx[3] = {a, b, c};
-    const double lo[3] = {G.lo0, G.lo1, G.lo2}, sp[3] = {G.sp0, G.sp1, G.sp2};
-    const double v0 = static_cast<double>(g0);
-    for (int ax = 0; ax < 3; ++ax) {
-        if (!((m >> ax) & 1u)) continue;
-        const double v1 = static_cast<double>(g[i + st[ax]]);
-        const double t = (static_cast<double>(iso) - v0) / (v1 - v0);
-        for (int k = 0; k < 3; ++k) {
-            const double f = k == ax ? static_cast<double>(idx[k]) + t : static_cast<double>(idx[k]);
-            verts[3 * static_cast<u64>(v) + k] = static_cast<float>(f * sp[k] + lo[k]);
+    for (int a = blockIdx.z * kMcEmitPlanes; a < min((blockIdx.z + 1) * kMcEmitPlanes, r0); ++a) {
+        const u64 seg = (static_cast<u64>(a) * r1 + b) * ntx + blockIdx.x;
+        if (segE[seg] == 0) continue;  // whole wave
+        const u64 st[3] = {static_cast<u64>(r1) * r2, static_cast<u64>(r2), 1};
+        const u64 i = (static_cast<u64>(a) * r1 + b) * r2 + c;
+        u32 m = 0;
+        float g0 = 0.0f;
+        if (c < r2) {
+            g0 = g[i];
+            const bool in0 = g0 > iso;
+            if (a + 1 < r0 && ((g[i + st[0]] > iso) != in0)) m |= 1u;
+            if (b + 1 < r1 && ((g[i + st[1]] > iso) != in0)) m |= 2u;
+            if (c + 1 < r2 && ((g[i + st[2]] > iso) != in0)) m |= 4u;
         }
-        ++v;
+        u32 v = segEoff[seg] + wave_prefix(static_cast<u32>(__popc(m)), 2);
+        if (!m) continue;
+        emask[i] = static_cast<uint8_t>(m);
+        eoff[i] = v;
+        const int idx[3] = {a, b, c};
+        const double lo[3] = {G.lo0, G.lo1, G.lo2}, sp[3] = {G.sp0, G.sp1, G.sp2};
+        const double v0 = static_cast<double>(g0);
+        for (int ax = 0; ax < 3; ++ax) {
+            if (!((m >> ax) & 1u)) continue;
+            const double v1 = static_cast<double>(g[i + st[ax]]);
+            const double t = (static_cast<double>(iso) - v0) / (v1 - v0);
+            for (int k = 0; k < 3; ++k) {
+                const double f = k == ax ? static_cast<double>(idx[k]) + t : static_cast<double>(idx[k]);
+                verts[3 * static_cast<u64>(v) + k] = static_cast<float>(f * sp[k] + lo[k]);
+            }
+            ++v;
+        }
     }
 }
 
@@ -600,34 +726,36 @@ __global__ __launch_bounds__(256) void k_mc_triangles(const float* __restrict__ 
                                                       const uint8_t* __restrict__ emask,
                                                       const u32* __restrict__ eoff,
                                                       u32* __restrict__ tris) {
-    const int a = blockIdx.z, b = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int b = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (b >= r1) return;
-    const u64 seg = (static_cast<u64>(a) * r1 + b) * ntx + blockIdx.x;
-    if (segT[seg] == 0) return;  // whole wave
     const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    const u64 st[3] = {static_cast<u64>(r1) * r2, static_cast<u64>(r2), 1};
-    const u64 i = (static_cast<u64>(a) * r1 + b) * r2 + c;
-    u32 nt = 0;
-    int cfg = 0;
-    if (c + 1 < r2 && a + 1 < r0 && b + 1 < r1) {
-        cfg = mc_config(g, i, st[0], st[1], iso);
-        nt = T->ntri[cfg];
-    }
-    const u32 t0 = segToff[seg] + wave_prefix(nt, 4);
-    for (u32 t = 0; t < nt; ++t) {
-        for (int q = 0; q < 3; ++q) {
-            const int e = T->tri[cfg][3 * t + q];
-            const int ax = e >> 2, bb = e & 3;
-            // owner corner: bit ax clear, the other two bits from bb (lower axis first)
-            int off[3] = {0, 0, 0}, k = 0;
-            for (int x = 0; x < 3; ++x) {
-                if (x == ax) continue;
-                off[x] = (bb >> k) & 1;
-                ++k;
+    for (int a = blockIdx.z * kMcEmitPlanes; a < min((blockIdx.z + 1) * kMcEmitPlanes, r0); ++a) {
+        const u64 seg = (static_cast<u64>(a) * r1 + b) * ntx + blockIdx.x;
+        if (segT[seg] == 0) continue;  // whole wave
+        const u64 st[3] = {static_cast<u64>(r1) * r2, static_cast<u64>(r2), 1};
+        const u64 i = (static_cast<u64>(a) * r1 + b) * r2 + c;
+        u32 nt = 0;
+        int cfg = 0;
+        if (c + 1 < r2 && a + 1 < r0 && b + 1 < r1) {
+            cfg = mc_config(g, i, st[0], st[1], iso);
+            nt = T->ntri[cfg];
+        }
+        const u32 t0 = segToff[seg] + wave_prefix(nt, 4);
+        for (u32 t = 0; t < nt; ++t) {
+            for (int q = 0; q < 3; ++q) {
+                const int e = T->tri[cfg][3 * t + q];
+                const int ax = e >> 2, bb = e & 3;
+                // owner corner: bit ax clear, the other two bits from bb (lower axis first)
+                int off[3] = {0, 0, 0}, k = 0;
+                for (int x = 0; x < 3; ++x) {
+                    if (x == ax) continue;
+                    off[x] = (bb >> k) & 1;
+                    ++k;
+                }
+                const u64 o = i + off[0] * st[0] + off[1] * st[1] + off[2] * st[2];
+                const u32 rank = static_cast<u32>(__popc(emask[o] & ((1u << ax) - 1u)));
+                tris[3 * static_cast<u64>(t0 + t) + q] = eoff[o] + rank;
             }
-            const u64 o = i + off[0] * st[0] + off[1] * st[1] + off[2] * st[2];
-            const u32 rank = static_cast<u32>(__popc(emask[o] & ((1u << ax) - 1u)));
-            tris[3 * static_cast<u64>(t0 + t) + q] = eoff[o] + rank;
         }
     }
 }
@@ -799,12 +927,33 @@ void SurfaceMesher::run(const float* xyz, size_t n, const MeshParams& P) {
         const double* in = pts_[cur].data();
         const double* ps = knn_ps_.data();
         const u32* cs = knn_cs_.data();
+        if (knn_far_.size() < n) knn_far_ = DeviceBuffer<u32>(n);
+        if (knn_farc_.size() < 1) knn_farc_ = DeviceBuffer<u32>(1);
+        u32* fc = knn_farc_.data();
+        u32* fl = knn_far_.data();
+        WSIM_HIP(hipMemsetAsync(fc, 0, sizeof(u32), s));
+        const u32 gf = std::min<u32>(gp, 256);  // 4 waves per block walk the list
         switch (P.k_neighbors) {
-        case 8: k_knn_smooth<8><<<gp, kMeshThreads, 0, s>>>(in, nn, kg, sorted, ps, cs, out); break;
-        case 10: k_knn_smooth<10><<<gp, kMeshThreads, 0, s>>>(in, nn, kg, sorted, ps, cs, out); break;
-        case 15: k_knn_smooth<15><<<gp, kMeshThreads, 0, s>>>(in, nn, kg, sorted, ps, cs, out); break;
-        case 20: k_knn_smooth<20><<<gp, kMeshThreads, 0, s>>>(in, nn, kg, sorted, ps, cs, out); break;
-        default: k_knn_smooth<30><<<gp, kMeshThreads, 0, s>>>(in, nn, kg, sorted, ps, cs, out); break;
+        case 8:
+            k_knn_smooth<8><<<gp, kMeshThreads, 0, s>>>(in, nn, kg, sorted, ps, cs, out, fc, fl);
+            k_knn_far<8><<<gf, kMeshThreads, 0, s>>>(in, kg, sorted, ps, cs, fc, fl, out);
+            break;
+        case 10:
+            k_knn_smooth<10><<<gp, kMeshThreads, 0, s>>>(in, nn, kg, sorted, ps, cs, out, fc, fl);
+            k_knn_far<10><<<gf, kMeshThreads, 0, s>>>(in, kg, sorted, ps, cs, fc, fl, out);
+            break;
+        case 15:
+            k_knn_smooth<15><<<gp, kMeshThreads, 0, s>>>(in, nn, kg, sorted, ps, cs, out, fc, fl);
+            k_knn_far<15><<<gf, kMeshThreads, 0, s>>>(in, kg, sorted, ps, cs, fc, fl, out);
+            break;
+        case 20:
+            k_knn_smooth<20><<<gp, kMeshThreads, 0, s>>>(in, nn, kg, sorted, ps, cs, out, fc, fl);
+            k_knn_far<20><<<gf, kMeshThreads, 0, s>>>(in, kg, sorted, ps, cs, fc, fl, out);
+            break;
+        default:
+            k_knn_smooth<30><<<gp, kMeshThreads, 0, s>>>(in, nn, kg, sorted, ps, cs, out, fc, fl);
+            k_knn_far<30><<<gf, kMeshThreads, 0, s>>>(in, kg, sorted, ps, cs, fc, fl, out);
+            break;
         }
         WSIM_LAUNCH_CHECK();
         cur ^= 1;
@@ -894,8 +1043,13 @@ void SurfaceMesher::run(const float* xyz, size_t n, const MeshParams& P) {
                       static_cast<u32>(res[0]));
         const u64 nparts = static_cast<u64>(g2.x) * g2.y * g2.z;
         fit(part_, nparts);
-        k_blur_rows<<<g2, 256, (256 + 2 * R) * sizeof(float), s>>>(
-            tmp_.data(), grid_.data(), res[1], res[2], dw.data(), R, part_.data());
+        const size_t lr = (256 + 2 * R) * sizeof(float);
+        if (R == 10)  // sigma 2.5, the tool's default: unrolled taps
+            k_blur_rows<10><<<g2, 256, lr, s>>>(tmp_.data(), grid_.data(), res[1], res[2],
+                                                dw.data(), R, part_.data());
+        else
+            k_blur_rows<0><<<g2, 256, lr, s>>>(tmp_.data(), grid_.data(), res[1], res[2],
+                                               dw.data(), R, part_.data());
         WSIM_LAUNCH_CHECK();
         gmax = max_of(part_.data(), nparts);
     } else {
@@ -940,7 +1094,7 @@ void SurfaceMesher::run(const float* xyz, size_t n, const MeshParams& P) {
     const u64 ntri = scan_u32(segT_.data(), segToff_.data(), nseg, s);
     fit(verts_, 3 * std::max<u64>(nvert, 1));
     fit(tris_, 3 * std::max<u64>(ntri, 1));
-    const dim3 ge(ntx, by, static_cast<u32>(res[0]));
+    const dim3 ge(ntx, by, static_cast<u32>((res[0] + kMcEmitPlanes - 1) / kMcEmitPlanes));
     k_mc_vertices<<<ge, 256, 0, s>>>(grid_.data(), G, iso, ntx, segE_.data(), segEoff_.data(),
                                      emask_.data(), eoff_.data(), verts_.data());
     WSIM_LAUNCH_CHECK();

@@ -140,7 +140,8 @@ private:
     DeviceBuffer<uint8_t> emask_;
     DeviceBuffer<double> pts_[2];  // binary64 points, smoothed in place
     DeviceBuffer<u32> knn_keys_[2], knn_vals_[2], knn_hist_, knn_cs_;
-    DeviceBuffer<double> knn_ps_;  // points in cell order
+    DeviceBuffer<double> knn_ps_;       // points in cell order
+    DeviceBuffer<u32> knn_far_, knn_farc_;  // points left to the wave-per-point search
 };
 // The marching-cubes table: triangle counts [256] and edge ids [256][24].
 const uint8_t* mc_table_counts();
