@@ -320,7 +320,8 @@ def mesh_leg(x, r: float, reps: int, cpu: bool, what: str = "dam-break positions
     (tools/particles_to_mesh.py:61-138: two 15-nearest-neighbour smoothing
     passes, voxel r/2, blur sigma 2.5 voxels, iso 0.18 max; no Laplacian
     smoothing) on a dam-break frame.  The CPU baseline runs the reference's
-    own numpy splat and scipy blur on the 50^3 corner of the block."""
+    own numpy splat and scipy blur on the points within 50 particle spacings
+    of the frame's lower corner."""
     import water_simulation_amd as ws
     from water_simulation_amd import mesh as WM
     buf = ws.DeviceBuffer.from_array(x)
