@@ -317,9 +317,9 @@ def lbvh_leg(x, reps: int) -> dict:
 
 def mesh_leg(x, r: float, reps: int, cpu: bool, what: str = "dam-break positions") -> dict:
     """The splat surface mesher with the tool's defaults
-    (tools/particles_to_mesh.py:61-138: two 15-nearest-neighbour smoothing
-    passes, voxel r/2, blur sigma 2.5 voxels, iso 0.18 max; no Laplacian
-    smoothing) on a dam-break frame.  The CPU baseline runs the reference's
+    (tools/particles_to_mesh.py:61-139: two 15-nearest-neighbour smoothing
+    passes, voxel r/2, blur sigma 2.5 voxels, iso 0.18 max, two Laplacian
+    mesh smoothing passes) on a dam-break frame.  The CPU baseline runs the reference's
     own numpy splat and scipy blur on the points within 50 particle spacings
     of the frame's lower corner."""
     import water_simulation_amd as ws
@@ -334,7 +334,7 @@ def mesh_leg(x, r: float, reps: int, cpu: bool, what: str = "dam-break positions
     el = (time.perf_counter() - t0) / reps
     voxels = int(np.prod(info["dims"]))
     out = {"workload": f"surface mesh of the {len(x)} {what}, radius {r}, "
-                       "2 x 15-NN position smoothing",
+                       "2 x 15-NN position smoothing, 2 Laplacian passes",
            "points": len(x), "grid": list(info["dims"]), "voxels": voxels,
            "vertices": info["n_vertices"], "triangles": info["n_triangles"], "runs": reps,
            "ms_per_mesh": el * 1e3, "voxels_per_s": voxels / el, "cpu_baseline": None}

@@ -340,7 +340,11 @@ void wsim_frame_writer_destroy(wsim_frame_writer writer);
  * inside corners (the reference calls skimage's marching_cubes).  Before
  * the bounds, smooth_pre_iters passes replace every point by the mean of its
  * k nearest points (smooth_positions_knn :61-74, exact kNN, ties by index);
- * the reference's Laplacian mesh smoothing is not applied.  weights, when
+ * after marching cubes, smooth_post_iters passes of Laplacian smoothing
+ * (:138-139, Open3D's filter_smooth_laplacian with its default lambda 0.5):
+ * v + 0.5 (sum w_n v_n / sum w_n - v), w_n = 1 / (|v - v_n| + 1e-12) over
+ * the vertices sharing a triangle with v, binary64, all vertices from the
+ * previous pass.  weights, when
  * not NULL, are the 2R + 1 blur weights (R = (int)(4 sigma + 0.5)) to use
  * instead of exp(). */
 typedef struct wsim_mesher_t* wsim_mesher;
@@ -354,6 +358,8 @@ typedef struct {
     const double* weights;
     int32_t smooth_pre_iters; /* k-NN position smoothing passes (--smooth-pre, 2) */
     int32_t k_neighbors;      /* --k-neighbors: 8, 10, 15 (default), 20 or 30 */
+    int32_t smooth_post_iters; /* Laplacian mesh smoothing passes (--smooth, 2) */
+    int32_t reserved2;
 } wsim_mesh_params;
 typedef struct {
     int32_t dims[3];

@@ -12,7 +12,12 @@ reference pins no version) with the marching-cubes variant of the GPU path,
 restated here: one vertex per grid edge whose end values straddle the iso
 level, ordered by (grid point, axis); triangles per cube from a table that
 joins, on every face, the two crossing edges bounding each run of inside
-corners, ordered by (cube, table order).
+corners, ordered by (cube, table order).  The Laplacian post-smoothing
+(:138-139) restates Open3D's TriangleMesh.filter_smooth_laplacian (Open3D,
+absent here; the reference pins no version) as its documentation gives it,
+with lambda 0.5: v + 0.5 (sum w_n v_n / sum w_n - v),
+w_n = 1 / (|v - v_n| + 1e-12), over the vertices sharing a triangle with v,
+binary64, neighbours in ascending index (Open3D iterates a hash set).
 
 Pinned by tests/test_mesh.py: blur_restated equals scipy bit for bit;
 every mesh of a field that stays inside the grid is closed and consistently
@@ -20,9 +25,13 @@ oriented; a sphere's area and enclosed volume converge to 4 pi r^2 and
 4/3 pi r^3 with the outward orientation, Euler characteristic 2 (a torus 0);
 complementary corner configurations cut the same edges; cKDTree's
 neighbours equal a brute-force search ordered by (distance, index) on
-inputs without distance ties (the GPU's order).  Parity with skimage's
-Lewiner tables is unpinned (not available), and so is cKDTree's order
-among exactly equal distances.
+inputs without distance ties (the GPU's order); the Laplacian step keeps a
+plane's vertices on the plane, leaves a vertex whose neighbours sit
+symmetrically around it in place, moves a vertex halfway to the
+inverse-distance-weighted neighbour mean, and shrinks a sphere's mesh.
+Parity with skimage's Lewiner tables and with Open3D's smoothing is
+unpinned (neither is available), and so is cKDTree's order among exactly
+equal distances.
 """
 from __future__ import annotations
 
@@ -224,8 +233,48 @@ def marching_cubes(grid: np.ndarray, iso, lo, spacing):
     return verts, np.array(out, np.uint32).reshape(-1, 3)
 
 
+def vertex_neighbours(tris: np.ndarray, nv: int):
+    """Every vertex's neighbours (the vertices sharing a triangle with it),
+    ascending: (table (nv, D) int64 padded with 0, valid (nv, D) bool)."""
+    t = np.asarray(tris, np.int64).reshape(-1, 3)
+    pairs = np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]],
+                            t[:, [1, 0]], t[:, [2, 1]], t[:, [0, 2]]])
+    key = np.unique(pairs[:, 0] * nv + pairs[:, 1])
+    src, dst = key // nv, key % nv
+    deg = np.bincount(src, minlength=nv)
+    start = np.concatenate([[0], np.cumsum(deg)])
+    D = int(deg.max()) if len(deg) else 0
+    table = np.zeros((nv, D), np.int64)
+    valid = np.zeros((nv, D), bool)
+    col = np.arange(len(src)) - start[src]
+    table[src, col] = dst
+    valid[src, col] = True
+    return table, valid
+
+
+def laplacian_smooth(verts: np.ndarray, tris: np.ndarray, iters: int, lam: float = 0.5):
+    """tools/particles_to_mesh.py:138-139 (Open3D filter_smooth_laplacian,
+    restated): binary64 passes over binary32 vertices, binary32 out."""
+    V = np.asarray(verts, np.float32).astype(np.float64)
+    if iters <= 0 or len(V) == 0:
+        return np.asarray(verts, np.float32)
+    table, valid = vertex_neighbours(tris, len(V))
+    for _ in range(iters):
+        tw = np.zeros(len(V))
+        s = np.zeros_like(V)
+        for k in range(table.shape[1]):  # sequential sums; padding adds exact zeros
+            q = V[table[:, k]]
+            d = V - q
+            dist = np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2])
+            w = np.where(valid[:, k], 1.0 / (dist + 1e-12), 0.0)
+            tw = tw + w
+            s = s + w[:, None] * q
+        V = V + lam * (s / tw[:, None] - V)
+    return V.astype(np.float32)
+
+
 def reconstruct_splat(points, radius=0.012, voxel_factor=0.5, sigma=2.5, iso_factor=0.18,
-                      swap_yz=True, smooth_pre=2, k=15):
+                      swap_yz=True, smooth_pre=2, k=15, smooth_post=2):
     """The whole path on the CPU -> dict(lo, extent, res, grid, iso, vertices, triangles)."""
     pts = np.asarray(points, np.float32).astype(np.float64)
     if swap_yz:
@@ -243,5 +292,6 @@ def reconstruct_splat(points, radius=0.012, voxel_factor=0.5, sigma=2.5, iso_fac
         out.update(vertices=np.zeros((0, 3), np.float32), triangles=np.zeros((0, 3), np.uint32))
         return out
     v, t = marching_cubes(grid, iso, lo, extent / res)
+    v = laplacian_smooth(v, t, smooth_post)
     out.update(vertices=v, triangles=t)
     return out

@@ -98,6 +98,67 @@ def test_complementary_configurations_cut_the_same_edges():
         assert a == cut, cfg
 
 
+def _grid_mesh(n=6, z=None):
+    """A triangulated n x n vertex grid in the plane z = 0 (or height z(x, y))."""
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    v = np.stack([i.ravel(), j.ravel(), np.zeros(n * n)], 1).astype(np.float32)
+    if z is not None:
+        v[:, 2] = z(v[:, 0], v[:, 1])
+    t = []
+    for a in range(n - 1):
+        for b in range(n - 1):
+            p = a * n + b
+            t += [(p, p + n, p + n + 1), (p, p + n + 1, p + 1)]
+    return v, np.array(t, np.uint32)
+
+
+def test_laplacian_neighbours_come_from_triangles():
+    t = np.array([[0, 1, 2], [0, 2, 3], [0, 3, 1], [1, 3, 2]], np.uint32)  # a tetrahedron
+    table, valid = M.vertex_neighbours(t, 4)
+    for v in range(4):
+        assert table[v][valid[v]].tolist() == [u for u in range(4) if u != v]
+
+
+def test_laplacian_step_is_half_way_to_inverse_distance_mean():
+    """A fan around vertex 0 with neighbours at distances 1, 2, 4: weights
+    1, 1/2, 1/4 (inverse distance, not squared, not uniform), lambda 0.5."""
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 2, 0], [0, 0, 4]], np.float32)
+    t = np.array([[0, 1, 2], [0, 2, 3], [0, 3, 1]], np.uint32)
+    out = M.laplacian_smooth(v, t, 1)
+    mean = (1.0 * v[1] + 0.5 * v[2] + 0.25 * v[3]) / 1.75
+    assert np.allclose(out[0], 0.5 * mean, rtol=1e-6, atol=1e-9)
+    # every vertex reads the previous pass (not vertex 0's new position):
+    # vertex 1 sees 0, 2, 3 at distances 1, sqrt 5, sqrt 17
+    w = np.array([1.0, 1 / np.sqrt(5.0), 1 / np.sqrt(17.0)])
+    mean1 = (w[0] * v[0] + w[1] * v[2] + w[2] * v[3]) / w.sum()
+    assert np.allclose(out[1], v[1] + 0.5 * (mean1 - v[1]), rtol=1e-6, atol=1e-9)
+
+
+def test_laplacian_keeps_planes_and_symmetric_vertices():
+    v, t = _grid_mesh(7)
+    out = M.laplacian_smooth(v, t, 3)
+    assert np.all(out[:, 2] == 0)
+    # an interior vertex of the regular grid has its neighbours in symmetric pairs
+    assert np.allclose(out[3 * 7 + 3], v[3 * 7 + 3], atol=1e-6)
+    # a bump flattens
+    vb, tb = _grid_mesh(7, z=lambda x, y: np.where((x == 3) & (y == 3), 1.0, 0.0))
+    ob = M.laplacian_smooth(vb, tb, 1)
+    assert 0 < ob[3 * 7 + 3, 2] < 1
+
+
+def test_laplacian_shrinks_a_sphere_slightly():
+    n, R = 48, 15.0
+    X, Y, Z = _centered(n)
+    g = (R - np.sqrt(X ** 2 + Y ** 2 + Z ** 2)).astype(np.float32)
+    v, t = M.marching_cubes(g, np.float32(0.0), np.zeros(3), np.ones(3))
+    c = np.array([(n - 1) / 2] * 3)
+    r0 = np.linalg.norm(v - c, axis=1).mean()
+    s = M.laplacian_smooth(v, t, 2)
+    r2 = np.linalg.norm(s - c, axis=1).mean()
+    assert r2 < r0 and r0 - r2 < 0.02 * R
+    assert _mesh_checks(s, t)[0]
+
+
 def test_ckdtree_neighbours_are_brute_force_order():
     """On points without distance ties, cKDTree.query's rows are the k
     nearest in ascending distance, self first: the order the GPU sums in."""
@@ -207,9 +268,11 @@ def test_gpu_mesh_cli_writes_ply(tmp_path):
         nv = int(head.split(b"element vertex ")[1].split(b"\n")[0])
         nf = int(head.split(b"element face ")[1].split(b"\n")[0])
         assert len(body) == 12 * nv + 13 * nf and nf > 0
-    with pytest.raises(SystemExit):
-        WM.main(["--frame", str(tmp_path / "frame_0001.bin"), "--out", str(tmp_path / "a.ply"),
-                 "--smooth", "2"])
+    # --smooth 0: the Laplacian passes off; vertex and face counts do not change
+    assert WM.main(["--frame", str(tmp_path / "frame_0001.bin"), "--out", str(tmp_path / "a.ply"),
+                    "--smooth", "0"]) == 0
+    assert (tmp_path / "a.ply").read_bytes()[:200].split(b"end_header")[0] == \
+        (tmp_path / "mesh" / "frame_0001.ply").read_bytes()[:200].split(b"end_header")[0]
 
 
 @pytest.mark.gpu
@@ -306,6 +369,17 @@ def test_gpu_knn_smoothing_isolated_points(k, iters):
     m.run(x, radius=0.012, smooth_pre=iters, k_neighbors=k)
     _compare(m, M.reconstruct_splat(x, radius=0.012, smooth_pre=iters, k=k),
              f"isolated k={k} x{iters}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("post", [0, 1, 3])
+def test_gpu_laplacian_smoothing_matches_oracle(post):
+    from water_simulation_amd import mesh as WM
+
+    x = _cloud(8, n_side=14)
+    m = WM.SurfaceMesher()
+    m.run(x, radius=0.012, smooth_post=post)
+    _compare(m, M.reconstruct_splat(x, radius=0.012, smooth_post=post), f"post={post}")
 
 
 @pytest.mark.gpu

@@ -116,6 +116,8 @@ class MeshParamsC(C.Structure):
         ("weights", C.c_void_p),
         ("smooth_pre_iters", C.c_int32),
         ("k_neighbors", C.c_int32),
+        ("smooth_post_iters", C.c_int32),
+        ("reserved2", C.c_int32),
     ]
 
 

@@ -681,6 +681,8 @@ void wsim_mesh_params_default(wsim_mesh_params* out) {
     out->weights = nullptr;
     out->smooth_pre_iters = d.smooth_pre_iters;
     out->k_neighbors = d.k_neighbors;
+    out->smooth_post_iters = d.smooth_post_iters;
+    out->reserved2 = 0;
 }
 wsim_status wsim_mesher_create(void* stream, wsim_mesher* out) {
     return guarded([&] {
@@ -706,6 +708,7 @@ wsim_status wsim_mesher_run(wsim_mesher m, const float* xyz, size_t count,
         q.weights = p->weights;
         q.smooth_pre_iters = p->smooth_pre_iters;
         q.k_neighbors = p->k_neighbors;
+        q.smooth_post_iters = p->smooth_post_iters;
         m->impl->run(xyz, count, q);
     });
 }

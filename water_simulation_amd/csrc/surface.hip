@@ -15,8 +15,9 @@
 //      end values straddle the iso level (linear interpolation), triangles
 //      from a table built from the cube faces (mc_table).
 // Before the bounds, the reference's k-nearest-neighbour position smoothing
-// (smooth_positions_knn :61-74) runs smooth_pre_iters times; its Laplacian
-// mesh post-smoothing is not part of this path.
+// (smooth_positions_knn :61-74) runs smooth_pre_iters times; after marching
+// cubes, smooth_post_iters passes of the Laplacian mesh smoothing it asks of
+// Open3D (:138-139, filter_smooth_laplacian, lambda 0.5).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -677,7 +678,8 @@ __global__ __launch_bounds__(256) void k_mc_vertices(const float* __restrict__ g
                                                      const u32* __restrict__ segEoff,
                                                      uint8_t* __restrict__ emask,
                                                      u32* __restrict__ eoff,
-                                                     float* __restrict__ verts) {
+                                                     float* __restrict__ verts,
+                                                     u32* __restrict__ vpt) {
     const int r0 = G.r0, r1 = G.r1, r2 = G.r2;
     const int b = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (b >= r1) return;
@@ -711,6 +713,7 @@ __global__ __launch_bounds__(256) void k_mc_vertices(const float* __restrict__ g
                 const double f = k == ax ? static_cast<double>(idx[k]) + t : static_cast<double>(idx[k]);
                 verts[3 * static_cast<u64>(v) + k] = static_cast<float>(f * sp[k] + lo[k]);
             }
+            if (vpt) vpt[v] = static_cast<u32>(i) << 2 | static_cast<u32>(ax);
             ++v;
         }
     }
@@ -758,6 +761,140 @@ __global__ __launch_bounds__(256) void k_mc_triangles(const float* __restrict__ 
             }
         }
     }
+}
+
+// ---- Laplacian mesh smoothing ---------------------------------------------
+// Open3D's TriangleMesh::filter_smooth_laplacian (the reference's
+// mesh.filter_smooth_laplacian(number_of_iterations=n), default lambda 0.5),
+// as its documentation states it: every vertex moves to
+// v + lambda (sum_n w_n v_n / sum_n w_n - v), w_n = 1 / (|v - v_n| + 1e-12),
+// over the vertices that share a triangle with it, all from the previous
+// pass, in binary64.  The sums here run over the neighbours in ascending
+// index (Open3D's hash-set order is unspecified).
+// Neighbours per vertex: a cube's table has at most 5 triangles, so a vertex
+// sits on one fan of at most 7 vertices per cube and has at most 6
+// neighbours there, 24 over the four cubes around its edge.
+constexpr int kAdjCap = 24;
+
+// Neighbours of every vertex, from the (up to) four cubes around its grid
+// edge: the other vertices of the table triangles that contain the edge.
+__global__ __launch_bounds__(256) void k_mc_adjacency(const float* __restrict__ g, int r0, int r1,
+                                                      int r2, float iso,
+                                                      const McDev* __restrict__ T, u64 nvert,
+                                                      const u32* __restrict__ vpt,
+                                                      const uint8_t* __restrict__ emask,
+                                                      const u32* __restrict__ eoff,
+                                                      u32* __restrict__ adj,
+                                                      u32* __restrict__ deg,
+                                                      u32* __restrict__ over) {
+    __shared__ __align__(16) McDev sT;  // the table, read per lane below
+    for (u32 k = threadIdx.x; k < sizeof(McDev) / 4; k += 256)
+        reinterpret_cast<u32*>(&sT)[k] = reinterpret_cast<const u32*>(T)[k];
+    __syncthreads();
+    const u64 v = static_cast<u64>(blockIdx.x) * 256 + threadIdx.x;
+    if (v >= nvert) return;
+    // this vertex's grid edge: from grid point i (< 2^30) along axis ax
+    const u32 code = vpt[v];
+    const u32 i32 = code >> 2;
+    const int ax = static_cast<int>(code & 3u);
+    const u32 plane = static_cast<u32>(r1) * static_cast<u32>(r2);
+    const int idx[3] = {static_cast<int>(i32 / plane), static_cast<int>((i32 % plane) / r2),
+                        static_cast<int>(i32 % static_cast<u32>(r2))};
+    const u64 i = i32;
+    const int r[3] = {r0, r1, r2};
+    const u64 st[3] = {plane, static_cast<u64>(r2), 1};
+    // ascending list in registers, 0xFFFFFFFF past its end
+    u32 L[kAdjCap];
+#pragma unroll
+    for (int k = 0; k < kAdjCap; ++k) L[k] = 0xFFFFFFFFu;
+    u32 n = 0;
+    const int u = ax == 0 ? 1 : 0, w = ax == 2 ? 1 : 2;  // the other axes, lower first
+    for (int o = 0; o < 4; ++o) {
+        const int ou = o & 1, ow = o >> 1;
+        // the cube whose lowest corner is this point - ou e_u - ow e_w
+        if (idx[u] - ou < 0 || idx[w] - ow < 0 || idx[u] - ou + 1 >= r[u] ||
+            idx[w] - ow + 1 >= r[w])
+            continue;
+        const u64 qi = i - ou * st[u] - ow * st[w];
+        const int cfg = mc_config(g, qi, st[0], st[1], iso);
+        const int mine = 4 * ax + (ou | (ow << 1));  // this edge within that cube
+        for (u32 t = 0; t < sT.ntri[cfg]; ++t) {
+            const uint8_t* tri = sT.tri[cfg] + 3 * t;
+            if (tri[0] != mine && tri[1] != mine && tri[2] != mine) continue;
+            for (int q = 0; q < 3; ++q) {
+                const int e = tri[q];
+                if (e == mine) continue;
+                const int eax = e >> 2, bb = e & 3;
+                int off[3] = {0, 0, 0}, k = 0;
+                for (int x = 0; x < 3; ++x) {
+                    if (x == eax) continue;
+                    off[x] = (bb >> k) & 1;
+                    ++k;
+                }
+                const u64 oi = qi + off[0] * st[0] + off[1] * st[1] + off[2] * st[2];
+                u32 y = eoff[oi] + static_cast<u32>(__popc(emask[oi] & ((1u << eax) - 1u)));
+                bool seen = false;
+#pragma unroll
+                for (int k2 = 0; k2 < kAdjCap; ++k2) seen |= L[k2] == y;
+                if (seen) continue;
+                if (n == kAdjCap) {
+                    atomicOr(over, 1u);
+                    continue;
+                }
+#pragma unroll
+                for (int k2 = 0; k2 < kAdjCap; ++k2) {  // sorted insert
+                    const bool less = y < L[k2];
+                    const u32 tk = L[k2];
+                    L[k2] = less ? y : tk;
+                    y = less ? tk : y;
+                }
+                ++n;
+            }
+        }
+    }
+    u32* row = adj + v * kAdjCap;
+#pragma unroll
+    for (int k = 0; k < kAdjCap; ++k)
+        if (static_cast<u32>(k) < n) row[k] = L[k];
+    deg[v] = n;
+}
+
+__global__ __launch_bounds__(kMeshThreads) void k_verts_widen(const float* __restrict__ in,
+                                                              u64 n, double* __restrict__ out) {
+    const u64 i = static_cast<u64>(blockIdx.x) * kMeshThreads + threadIdx.x;
+    if (i < n) out[i] = static_cast<double>(in[i]);
+}
+
+__global__ __launch_bounds__(kMeshThreads) void k_verts_narrow(const double* __restrict__ in,
+                                                               u64 n, float* __restrict__ out) {
+    const u64 i = static_cast<u64>(blockIdx.x) * kMeshThreads + threadIdx.x;
+    if (i < n) out[i] = static_cast<float>(in[i]);
+}
+
+__global__ __launch_bounds__(kMeshThreads) void k_mesh_laplacian(const double* __restrict__ V,
+                                                                 u64 nv,
+                                                                 const u32* __restrict__ adj,
+                                                                 const u32* __restrict__ deg,
+                                                                 double* __restrict__ out) {
+    const u64 v = static_cast<u64>(blockIdx.x) * kMeshThreads + threadIdx.x;
+    if (v >= nv) return;
+    const double px = V[3 * v], py = V[3 * v + 1], pz = V[3 * v + 2];
+    double sx = 0.0, sy = 0.0, sz = 0.0, tw = 0.0;
+    const u32* L = adj + v * kAdjCap;
+    const u32 n = deg[v];
+    for (u32 k = 0; k < n; ++k) {
+        const u64 j = L[k];
+        const double qx = V[3 * j], qy = V[3 * j + 1], qz = V[3 * j + 2];
+        const double dx = px - qx, dy = py - qy, dz = pz - qz;
+        const double wgt = 1.0 / (sqrt(dx * dx + dy * dy + dz * dz) + 1e-12);
+        tw = tw + wgt;
+        sx = sx + wgt * qx;
+        sy = sy + wgt * qy;
+        sz = sz + wgt * qz;
+    }
+    out[3 * v] = px + 0.5 * (sx / tw - px);
+    out[3 * v + 1] = py + 0.5 * (sy / tw - py);
+    out[3 * v + 2] = pz + 0.5 * (sz / tw - pz);
 }
 
 // ---- exclusive scan of u32 counts (1024 per block) ----------------------------
@@ -849,7 +986,8 @@ void SurfaceMesher::run(const float* xyz, size_t n, const MeshParams& P) {
     if (n == 0) fail(kInvalidArgument, "SurfaceMesher: no points");
     if (!(P.particle_radius > 0.0) || !(P.grid_voxel_factor > 0.0) || !(P.smooth_sigma >= 0.0))
         fail(kInvalidArgument, "SurfaceMesher: radius, voxel factor and sigma must be positive");
-    if (P.smooth_pre_iters < 0) fail(kInvalidArgument, "SurfaceMesher: negative smoothing iterations");
+    if (P.smooth_pre_iters < 0 || P.smooth_post_iters < 0)
+        fail(kInvalidArgument, "SurfaceMesher: negative smoothing iterations");
     if (P.smooth_pre_iters > 0) {
         const int k = P.k_neighbors;
         if (k != 8 && k != 10 && k != 15 && k != 20 && k != 30)
@@ -1095,13 +1233,49 @@ void SurfaceMesher::run(const float* xyz, size_t n, const MeshParams& P) {
     fit(verts_, 3 * std::max<u64>(nvert, 1));
     fit(tris_, 3 * std::max<u64>(ntri, 1));
     const dim3 ge(ntx, by, static_cast<u32>((res[0] + kMcEmitPlanes - 1) / kMcEmitPlanes));
+    const bool post = P.smooth_post_iters > 0 && nvert > 0;
+    if (post) {
+        if (nv > (1ull << 30)) fail(kOutOfRange, "SurfaceMesher: grid too large for smoothing");
+        fit(vpt_, nvert);
+    }
     k_mc_vertices<<<ge, 256, 0, s>>>(grid_.data(), G, iso, ntx, segE_.data(), segEoff_.data(),
-                                     emask_.data(), eoff_.data(), verts_.data());
+                                     emask_.data(), eoff_.data(), verts_.data(),
+                                     post ? vpt_.data() : nullptr);
     WSIM_LAUNCH_CHECK();
     k_mc_triangles<<<ge, 256, 0, s>>>(grid_.data(), res[0], res[1], res[2], iso, dT.data(), ntx,
                                       segT_.data(), segToff_.data(), emask_.data(), eoff_.data(),
                                       tris_.data());
     WSIM_LAUNCH_CHECK();
+    // 7. Laplacian mesh smoothing (:138-139)
+    if (post) {
+        fit(adj_, nvert * kAdjCap);
+        fit(deg_, nvert);
+        fit(adj_over_, 1);
+        for (auto& b : vd_) fit(b, 3 * nvert);
+        WSIM_HIP(hipMemsetAsync(adj_over_.data(), 0, sizeof(u32), s));
+        k_mc_adjacency<<<static_cast<u32>((nvert + 255) / 256), 256, 0, s>>>(
+            grid_.data(), res[0], res[1], res[2], iso, dT.data(), nvert, vpt_.data(),
+            emask_.data(), eoff_.data(), adj_.data(), deg_.data(), adj_over_.data());
+        WSIM_LAUNCH_CHECK();
+        u32 over = 0;
+        WSIM_HIP(hipMemcpyAsync(&over, adj_over_.data(), sizeof(u32), hipMemcpyDeviceToHost, s));
+        WSIM_HIP(hipStreamSynchronize(s));
+        if (over) fail(kRuntimeError, "SurfaceMesher: a vertex with more than 24 neighbours");
+        const u64 n3 = 3 * nvert;
+        const u32 g3 = static_cast<u32>((n3 + kMeshThreads - 1) / kMeshThreads);
+        const u32 g1 = static_cast<u32>((nvert + kMeshThreads - 1) / kMeshThreads);
+        k_verts_widen<<<g3, kMeshThreads, 0, s>>>(verts_.data(), n3, vd_[0].data());
+        WSIM_LAUNCH_CHECK();
+        int cur = 0;
+        for (int it = 0; it < P.smooth_post_iters; ++it) {
+            k_mesh_laplacian<<<g1, kMeshThreads, 0, s>>>(vd_[cur].data(), nvert, adj_.data(),
+                                                         deg_.data(), vd_[cur ^ 1].data());
+            WSIM_LAUNCH_CHECK();
+            cur ^= 1;
+        }
+        k_verts_narrow<<<g3, kMeshThreads, 0, s>>>(vd_[cur].data(), n3, verts_.data());
+        WSIM_LAUNCH_CHECK();
+    }
     WSIM_HIP(hipStreamSynchronize(s));
     info_.n_vertices = nvert;
     info_.n_triangles = ntri;

@@ -114,6 +114,7 @@ struct MeshParams {
     const double* weights = nullptr;  // the 2R+1 blur weights, R = int(4 sigma + 0.5); NULL: exp()
     int smooth_pre_iters = 2;         // k-NN position smoothing passes (--smooth-pre)
     int k_neighbors = 15;             // --k-neighbors: 8, 10, 15, 20 or 30
+    int smooth_post_iters = 2;        // Laplacian mesh smoothing passes (--smooth)
 };
 struct MeshInfo {
     int dims[3];
@@ -142,6 +143,9 @@ private:
     DeviceBuffer<u32> knn_keys_[2], knn_vals_[2], knn_hist_, knn_cs_;
     DeviceBuffer<double> knn_ps_;       // points in cell order
     DeviceBuffer<u32> knn_far_, knn_farc_;  // points left to the wave-per-point search
+    DeviceBuffer<u32> adj_, deg_, adj_over_;  // vertex neighbours for the Laplacian passes
+    DeviceBuffer<u32> vpt_;                   // vertex -> grid point << 2 | axis
+    DeviceBuffer<double> vd_[2];
 };
 // The marching-cubes table: triangle counts [256] and edge ids [256][24].
 const uint8_t* mc_table_counts();

@@ -253,6 +253,7 @@ public:
         const double* weights = nullptr;   // 2R + 1 blur weights, R = int(4 sigma + 0.5)
         int smooth_pre_iters = 2;          // k-NN position smoothing passes
         int k_neighbors = 15;              // 8, 10, 15, 20 or 30
+        int smooth_post_iters = 2;         // Laplacian mesh smoothing passes
     };
     explicit SurfaceMesher(void* stream = nullptr) { detail::check(wsim_mesher_create(stream, &h_)); }
     ~SurfaceMesher() { wsim_mesher_destroy(h_); }
@@ -270,6 +271,7 @@ public:
         c.weights = p.weights;
         c.smooth_pre_iters = p.smooth_pre_iters;
         c.k_neighbors = p.k_neighbors;
+        c.smooth_post_iters = p.smooth_post_iters;
         detail::check(wsim_mesher_run(h_, positions_device, count, &c));
         wsim_mesh_info info;
         detail::check(wsim_mesher_info(h_, &info));

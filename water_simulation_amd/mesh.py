@@ -3,9 +3,9 @@
 The splat method of tools/particles_to_mesh.py (reconstruct_splat :77-138,
 load_frame :46-58, the command line :194-264) on libwsim's SurfaceMesher:
 splat, Gaussian blur (the weights scipy.ndimage uses), iso level, marching
-cubes, after the k-nearest-neighbour position smoothing.  The reference's
-Laplacian mesh post-smoothing (--smooth must be 0) and its Poisson method
-are not provided.
+cubes, after the k-nearest-neighbour position smoothing and followed by the
+Laplacian mesh smoothing (Open3D's filter_smooth_laplacian, :138-139).  The
+reference's Poisson method is not provided.
 
     python -m water_simulation_amd.mesh --frame out/frame_0030.bin --out mesh.ply --radius 0.012
     python -m water_simulation_amd.mesh --seq out/dam_break --out-dir out/mesh --stride 2
@@ -61,7 +61,7 @@ class SurfaceMesher:
 
     def run(self, points, radius: float = 0.012, grid_voxel_factor: float = 0.5,
             smooth_sigma: float = 2.5, iso_factor: float = 0.18, swap_yz: bool = True,
-            smooth_pre: int = 2, k_neighbors: int = 15) -> dict:
+            smooth_pre: int = 2, k_neighbors: int = 15, smooth_post: int = 2) -> dict:
         """points: (N, 3) binary32 array, a ParticleStore, or (device pointer, N)."""
         if isinstance(points, ParticleStore):
             ptr, n = points.positions(), points.count()
@@ -80,6 +80,7 @@ class SurfaceMesher:
         p.swap_yz = 1 if swap_yz else 0
         p.smooth_pre_iters = int(smooth_pre)
         p.k_neighbors = int(k_neighbors)
+        p.smooth_post_iters = int(smooth_post)
         w = gaussian_weights(smooth_sigma) if smooth_sigma > 0 else np.zeros(1)
         p.weights = w.ctypes.data
         check(_L().wsim_mesher_run(self._h, ptr, n, C.byref(p)))
@@ -154,12 +155,10 @@ def main(argv=None) -> int:
                    help="k-NN position smoothing iterations (Zhu-Bridson)")
     p.add_argument("--k-neighbors", type=int, default=15,
                    help="K-NN count for position smoothing (8, 10, 15, 20 or 30)")
-    p.add_argument("--smooth", type=int, default=0,
-                   help="Laplacian mesh smoothing iterations (not provided: must be 0)")
+    p.add_argument("--smooth", type=int, default=2,
+                   help="post-MC Laplacian mesh smoothing iterations (0 disables)")
     p.add_argument("--stride", type=int, default=1)
     args = p.parse_args(argv)
-    if args.smooth:
-        p.error("--smooth (Laplacian mesh smoothing) is not provided by this mesher; pass 0")
     mesher = SurfaceMesher()
 
     def one(frame, out):
@@ -167,7 +166,7 @@ def main(argv=None) -> int:
         t0 = time.time()
         info = mesher.run(pts, args.radius, args.grid_voxel_factor, args.smooth_sigma,
                           args.iso_factor, smooth_pre=args.smooth_pre,
-                          k_neighbors=args.k_neighbors)
+                          k_neighbors=args.k_neighbors, smooth_post=args.smooth)
         v, t = mesher.vertices(), mesher.triangles()
         dt = time.time() - t0
         Path(out).parent.mkdir(parents=True, exist_ok=True)
