@@ -383,6 +383,20 @@ def test_gpu_laplacian_smoothing_matches_oracle(post):
 
 
 @pytest.mark.gpu
+def test_gpu_knn_smoothing_points_on_a_line():
+    """A thin grid (one cell across) with uneven gaps: many points need far
+    more than three rings, searched row by row within the grid."""
+    from water_simulation_amd import mesh as WM
+
+    rng = np.random.default_rng(12)
+    xs = np.float32(0.1) + np.cumsum(rng.uniform(0.002, 0.2, 150)).astype(np.float32)
+    x = np.stack([xs, np.full(150, 0.3, np.float32), np.full(150, 0.2, np.float32)], 1)
+    m = WM.SurfaceMesher()
+    m.run(x, radius=0.012, smooth_pre=2, k_neighbors=8)
+    _compare(m, M.reconstruct_splat(x, radius=0.012, smooth_pre=2, k=8), "line")
+
+
+@pytest.mark.gpu
 def test_gpu_knn_smoothing_rejects_bad_k():
     from water_simulation_amd import mesh as WM
     from water_simulation_amd._ffi import InvalidArgument

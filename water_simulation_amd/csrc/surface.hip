@@ -356,11 +356,12 @@ __global__ __launch_bounds__(kMeshThreads) void k_knn_far(const double* __restri
             bi[k] = 0xFFFFFFFFu;
         }
         for (int R = 0; R <= rmax; ++R) {
-            const int side = 2 * R + 1;
+            // the shell's (a, b) rows inside the grid: na * nb <= d0 * d1 <= 2^26
+            const int a0 = max(h0 - R, 0), na = min(h0 + R, g.d0 - 1) - a0 + 1;
+            const int b0 = max(h1 - R, 0), nb = min(h1 + R, g.d1 - 1) - b0 + 1;
             const int c0 = max(h2 - R, 0), c1 = min(h2 + R, g.d2 - 1);
-            for (int t = lane; t < side * side; t += 64) {
-                const int a = h0 - R + t / side, b = h1 - R + t % side;
-                if (a < 0 || a >= g.d0 || b < 0 || b >= g.d1) continue;
+            for (int t = lane; t < na * nb; t += 64) {
+                const int a = a0 + t / nb, b = b0 + t % nb;
                 const u32 row = (static_cast<u32>(a) * g.d1 + b) * g.d2;
                 if (R == 0 || a == h0 - R || a == h0 + R || b == h1 - R || b == h1 + R) {
                     knn_scan<K>(cs[row + c0], cs[row + c1 + 1], px, py, pz, Ps, sorted, bd, bi);
