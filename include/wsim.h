@@ -331,7 +331,8 @@ wsim_status wsim_frame_writer_finish(wsim_frame_writer writer);
 void wsim_frame_writer_destroy(wsim_frame_writer writer);
 
 /* ---- fluid surface mesh: the splat method of tools/particles_to_mesh.py
- * (reconstruct_splat :77-138, load_frame :46-58) on the device ----
+ * (reconstruct_splat :77-139, smooth_positions_knn :61-74, load_frame :46-58)
+ * on the device ----
  * Bounds padded by 4 r, +1 per point in voxel ((p - lo) / extent * res)
  * with res = max(ceil(extent / (factor r)), 8), the Gaussian blur of
  * scipy.ndimage.gaussian_filter (truncate 4, 'reflect'), iso = max *
