@@ -276,6 +276,40 @@ def test_gpu_mesh_cli_writes_ply(tmp_path):
 
 
 @pytest.mark.gpu
+def test_gpu_mesh_frame_cpp_app_matches_python(tmp_path):
+    """bin/mesh_frame (the C++ water::SurfaceMesher over a ParticleStore,
+    default weights computed by the library in scipy's order) writes the
+    same PLY bytes as the Python command line; its flags reach the mesher."""
+    import subprocess
+    from pathlib import Path
+
+    import water_simulation_amd as ws
+    from water_simulation_amd import mesh as WM
+
+    app = Path(__file__).resolve().parents[1] / "water_simulation_amd" / "bin" / "mesh_frame"
+    assert app.exists(), "mesh_frame is not built"
+    x = _cloud(9, n_side=12)
+    f = tmp_path / "frame_0001.bin"
+    ws.write_frame(str(f), x)
+    r = subprocess.run([str(app), "--frame", str(f), "--out", str(tmp_path / "cpp.ply")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert WM.main(["--frame", str(f), "--out", str(tmp_path / "py.ply")]) == 0
+    assert (tmp_path / "cpp.ply").read_bytes() == (tmp_path / "py.ply").read_bytes()
+    r = subprocess.run([str(app), "--frame", str(f), "--out", str(tmp_path / "cpp2.ply"),
+                        "--smooth", "0", "--smooth-pre", "1", "--k-neighbors", "8", "--no-swap",
+                        "--iso-factor", "0.3"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    m = WM.SurfaceMesher()
+    m.run(x, smooth_post=0, smooth_pre=1, k_neighbors=8, swap_yz=False, iso_factor=0.3)
+    WM.write_ply(tmp_path / "py2.ply", m.vertices(), m.triangles())
+    assert (tmp_path / "cpp2.ply").read_bytes() == (tmp_path / "py2.ply").read_bytes()
+    r = subprocess.run([str(app), "--frame", str(f), "--out", str(tmp_path / "bad.ply"),
+                        "--k-neighbors", "7"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "k_neighbors" in r.stderr
+
+
+@pytest.mark.gpu
 def test_gpu_mesh_rejects_empty():
     from water_simulation_amd import mesh as WM
     from water_simulation_amd._ffi import InvalidArgument

@@ -3,6 +3,7 @@
 Products (all in-tree, git-ignored, shipped with the tree):
   water_simulation_amd/lib/libwsim.so   -- the product library (C ABI, include/wsim.h)
   water_simulation_amd/bin/sim_cli      -- the scene driver
+  water_simulation_amd/bin/mesh_frame   -- one frame to a PLY surface mesh
   oracle/liboracle.so                   -- CPU oracle (test infrastructure)
   oracle/_ref/libref_host.so            -- reference host sources (only where
                                            /root/reference exists)
@@ -86,6 +87,15 @@ def build_sim_cli() -> Path:
     return out
 
 
+def build_mesh_frame() -> Path:
+    BIN.mkdir(exist_ok=True)
+    out = BIN / "mesh_frame"
+    _run(["g++", "-std=c++17", "-O2", "-Wall", f"-I{ROOT / 'include'}", f"-I{PKG / 'include'}",
+          str(PKG / "apps" / "mesh_frame.cpp"), f"-L{LIB}", "-lwsim", "-Wl,-rpath,$ORIGIN/../lib",
+          "-o", str(out)])
+    return out
+
+
 def build_oracle() -> Path:
     out = ORACLE / "liboracle.so"
     tmp = ORACLE / "liboracle.so.tmp"
@@ -108,6 +118,7 @@ def build_all(jobs: int = 8) -> None:
     build_reference_host()
     build_library(jobs)
     build_sim_cli()
+    build_mesh_frame()
 
 
 if __name__ == "__main__":

@@ -962,16 +962,38 @@ static u64 scan_u32(const u32* in, u32* out, u64 n, hipStream_t s) {
     return run;
 }
 
+// numpy's float64 sum: sequential below 8 elements, eight running sums
+// combined pairwise up to 128, halves (a multiple of 8 first) beyond.
+static double numpy_sum(const double* a, size_t n) {
+    if (n < 8) {
+        double r = 0.0;
+        for (size_t i = 0; i < n; ++i) r = r + a[i];
+        return r;
+    }
+    if (n <= 128) {
+        double r[8];
+        for (int j = 0; j < 8; ++j) r[j] = a[j];
+        size_t i = 8;
+        for (; i < n - n % 8; i += 8)
+            for (int j = 0; j < 8; ++j) r[j] = r[j] + a[i + j];
+        double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+        for (; i < n; ++i) res = res + a[i];
+        return res;
+    }
+    size_t n2 = n / 2;
+    n2 -= n2 % 8;
+    return numpy_sum(a, n2) + numpy_sum(a + n2, n - n2);
+}
+
 std::vector<double> gaussian_weights(double sigma, int* radius) {
-    // scipy.ndimage._gaussian_kernel1d, order 0, truncate 4.0
+    // scipy.ndimage._gaussian_kernel1d, order 0, truncate 4.0: phi / phi.sum()
     const int R = static_cast<int>(4.0 * sigma + 0.5);
     std::vector<double> w(2 * R + 1);
     const double s2 = sigma * sigma;
-    double sum = 0.0;
     for (int k = -R; k <= R; ++k) {
         w[k + R] = std::exp(-0.5 / s2 * static_cast<double>(k * k));
     }
-    for (double x : w) sum += x;
+    const double sum = numpy_sum(w.data(), w.size());
     for (double& x : w) x = x / sum;
     *radius = R;
     return w;
