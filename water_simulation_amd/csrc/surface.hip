@@ -528,10 +528,13 @@ __global__ __launch_bounds__(256) void k_blur_strided(const In* __restrict__ in,
     }
 }
 
-// The contiguous pass (axis 2): 256 consecutive c of one (a, b) row plus 2R
-// halo in LDS, one output per thread; each workgroup also writes the maximum
-// of its outputs to part[] (blurred densities are >= 0, so 0 is neutral).
+// The contiguous pass (axis 2): a workgroup takes kBlurRows (a, b) rows in
+// turn, 256 consecutive c plus 2R halo in LDS at a time, one output per
+// thread, and writes the maximum of its outputs to part[] (blurred densities
+// are >= 0, so 0 is neutral).
 // RT > 0 fixes the tap radius at compile time (RT == R); RT == 0 takes R.
+constexpr int kBlurRows = 4;  // (a, b) rows per workgroup of the contiguous pass
+
 template <int RT>
 __global__ __launch_bounds__(256) void k_blur_rows(const float* __restrict__ in,
                                                    float* __restrict__ out, int r1, int r2,
@@ -541,27 +544,33 @@ __global__ __launch_bounds__(256) void k_blur_rows(const float* __restrict__ in,
     constexpr int kTapUnroll = RT > 0 ? RT : 1;
     extern __shared__ float sb[];  // [256 + 2R]
     __shared__ float wmax[4];
-    const int c0 = blockIdx.x * 256;
-    const u64 row = (static_cast<u64>(blockIdx.z) * r1 + blockIdx.y) * r2;
-    for (int k = threadIdx.x; k < 256 + 2 * R; k += 256)
-        sb[k] = in[row + reflect_fast(c0 - R + k, r2)];
-    __syncthreads();
-    const int k = threadIdx.x + R;
-    double acc = static_cast<double>(sb[k]) * w[R];
+    float vmax = 0.0f;
+    const int b1 = min((static_cast<int>(blockIdx.x) + 1) * kBlurRows, r1);
+    for (int b = blockIdx.x * kBlurRows; b < b1; ++b) {
+        const u64 row = (static_cast<u64>(blockIdx.y) * r1 + b) * r2;
+        for (int c0 = 0; c0 < r2; c0 += 256) {
+            __syncthreads();  // the previous segment's taps are read
+            for (int k = threadIdx.x; k < 256 + 2 * R; k += 256)
+                sb[k] = in[row + reflect_fast(c0 - R + k, r2)];
+            __syncthreads();
+            const int k = threadIdx.x + R;
+            double acc = static_cast<double>(sb[k]) * w[R];
 #pragma unroll kTapUnroll
-    for (int j = 1; j <= R; ++j)
-        acc += (static_cast<double>(sb[k - j]) + static_cast<double>(sb[k + j])) * w[R + j];
-    const int c = c0 + threadIdx.x;
-    float v = static_cast<float>(acc);
-    if (c < r2)
-        out[row + c] = v;
-    else
-        v = 0.0f;
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+            for (int j = 1; j <= R; ++j)
+                acc += (static_cast<double>(sb[k - j]) + static_cast<double>(sb[k + j])) * w[R + j];
+            const int c = c0 + threadIdx.x;
+            const float v = static_cast<float>(acc);
+            if (c < r2) {
+                out[row + c] = v;
+                vmax = fmaxf(vmax, v);
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = vmax;
     __syncthreads();
     if (threadIdx.x == 0)
-        part[(static_cast<u64>(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] =
+        part[static_cast<u64>(blockIdx.y) * gridDim.x + blockIdx.x] =
             fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
 }
 
@@ -1200,9 +1209,9 @@ void SurfaceMesher::run(const float* xyz, size_t n, const MeshParams& P) {
         k_blur_strided<float><<<g1, 256, lds, s>>>(grid_.data(), tmp_.data(), res[0], res[1],
                                                    res[2], 1, dw.data(), R);
         WSIM_LAUNCH_CHECK();
-        const dim3 g2(static_cast<u32>((res[2] + 255) / 256), static_cast<u32>(res[1]),
+        const dim3 g2(static_cast<u32>((res[1] + kBlurRows - 1) / kBlurRows),
                       static_cast<u32>(res[0]));
-        const u64 nparts = static_cast<u64>(g2.x) * g2.y * g2.z;
+        const u64 nparts = static_cast<u64>(g2.x) * g2.y;
         fit(part_, nparts);
         const size_t lr = (256 + 2 * R) * sizeof(float);
         if (R == 10)  // sigma 2.5, the tool's default: unrolled taps
