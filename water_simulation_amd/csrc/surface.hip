@@ -602,7 +602,7 @@ struct McDev {
 // edges from it along +a, +b, +c (edge mask bits 0, 1, 2); a cube starts at
 // its lowest corner.
 constexpr int kMcPlanes = 16;     // a planes walked by each k_mc_segments thread
-constexpr int kMcEmitPlanes = 8;  // a planes (segments) per wave of the emit passes
+constexpr int kMcEmitPlanes = 64;  // a planes (segments) per wave of the emit passes: one per lane
 
 // exclusive prefix over the wave of small per-lane counts (< 2^bits)
 __device__ __forceinline__ u32 wave_prefix(u32 v, int bits) {
@@ -693,10 +693,16 @@ __global__ __launch_bounds__(256) void k_mc_vertices(const float* __restrict__ g
     const int r0 = G.r0, r1 = G.r1, r2 = G.r2;
     const int b = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (b >= r1) return;
-    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    for (int a = blockIdx.z * kMcEmitPlanes; a < min((blockIdx.z + 1) * kMcEmitPlanes, r0); ++a) {
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * 64 + lane;
+    // the wave's segments with vertices: lane l reads plane a0 + l's count
+    const int a0 = blockIdx.z * kMcEmitPlanes;
+    const u32 cnt = a0 + lane < r0
+                        ? segE[(static_cast<u64>(a0 + lane) * r1 + b) * ntx + blockIdx.x]
+                        : 0u;
+    for (unsigned long long act = __ballot(cnt != 0); act; act &= act - 1) {
+        const int a = a0 + __ffsll(act) - 1;
         const u64 seg = (static_cast<u64>(a) * r1 + b) * ntx + blockIdx.x;
-        if (segE[seg] == 0) continue;  // whole wave
         const u64 st[3] = {static_cast<u64>(r1) * r2, static_cast<u64>(r2), 1};
         const u64 i = (static_cast<u64>(a) * r1 + b) * r2 + c;
         u32 m = 0;
@@ -741,10 +747,16 @@ __global__ __launch_bounds__(256) void k_mc_triangles(const float* __restrict__ 
                                                       u32* __restrict__ tris) {
     const int b = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (b >= r1) return;
-    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    for (int a = blockIdx.z * kMcEmitPlanes; a < min((blockIdx.z + 1) * kMcEmitPlanes, r0); ++a) {
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * 64 + lane;
+    // the wave's segments with triangles: lane l reads plane a0 + l's count
+    const int a0 = blockIdx.z * kMcEmitPlanes;
+    const u32 cnt = a0 + lane < r0
+                        ? segT[(static_cast<u64>(a0 + lane) * r1 + b) * ntx + blockIdx.x]
+                        : 0u;
+    for (unsigned long long act = __ballot(cnt != 0); act; act &= act - 1) {
+        const int a = a0 + __ffsll(act) - 1;
         const u64 seg = (static_cast<u64>(a) * r1 + b) * ntx + blockIdx.x;
-        if (segT[seg] == 0) continue;  // whole wave
         const u64 st[3] = {static_cast<u64>(r1) * r2, static_cast<u64>(r2), 1};
         const u64 i = (static_cast<u64>(a) * r1 + b) * r2 + c;
         u32 nt = 0;
