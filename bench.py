@@ -66,6 +66,27 @@ def make_workload(n_side: int):
     return x, xb, cfg
 
 
+def make_mid_workload():
+    """The mid dam break: 40 x 80 x 40 = 128,000 binary32 particles (r =
+    0.005 m) on a 0.01 m integer-index lattice, one spacing inside the walls,
+    forming a 0.4 x 0.8 x 0.4 m column in the corner of a 1.6 x 0.9 x 0.8 m
+    tank walled with Akinci particles at 0.01 m x 3 layers (the tank is 0.9 m
+    tall instead of 0.8 m: the sampler puts three inward layers on every face,
+    boundary_sampler.cpp:13-30, and an 0.8 m tank's ceiling layers would cut
+    the column's top).  XSPH 0.01, damping 0.5, eta 0.1 %."""
+    import water_simulation_amd as ws
+
+    r = 0.005
+    g = np.stack(np.meshgrid(np.arange(40, dtype=np.float32), np.arange(80, dtype=np.float32),
+                             np.arange(40, dtype=np.float32), indexing="ij"), -1).reshape(-1, 3)
+    x = (np.float32(0.03) + np.float32(0.01) * g).astype(np.float32)
+    xb = ws.sample_aabb_boundary((0, 0, 0), (1.6, 0.9, 0.8), 0.01, 3)
+    cfg = dict(rest_density=1000.0, particle_radius=r, smoothing_length=2 * r, viscosity=0.01,
+               damping=0.5, gravity=(0.0, -9.81, 0.0), max_density_iters=100,
+               eta_density=1e-3, domain_min=(0, 0, 0), domain_max=(1.6, 0.9, 0.8))
+    return x, xb, cfg
+
+
 def cpu_baseline(x, v, xb, cfg, dts, budget_s: float):
     """Binary32 oracle on the host cores from the given state (bounded sample)."""
     from oracle import oracle as O

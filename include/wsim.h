@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define WSIM_ABI_VERSION 1
+#define WSIM_ABI_VERSION 2
 
 typedef enum {
     WSIM_OK = 0,
@@ -171,6 +171,17 @@ typedef struct {
     float max_velocity;          /* max_velocity() (dfsph.h:39) */
     uint32_t nan_count;          /* particles with a non-finite position or velocity */
     uint64_t substeps;
+    /* Regression diagnostics after the substep (the checks of
+     * tests/regression/test_settled_puddle.cu:197-206), no reference
+     * counterpart in the stats API: */
+    uint32_t escapees;           /* particles whose position lies on or outside the
+                                    advection's clamp box [domain_min + 1e-3,
+                                    domain_max - 1e-3] (dfsph_kernels.cu:376-385):
+                                    they reached the domain limit */
+    uint32_t reserved;
+    double total_mass;           /* binary64: m x (particles with a finite state) */
+    double momentum[3];          /* binary64: sum of m v over those particles */
+    double kinetic_energy;       /* binary64: sum of m |v|^2 / 2 */
 } wsim_step_stats;
 
 typedef struct wsim_solver_t* wsim_solver;
@@ -192,6 +203,18 @@ wsim_status wsim_solver_density_at(wsim_solver solver, size_t i, float* out);
 wsim_status wsim_solver_alpha_at(wsim_solver solver, size_t i, float* out);
 wsim_status wsim_solver_boundary_mass_at(wsim_solver solver, size_t i, float* out);
 wsim_status wsim_solver_stats(wsim_solver solver, wsim_step_stats* out);
+/* Neighbour counts of ITER_NEIGHBORS (solvers/dfsph/src/dfsph_kernels.cu:13-35)
+ * as the density/alpha walk of the next substep sees them, on the fluid
+ * store's current positions, per particle in original order: cand = slots
+ * of the in-grid 27-cell stencil (self included), nbr = candidates with
+ * sqrtf(|r|^2) / smoothing_length < 2 (binary32), for the fluid store and
+ * for the boundary store (0 where the particle's stencil holds no boundary
+ * cell).  Device arrays of count() entries, each may be NULL.  Synchronous;
+ * the store and the results of the following steps are unchanged.  The
+ * reference exposes these only implicitly, through its neighbour loops. */
+wsim_status wsim_solver_neighbor_counts(wsim_solver solver, uint32_t* cand_fluid,
+                                        uint32_t* nbr_fluid, uint32_t* cand_boundary,
+                                        uint32_t* nbr_boundary);
 /* fluid_grid() / boundary_grid() (dfsph.h:41-42): borrowed, valid while the solver lives;
  * boundary grid is NULL without a boundary store.  Do not destroy. */
 wsim_grid wsim_solver_fluid_grid(wsim_solver solver);

@@ -53,6 +53,10 @@ class OrcStats(C.Structure):
         ("mean_divergence_error", C.c_double),
         ("max_velocity", C.c_float),
         ("nan_count", C.c_uint32),
+        ("escapees", C.c_uint32),
+        ("total_mass", C.c_double),
+        ("momentum", C.c_double * 3),
+        ("kinetic_energy", C.c_double),
     ]
 
 
@@ -326,7 +330,8 @@ class Solver:
     def stats(self) -> dict:
         s = OrcStats()
         lib().orc_solver_stats(self._h, C.byref(s))
-        return {f[0]: getattr(s, f[0]) for f in s._fields_}
+        return {f[0]: (list(getattr(s, f[0])) if f[0] == "momentum" else getattr(s, f[0]))
+                for f in s._fields_}
 
     # gradient checks
     def seed(self, dx=None, dv=None, d_viscosity: float = 0.0) -> None:

@@ -744,20 +744,49 @@ template <class R> struct Solver {
         if (tape_mode == 1) tape->clamps.push_back(clamp_buf);
     }
 
-    // dfsph_kernels.cu:396-400 + CUB max (dfsph.cu:303-326)
+    // dfsph_kernels.cu:396-400 + CUB max (dfsph.cu:303-326), plus the
+    // regression diagnostics: escapees (a position on or outside the
+    // advection's clamp box, dfsph_kernels.cu:376-385: the particle reached
+    // the domain limit) and binary64 mass, momentum and kinetic energy of the
+    // particles with a finite state, summed in index order.
     void finish_stats() {
         R m = 0;
-        u32 nan_count = 0;
+        u32 nan_count = 0, esc = 0;
+        double mass_sum = 0, px = 0, py = 0, pz = 0, ke = 0;
+        const R eps = R(1e-3f);
+        const R lo[3] = {R(cfg.domain_min[0]) + eps, R(cfg.domain_min[1]) + eps,
+                         R(cfg.domain_min[2]) + eps};
+        const R hi[3] = {R(cfg.domain_max[0]) - eps, R(cfg.domain_max[1]) - eps,
+                         R(cfg.domain_max[2]) - eps};
+        const double md = double(mass);
         for (size_t i = 0; i < n; ++i) {
             const V3<R>& vv = v[i];
             const R s = o_sqrt(vv.x * vv.x + vv.y * vv.y + vv.z * vv.z);
             m = o_fmax(m, s);
+            const V3<R>& p = x[i];
+            if (p.x <= lo[0] || p.y <= lo[1] || p.z <= lo[2] || p.x >= hi[0] || p.y >= hi[1] ||
+                p.z >= hi[2])
+                ++esc;
             if (!o_isfinite(x[i].x) || !o_isfinite(x[i].y) || !o_isfinite(x[i].z) ||
-                !o_isfinite(vv.x) || !o_isfinite(vv.y) || !o_isfinite(vv.z))
+                !o_isfinite(vv.x) || !o_isfinite(vv.y) || !o_isfinite(vv.z)) {
                 ++nan_count;
+                continue;
+            }
+            const double a = double(vv.x), b = double(vv.y), c = double(vv.z);
+            mass_sum += md;
+            px += md * a;
+            py += md * b;
+            pz += md * c;
+            ke += 0.5 * md * ((a * a + b * b) + c * c);
         }
         st.max_velocity = float(m);
         st.nan_count = nan_count;
+        st.escapees = esc;
+        st.total_mass = mass_sum;
+        st.momentum[0] = px;
+        st.momentum[1] = py;
+        st.momentum[2] = pz;
+        st.kinetic_energy = ke;
     }
 
     void density_alpha() {

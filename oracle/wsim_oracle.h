@@ -75,6 +75,11 @@ typedef struct {
     double   mean_divergence_error;
     float    max_velocity;
     uint32_t nan_count;
+    /* after the substep, in binary64 over the particles with a finite state */
+    uint32_t escapees;               /* position on or outside the advection clamp box */
+    double   total_mass;             /* m x (particles with a finite state) */
+    double   momentum[3];            /* sum m v */
+    double   kinetic_energy;         /* sum m |v|^2 / 2 */
 } orc_stats;
 
 void orc_set_threads(int n);

@@ -101,6 +101,27 @@ class Pair:
         if rs["density_iters"]:
             assert abs(gs["mean_density_error"] - rs["mean_density_error"]) <= \
                 1e-12 * max(1.0, abs(rs["mean_density_error"])), (tag, gs, rs)
+        assert_diagnostics(gs, rs, tag)
+
+
+def assert_diagnostics(gs, rs, tag=""):
+    """Regression diagnostics of a substep: the escapee count exactly, and
+    the binary64 mass, momentum and kinetic energy, whose sums run in a
+    different order on the device (tile partials) than in the oracle (index
+    order).  Reordering a binary64 sum of n terms moves it by at most about
+    n * 2^-53 of the sum of the terms' magnitudes: 1e-10 of that scale
+    covers 10^6 particles (the momentum's scale is mass * max |v|, since a
+    settled fluid's momentum is a cancellation)."""
+    rel = 1e-10
+    assert gs["escapees"] == rs["escapees"], (tag, gs, rs)
+    scale_m = max(abs(rs["total_mass"]), 1e-300)
+    assert abs(gs["total_mass"] - rs["total_mass"]) <= rel * scale_m, (tag, gs, rs)
+    vscale = max(abs(rs["max_velocity"]), 1e-6)
+    for a in range(3):
+        assert abs(gs["momentum"][a] - rs["momentum"][a]) <= rel * scale_m * vscale, \
+            (tag, a, gs, rs)
+    assert abs(gs["kinetic_energy"] - rs["kinetic_energy"]) <= rel * rs["kinetic_energy"] + 1e-300, \
+        (tag, gs, rs)
 
 
 def radix_tree_top_down(sorted_codes, points):

@@ -165,6 +165,8 @@ def test_torch_rollout_density_loss_matches_dual():
     assert res["loss_rel_err"] <= 1e-9
     for got, want in res["directional"]:
         assert abs(got - want) <= 1e-5 * abs(want), (got, want)
+    assert res["float_viscosity_backward"]
+    assert res["stale_tape_refused"]
 
 
 def test_viscosity_recovery():

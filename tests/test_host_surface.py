@@ -39,7 +39,7 @@ def test_library_exports_every_declared_symbol():
     assert set(syms) <= exported
     # every ctypes declaration names a declared symbol
     assert set(_ffi.SIGNATURES) <= set(syms)
-    assert _ffi.lib().wsim_abi_version() == 1
+    assert _ffi.lib().wsim_abi_version() == 2
 
 
 def test_last_error_and_status_codes():

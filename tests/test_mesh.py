@@ -354,6 +354,22 @@ def test_gpu_mesh_rejects_empty():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("bad", [np.nan, np.inf])
+@pytest.mark.parametrize("pre", [2, 0])
+def test_gpu_mesh_rejects_non_finite_points(bad, pre):
+    """One non-finite coordinate among finite points: the bounds pass counts
+    it (min/max skip NaN) and the host refuses the set before any k-NN search
+    or splat runs."""
+    from water_simulation_amd import mesh as WM
+    from water_simulation_amd._ffi import InvalidArgument
+
+    pts = _cloud(3, n_side=6)
+    pts[17, 1] = bad
+    with pytest.raises(InvalidArgument):
+        WM.SurfaceMesher().run(pts, smooth_pre=pre)
+
+
+@pytest.mark.gpu
 def test_gpu_mesh_of_a_solver_store_after_steps():
     """The consumer downstream of the solver: mesh the store's device
     positions after a few substeps, against the oracle on the same frame."""

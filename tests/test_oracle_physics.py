@@ -290,3 +290,33 @@ def test_binary32_error_against_binary64_per_field():
     for f in (O.FIELD_KAPPA, O.FIELD_V, O.FIELD_RHO_ADV):
         d = np.abs(a.get(f).astype(np.float64) - b.get(f)).max()
         assert d <= 1e-4 * np.abs(b.get(f)).max()
+
+
+def test_regression_diagnostics_closed_forms():
+    """Escapees, mass, momentum and kinetic energy after a substep: isolated
+    particles (no neighbours, no gravity, no damping) keep their velocities,
+    so the binary64 sums have closed forms; a particle that reaches the
+    domain clamp counts as an escapee (and its outward velocity is zeroed,
+    dfsph_kernels.cu:376-385); a NaN particle is neither summed nor an
+    escapee.  A wrong factor 1/2, a dropped axis or a sum over the NaN
+    particle fails it."""
+    c = O.config(rest_density=1000, particle_radius=0.01, smoothing_length=0.02, viscosity=0.0,
+                 damping=0.0, gravity=(0, 0, 0), max_density_iters=0)
+    x = np.array([[0.2, 0.2, 0.2], [0.5, 0.5, 0.5], [0.8, 0.2, 0.5], [0.0015, 0.5, 0.5],
+                  [0.3, 0.7, 0.3]], np.float32)
+    v = np.array([[1.0, 0.0, 0.0], [0.0, -2.0, 0.5], [0.25, 0.25, -1.0], [-3.0, 0.0, 0.0],
+                  [np.nan, 0.0, 0.0]], np.float32)
+    s = O.Solver(c, x, v, None, O.F64)
+    s.step(0.001)
+    st = s.stats()
+    m = 1000.0 * (2.0 * float(np.float32(0.01))) ** 3  # the binary64 instance's mass
+    assert m == pytest.approx(s.particle_mass(), rel=1e-7)
+    vs = v[:3].astype(np.float64)  # particle 3 is clamped at x = 0.001: v.x -> 0
+    assert st["escapees"] == 1
+    assert st["nan_count"] == 1
+    assert st["total_mass"] == pytest.approx(4 * m, rel=1e-15)
+    mom = m * vs.sum(0)
+    assert st["momentum"] == pytest.approx(list(mom), rel=1e-14)
+    assert st["kinetic_energy"] == pytest.approx(0.5 * m * (vs ** 2).sum(), rel=1e-14)
+    out = s.get(O.FIELD_X)
+    assert out[3, 0] == float(np.float32(1e-3))  # domain_min + 1e-3f

@@ -66,8 +66,25 @@ def main():
         want = float((gx_loss * d.tangent(O.FIELD_X)).sum())
         got = float((gv0 * uv).sum() + ge * 0.5)
         pairs.append((got, want))
+    # a Python float viscosity takes no gradient: backward returns None for it
+    v1 = torch.tensor(p.v_guess, device=dev, requires_grad=True)
+    xf, _ = Fn.apply(x0, v1, float(cfg["viscosity"]), ro, p.dts[:5])
+    xf.sum().backward()
+    float_visc_ok = bool(v1.grad is not None and torch.isfinite(v1.grad).all())
+    # a second forward pass on the same Rollout before backward: the first
+    # graph's backward must refuse the replaced checkpoints
+    v2 = torch.tensor(p.v_guess, device=dev, requires_grad=True)
+    xa, _ = Fn.apply(x0, v2, eps, ro, p.dts[:5])
+    Fn.apply(x0, v2, eps, ro, p.dts[:5])
+    try:
+        xa.sum().backward()
+        stale_refused = False
+    except RuntimeError as e:
+        stale_refused = "another forward pass" in str(e)
     print(json.dumps({"same_trajectory": same, "loss": lnp,
-                      "loss_rel_err": abs(lnp - float(loss)) / lnp, "directional": pairs}))
+                      "loss_rel_err": abs(lnp - float(loss)) / lnp, "directional": pairs,
+                      "float_viscosity_backward": float_visc_ok,
+                      "stale_tape_refused": stale_refused}))
 
 
 if __name__ == "__main__":

@@ -439,8 +439,7 @@ class DFSPHSolver:
         dts = dts[: n.value].copy()
         if not with_stats:
             return dts
-        return dts, [{f[0]: getattr(recs[k], f[0]) for f in recs[k]._fields_}
-                     for k in range(n.value)]
+        return dts, [recs[k].as_dict() for k in range(n.value)]
 
     def max_velocity(self) -> float:
         return _L().wsim_solver_max_velocity(self._h)
@@ -468,7 +467,23 @@ class DFSPHSolver:
     def stats(self) -> dict:
         s = _ffi.StepStatsC()
         check(_L().wsim_solver_stats(self._h, C.byref(s)))
-        return {f[0]: getattr(s, f[0]) for f in s._fields_}
+        return s.as_dict()
+
+    def neighbor_counts(self) -> dict:
+        """Per-particle neighbour counts of the density/alpha walk on the
+        store's current positions (original order): stencil candidates and
+        in-support neighbours in the fluid and the boundary store
+        (ITER_NEIGHBORS, dfsph_kernels.cu:13-35)."""
+        n = self._fluid.count()
+        bufs = [DeviceBuffer(max(4 * n, 4)) for _ in range(4)]
+        check(_L().wsim_solver_neighbor_counts(self._h, *(b.ptr for b in bufs)))
+        out = {}
+        for k, b in zip(("cand_fluid", "nbr_fluid", "cand_boundary", "nbr_boundary"), bufs):
+            a = np.empty(n, np.uint32)
+            if n:
+                memcpy_d2h(a, b.ptr)
+            out[k] = a
+        return out
 
     def fluid_grid(self) -> CellGrid:
         return CellGrid(None, None, None, _view=_L().wsim_solver_fluid_grid(self._h), _owner=self)
@@ -570,7 +585,7 @@ class BatchSolver(DFSPHSolver):
     def scene_stats(self, s: int) -> dict:
         st = _ffi.StepStatsC()
         check(_L().wsim_batch_scene_stats(self._h, int(s), C.byref(st)))
-        return {f[0]: getattr(st, f[0]) for f in st._fields_}
+        return st.as_dict()
 
     def all_stats(self) -> list:
         return [self.scene_stats(s) for s in range(self.num_scenes)]

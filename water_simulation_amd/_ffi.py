@@ -50,7 +50,21 @@ class StepStatsC(C.Structure):
         ("max_velocity", C.c_float),
         ("nan_count", C.c_uint32),
         ("substeps", C.c_uint64),
+        ("escapees", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("total_mass", C.c_double),
+        ("momentum", C.c_double * 3),
+        ("kinetic_energy", C.c_double),
     ]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for name, _ in self._fields_:
+            if name == "reserved":
+                continue
+            v = getattr(self, name)
+            out[name] = list(v) if name == "momentum" else v
+        return out
 
 
 class KernelShapesC(C.Structure):
@@ -207,6 +221,7 @@ SIGNATURES = {
     "wsim_solver_alpha_at": (st, [vp, sz, C.POINTER(C.c_float)]),
     "wsim_solver_boundary_mass_at": (st, [vp, sz, C.POINTER(C.c_float)]),
     "wsim_solver_stats": (st, [vp, C.POINTER(StepStatsC)]),
+    "wsim_solver_neighbor_counts": (st, [vp, vp, vp, vp, vp]),
     "wsim_solver_fluid_grid": (vp, [vp]),
     "wsim_solver_boundary_grid": (vp, [vp]),
     "wsim_solver_set_timing": (st, [vp, C.c_int]),

@@ -70,6 +70,8 @@ class Rollout:
         d = np.ascontiguousarray(dts, np.float32)
         check(_L().wsim_rollout_forward(self._h, x0, v0, d.ctypes.data_as(C.POINTER(C.c_float)),
                                         int(d.size)))
+        # the checkpoints of this forward pass replace the previous ones
+        self.generation = getattr(self, "generation", 0) + 1
 
     def backward_ptr(self, gx: int, gv: int, gx0: int, gv0: int) -> float:
         g = C.c_double()
@@ -146,12 +148,19 @@ def rollout_function():
             _device_copy(xT.data_ptr(), store.positions(), 12 * n)
             _device_copy(vT.data_ptr(), store.velocities(), 12 * n)
             ctx.rollout = rollout
-            ctx.vdtype = viscosity.dtype if torch.is_tensor(viscosity) else torch.float64
+            ctx.generation = rollout.generation
+            ctx.visc_is_tensor = torch.is_tensor(viscosity)
+            ctx.vdtype = viscosity.dtype if ctx.visc_is_tensor else torch.float64
             return xT, vT
 
         @staticmethod
         def backward(ctx, gxT, gvT):
             rollout = ctx.rollout
+            if getattr(rollout, "generation", None) != ctx.generation:
+                raise RuntimeError(
+                    "RolloutFunction.backward: the Rollout ran another forward pass after this "
+                    "one; its checkpoints belong to that pass.  Use one Rollout per "
+                    "outstanding graph, or call backward before the next apply().")
             n = rollout.n
             dev = gxT.device if gxT is not None else gvT.device
             gx = (gxT if gxT is not None else torch.zeros((n, 3), device=dev)).double().contiguous()
@@ -161,8 +170,9 @@ def rollout_function():
             torch.cuda.synchronize()
             gvisc = rollout.backward_ptr(gx.data_ptr(), gv.data_ptr(), gx0.data_ptr(),
                                          gv0.data_ptr())
-            return (gx0.float(), gv0.float(), torch.tensor(gvisc, dtype=ctx.vdtype, device=dev),
-                    None, None)
+            gv_visc = (torch.tensor(gvisc, dtype=ctx.vdtype, device=dev) if ctx.visc_is_tensor
+                       else None)  # a Python float viscosity takes no gradient
+            return gx0.float(), gv0.float(), gv_visc, None, None
 
     return RolloutFunction
 

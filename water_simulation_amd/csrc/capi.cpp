@@ -374,6 +374,11 @@ static void fill_stats(const wsim::StepStats& st, wsim_step_stats* out) {
     out->max_velocity = st.max_velocity;
     out->nan_count = st.nan_count;
     out->substeps = st.substeps;
+    out->escapees = st.escapees;
+    out->reserved = 0;
+    out->total_mass = st.total_mass;
+    for (int a = 0; a < 3; ++a) out->momentum[a] = st.momentum[a];
+    out->kinetic_energy = st.kinetic_energy;
 }
 
 wsim_status wsim_solver_create(wsim_store fluid, wsim_store boundary, const wsim_dfsph_config* c,
@@ -564,6 +569,14 @@ wsim_status wsim_batch_scene_stats(wsim_solver b, uint32_t scene, wsim_step_stat
         need(b, "batch");
         need(out, "out");
         fill_stats(b->impl->scene_stats(scene), out);
+    });
+}
+
+wsim_status wsim_solver_neighbor_counts(wsim_solver s, uint32_t* cand_fluid, uint32_t* nbr_fluid,
+                                        uint32_t* cand_boundary, uint32_t* nbr_boundary) {
+    return guarded([&] {
+        need(s, "solver");
+        s->impl->neighbor_counts(cand_fluid, nbr_fluid, cand_boundary, nbr_boundary);
     });
 }
 

@@ -155,18 +155,44 @@ __global__ __launch_bounds__(NS) void k_pack_boundary(u32 nb, const u32* __restr
     store_mb[perm[s]] = mb[s];
 }
 
+// Neighbour counts of working slots to original order (as k_reorder maps
+// slots to particles).
+__global__ __launch_bounds__(NS) void k_scatter_counts(Nbr P, const u32* __restrict__ perm,
+                                                       const uint4* __restrict__ counts,
+                                                       u32* __restrict__ cand_f,
+                                                       u32* __restrict__ nbr_f,
+                                                       u32* __restrict__ cand_b,
+                                                       u32* __restrict__ nbr_b) {
+    const u32 i = blockIdx.x * NS + threadIdx.x;
+    if (i >= P.nslots) return;
+    const SceneSeg& G = P.seg[scene_of_slot(P, i)];
+    const u32 l = i - G.off;
+    if (l >= G.n) return;
+    const u32 id = perm[G.start + l];
+    const uint4 c = counts[i];
+    if (cand_f) cand_f[id] = c.x;
+    if (nbr_f) nbr_f[id] = c.y;
+    if (cand_b) cand_b[id] = c.z;
+    if (nbr_b) nbr_b[id] = c.w;
+}
+
 __device__ __forceinline__ u32 pack_counts(u32 cf, u32 cb) {
     return (cf < 0xFFFFu ? cf : 0xFFFFu) | ((cb < 0xFFFFu ? cb : 0xFFFFu) << 16);
 }
 
 // rho (dfsph_kernels.cu:68-112) and alpha (dfsph_kernels.cu:117-169) in one
 // stencil walk: they share r, |r| and q for every pair.  Also records the
-// pair lists used by every iteration of the pressure solves.
-template <int NT>
+// pair lists used by every iteration of the pressure solves.  With COUNT the
+// walk also writes, per working slot, the neighbour counts of ITER_NEIGHBORS
+// (dfsph_kernels.cu:13-35) for the fluid and the boundary store: candidates
+// (every slot of the in-grid stencil cells, self included) and neighbours
+// (candidates with sqrtf(r^2) / l < 2, i.e. r^2 < far2).
+template <int NT, bool COUNT = false>
 __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
                                                       const float4* __restrict__ xs,
                                                       float* __restrict__ rho_out,
-                                                      float* __restrict__ alpha_out) {
+                                                      float* __restrict__ alpha_out,
+                                                      uint4* __restrict__ counts = nullptr) {
     const u32 i = slot_index<NT>(P.remap);
     if (i >= P.nslots) return;
     const SceneSeg& G = P.seg[scene_of_slot(P, i)];
@@ -177,11 +203,16 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
     float rho = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f, sq = 0.0f;
     u32 cf = 0, cb = 0;
     bool unlisted = false;  // an in-support fluid pair with a zero gradient
+    uint4 nc = make_uint4(0u, 0u, 0u, 0u);  // COUNT: candidates, neighbours (fluid, boundary)
     const StageRanges R = L.plan[i / kTile];
     for_stencil_rows(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
         const float4 q = xs[s];
         const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
         const float r2 = dist2(rx, ry, rz);
+        if (COUNT) {
+            ++nc.x;
+            nc.y += r2 < P.sp.far2 ? 1u : 0u;
+        }
         if (r2 >= P.sp.far2) return;  // W = 0 and c = 0
         const float d = pair_sqrt(r2, P.sp);
         rho += P.mass * spline_W_near(d, P.sp);
@@ -213,6 +244,10 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
             const float4 q = P.xb[s];
             const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
             const float r2 = dist2(rx, ry, rz);
+            if (COUNT) {
+                ++nc.z;
+                nc.w += r2 < P.sp.far2 ? 1u : 0u;
+            }
             if (r2 >= P.sp.far2) return;
             const float d = pair_sqrt(r2, P.sp);
             rho += q.w * spline_W_near(d, P.sp);
@@ -229,6 +264,7 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
         });
     }
     L.cnt[i] = pack_counts(cf, cb);
+    if (COUNT) counts[i] = nc;
     rho_out[i] = rho;
     const float denom = sx * sx + sy * sy + sz * sz + sq;
     alpha_out[i] = (denom > 1e-9f) ? (rho / denom) : 0.0f;
@@ -1070,17 +1106,59 @@ __device__ __forceinline__ bool advect_due(const DeviceState& S, int force) {
     return !S.advected && (force || S.conv_iter != kNone);
 }
 
-// Per-tile (max |v|, NaN count) partials (one 256-slot tile per workgroup);
-// k_finish_vstats reduces them per scene.
-__device__ __forceinline__ void vstats_partial(float mag, u32 bad, float* __restrict__ pmax,
-                                               u32* __restrict__ pnan) {
+// Per-workgroup partials of the substep statistics (one 256-slot tile or
+// 256 original indices per workgroup); k_finish_vstats reduces them per
+// scene in a fixed order.
+
+// One particle's contribution after the substep: |v|, non-finite state,
+// escape (position on or outside the domain clamp box, a NaN never counts)
+// and, for a finite state, m, m v and m |v|^2 / 2 in binary64.
+struct PStat {
+    float mag = 0.0f;
+    u32 bad = 0, esc = 0;
+    double m = 0.0, px = 0.0, py = 0.0, pz = 0.0, ke = 0.0;
+};
+__device__ __forceinline__ PStat particle_stat(float mass, float x, float y, float z, float vx,
+                                               float vy, float vz, const float* lo,
+                                               const float* hi) {
+    PStat r;
+    r.mag = norm3(vx, vy, vz);
+    r.bad = finite3(x, y, z) && finite3(vx, vy, vz) ? 0u : 1u;
+    r.esc = (x <= lo[0] || y <= lo[1] || z <= lo[2] || x >= hi[0] || y >= hi[1] || z >= hi[2])
+                ? 1u : 0u;
+    if (!r.bad) {
+        const double m = static_cast<double>(mass), a = vx, b = vy, c = vz;
+        r.m = m;
+        r.px = m * a;
+        r.py = m * b;
+        r.pz = m * c;
+        r.ke = 0.5 * m * ((a * a + b * b) + c * c);
+    }
+    return r;
+}
+
+__device__ __forceinline__ void vstats_partial(const PStat& q, const VStats& V) {
     __shared__ float s_f[NS / 64];
     __shared__ u32 s_u[NS / 64];
-    const float m = block_max<NS>(mag, s_f);
-    const u32 c = block_sum_u32<NS>(bad, s_u);
+    __shared__ double s_d[NS / 64];
+    const float m = block_max<NS>(q.mag, s_f);
+    const u32 c = block_sum_u32<NS>(q.bad, s_u);
+    const u32 e = block_sum_u32<NS>(q.esc, s_u);
+    const double d0 = block_sum<NS>(q.m, s_d);
+    const double d1 = block_sum<NS>(q.px, s_d);
+    const double d2 = block_sum<NS>(q.py, s_d);
+    const double d3 = block_sum<NS>(q.pz, s_d);
+    const double d4 = block_sum<NS>(q.ke, s_d);
     if (threadIdx.x == 0) {
-        pmax[blockIdx.x] = m;
-        pnan[blockIdx.x] = c;
+        V.pmax[blockIdx.x] = m;
+        V.pnan[blockIdx.x] = c;
+        V.pesc[blockIdx.x] = e;
+        double* o = V.pmom + kMomParts * static_cast<u64>(blockIdx.x);
+        o[0] = d0;
+        o[1] = d1;
+        o[2] = d2;
+        o[3] = d3;
+        o[4] = d4;
     }
 }
 
@@ -1088,25 +1166,39 @@ __device__ __forceinline__ void vstats_partial(float mag, u32 bad, float* __rest
 // processed (the advect_due test, evaluated before this kernel marks them).
 template <int NT>
 __global__ __launch_bounds__(NT) void k_finish_vstats(const SceneSeg* __restrict__ seg,
-                                                      const float* __restrict__ pmax,
-                                                      const u32* __restrict__ pnan,
-                                                      DeviceState* st, int guarded, int force) {
+                                                      VStats V, DeviceState* st, int guarded,
+                                                      int force) {
     __shared__ float s_f[NT / 64];
     __shared__ u32 s_u[NT / 64];
+    __shared__ double s_d[NT / 64];
     const SceneSeg& G = seg[blockIdx.x];
     DeviceState& S = st[blockIdx.x];
     if (guarded && !advect_due(S, force)) return;  // block-uniform
     float m = 0.0f;
-    u32 c = 0;
+    u32 c = 0, e = 0;
+    double d[kMomParts] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (u32 t = threadIdx.x; t < G.ntiles; t += NT) {
-        m = fmaxf(m, pmax[G.tile0 + t]);
-        c += pnan[G.tile0 + t];
+        const u32 b = G.tile0 + t;
+        m = fmaxf(m, V.pmax[b]);
+        c += V.pnan[b];
+        e += V.pesc[b];
+#pragma unroll
+        for (int k = 0; k < kMomParts; ++k) d[k] += V.pmom[kMomParts * static_cast<u64>(b) + k];
     }
     m = block_max<NT>(m, s_f);
     c = block_sum_u32<NT>(c, s_u);
+    e = block_sum_u32<NT>(e, s_u);
+#pragma unroll
+    for (int k = 0; k < kMomParts; ++k) d[k] = block_sum<NT>(d[k], s_d);
     if (threadIdx.x == 0) {
         S.max_v = m;
         S.nan_count = c;
+        S.escapees = e;
+        S.mass = d[0];
+        S.mom[0] = d[1];
+        S.mom[1] = d[2];
+        S.mom[2] = d[3];
+        S.ke = d[4];
         if (guarded) S.advected = 1u;
     }
 }
@@ -1132,11 +1224,9 @@ __global__ __launch_bounds__(NS) void k_advect(Nbr P, const float4* __restrict__
                                                const float* __restrict__ kappa,
                                                const float* __restrict__ rho_adv, int wb_solve,
                                                const DeviceState* __restrict__ st, int force,
-                                               float* __restrict__ pmax,
-                                               u32* __restrict__ pnan) {
+                                               VStats V) {
     const u32 i = blockIdx.x * NS + threadIdx.x;
-    float mag = 0.0f;
-    u32 bad = 0;
+    PStat q;
     {
         const u32 sc = scene_of_slot(P, i);
         if (!advect_due(st[sc], force)) return;  // the workgroup is one tile of one scene
@@ -1174,11 +1264,11 @@ __global__ __launch_bounds__(NS) void k_advect(Nbr P, const float4* __restrict__
                 out.kappa[id] = kappa[i];
                 out.rho_adv[id] = rho_adv[i];
             }
-            mag = norm3(v.x, v.y, v.z);
-            bad = finite3(px, py, pz) && finite3(v.x, v.y, v.z) ? 0u : 1u;
+            const float lo[3] = {lx, ly, lz}, hi[3] = {hx, hy, hz};
+            q = particle_stat(P.mass, px, py, pz, v.x, v.y, v.z, lo, hi);
         }
     }
-    vstats_partial(mag, bad, pmax, pnan);
+    vstats_partial(q, V);
 }
 
 // k_advect for a single scene, one thread per original index: the sorted
@@ -1192,11 +1282,9 @@ __global__ __launch_bounds__(NS) void k_advect_gather(
     const u32* __restrict__ inv, float vmax, float lx, float ly, float lz, float hx, float hy,
     float hz, StoreOut out, const float* __restrict__ rho, const float* __restrict__ alpha,
     const float* __restrict__ kappa, const float* __restrict__ rho_adv, int wb_solve,
-    const DeviceState* __restrict__ st, int force, float* __restrict__ pmax,
-    u32* __restrict__ pnan) {
+    const DeviceState* __restrict__ st, int force, VStats V) {
     const u32 id = blockIdx.x * NS + threadIdx.x;
-    float mag = 0.0f;
-    u32 bad = 0;
+    PStat q;
     if (!advect_due(st[0], force)) return;  // grid-uniform
     const SceneSeg& G = P.seg[0];
     if (id < G.n) {
@@ -1228,10 +1316,10 @@ __global__ __launch_bounds__(NS) void k_advect_gather(
             out.kappa[id] = kappa[i];
             out.rho_adv[id] = rho_adv[i];
         }
-        mag = norm3(v.x, v.y, v.z);
-        bad = finite3(px, py, pz) && finite3(v.x, v.y, v.z) ? 0u : 1u;
+        const float lo[3] = {lx, ly, lz}, hi[3] = {hx, hy, hz};
+        q = particle_stat(P.mass, px, py, pz, v.x, v.y, v.z, lo, hi);
     }
-    vstats_partial(mag, bad, pmax, pnan);
+    vstats_partial(q, V);
 }
 
 // End of the divergence solve: velocities and the recomputed attributes back
@@ -1243,11 +1331,10 @@ __global__ __launch_bounds__(NS) void k_div_writeback(Nbr P, const float4* __res
                                                       const float* __restrict__ alpha,
                                                       const float* __restrict__ kappa_v,
                                                       const float* __restrict__ rho_dot,
-                                                      int wb_solve, float* __restrict__ pmax,
-                                                      u32* __restrict__ pnan) {
+                                                      int wb_solve, float lx, float ly, float lz,
+                                                      float hx, float hy, float hz, VStats V) {
     const u32 i = blockIdx.x * NS + threadIdx.x;
-    float mag = 0.0f;
-    u32 bad = 0;
+    PStat q;
     {
         const SceneSeg& G = P.seg[scene_of_slot(P, i)];
         const u32 l = i - G.off;
@@ -1265,11 +1352,11 @@ __global__ __launch_bounds__(NS) void k_div_writeback(Nbr P, const float4* __res
                 out.kappa[id] = kappa_v[i];
                 out.rho_adv[id] = rho_dot[i];
             }
-            mag = norm3(v.x, v.y, v.z);
-            bad = finite3(x4.x, x4.y, x4.z) && finite3(v.x, v.y, v.z) ? 0u : 1u;
+            const float lo[3] = {lx, ly, lz}, hi[3] = {hx, hy, hz};
+            q = particle_stat(P.mass, x4.x, x4.y, x4.z, v.x, v.y, v.z, lo, hi);
         }
     }
-    vstats_partial(mag, bad, pmax, pnan);
+    vstats_partial(q, V);
 }
 
 // Opens a solve: every non-empty scene unconverged, empty scenes done.
@@ -1626,6 +1713,8 @@ void DFSPHSolver::init(const std::vector<u32>& counts, const std::vector<float>&
     part_d_ = DeviceBuffer<double>(slots_cap / kTile);
     part_f_ = DeviceBuffer<float>(slots_cap / 64);
     part_u_ = DeviceBuffer<u32>(slots_cap / 64);
+    part_e_ = DeviceBuffer<u32>(slots_cap / 64);
+    part_m_ = DeviceBuffer<double>(kMomParts * (slots_cap / 64));
     hseg_ = PinnedBuffer<SceneSeg>(scenes);
     dseg_ = DeviceBuffer<SceneSeg>(scenes);
     dstate_ = DeviceBuffer<DeviceState>(scenes);
@@ -1964,6 +2053,48 @@ void DFSPHSolver::phase_prefix_solve(bool gravity) {
     WSIM_LAUNCH_CHECK();
 }
 
+void DFSPHSolver::neighbor_counts(u32* cand_f, u32* nbr_f, u32* cand_b, u32* nbr_b) {
+    // the substep's grid phase on the store's current positions, with the
+    // counting walk; the next substep rebuilds all of it from the store
+    if (!batched_) {
+        const size_t nsz = fluid_.count();
+        if (nsz == 0) return;
+        if (counts_.empty() || counts_[0] != nsz) layout({static_cast<u32>(nsz)});
+    } else {
+        if (fluid_.count() != total_)
+            fail(kInvalidArgument, "batched DFSPHSolver: the store count changed");
+        if (total_ == 0) return;
+    }
+    // the scene table's layout part (a solver that never stepped has not
+    // uploaded it yet; the per-substep part is not read by this walk)
+    WSIM_HIP(hipMemcpyAsync(dseg_.data(), hseg_.data(), counts_.size() * sizeof(SceneSeg),
+                            hipMemcpyHostToDevice, stream_));
+    if (batched_) fluid_grid_.build_segmented(fluid_.positions(), total_, dstarts_.data(),
+                                              static_cast<u32>(counts_.size()));
+    else fluid_grid_.build(fluid_.positions(), total_);
+    const Nbr P = nbr();
+    k_reorder<<<nslots_ / NS, NS, 0, stream_>>>(P, fluid_grid_.sorted_indices(), fluid_.positions(),
+                                                 fluid_.velocities(), xs_.data(), nullptr, 0, 1,
+                                                 nullptr);
+    WSIM_LAUNCH_CHECK();
+    k_stage_plan<<<div_up(nslots_ / kTile, NS), NS, 0, stream_>>>(P, plan_.data());
+    WSIM_LAUNCH_CHECK();
+    DeviceBuffer<uint4> counts(std::max<u32>(nslots_, 1u));
+    k_density_alpha<NS, true><<<nslots_ / NS, NS, 0, stream_>>>(P, pair_list(), xs_.data(),
+                                                                rho_.data(), alpha_.data(),
+                                                                counts.data());
+    WSIM_LAUNCH_CHECK();
+    k_scatter_counts<<<nslots_ / NS, NS, 0, stream_>>>(P, fluid_grid_.sorted_indices(),
+                                                        counts.data(), cand_f, nbr_f, cand_b,
+                                                        nbr_b);
+    WSIM_LAUNCH_CHECK();
+    WSIM_HIP(hipStreamSynchronize(stream_));
+}
+
+VStats DFSPHSolver::vstats() {
+    return VStats{part_f_.data(), part_u_.data(), part_e_.data(), part_m_.data()};
+}
+
 Nbr DFSPHSolver::nbr() const {
     return make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), bnear_.data(), nb_, cfg_, mass_,
                     nslots_, dseg_.data(), batched_ ? tile_scene_.data() : nullptr, shapes_.remap);
@@ -2015,21 +2146,21 @@ void DFSPHSolver::phase_advect(bool force) {
             cfg_.domain_min[1] + eps, cfg_.domain_min[2] + eps, cfg_.domain_max[0] - eps,
             cfg_.domain_max[1] - eps, cfg_.domain_max[2] - eps, out, rho_.data(), alpha_.data(),
             kappa_.data(), rho_adv_.data(), cfg_.max_density_iters > 0 ? 1 : 0, dstate_.data(),
-            force ? 1 : 0, part_f_.data(), part_u_.data());
+            force ? 1 : 0, vstats());
     } else {
         k_advect<<<nslots_ / NS, NS, 0, stream_>>>(
             P, xs_.data(), vs_.data(), fluid_grid_.sorted_indices(), 5.0f, cfg_.domain_min[0] + eps,
             cfg_.domain_min[1] + eps, cfg_.domain_min[2] + eps, cfg_.domain_max[0] - eps,
             cfg_.domain_max[1] - eps, cfg_.domain_max[2] - eps, out, rho_.data(), alpha_.data(),
             kappa_.data(), rho_adv_.data(), cfg_.max_density_iters > 0 ? 1 : 0, dstate_.data(),
-            force ? 1 : 0, part_f_.data(), part_u_.data());
+            force ? 1 : 0, vstats());
     }
     WSIM_LAUNCH_CHECK();
     tend();
     // also marks the scenes advected (after the divergence solve, the
     // writeback reports max |v| instead)
     k_finish_vstats<1024><<<static_cast<u32>(counts_.size()), 1024, 0, stream_>>>(
-        dseg_.data(), part_f_.data(), part_u_.data(), dstate_.data(), 1, force ? 1 : 0);
+        dseg_.data(), vstats(), dstate_.data(), 1, force ? 1 : 0);
     WSIM_LAUNCH_CHECK();
 }
 
@@ -2037,15 +2168,17 @@ void DFSPHSolver::phase_div_tail() {
     const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), bnear_.data(), nb_, cfg_, mass_, nslots_,
                            dseg_.data(), batched_ ? tile_scene_.data() : nullptr, shapes_.remap);
     const StoreOut out = store_out(fluid_, h_density_, h_alpha_, h_kappa_v_, h_density_adv_);
+    const float eps = 1e-3f;  // the advection's clamp box (dfsph_kernels.cu:377)
     tbegin(kKDivWriteback);
     k_div_writeback<<<nslots_ / NS, NS, 0, stream_>>>(
         P, xs_.data(), vs_.data(), fluid_grid_.sorted_indices(), out, rho_.data(), alpha_.data(),
-        kappa_.data(), rho_adv_.data(), cfg_.max_divergence_iters > 0 ? 1 : 0, part_f_.data(),
-        part_u_.data());
+        kappa_.data(), rho_adv_.data(), cfg_.max_divergence_iters > 0 ? 1 : 0,
+        cfg_.domain_min[0] + eps, cfg_.domain_min[1] + eps, cfg_.domain_min[2] + eps,
+        cfg_.domain_max[0] - eps, cfg_.domain_max[1] - eps, cfg_.domain_max[2] - eps, vstats());
     WSIM_LAUNCH_CHECK();
     tend();
     k_finish_vstats<1024><<<static_cast<u32>(counts_.size()), 1024, 0, stream_>>>(
-        dseg_.data(), part_f_.data(), part_u_.data(), dstate_.data(), 0, 1);
+        dseg_.data(), vstats(), dstate_.data(), 0, 1);
     WSIM_LAUNCH_CHECK();
 }
 
@@ -2150,6 +2283,10 @@ void DFSPHSolver::take_state() {
         }
         st.max_velocity = h.max_v;
         st.nan_count = h.nan_count;
+        st.escapees = empty ? 0u : h.escapees;
+        st.total_mass = empty ? 0.0 : h.mass;
+        for (int a = 0; a < 3; ++a) st.momentum[a] = empty ? 0.0 : h.mom[a];
+        st.kinetic_energy = empty ? 0.0 : h.ke;
         st.substeps += 1;
         max_d = std::max(max_d, st.density_iters);
         max_v = std::max(max_v, st.divergence_iters);

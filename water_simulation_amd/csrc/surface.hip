@@ -134,27 +134,37 @@ __global__ __launch_bounds__(kMeshThreads) void k_mesh_points(const float* __res
     P[3 * i + 2] = swap ? y : z;
 }
 
+// Per-workgroup partials: min x, y, z, max x, y, z and the number of
+// non-finite coordinates (fmin/fmax skip NaN, so the count is what lets the
+// host refuse a point set holding one before any k-NN search runs).
+constexpr int kBoundsPart = 7;
 __global__ __launch_bounds__(kMeshThreads) void k_mesh_bounds(const double* __restrict__ P, u64 n,
                                                               double* __restrict__ part) {
-    __shared__ double s[6][kMeshThreads];
+    __shared__ double s[kBoundsPart][kMeshThreads];
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double bad = 0.0;
     const u64 stride = static_cast<u64>(gridDim.x) * kMeshThreads;
     for (u64 i = static_cast<u64>(blockIdx.x) * kMeshThreads + threadIdx.x; i < n; i += stride) {
         for (int a = 0; a < 3; ++a) {
-            mn[a] = fmin(mn[a], P[3 * i + a]);
-            mx[a] = fmax(mx[a], P[3 * i + a]);
+            const double c = P[3 * i + a];
+            mn[a] = fmin(mn[a], c);
+            mx[a] = fmax(mx[a], c);
+            bad += isfinite(c) ? 0.0 : 1.0;
         }
     }
     for (int a = 0; a < 3; ++a) {
         s[a][threadIdx.x] = mn[a];
         s[3 + a][threadIdx.x] = mx[a];
     }
+    s[6][threadIdx.x] = bad;
     __syncthreads();
-    if (threadIdx.x < 6) {
+    if (threadIdx.x < kBoundsPart) {
         double r = s[threadIdx.x][0];
-        for (int k = 1; k < kMeshThreads; ++k)
-            r = threadIdx.x < 3 ? fmin(r, s[threadIdx.x][k]) : fmax(r, s[threadIdx.x][k]);
-        part[6 * blockIdx.x + threadIdx.x] = r;
+        for (int k = 1; k < kMeshThreads; ++k) {
+            const double o = s[threadIdx.x][k];
+            r = threadIdx.x < 3 ? fmin(r, o) : (threadIdx.x < 6 ? fmax(r, o) : r + o);
+        }
+        part[kBoundsPart * blockIdx.x + threadIdx.x] = r;
     }
 }
 
@@ -314,6 +324,7 @@ __global__ __launch_bounds__(kMeshThreads) void k_knn_smooth(const double* __res
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const u32 j = bi[k];
+        if (j == 0xFFFFFFFFu) continue;  // never with finite input (the host refuses others)
         sx = sx + P[3ull * j];
         sy = sy + P[3ull * j + 1];
         sz = sz + P[3ull * j + 2];
@@ -402,9 +413,11 @@ __global__ __launch_bounds__(kMeshThreads) void k_knn_far(const double* __restri
             }
             bd[K - 1] = mine ? INFINITY : bd[K - 1];
             bi[K - 1] = mine ? 0xFFFFFFFFu : bi[K - 1];
-            sx = sx + P[3ull * id];
-            sy = sy + P[3ull * id + 1];
-            sz = sz + P[3ull * id + 2];
+            if (id != 0xFFFFFFFFu) {  // always, with finite input
+                sx = sx + P[3ull * id];
+                sy = sy + P[3ull * id + 1];
+                sz = sz + P[3ull * id + 2];
+            }
         }
         if (lane == 0) {
             const u32 i = sorted[q];
@@ -1050,11 +1063,11 @@ void SurfaceMesher::run(const float* xyz, size_t n, const MeshParams& P) {
     k_mesh_points<<<gp, kMeshThreads, 0, s>>>(xyz, n, P.swap_yz ? 1 : 0, pts_[0].data());
     WSIM_LAUNCH_CHECK();
     const u32 nbb = std::min<u32>(gp, 1024);
-    DeviceBuffer<double> part(6 * static_cast<size_t>(nbb));
+    DeviceBuffer<double> part(kBoundsPart * static_cast<size_t>(nbb));
     auto bounds = [&](const double* pts, double* mn, double* mx) {
         k_mesh_bounds<<<nbb, kMeshThreads, 0, s>>>(pts, n, part.data());
         WSIM_LAUNCH_CHECK();
-        std::vector<double> hp(6 * static_cast<size_t>(nbb));
+        std::vector<double> hp(kBoundsPart * static_cast<size_t>(nbb));
         WSIM_HIP(hipMemcpyAsync(hp.data(), part.data(), hp.size() * sizeof(double),
                                 hipMemcpyDeviceToHost, s));
         WSIM_HIP(hipStreamSynchronize(s));
@@ -1062,11 +1075,15 @@ void SurfaceMesher::run(const float* xyz, size_t n, const MeshParams& P) {
             mn[a] = INFINITY;
             mx[a] = -INFINITY;
         }
-        for (u32 k = 0; k < nbb; ++k)
+        double bad = 0.0;
+        for (u32 k = 0; k < nbb; ++k) {
             for (int a = 0; a < 3; ++a) {
-                mn[a] = std::fmin(mn[a], hp[6 * k + a]);
-                mx[a] = std::fmax(mx[a], hp[6 * k + 3 + a]);
+                mn[a] = std::fmin(mn[a], hp[kBoundsPart * k + a]);
+                mx[a] = std::fmax(mx[a], hp[kBoundsPart * k + 3 + a]);
             }
+            bad += hp[kBoundsPart * k + 6];
+        }
+        if (bad != 0.0) fail(kInvalidArgument, "SurfaceMesher: non-finite positions");
         for (int a = 0; a < 3; ++a)
             if (!std::isfinite(mn[a]) || !std::isfinite(mx[a]))
                 fail(kInvalidArgument, "SurfaceMesher: non-finite positions");

@@ -58,6 +58,17 @@ private:
 // cell_start[C+1] with cell_start[c] = #particles whose key < c.
 namespace dev {
 struct Nbr;  // kernel view of the grids and stores (stencil.hpp)
+
+// Per-workgroup partials of the substep statistics, reduced per scene after
+// advection: max |v|, non-finite count, escapee count, and kMomParts
+// binary64 sums (mass, momentum x, y, z, kinetic energy) per workgroup.
+constexpr int kMomParts = 5;
+struct VStats {
+    float* pmax;
+    u32* pnan;
+    u32* pesc;
+    double* pmom;
+};
 }
 
 class CellGrid {
@@ -225,6 +236,11 @@ struct StepStats {
     float max_velocity = 0.0f;
     u32 nan_count = 0;
     u64 substeps = 0;
+    // after the substep, binary64 over the particles with a finite state:
+    u32 escapees = 0;           // positions on or outside the domain clamp box
+    double total_mass = 0.0;    // m * (particles with finite position and velocity)
+    double momentum[3] = {0.0, 0.0, 0.0};  // sum m v
+    double kinetic_energy = 0.0;           // sum m |v|^2 / 2
 };
 
 // Working-set slots are grouped in tiles of kTile; each scene starts on a
@@ -266,7 +282,10 @@ struct DeviceState {
     double err;         // mean error of the latest executed density iteration
     double div_err;     // same for the divergence solve
     u32 advected;       // this substep's advection has run for the scene
-    u32 pad;
+    u32 escapees;       // see StepStats
+    double mass;
+    double mom[3];
+    double ke;
 };
 
 enum KernelKind : int {
@@ -438,6 +457,13 @@ public:
     void autotune(int reps);
     const std::vector<TuneRecord>& tune_log() const { return tune_log_; }
 
+    // Neighbour counts of the density/alpha walk on the store's current
+    // positions, per particle in original order (device arrays, each may be
+    // null): stencil candidates and in-support neighbours in the fluid and
+    // the boundary store.  Synchronous; leaves the store and the results of
+    // the next step unchanged.
+    void neighbor_counts(u32* cand_f, u32* nbr_f, u32* cand_b, u32* nbr_b);
+
     // XSPH viscosity of a plain solver (the parameter the adjoint differentiates).
     void set_viscosity(float eps);
     // Plain solver only: a substep that records `tape` (eager launches).
@@ -477,6 +503,7 @@ private:
     void run_phase(u64 key, const std::function<void()>& body);
     void drop_graphs();
     PairList pair_list();
+    dev::VStats vstats();
     void tbegin(int kind, int iter = 0) {
         if (timing_ && !capturing_) timer_.begin(kind, iter, stream_);
     }
@@ -521,7 +548,8 @@ private:
     DeviceBuffer<float> rho_, alpha_, kappa_, rho_adv_;
     DeviceBuffer<double> part_d_;
     DeviceBuffer<float> part_f_;
-    DeviceBuffer<u32> part_u_;
+    DeviceBuffer<u32> part_u_, part_e_;
+    DeviceBuffer<double> part_m_;
     DeviceBuffer<DeviceState> dstate_;
     PinnedBuffer<DeviceState> hstate_;
 
