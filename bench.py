@@ -389,6 +389,9 @@ def main() -> int:
     ap.add_argument("--n-side", type=int, default=100)
     ap.add_argument("--pressure-form", type=int, default=1,
                     help="1: kappa/rho (meets eta; the headline), 0: the reference's kappa/rho^2")
+    ap.add_argument("--fast-math", type=int, default=0,
+                    help="1: tolerance-mode pair arithmetic of the neighbour kernels (hardware "
+                         "rsqrt/rcp and fma, as the reference's -use_fast_math build)")
     ap.add_argument("--reference-steps", type=int, default=20,
                     help="substeps timed with the reference kappa/rho^2 form (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -456,6 +459,7 @@ def main() -> int:
         return 0
 
     x, xb, cfg = make_workload(args.n_side)
+    cfg = dict(cfg, fast_math=bool(args.fast_math))
     run = dam_break(ws, x, xb, dict(cfg, pressure_form=args.pressure_form), args.steps,
                     args.warmup, timing=True)
     n, nb = len(x), len(xb)

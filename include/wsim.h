@@ -161,6 +161,14 @@ typedef struct {
     uint32_t enable_surface_tension;  /* extension: Akinci 2013 cohesion + curvature with
                                          coefficient surface_tension (the reference parses the
                                          coefficient and applies no force) */
+    uint32_t fast_math;               /* extension: 1 = tolerance mode of the neighbour kernels
+                                         (density/alpha, XSPH, both pressure solves): the pair
+                                         terms use the hardware reciprocal square root and
+                                         reciprocal and fused multiply-adds, as the reference's
+                                         own -use_fast_math build (CMakeLists.txt:56) does;
+                                         fields then differ from the IEEE evaluation in the
+                                         last bits.  0 (default) = bit-exact IEEE binary32.
+                                         The adjoint requires 0. */
 } wsim_dfsph_config;
 
 typedef struct {

@@ -247,9 +247,13 @@ def next_dt(frame_dt, dt_max, cfl_lambda, max_substeps, max_velocity, particle_r
 
 
 def config(**kw) -> OrcConfig:
+    """The oracle evaluates the reference's expressions in IEEE arithmetic:
+    the solver's tolerance-mode switch (fast_math) has no meaning here."""
     c = OrcConfig()
     lib().orc_config_default(C.byref(c))
     for k, v in kw.items():
+        if k == "fast_math":
+            continue
         if k in ("gravity", "domain_min", "domain_max"):
             setattr(c, k, (C.c_float * 3)(*[float(x) for x in v]))
         elif k in ("max_density_iters", "max_divergence_iters", "pressure_form",

@@ -371,6 +371,7 @@ class DFSPHConfig:
     enable_divergence_solve: bool = False
     pressure_form: int = 0  # 0: reference kappa/rho^2, 1: kappa/rho
     enable_surface_tension: bool = False  # Akinci 2013, coefficient surface_tension
+    fast_math: bool = False  # tolerance mode of the pair terms (hardware rsqrt/rcp, fma)
 
     def to_c(self) -> _ffi.DFSPHConfigC:
         c = _ffi.DFSPHConfigC()
@@ -380,7 +381,7 @@ class DFSPHConfig:
                 setattr(c, f.name, (C.c_float * 3)(*[float(x) for x in v]))
             elif f.name in ("max_density_iters", "max_divergence_iters", "pressure_form"):
                 setattr(c, f.name, int(v))
-            elif f.name in ("enable_divergence_solve", "enable_surface_tension"):
+            elif f.name in ("enable_divergence_solve", "enable_surface_tension", "fast_math"):
                 setattr(c, f.name, 1 if v else 0)
             else:
                 setattr(c, f.name, float(v))

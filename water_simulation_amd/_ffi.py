@@ -38,6 +38,7 @@ class DFSPHConfigC(C.Structure):
         ("enable_divergence_solve", C.c_uint32),
         ("pressure_form", C.c_uint32),
         ("enable_surface_tension", C.c_uint32),
+        ("fast_math", C.c_uint32),
     ]
 
 

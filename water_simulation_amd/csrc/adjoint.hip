@@ -676,6 +676,9 @@ Rollout::Rollout(DFSPHSolver& solver, ParticleStore& fluid, u32 every)
     if (solver.config().enable_divergence_solve || solver.config().enable_surface_tension)
         fail(kInvalidArgument,
              "Rollout: the adjoint covers the density solve without surface tension");
+    if (solver.config().fast_math)
+        fail(kInvalidArgument,
+             "Rollout: the adjoint differentiates the bit-exact pair arithmetic (fast_math 0)");
 }
 
 void Rollout::forward(const float* x0, const float* v0, const float* dts, u32 substeps) {

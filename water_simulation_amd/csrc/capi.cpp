@@ -340,10 +340,12 @@ void wsim_dfsph_config_default(wsim_dfsph_config* c) {
     c->enable_divergence_solve = d.enable_divergence_solve;
     c->pressure_form = d.pressure_form;
     c->enable_surface_tension = d.enable_surface_tension;
+    c->fast_math = d.fast_math;
 }
 
 static wsim::DFSPHConfig to_config(const wsim_dfsph_config* c) {
     if (c->pressure_form > 1) wsim::fail(wsim::kInvalidArgument, "pressure_form must be 0 or 1");
+    if (c->fast_math > 1) wsim::fail(wsim::kInvalidArgument, "fast_math must be 0 or 1");
     wsim::DFSPHConfig d;
     d.rest_density = c->rest_density;
     d.particle_radius = c->particle_radius;
@@ -363,6 +365,7 @@ static wsim::DFSPHConfig to_config(const wsim_dfsph_config* c) {
     d.enable_divergence_solve = c->enable_divergence_solve;
     d.pressure_form = c->pressure_form;
     d.enable_surface_tension = c->enable_surface_tension;
+    d.fast_math = c->fast_math;
     return d;
 }
 

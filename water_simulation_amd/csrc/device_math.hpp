@@ -70,6 +70,11 @@ struct Spline {
     float far2;
     float inv_l;  // RN(1 / l)
     int fast;     // the pair-path sqrt and quotients below apply (pair_ops_in_range)
+    // Tolerance-mode constants (PairMath<true>): grad W = r * c with
+    // c = ck_in * (2.25 q - 3) for q < 1 (sigma (-3 q + 2.25 q^2) / (l d),
+    // q / d = 1 / l) and c = ck_out * (2 - q)^2 / d for 1 <= q < 2.
+    float ck_in;   // sigma / l^2
+    float ck_out;  // -0.75 sigma / l
     static Spline make(float l);
 };
 
@@ -155,6 +160,8 @@ inline Spline Spline::make(float l) {
     s.far2 = (sqrtf(x) / l >= 2.0f) ? x : 3.0e38f;
     s.inv_l = 1.0f / l;
     s.fast = pair_ops_in_range(l) ? 1 : 0;
+    s.ck_in = static_cast<float>(static_cast<double>(s.sigma) / (static_cast<double>(l) * l));
+    s.ck_out = static_cast<float>(-0.75 * static_cast<double>(s.sigma) / l);
     return s;
 }
 
@@ -206,6 +213,92 @@ __device__ inline float spline_gradc_near(float d, const Spline& sp) {
 __device__ inline float norm3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
 // The radicand of norm3, in the same operation order.
 __device__ inline float dist2(float x, float y, float z) { return x * x + y * y + z * z; }
+
+// ---- pair arithmetic of the neighbour kernels, two modes -----------------
+// PairMath<false> (default): the reference's expressions in its operand
+// order, no contraction, correctly rounded sqrt and quotients: every field
+// equals the binary32 oracle bit for bit.
+// PairMath<true> (tolerance mode, DFSPHConfig::fast_math): the same
+// formulas evaluated the way the reference's own build does it
+// (-use_fast_math, CMakeLists.txt:56): one v_rsq_f32 per pair gives 1/d
+// and d = r2 / d, q = d * RN(1/l) without correction, the gradient factor
+// without a divide (q / d = 1 / l inside the core, 1 / d = rsq outside),
+// and products summed with fma.  Its fields differ from the binary32
+// oracle in the last bits and are checked against the binary64 oracle
+// within the tolerance the binary32 path itself meets.
+template <bool F> struct PairMath;
+
+template <> struct PairMath<false> {
+    static constexpr bool fast = false;
+    __device__ static inline float r2(float x, float y, float z) { return dist2(x, y, z); }
+    // W for a candidate inside the support (r2 < far2)
+    __device__ static inline float w(float r2, const Spline& sp) {
+        return spline_W_near(pair_sqrt(r2, sp), sp);
+    }
+    __device__ static inline float c(float r2, const Spline& sp) {
+        return spline_gradc_near(pair_sqrt(r2, sp), sp);
+    }
+    __device__ static inline void wc(float r2, const Spline& sp, float& wv, float& cv) {
+        const float d = pair_sqrt(r2, sp);
+        wv = spline_W_near(d, sp);
+        cv = spline_gradc_near(d, sp);
+    }
+    // acc + a * b, rounded twice (the reference's acc += a * b)
+    __device__ static inline float mad(float a, float b, float acc) { return acc + a * b; }
+    __device__ static inline float div(float a, float b) { return a / b; }
+};
+
+template <> struct PairMath<true> {
+    static constexpr bool fast = true;
+    __device__ static inline float r2(float x, float y, float z) {
+        return __builtin_fmaf(x, x, __builtin_fmaf(y, y, z * z));
+    }
+    // d and 1/d from r2 > 0; d = 0 (and 1/d = 0) for r2 = 0, where the
+    // gradient factor is 0 (coincident particles, the self pair)
+    __device__ static inline void dist(float r2, float& d, float& inv_d) {
+        const float rs = __builtin_amdgcn_rsqf(r2);
+        const bool pos = r2 > 0.0f;
+        d = pos ? r2 * rs : 0.0f;
+        inv_d = pos ? rs : 0.0f;
+    }
+    __device__ static inline float w_of_q(float q, const Spline& sp) {
+        const float q2 = q * q;
+        const float inner = sp.sigma * __builtin_fmaf(q2, __builtin_fmaf(0.75f, q, -1.5f), 1.0f);
+        const float t = 2.0f - q;
+        const float outer = sp.sigma_025 * (t * t * t);
+        return q < 1.0f ? inner : outer;
+    }
+    __device__ static inline float c_of_q(float q, float d, float inv_d, const Spline& sp) {
+        const float inner = sp.ck_in * __builtin_fmaf(2.25f, q, -3.0f);
+        const float t = 2.0f - q;
+        const float outer = sp.ck_out * (t * t) * inv_d;
+        const float c = q < 1.0f ? inner : outer;
+        return d < 1e-12f ? 0.0f : c;
+    }
+    __device__ static inline float w(float r2, const Spline& sp) {
+        float d, inv_d;
+        dist(r2, d, inv_d);
+        return w_of_q(d * sp.inv_l, sp);
+    }
+    __device__ static inline float c(float r2, const Spline& sp) {
+        float d, inv_d;
+        dist(r2, d, inv_d);
+        return c_of_q(d * sp.inv_l, d, inv_d, sp);
+    }
+    __device__ static inline void wc(float r2, const Spline& sp, float& wv, float& cv) {
+        float d, inv_d;
+        dist(r2, d, inv_d);
+        const float q = d * sp.inv_l;
+        wv = w_of_q(q, sp);
+        cv = c_of_q(q, d, inv_d, sp);
+    }
+    __device__ static inline float mad(float a, float b, float acc) {
+        return __builtin_fmaf(a, b, acc);
+    }
+    __device__ static inline float div(float a, float b) {
+        return a * __builtin_amdgcn_rcpf(b);
+    }
+};
 
 // Workgroup id remap so that each XCD walks a contiguous run of tiles: the
 // dispatcher hands workgroup b to XCD (b % 8), so neighbouring tiles (which

@@ -180,6 +180,30 @@ __device__ __forceinline__ u32 pack_counts(u32 cf, u32 cb) {
     return (cf < 0xFFFFu ? cf : 0xFFFFu) | ((cb < 0xFFFFu ? cb : 0xFFFFu) << 16);
 }
 
+// Density and alpha sums of one pair with mass m (dfsph_kernels.cu:95-107,
+// 145-163): rho += m W, G += m grad W, Q += |m grad W|^2.
+template <bool F>
+__device__ __forceinline__ void grad_terms(float& rho, float& sx, float& sy, float& sz, float& sq,
+                                           float m, float wv, float c, float rx, float ry,
+                                           float rz) {
+    if (F) {
+        rho = __builtin_fmaf(m, wv, rho);
+        const float mc = m * c;
+        const float mx = rx * mc, my = ry * mc, mz = rz * mc;
+        sx += mx;
+        sy += my;
+        sz += mz;
+        sq = __builtin_fmaf(mx, mx, __builtin_fmaf(my, my, __builtin_fmaf(mz, mz, sq)));
+    } else {
+        rho += m * wv;
+        const float mx = m * (rx * c), my = m * (ry * c), mz = m * (rz * c);
+        sx += mx;
+        sy += my;
+        sz += mz;
+        sq += mx * mx + my * my + mz * mz;
+    }
+}
+
 // rho (dfsph_kernels.cu:68-112) and alpha (dfsph_kernels.cu:117-169) in one
 // stencil walk: they share r, |r| and q for every pair.  Also records the
 // pair lists used by every iteration of the pressure solves.  With COUNT the
@@ -187,7 +211,7 @@ __device__ __forceinline__ u32 pack_counts(u32 cf, u32 cb) {
 // (dfsph_kernels.cu:13-35) for the fluid and the boundary store: candidates
 // (every slot of the in-grid stencil cells, self included) and neighbours
 // (candidates with sqrtf(r^2) / l < 2, i.e. r^2 < far2).
-template <int NT, bool COUNT = false>
+template <int NT, bool F, bool COUNT = false>
 __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
                                                       const float4* __restrict__ xs,
                                                       float* __restrict__ rho_out,
@@ -208,21 +232,16 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
     for_stencil_rows(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
         const float4 q = xs[s];
         const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-        const float r2 = dist2(rx, ry, rz);
+        const float r2 = PairMath<F>::r2(rx, ry, rz);
         if (COUNT) {
             ++nc.x;
             nc.y += r2 < P.sp.far2 ? 1u : 0u;
         }
         if (r2 >= P.sp.far2) return;  // W = 0 and c = 0
-        const float d = pair_sqrt(r2, P.sp);
-        rho += P.mass * spline_W_near(d, P.sp);
         // the self pair (d = 0) has c = 0: its terms add +0, and it is not listed
-        const float c = spline_gradc_near(d, P.sp);
-        const float mx = P.mass * (rx * c), my = P.mass * (ry * c), mz = P.mass * (rz * c);
-        sx += mx;
-        sy += my;
-        sz += mz;
-        sq += mx * mx + my * my + mz * mz;
+        float wv, c;
+        PairMath<F>::wc(r2, P.sp, wv, c);
+        grad_terms<F>(rho, sx, sy, sz, sq, P.mass, wv, c, rx, ry, rz);
         if (c != 0.0f) {
             if (cf < L.kf) {
                 if (R.total)
@@ -243,20 +262,15 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
         for_stencil_rows(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
             const float4 q = P.xb[s];
             const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-            const float r2 = dist2(rx, ry, rz);
+            const float r2 = PairMath<F>::r2(rx, ry, rz);
             if (COUNT) {
                 ++nc.z;
                 nc.w += r2 < P.sp.far2 ? 1u : 0u;
             }
             if (r2 >= P.sp.far2) return;
-            const float d = pair_sqrt(r2, P.sp);
-            rho += q.w * spline_W_near(d, P.sp);
-            const float c = spline_gradc_near(d, P.sp);
-            const float mx = q.w * (rx * c), my = q.w * (ry * c), mz = q.w * (rz * c);
-            sx += mx;
-            sy += my;
-            sz += mz;
-            sq += mx * mx + my * my + mz * mz;
+            float wv, c;
+            PairMath<F>::wc(r2, P.sp, wv, c);
+            grad_terms<F>(rho, sx, sy, sz, sq, q.w, wv, c, rx, ry, rz);
             if (c != 0.0f) {
                 if (cb < L.kb) L.b[pair_at(cb, i, L.kb)] = make_uint2(s, __float_as_uint(c));
                 ++cb;
@@ -266,22 +280,40 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
     L.cnt[i] = pack_counts(cf, cb);
     if (COUNT) counts[i] = nc;
     rho_out[i] = rho;
-    const float denom = sx * sx + sy * sy + sz * sz + sq;
+    const float denom = F ? __builtin_fmaf(sx, sx, __builtin_fmaf(sy, sy, __builtin_fmaf(sz, sz, sq)))
+                          : sx * sx + sy * sy + sz * sz + sq;
     alpha_out[i] = (denom > 1e-9f) ? (rho / denom) : 0.0f;
 }
 
-// One XSPH pair term (dfsph_kernels.cu:326-337).
+// One XSPH pair term (dfsph_kernels.cu:326-337) for a pair at squared
+// distance r2 inside the support.
+template <bool F>
+__device__ __forceinline__ void xsph_term_r2(float& dx, float& dy, float& dz, float r2,
+                                             const float4& vi, float rho_i, const float4& vj,
+                                             float rho_j, float eps, const Nbr& P,
+                                             bool keep = true) {
+    const float w = PairMath<F>::w(r2, P.sp);
+    const float rho_avg = 0.5f * (rho_i + rho_j);
+    const float c = eps * PairMath<F>::div(P.mass, fmaxf(rho_avg, 1e-9f)) * w;
+    if (F) {
+        const float k = keep ? c : 0.0f;
+        dx = __builtin_fmaf(k, vj.x - vi.x, dx);
+        dy = __builtin_fmaf(k, vj.y - vi.y, dy);
+        dz = __builtin_fmaf(k, vj.z - vi.z, dz);
+    } else {
+        // a dropped term adds -0.0f, which leaves every value unchanged
+        dx += keep ? c * (vj.x - vi.x) : -0.0f;
+        dy += keep ? c * (vj.y - vi.y) : -0.0f;
+        dz += keep ? c * (vj.z - vi.z) : -0.0f;
+    }
+}
+template <bool F>
 __device__ __forceinline__ void xsph_term(float& dx, float& dy, float& dz, const float4& p,
                                           const float4& vi, float rho_i, const float4& q,
                                           const float4& vj, float rho_j, float eps,
                                           const Nbr& P, bool keep = true) {
-    const float w = spline_W_near(pair_sqrt(dist2(p.x - q.x, p.y - q.y, p.z - q.z), P.sp), P.sp);
-    const float rho_avg = 0.5f * (rho_i + rho_j);
-    const float c = eps * (P.mass / fmaxf(rho_avg, 1e-9f)) * w;
-    // a dropped term adds -0.0f, which leaves every value unchanged
-    dx += keep ? c * (vj.x - vi.x) : -0.0f;
-    dy += keep ? c * (vj.y - vi.y) : -0.0f;
-    dz += keep ? c * (vj.z - vi.z) : -0.0f;
+    xsph_term_r2<F>(dx, dy, dz, PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z), vi, rho_i, vj,
+                    rho_j, eps, P, keep);
 }
 
 // XSPH as a Jacobi sweep: every neighbour velocity is the pre-XSPH one.  A
@@ -289,7 +321,7 @@ __device__ __forceinline__ void xsph_term(float& dx, float& dy, float& dz, const
 // The fluid pair list holds every in-support pair in stencil order (pairs
 // outside the support add +-0); PU pairs per trip have their loads issued
 // together.  Particles without a complete list walk the stencil.
-template <int NT, int PU>
+template <int NT, int PU, bool F>
 __global__ __launch_bounds__(NT) void k_xsph(Nbr P, PairList L, const float4* __restrict__ xs,
                                              const float4* __restrict__ vin,
                                              const float* __restrict__ rho,
@@ -325,12 +357,13 @@ __global__ __launch_bounds__(NT) void k_xsph(Nbr P, PairList L, const float4* __
                 rj[u] = rho[j[u]];
             }
 #pragma unroll
-            for (int u = 0; u < PU; ++u) xsph_term(dx, dy, dz, p, vi, rho_i, q[u], vj[u], rj[u], eps, P);
+            for (int u = 0; u < PU; ++u)
+                xsph_term<F>(dx, dy, dz, p, vi, rho_i, q[u], vj[u], rj[u], eps, P);
         }
         for (; k < cf; ++k) {
             u32 j;
             fluid_entry(L, R, k, i, j, c_unused);
-            xsph_term(dx, dy, dz, p, vi, rho_i, xs[j], vin[j], rho[j], eps, P);
+            xsph_term<F>(dx, dy, dz, p, vi, rho_i, xs[j], vin[j], rho[j], eps, P);
         }
         vout[i] = make_float4(vi.x + dx, vi.y + dy, vi.z + dz, 0.0f);
         return;
@@ -340,15 +373,9 @@ __global__ __launch_bounds__(NT) void k_xsph(Nbr P, PairList L, const float4* __
     for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
         if (s == i) return;
         const float4 q = xs[s];
-        const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+        const float r2 = PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
         if (r2 >= P.sp.far2) return;  // W = 0
-        const float4 vj = vin[s];
-        const float w = spline_W_near(pair_sqrt(r2, P.sp), P.sp);
-        const float rho_avg = 0.5f * (rho_i + rho[s]);
-        const float c = eps * (P.mass / fmaxf(rho_avg, 1e-9f)) * w;
-        dx += c * (vj.x - vi.x);
-        dy += c * (vj.y - vi.y);
-        dz += c * (vj.z - vi.z);
+        xsph_term_r2<F>(dx, dy, dz, r2, vi, rho_i, vin[s], rho[s], eps, P);
     });
     vout[i] = make_float4(vi.x + dx, vi.y + dy, vi.z + dz, 0.0f);
 }
@@ -356,7 +383,7 @@ __global__ __launch_bounds__(NT) void k_xsph(Nbr P, PairList L, const float4* __
 // k_xsph with the tile's fluid neighbourhood staged in LDS: positions, and
 // pre-XSPH velocities with the density in w.  The same terms in the same
 // order; tiles without a staging plan gather from global memory.
-template <int PU>
+template <int PU, bool F>
 __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const float4* __restrict__ xs,
                                                    const float4* __restrict__ vin,
                                                    const float* __restrict__ rho,
@@ -411,15 +438,15 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
                 const float4 w1 = make_float4(q1.w, b1.x, b1.y, 0.0f);
                 const float4 w2 = make_float4(q2.w, b2.x, b2.y, 0.0f);
                 const float4 w3 = make_float4(q3.w, b3.x, b3.y, 0.0f);
-                xsph_term(dx, dy, dz, p, vi, rho_i, q0, w0, r0, eps, P);
-                xsph_term(dx, dy, dz, p, vi, rho_i, q1, w1, r1, eps, P, m > 1u);
-                xsph_term(dx, dy, dz, p, vi, rho_i, q2, w2, r2, eps, P, m > 2u);
-                xsph_term(dx, dy, dz, p, vi, rho_i, q3, w3, r3, eps, P, m > 3u);
+                xsph_term<F>(dx, dy, dz, p, vi, rho_i, q0, w0, r0, eps, P);
+                xsph_term<F>(dx, dy, dz, p, vi, rho_i, q1, w1, r1, eps, P, m > 1u);
+                xsph_term<F>(dx, dy, dz, p, vi, rho_i, q2, w2, r2, eps, P, m > 2u);
+                xsph_term<F>(dx, dy, dz, p, vi, rho_i, q3, w3, r3, eps, P, m > 3u);
             });
         } else {
             for (; k < cf; ++k) {
                 const u32 j = ld_pair(L.f + pair_at(k, i, L.kf)).x;
-                xsph_term(dx, dy, dz, p, vi, rho_i, xs[j], vin[j], rho[j], eps, P);
+                xsph_term<F>(dx, dy, dz, p, vi, rho_i, xs[j], vin[j], rho[j], eps, P);
             }
         }
         vout[i] = make_float4(vi.x + dx, vi.y + dy, vi.z + dz, 0.0f);
@@ -430,15 +457,9 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
     for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
         if (s == i) return;
         const float4 q = xs[s];
-        const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+        const float r2 = PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
         if (r2 >= P.sp.far2) return;  // W = 0
-        const float4 vj = vin[s];
-        const float w = spline_W_near(pair_sqrt(r2, P.sp), P.sp);
-        const float rho_avg = 0.5f * (rho_i + rho[s]);
-        const float c = eps * (P.mass / fmaxf(rho_avg, 1e-9f)) * w;
-        dx += c * (vj.x - vi.x);
-        dy += c * (vj.y - vi.y);
-        dz += c * (vj.z - vi.z);
+        xsph_term_r2<F>(dx, dy, dz, r2, vi, rho_i, vin[s], rho[s], eps, P);
     });
     vout[i] = make_float4(vi.x + dx, vi.y + dy, vi.z + dz, 0.0f);
 }
@@ -552,17 +573,31 @@ __global__ __launch_bounds__(NT) void k_tension_force(Nbr P, PairList L,
 }
 
 // rho_dot terms (dfsph_kernels.cu:210, 222) for one pair with gradient scalar c.
+template <bool F>
 __device__ __forceinline__ void rho_dot_fluid(float& rd, const float4& p, const float4& vi,
                                               const float4& q, const float4& vj, float c,
                                               float mass, bool keep = true) {
-    const float gx = (p.x - q.x) * c, gy = (p.y - q.y) * c, gz = (p.z - q.z) * c;
     const float ux = vi.x - vj.x, uy = vi.y - vj.y, uz = vi.z - vj.z;
-    rd += keep ? mass * (ux * gx + uy * gy + uz * gz) : -0.0f;  // -0.0f: no change
+    if (F) {  // m (u . r) c
+        const float dot = __builtin_fmaf(ux, p.x - q.x,
+                                         __builtin_fmaf(uy, p.y - q.y, uz * (p.z - q.z)));
+        rd = __builtin_fmaf(keep ? mass * c : 0.0f, dot, rd);
+    } else {
+        const float gx = (p.x - q.x) * c, gy = (p.y - q.y) * c, gz = (p.z - q.z) * c;
+        rd += keep ? mass * (ux * gx + uy * gy + uz * gz) : -0.0f;  // -0.0f: no change
+    }
 }
+template <bool F>
 __device__ __forceinline__ void rho_dot_boundary(float& rd, const float4& p, const float4& vi,
                                                  const float4& q, float c) {
-    const float gx = (p.x - q.x) * c, gy = (p.y - q.y) * c, gz = (p.z - q.z) * c;
-    rd += q.w * (vi.x * gx + vi.y * gy + vi.z * gz);
+    if (F) {
+        const float dot = __builtin_fmaf(vi.x, p.x - q.x,
+                                         __builtin_fmaf(vi.y, p.y - q.y, vi.z * (p.z - q.z)));
+        rd = __builtin_fmaf(q.w * c, dot, rd);
+    } else {
+        const float gx = (p.x - q.x) * c, gy = (p.y - q.y) * c, gz = (p.z - q.z) * c;
+        rd += q.w * (vi.x * gx + vi.y * gy + vi.z * gz);
+    }
 }
 
 // One predictor sweep.  MODE 0: density change + kappa (dfsph_kernels.cu:
@@ -571,7 +606,7 @@ __device__ __forceinline__ void rho_dot_boundary(float& rd, const float4& p, con
 // following k_apply and one binary64 error partial per tile: the four waves
 // of a tile are summed in a fixed order, so the partials do not depend on
 // the workgroup size.  Scenes that converged before `iter` skip the work.
-template <int MODE, int NT, int PU>
+template <int MODE, int NT, int PU, bool F>
 __global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* __restrict__ xs,
                                                const float4* __restrict__ vs,
                                                const float* __restrict__ rho,
@@ -616,7 +651,7 @@ __global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* 
                     }
 #pragma unroll
                     for (int u = 0; u < PU; ++u)
-                        rho_dot_fluid(rd, p, vi, q[u], w[u], R.total ? pair_c(p, q[u], P.sp) : c[u],
+                        rho_dot_fluid<F>(rd, p, vi, q[u], w[u], R.total ? pair_c_f<F>(p, q[u], P.sp) : c[u],
                                       P.mass);
                 }
                 for (; k < cf; ++k) {
@@ -624,7 +659,7 @@ __global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* 
                     float c;
                     fluid_entry(L, R, k, i, j, c);
                     const float4 q = xs[j];
-                    rho_dot_fluid(rd, p, vi, q, vs[j], R.total ? pair_c(p, q, P.sp) : c, P.mass);
+                    rho_dot_fluid<F>(rd, p, vi, q, vs[j], R.total ? pair_c_f<F>(p, q, P.sp) : c, P.mass);
                 }
                 k = 0;
                 for (; k + PU <= cb; k += PU) {
@@ -636,28 +671,28 @@ __global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* 
                     for (int u = 0; u < PU; ++u) q[u] = P.xb[e[u].x];
 #pragma unroll
                     for (int u = 0; u < PU; ++u)
-                        rho_dot_boundary(rd, p, vi, q[u], __uint_as_float(e[u].y));
+                        rho_dot_boundary<F>(rd, p, vi, q[u], __uint_as_float(e[u].y));
                 }
                 for (; k < cb; ++k) {
                     const uint2 e = ld_pair(L.b + pair_at(k, i, L.kb));
-                    rho_dot_boundary(rd, p, vi, P.xb[e.x], __uint_as_float(e.y));
+                    rho_dot_boundary<F>(rd, p, vi, P.xb[e.x], __uint_as_float(e.y));
                 }
             } else {
                 for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
                     if (s == i) return;
                     const float4 q = xs[s];
-                    const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+                    const float r2 = PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
                     if (r2 >= P.sp.far2) return;
-                    const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
-                    rho_dot_fluid(rd, p, vi, q, vs[s], c, P.mass);
+                    const float c = stencil_c<F>(r2, P.sp);
+                    rho_dot_fluid<F>(rd, p, vi, q, vs[s], c, P.mass);
                 });
                 if (P.nb) {
                     for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
                         const float4 q = P.xb[s];
-                        const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+                        const float r2 = PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
                         if (r2 >= P.sp.far2) return;
-                        const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
-                        rho_dot_boundary(rd, p, vi, q, c);
+                        const float c = stencil_c<F>(r2, P.sp);
+                        rho_dot_boundary<F>(rd, p, vi, q, c);
                     });
                 }
             }
@@ -692,7 +727,7 @@ __global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* 
 
 // k_change with the tile's fluid neighbourhood staged in LDS (positions and
 // velocities); the same arithmetic in the same order as k_change.
-template <int MODE, int PU>
+template <int MODE, int PU, bool F>
 __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
                                                      const float4* __restrict__ xs,
                                                      const float4* __restrict__ vs,
@@ -747,7 +782,7 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
                 // compact list: LDS entries only; c is recomputed from the staged
                 // positions with the expression that produced it
                 auto term = [&](const float4& q, const float4& w, bool keep) {
-                    rho_dot_fluid(rd, p, vi, q, w, pair_c(p, q, P.sp), P.mass, keep);
+                    rho_dot_fluid<F>(rd, p, vi, q, w, pair_c_f<F>(p, q, P.sp), P.mass, keep);
                 };
                 auto at = [&](u32 a, float4& q, float4& w) {
                     const float4 x = s_a[a];
@@ -779,11 +814,11 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
                     }
 #pragma unroll
                     for (int u = 0; u < PU; ++u)
-                        rho_dot_fluid(rd, p, vi, q[u], w[u], __uint_as_float(e[u].y), P.mass);
+                        rho_dot_fluid<F>(rd, p, vi, q[u], w[u], __uint_as_float(e[u].y), P.mass);
                 }
                 for (; k < cf; ++k) {
                     const uint2 e = ld_pair(L.f + pair_at(k, i, L.kf));
-                    rho_dot_fluid(rd, p, vi, xs[e.x], vs[e.x], __uint_as_float(e.y), P.mass);
+                    rho_dot_fluid<F>(rd, p, vi, xs[e.x], vs[e.x], __uint_as_float(e.y), P.mass);
                 }
             }
             k = 0;
@@ -796,28 +831,28 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
                 for (int u = 0; u < PU; ++u) q[u] = P.xb[e[u].x];
 #pragma unroll
                 for (int u = 0; u < PU; ++u)
-                    rho_dot_boundary(rd, p, vi, q[u], __uint_as_float(e[u].y));
+                    rho_dot_boundary<F>(rd, p, vi, q[u], __uint_as_float(e[u].y));
             }
             for (; k < cb; ++k) {
                 const uint2 e = ld_pair(L.b + pair_at(k, i, L.kb));
-                rho_dot_boundary(rd, p, vi, P.xb[e.x], __uint_as_float(e.y));
+                rho_dot_boundary<F>(rd, p, vi, P.xb[e.x], __uint_as_float(e.y));
             }
         } else {
             for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
                 if (s == i) return;
                 const float4 q = xs[s];
-                const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+                const float r2 = PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
                 if (r2 >= P.sp.far2) return;
-                const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
-                rho_dot_fluid(rd, p, vi, q, vs[s], c, P.mass);
+                const float c = stencil_c<F>(r2, P.sp);
+                rho_dot_fluid<F>(rd, p, vi, q, vs[s], c, P.mass);
             });
             if (P.nb) {
                 for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
                     const float4 q = P.xb[s];
-                    const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+                    const float r2 = PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
                     if (r2 >= P.sp.far2) return;
-                    const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
-                    rho_dot_boundary(rd, p, vi, q, c);
+                    const float c = stencil_c<F>(r2, P.sp);
+                    rho_dot_boundary<F>(rd, p, vi, q, c);
                 });
             }
         }
@@ -869,20 +904,28 @@ __device__ __forceinline__ void finish_error(int mode, u32 iter, double thr, con
 }
 
 // Pressure velocity terms (dfsph_kernels.cu:275-278, 291-294) for one pair.
+template <bool F>
 __device__ __forceinline__ void kappa_term(float& dx, float& dy, float& dz, const float4& p,
                                            const float4& q, float c, float c2, float dt,
                                            bool keep = true) {
-    const float gx = (p.x - q.x) * c, gy = (p.y - q.y) * c, gz = (p.z - q.z) * c;
-    // a dropped term subtracts +0.0f, which leaves every value unchanged
-    dx -= keep ? dt * c2 * gx : 0.0f;
-    dy -= keep ? dt * c2 * gy : 0.0f;
-    dz -= keep ? dt * c2 * gz : 0.0f;
+    if (F) {  // -dt c2 c r
+        const float k = keep ? -(dt * c2 * c) : 0.0f;
+        dx = __builtin_fmaf(k, p.x - q.x, dx);
+        dy = __builtin_fmaf(k, p.y - q.y, dy);
+        dz = __builtin_fmaf(k, p.z - q.z, dz);
+    } else {
+        const float gx = (p.x - q.x) * c, gy = (p.y - q.y) * c, gz = (p.z - q.z) * c;
+        // a dropped term subtracts +0.0f, which leaves every value unchanged
+        dx -= keep ? dt * c2 * gx : 0.0f;
+        dy -= keep ? dt * c2 * gy : 0.0f;
+        dz -= keep ? dt * c2 * gz : 0.0f;
+    }
 }
 
 // Pressure velocity update (dfsph_kernels.cu:237-301); kappa-hat of every
 // particle was precomputed by k_change with the same expression the
 // reference evaluates per pair.
-template <int NT, int PU>
+template <int NT, int PU, bool F>
 __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* __restrict__ xk,
                                               const float* __restrict__ kappa, float rho0,
                                               int pform, u32 iter, int mode, double thr,
@@ -924,7 +967,7 @@ __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* _
             for (int u = 0; u < PU; ++u) q[u] = xk[j[u]];
 #pragma unroll
             for (int u = 0; u < PU; ++u)
-                kappa_term(dx, dy, dz, p, q[u], R.total ? pair_c(p, q[u], P.sp) : c[u],
+                kappa_term<F>(dx, dy, dz, p, q[u], R.total ? pair_c_f<F>(p, q[u], P.sp) : c[u],
                            P.mass * (ki + q[u].w), dt);
         }
         for (; k < cf; ++k) {
@@ -932,7 +975,7 @@ __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* _
             float c;
             fluid_entry(L, R, k, i, j, c);
             const float4 q = xk[j];
-            kappa_term(dx, dy, dz, p, q, R.total ? pair_c(p, q, P.sp) : c, P.mass * (ki + q.w), dt);
+            kappa_term<F>(dx, dy, dz, p, q, R.total ? pair_c_f<F>(p, q, P.sp) : c, P.mass * (ki + q.w), dt);
         }
         k = 0;
         for (; k + PU <= cb; k += PU) {
@@ -944,29 +987,29 @@ __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* _
             for (int u = 0; u < PU; ++u) q[u] = P.xb[e[u].x];
 #pragma unroll
             for (int u = 0; u < PU; ++u)
-                kappa_term(dx, dy, dz, p, q[u], __uint_as_float(e[u].y), q[u].w * (ki + bf), dt);
+                kappa_term<F>(dx, dy, dz, p, q[u], __uint_as_float(e[u].y), q[u].w * (ki + bf), dt);
         }
         for (; k < cb; ++k) {
             const uint2 e = ld_pair(L.b + pair_at(k, i, L.kb));
             const float4 q = P.xb[e.x];
-            kappa_term(dx, dy, dz, p, q, __uint_as_float(e.y), q.w * (ki + bf), dt);
+            kappa_term<F>(dx, dy, dz, p, q, __uint_as_float(e.y), q.w * (ki + bf), dt);
         }
     } else {
         for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
             if (s == i) return;
             const float4 q = xk[s];
-            const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+            const float r2 = PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
             if (r2 >= P.sp.far2) return;
-            const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
-            kappa_term(dx, dy, dz, p, q, c, P.mass * (ki + q.w), dt);
+            const float c = stencil_c<F>(r2, P.sp);
+            kappa_term<F>(dx, dy, dz, p, q, c, P.mass * (ki + q.w), dt);
         });
         if (P.nb) {
             for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
                 const float4 q = P.xb[s];
-                const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+                const float r2 = PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
                 if (r2 >= P.sp.far2) return;
-                const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
-                kappa_term(dx, dy, dz, p, q, c, q.w * (ki + bf), dt);
+                const float c = stencil_c<F>(r2, P.sp);
+                kappa_term<F>(dx, dy, dz, p, q, c, q.w * (ki + bf), dt);
             });
         }
     }
@@ -978,7 +1021,7 @@ __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* _
 }
 
 // k_apply with the tile's fluid (x, kappa-hat) neighbourhood staged in LDS.
-template <int PU>
+template <int PU, bool F>
 __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
                                                     const float4* __restrict__ xk,
                                                     const float* __restrict__ kappa, float rho0,
@@ -1028,7 +1071,7 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
         u32 k = 0;
         if (staged) {
             auto term = [&](const float4& q, bool keep) {
-                kappa_term(dx, dy, dz, p, q, pair_c(p, q, P.sp), P.mass * (ki + q.w), dt, keep);
+                kappa_term<F>(dx, dy, dz, p, q, pair_c_f<F>(p, q, P.sp), P.mass * (ki + q.w), dt, keep);
             };
             for_stage_quads(L, i, cf, [&](u32 a0, u32 a1, u32 a2, u32 a3, u32 m) {
                 const float4 q0 = s_x[a0], q1 = s_x[a1], q2 = s_x[a2], q3 = s_x[a3];
@@ -1047,12 +1090,12 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
                 for (int u = 0; u < PU; ++u) q[u] = xk[e[u].x];
 #pragma unroll
                 for (int u = 0; u < PU; ++u)
-                    kappa_term(dx, dy, dz, p, q[u], __uint_as_float(e[u].y), P.mass * (ki + q[u].w), dt);
+                    kappa_term<F>(dx, dy, dz, p, q[u], __uint_as_float(e[u].y), P.mass * (ki + q[u].w), dt);
             }
             for (; k < cf; ++k) {
                 const uint2 e = ld_pair(L.f + pair_at(k, i, L.kf));
                 const float4 q = xk[e.x];
-                kappa_term(dx, dy, dz, p, q, __uint_as_float(e.y), P.mass * (ki + q.w), dt);
+                kappa_term<F>(dx, dy, dz, p, q, __uint_as_float(e.y), P.mass * (ki + q.w), dt);
             }
         }
         k = 0;
@@ -1065,29 +1108,29 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
             for (int u = 0; u < PU; ++u) q[u] = P.xb[e[u].x];
 #pragma unroll
             for (int u = 0; u < PU; ++u)
-                kappa_term(dx, dy, dz, p, q[u], __uint_as_float(e[u].y), q[u].w * (ki + bf), dt);
+                kappa_term<F>(dx, dy, dz, p, q[u], __uint_as_float(e[u].y), q[u].w * (ki + bf), dt);
         }
         for (; k < cb; ++k) {
             const uint2 e = ld_pair(L.b + pair_at(k, i, L.kb));
             const float4 q = P.xb[e.x];
-            kappa_term(dx, dy, dz, p, q, __uint_as_float(e.y), q.w * (ki + bf), dt);
+            kappa_term<F>(dx, dy, dz, p, q, __uint_as_float(e.y), q.w * (ki + bf), dt);
         }
     } else {
         for_stencil(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
             if (s == i) return;
             const float4 q = xk[s];
-            const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+            const float r2 = PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
             if (r2 >= P.sp.far2) return;
-            const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
-            kappa_term(dx, dy, dz, p, q, c, P.mass * (ki + q.w), dt);
+            const float c = stencil_c<F>(r2, P.sp);
+            kappa_term<F>(dx, dy, dz, p, q, c, P.mass * (ki + q.w), dt);
         });
         if (P.nb) {
             for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
                 const float4 q = P.xb[s];
-                const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+                const float r2 = PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
                 if (r2 >= P.sp.far2) return;
-                const float c = spline_gradc(pair_sqrt(r2, P.sp), P.sp);
-                kappa_term(dx, dy, dz, p, q, c, q.w * (ki + bf), dt);
+                const float c = stencil_c<F>(r2, P.sp);
+                kappa_term<F>(dx, dy, dz, p, q, c, q.w * (ki + bf), dt);
             });
         }
     }
@@ -1376,30 +1419,43 @@ __global__ __launch_bounds__(NS) void k_reset_solve(const SceneSeg* __restrict__
 // ---- launch-shape dispatch ------------------------------------------------
 u32 grid_for(u32 nslots, u32 nt) { return div_up(nslots, nt); }
 
-void launch_density_alpha(u32 nt, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
-                          const float4* xs, float* rho, float* alpha) {
+template <bool F>
+void density_alpha_mode(u32 nt, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
+                        const float4* xs, float* rho, float* alpha) {
     switch (nt) {
-    case 64: k_density_alpha<64><<<grid_for(nslots, 64), 64, 0, s>>>(P, L, xs, rho, alpha); break;
-    case 128: k_density_alpha<128><<<grid_for(nslots, 128), 128, 0, s>>>(P, L, xs, rho, alpha); break;
-    case 256: k_density_alpha<256><<<grid_for(nslots, 256), 256, 0, s>>>(P, L, xs, rho, alpha); break;
-    case 512: k_density_alpha<512><<<grid_for(nslots, 512), 512, 0, s>>>(P, L, xs, rho, alpha); break;
+    case 64: k_density_alpha<64, F><<<grid_for(nslots, 64), 64, 0, s>>>(P, L, xs, rho, alpha); break;
+    case 128: k_density_alpha<128, F><<<grid_for(nslots, 128), 128, 0, s>>>(P, L, xs, rho, alpha); break;
+    case 256: k_density_alpha<256, F><<<grid_for(nslots, 256), 256, 0, s>>>(P, L, xs, rho, alpha); break;
+    case 512: k_density_alpha<512, F><<<grid_for(nslots, 512), 512, 0, s>>>(P, L, xs, rho, alpha); break;
     default: fail(kInvalidArgument, "density_alpha workgroup size");
     }
     WSIM_LAUNCH_CHECK();
 }
+// fast: the tolerance-mode pair arithmetic (PairMath<true>)
+void launch_density_alpha(u32 nt, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
+                          const float4* xs, float* rho, float* alpha, bool fast) {
+    if (fast) density_alpha_mode<true>(nt, s, nslots, P, L, xs, rho, alpha);
+    else density_alpha_mode<false>(nt, s, nslots, P, L, xs, rho, alpha);
+}
 
-void launch_xsph(u32 nt, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
-                 const float4* xs, const float4* vin, const float* rho, float4* vout) {
+template <bool F>
+void xsph_mode(u32 nt, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
+               const float4* xs, const float4* vin, const float* rho, float4* vout) {
     constexpr int XPU = 4;
     switch (nt) {
-    case 0: k_xsph_st<XPU><<<nslots / kTile, kTile, 0, s>>>(P, L, xs, vin, rho, vout); break;
-    case 64: k_xsph<64, XPU><<<grid_for(nslots, 64), 64, 0, s>>>(P, L, xs, vin, rho, vout); break;
-    case 128: k_xsph<128, XPU><<<grid_for(nslots, 128), 128, 0, s>>>(P, L, xs, vin, rho, vout); break;
-    case 256: k_xsph<256, XPU><<<grid_for(nslots, 256), 256, 0, s>>>(P, L, xs, vin, rho, vout); break;
-    case 512: k_xsph<512, XPU><<<grid_for(nslots, 512), 512, 0, s>>>(P, L, xs, vin, rho, vout); break;
+    case 0: k_xsph_st<XPU, F><<<nslots / kTile, kTile, 0, s>>>(P, L, xs, vin, rho, vout); break;
+    case 64: k_xsph<64, XPU, F><<<grid_for(nslots, 64), 64, 0, s>>>(P, L, xs, vin, rho, vout); break;
+    case 128: k_xsph<128, XPU, F><<<grid_for(nslots, 128), 128, 0, s>>>(P, L, xs, vin, rho, vout); break;
+    case 256: k_xsph<256, XPU, F><<<grid_for(nslots, 256), 256, 0, s>>>(P, L, xs, vin, rho, vout); break;
+    case 512: k_xsph<512, XPU, F><<<grid_for(nslots, 512), 512, 0, s>>>(P, L, xs, vin, rho, vout); break;
     default: fail(kInvalidArgument, "xsph workgroup size");
     }
     WSIM_LAUNCH_CHECK();
+}
+void launch_xsph(u32 nt, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
+                 const float4* xs, const float4* vin, const float* rho, float4* vout, bool fast) {
+    if (fast) xsph_mode<true>(nt, s, nslots, P, L, xs, vin, rho, vout);
+    else xsph_mode<false>(nt, s, nslots, P, L, xs, vin, rho, vout);
 }
 
 struct ChangeArgs {
@@ -1415,54 +1471,61 @@ struct ChangeArgs {
     float* rho_adv;
     float* kappa;
     float4* xk;
+    bool fast;  // tolerance-mode pair arithmetic
 };
 
-template <int MODE, int NT, int PU>
+template <int MODE, int NT, int PU, bool F>
 void change_one(hipStream_t s, u32 nslots, const Nbr& P, const PairList& L, const ChangeArgs& a) {
-    k_change<MODE, NT, PU><<<grid_for(nslots, NT), NT, 0, s>>>(P, L, a.xs, a.vs, a.rho, a.alpha,
-                                                               a.rho0, a.pform, a.iter, a.st,
-                                                               a.part, a.rho_adv, a.kappa, a.xk);
+    k_change<MODE, NT, PU, F><<<grid_for(nslots, NT), NT, 0, s>>>(
+        P, L, a.xs, a.vs, a.rho, a.alpha, a.rho0, a.pform, a.iter, a.st, a.part, a.rho_adv,
+        a.kappa, a.xk);
 }
 
-template <int MODE, int NT>
+template <int MODE, int NT, bool F>
 void change_pu(u32 pu, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
                const ChangeArgs& a) {
     switch (pu) {
-    case 2: change_one<MODE, NT, 2>(s, nslots, P, L, a); break;
-    case 4: change_one<MODE, NT, 4>(s, nslots, P, L, a); break;
-    case 8: change_one<MODE, NT, 8>(s, nslots, P, L, a); break;
+    case 2: change_one<MODE, NT, 2, F>(s, nslots, P, L, a); break;
+    case 4: change_one<MODE, NT, 4, F>(s, nslots, P, L, a); break;
+    case 8: change_one<MODE, NT, 8, F>(s, nslots, P, L, a); break;
     default: fail(kInvalidArgument, "change pairs per trip");
     }
 }
 
-template <int MODE, int PU>
+template <int MODE, int PU, bool F>
 void change_staged(hipStream_t s, u32 nslots, const Nbr& P, const PairList& L, const ChangeArgs& a) {
-    k_change_st<MODE, PU><<<nslots / kTile, kTile, 0, s>>>(P, L, a.xs, a.vs, a.rho, a.alpha, a.rho0,
-                                                          a.pform, a.iter, a.st, a.part, a.rho_adv,
-                                                          a.kappa, a.xk);
+    k_change_st<MODE, PU, F><<<nslots / kTile, kTile, 0, s>>>(
+        P, L, a.xs, a.vs, a.rho, a.alpha, a.rho0, a.pform, a.iter, a.st, a.part, a.rho_adv,
+        a.kappa, a.xk);
 }
 
 // nt == 0 selects the LDS-staged kernel (one 256-slot tile per workgroup).
-template <int MODE>
-void launch_change(u32 nt, u32 pu, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
-                   const ChangeArgs& a) {
+template <int MODE, bool F>
+void change_mode(u32 nt, u32 pu, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
+                 const ChangeArgs& a) {
     if (nt == 0) {
         switch (pu) {
-        case 2: change_staged<MODE, 2>(s, nslots, P, L, a); break;
-        case 4: change_staged<MODE, 4>(s, nslots, P, L, a); break;
-        case 8: change_staged<MODE, 8>(s, nslots, P, L, a); break;
+        case 2: change_staged<MODE, 2, F>(s, nslots, P, L, a); break;
+        case 4: change_staged<MODE, 4, F>(s, nslots, P, L, a); break;
+        case 8: change_staged<MODE, 8, F>(s, nslots, P, L, a); break;
         default: fail(kInvalidArgument, "change pairs per trip");
         }
         WSIM_LAUNCH_CHECK();
         return;
     }
     switch (nt) {
-    case 256: change_pu<MODE, 256>(pu, s, nslots, P, L, a); break;
-    case 512: change_pu<MODE, 512>(pu, s, nslots, P, L, a); break;
-    case 1024: change_pu<MODE, 1024>(pu, s, nslots, P, L, a); break;
+    case 256: change_pu<MODE, 256, F>(pu, s, nslots, P, L, a); break;
+    case 512: change_pu<MODE, 512, F>(pu, s, nslots, P, L, a); break;
+    case 1024: change_pu<MODE, 1024, F>(pu, s, nslots, P, L, a); break;
     default: fail(kInvalidArgument, "change workgroup size");
     }
     WSIM_LAUNCH_CHECK();
+}
+template <int MODE>
+void launch_change(u32 nt, u32 pu, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
+                   const ChangeArgs& a) {
+    if (a.fast) change_mode<MODE, true>(nt, pu, s, nslots, P, L, a);
+    else change_mode<MODE, false>(nt, pu, s, nslots, P, L, a);
 }
 
 struct ApplyArgs {
@@ -1477,53 +1540,61 @@ struct ApplyArgs {
     DeviceState* st;
     const float4* vin;
     float4* vout;
+    bool fast;  // tolerance-mode pair arithmetic
 };
 
-template <int NT, int PU>
+template <int NT, int PU, bool F>
 void apply_one(hipStream_t s, u32 nslots, const Nbr& P, const PairList& L, const ApplyArgs& a) {
-    k_apply<NT, PU><<<grid_for(nslots, NT), NT, 0, s>>>(P, L, a.xk, a.kappa, a.rho0, a.pform,
-                                                        a.iter, a.mode, a.thr, a.part, a.st, a.vin,
-                                                        a.vout);
+    k_apply<NT, PU, F><<<grid_for(nslots, NT), NT, 0, s>>>(P, L, a.xk, a.kappa, a.rho0, a.pform,
+                                                           a.iter, a.mode, a.thr, a.part, a.st,
+                                                           a.vin, a.vout);
 }
 
-template <int NT>
+template <int NT, bool F>
 void apply_pu(u32 pu, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
               const ApplyArgs& a) {
     switch (pu) {
-    case 2: apply_one<NT, 2>(s, nslots, P, L, a); break;
-    case 4: apply_one<NT, 4>(s, nslots, P, L, a); break;
-    case 8: apply_one<NT, 8>(s, nslots, P, L, a); break;
+    case 2: apply_one<NT, 2, F>(s, nslots, P, L, a); break;
+    case 4: apply_one<NT, 4, F>(s, nslots, P, L, a); break;
+    case 8: apply_one<NT, 8, F>(s, nslots, P, L, a); break;
     default: fail(kInvalidArgument, "apply pairs per trip");
     }
 }
 
-template <int PU>
+template <int PU, bool F>
 void apply_staged(hipStream_t s, u32 nslots, const Nbr& P, const PairList& L, const ApplyArgs& a) {
-    k_apply_st<PU><<<nslots / kTile, kTile, 0, s>>>(P, L, a.xk, a.kappa, a.rho0, a.pform, a.iter,
-                                                    a.mode, a.thr, a.part, a.st, a.vin, a.vout);
+    k_apply_st<PU, F><<<nslots / kTile, kTile, 0, s>>>(P, L, a.xk, a.kappa, a.rho0, a.pform,
+                                                       a.iter, a.mode, a.thr, a.part, a.st, a.vin,
+                                                       a.vout);
 }
 
 // nt == 0 selects the LDS-staged kernel.
-void launch_apply(u32 nt, u32 pu, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
-                  const ApplyArgs& a) {
+template <bool F>
+void apply_mode(u32 nt, u32 pu, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
+                const ApplyArgs& a) {
     if (nt == 0) {
         switch (pu) {
-        case 2: apply_staged<2>(s, nslots, P, L, a); break;
-        case 4: apply_staged<4>(s, nslots, P, L, a); break;
-        case 8: apply_staged<8>(s, nslots, P, L, a); break;
+        case 2: apply_staged<2, F>(s, nslots, P, L, a); break;
+        case 4: apply_staged<4, F>(s, nslots, P, L, a); break;
+        case 8: apply_staged<8, F>(s, nslots, P, L, a); break;
         default: fail(kInvalidArgument, "apply pairs per trip");
         }
         WSIM_LAUNCH_CHECK();
         return;
     }
     switch (nt) {
-    case 64: apply_pu<64>(pu, s, nslots, P, L, a); break;
-    case 128: apply_pu<128>(pu, s, nslots, P, L, a); break;
-    case 256: apply_pu<256>(pu, s, nslots, P, L, a); break;
-    case 512: apply_pu<512>(pu, s, nslots, P, L, a); break;
+    case 64: apply_pu<64, F>(pu, s, nslots, P, L, a); break;
+    case 128: apply_pu<128, F>(pu, s, nslots, P, L, a); break;
+    case 256: apply_pu<256, F>(pu, s, nslots, P, L, a); break;
+    case 512: apply_pu<512, F>(pu, s, nslots, P, L, a); break;
     default: fail(kInvalidArgument, "apply workgroup size");
     }
     WSIM_LAUNCH_CHECK();
+}
+void launch_apply(u32 nt, u32 pu, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
+                  const ApplyArgs& a) {
+    if (a.fast) apply_mode<true>(nt, pu, s, nslots, P, L, a);
+    else apply_mode<false>(nt, pu, s, nslots, P, L, a);
 }
 
 // dfsph.cu:55-65
@@ -2011,7 +2082,7 @@ void DFSPHSolver::phase_prefix_grid() {
     WSIM_LAUNCH_CHECK();
     tbegin(kKDensityAlpha);
     launch_density_alpha(shapes_.density_alpha_nt, stream_, nslots_, P, pair_list(), xs_.data(),
-                         rho_.data(), alpha_.data());
+                         rho_.data(), alpha_.data(), cfg_.fast_math != 0);
     tend();
 }
 
@@ -2031,7 +2102,7 @@ void DFSPHSolver::phase_prefix_solve(bool gravity) {
     if (xsph) {
         tbegin(kKXsph);
         launch_xsph(shapes_.xsph_nt, stream_, nslots_, P, pair_list(), xs_.data(), vpre_.data(),
-                    rho_.data(), vs_.data());
+                    rho_.data(), vs_.data(), cfg_.fast_math != 0);
         tend();
     }
     if (gravity && cfg_.enable_surface_tension) {
@@ -2080,7 +2151,7 @@ void DFSPHSolver::neighbor_counts(u32* cand_f, u32* nbr_f, u32* cand_b, u32* nbr
     k_stage_plan<<<div_up(nslots_ / kTile, NS), NS, 0, stream_>>>(P, plan_.data());
     WSIM_LAUNCH_CHECK();
     DeviceBuffer<uint4> counts(std::max<u32>(nslots_, 1u));
-    k_density_alpha<NS, true><<<nslots_ / NS, NS, 0, stream_>>>(P, pair_list(), xs_.data(),
+    k_density_alpha<NS, false, true><<<nslots_ / NS, NS, 0, stream_>>>(P, pair_list(), xs_.data(),
                                                                 rho_.data(), alpha_.data(),
                                                                 counts.data());
     WSIM_LAUNCH_CHECK();
@@ -2108,9 +2179,10 @@ void DFSPHSolver::phase_iterations(int mode, u32 begin, u32 end) {
                                  : static_cast<double>(cfg_.eta_divergence * cfg_.rest_density);
     ChangeArgs ca{xs_.data(), vs_.data(), rho_.data(), alpha_.data(), cfg_.rest_density,
                   static_cast<int>(cfg_.pressure_form), 0, dstate_.data(), part_d_.data(),
-                  rho_adv_.data(), kappa_.data(), xk_.data()};
+                  rho_adv_.data(), kappa_.data(), xk_.data(), cfg_.fast_math != 0};
     ApplyArgs aa{xk_.data(), kappa_.data(), cfg_.rest_density, static_cast<int>(cfg_.pressure_form),
-                 0, mode, thr, part_d_.data(), dstate_.data(), vs_.data(), vs_.data()};
+                 0, mode, thr, part_d_.data(), dstate_.data(), vs_.data(), vs_.data(),
+                 cfg_.fast_math != 0};
     for (u32 it = begin; it < end; ++it) {
         ca.iter = aa.iter = it;
         tbegin(mode == 0 ? kKDensityChange : kKDivChange, static_cast<int>(it));
@@ -2327,9 +2399,9 @@ void DFSPHSolver::iterations_taped(u32 begin, u32 end, SubstepTape& t) {
     const size_t ns = nslots_;
     ChangeArgs ca{xs_.data(), nullptr, rho_.data(), alpha_.data(), cfg_.rest_density,
                   static_cast<int>(cfg_.pressure_form), 0, dstate_.data(), part_d_.data(),
-                  nullptr, kappa_.data(), nullptr};
+                  nullptr, kappa_.data(), nullptr, false};
     ApplyArgs aa{nullptr, kappa_.data(), cfg_.rest_density, static_cast<int>(cfg_.pressure_form), 0, 0,
-                 thr, part_d_.data(), dstate_.data(), nullptr, nullptr};
+                 thr, part_d_.data(), dstate_.data(), nullptr, nullptr, false};
     for (u32 it = begin; it < end; ++it) {
         ca.iter = aa.iter = it;
         ca.vs = t.w.data() + it * ns;
@@ -2352,6 +2424,9 @@ void DFSPHSolver::step_taped(float dt, SubstepTape& t) {
     if (cfg_.enable_divergence_solve || cfg_.enable_surface_tension)
         fail(kInvalidArgument,
              "step_taped: the adjoint covers the density solve without surface tension");
+    if (cfg_.fast_math)
+        fail(kInvalidArgument,
+             "step_taped: the adjoint differentiates the bit-exact pair arithmetic (fast_math 0)");
     const size_t nsz = fluid_.count();
     if (nsz == 0) fail(kInvalidArgument, "step_taped: empty store");
     if (counts_.empty() || counts_[0] != nsz) layout({static_cast<u32>(nsz)});
@@ -2447,10 +2522,10 @@ void DFSPHSolver::autotune(int reps) {
     };
     ChangeArgs ca{xs_.data(), vs_.data(), rho_.data(), alpha_.data(), cfg_.rest_density,
                   static_cast<int>(cfg_.pressure_form), 0, dstate_.data(), part_d_.data(),
-                  rho_adv_.data(), kappa_.data(), xk_.data()};
+                  rho_adv_.data(), kappa_.data(), xk_.data(), cfg_.fast_math != 0};
     ApplyArgs aa{xk_.data(), kappa_.data(), cfg_.rest_density, static_cast<int>(cfg_.pressure_form),
                  0, 0, static_cast<double>(cfg_.eta_density * cfg_.rest_density), part_d_.data(),
-                 dstate_.data(), vs_.data(), vs_.data()};
+                 dstate_.data(), vs_.data(), vs_.data(), cfg_.fast_math != 0};
     try {
         // cell_start pass
         // cell_start entries: C + 1 for one scene, scenes * (C + 1) + 1 segmented
@@ -2473,7 +2548,7 @@ void DFSPHSolver::autotune(int reps) {
             for (u32 nt : {64u, 128u, 256u, 512u}) {
                 const double ms = time_ms([&] {
                     launch_density_alpha(nt, stream_, nslots_, P, L, xs_.data(), rho_.data(),
-                                         alpha_.data());
+                                         alpha_.data(), cfg_.fast_math != 0);
                 });
                 tune_log_.push_back({kKDensityAlpha, nt, 0, remap, 0, ms});
                 if (ms < b_da) b_da = ms, cand.density_alpha_nt = nt;
@@ -2482,7 +2557,7 @@ void DFSPHSolver::autotune(int reps) {
                 for (u32 nt : {0u, 64u, 128u, 256u, 512u}) {
                     const double ms = time_ms([&] {
                         launch_xsph(nt, stream_, nslots_, P, L, xs_.data(), vpre_.data(),
-                                    rho_.data(), vs_.data());
+                                    rho_.data(), vs_.data(), cfg_.fast_math != 0);
                     });
                     tune_log_.push_back({kKXsph, nt, 0, remap, 0, ms});
                     if (ms < b_xs) b_xs = ms, cand.xsph_nt = nt;

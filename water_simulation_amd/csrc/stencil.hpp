@@ -275,6 +275,19 @@ __device__ __forceinline__ void for_stage_quads(const PairList& L, u32 i, u32 cf
 __device__ __forceinline__ float pair_c(const float4& p, const float4& q, const Spline& sp) {
     return spline_gradc_near(pair_sqrt(dist2(p.x - q.x, p.y - q.y, p.z - q.z), sp), sp);
 }
+// pair_c in either pair-math mode (the list build and the list readers of a
+// mode use the same expression, so a recomputed c equals the recorded one)
+template <bool F>
+__device__ __forceinline__ float pair_c_f(const float4& p, const float4& q, const Spline& sp) {
+    if (F) return PairMath<true>::c(PairMath<true>::r2(p.x - q.x, p.y - q.y, p.z - q.z), sp);
+    return pair_c(p, q, sp);
+}
+// Gradient factor of a stencil candidate inside the support (r2 < far2).
+template <bool F>
+__device__ __forceinline__ float stencil_c(float r2, const Spline& sp) {
+    if (F) return PairMath<true>::c(r2, sp);
+    return spline_gradc(pair_sqrt(r2, sp), sp);
+}
 
 // The staging entries of one thread: tid, tid + 256, tid + 512, ...  src
 // holds their working slots (entries past the plan's end repeat its last

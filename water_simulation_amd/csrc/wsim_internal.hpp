@@ -226,6 +226,10 @@ struct DFSPHConfig {
     u32 enable_divergence_solve = 0;
     u32 pressure_form = 0;  // 0: kappa/rho^2 (reference), 1: kappa/rho
     u32 enable_surface_tension = 0;  // Akinci 2013 cohesion + curvature, gamma = surface_tension
+    // Tolerance mode of the neighbour kernels (PairMath<true>): hardware
+    // rsqrt/rcp and fma contraction on the pair terms, as the reference's
+    // -use_fast_math build; 0 keeps every field bit-exact to the oracle.
+    u32 fast_math = 0;
 };
 
 struct StepStats {

@@ -316,6 +316,7 @@ public:
         bool enable_divergence_solve = false;  // extension
         u32 pressure_form = 0;                 // extension: 0 kappa/rho^2, 1 kappa/rho
         bool enable_surface_tension = false;   // extension: Akinci 2013 with surface_tension
+        bool fast_math = false;                // extension: tolerance-mode pair arithmetic
     };
 
     DFSPHSolver(ParticleStore& fluid, ParticleStore* boundary, Config cfg, Stream stream = nullptr) {
@@ -343,6 +344,7 @@ public:
         c.enable_divergence_solve = cfg.enable_divergence_solve ? 1u : 0u;
         c.pressure_form = cfg.pressure_form;
         c.enable_surface_tension = cfg.enable_surface_tension ? 1u : 0u;
+        c.fast_math = cfg.fast_math ? 1u : 0u;
         detail::check(wsim_solver_create(fluid.native(), boundary ? boundary->native() : nullptr,
                                          &c, stream, &h_));
     }
