@@ -113,6 +113,20 @@ void wsim_grid_cells_per_axis(wsim_grid grid, int32_t out[3]);
 float wsim_grid_cell_length(wsim_grid grid);
 uint32_t wsim_grid_total_cells(wsim_grid grid);
 size_t wsim_grid_count(wsim_grid grid);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The result of wsim_grid_create + wsim_grid_build (cell keys,
+ * stable (key, index) sort, cell_start) written to caller arrays:
+ * sorted_keys and sorted_indices hold count entries, cell_start
+ * cells_per_axis[0] * [1] * [2] + 1 (all zero for count = 0).  Allocates
+ * nothing: the sort's scratch lives in a fixed per-device array (18 MiB,
+ * held until the process ends).  At most 2,097,152 particles
+ * (WSIM_ERR_INVALID_ARGUMENT above that).  Calls that overlap on one device
+ * race on the scratch: serialize them (one stream, or events). */
+wsim_status wsim_cell_sort_noalloc(const float* positions_xyz, size_t count,
+                                   const float origin[3], const int32_t cells_per_axis[3],
+                                   float cell_length, uint32_t* sorted_keys,
+                                   uint32_t* sorted_indices, uint32_t* cell_start, void* stream);
 
 /* ---- SpatialAccel (core/include/water/core/spatial_accel.h:19-53) ---- */
 typedef struct wsim_accel_t* wsim_accel;

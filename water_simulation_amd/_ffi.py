@@ -193,6 +193,7 @@ SIGNATURES = {
     "wsim_grid_destroy": (None, [vp]),
     "wsim_grid_reserve": (st, [vp, sz]),
     "wsim_grid_build": (st, [vp, vp, sz]),
+    "wsim_cell_sort_noalloc": (st, [vp, sz, F3, I3, C.c_float, vp, vp, vp, vp]),
     "wsim_grid_cell_start": (vp, [vp]),
     "wsim_grid_sorted_indices": (vp, [vp]),
     "wsim_grid_sorted_keys": (vp, [vp]),

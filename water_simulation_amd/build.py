@@ -40,7 +40,7 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 
-HIP_SOURCES = ["radix_sort.hip", "grid.hip", "morton.hip", "store.hip", "solver.hip", "adjoint.hip",
+HIP_SOURCES = ["radix_sort.hip", "grid.hip", "grid_noalloc.hip", "morton.hip", "store.hip", "solver.hip", "adjoint.hip",
                "copy.hip", "pair_ops.hip", "surface.hip"]
 HOST_SOURCES = ["host_surface.cpp", "capi.cpp", "frames.cpp"]
 

@@ -263,6 +263,17 @@ wsim_status wsim_grid_build(wsim_grid grid, const float* positions_xyz, size_t c
         grid->mut()->build(positions_xyz, count);
     });
 }
+wsim_status wsim_cell_sort_noalloc(const float* positions_xyz, size_t count, const float origin[3],
+                                   const int32_t cells_per_axis[3], float cell_length,
+                                   uint32_t* sorted_keys, uint32_t* sorted_indices,
+                                   uint32_t* cell_start, void* stream) {
+    return guarded([&] {
+        need(origin, "origin");
+        need(cells_per_axis, "cells_per_axis");
+        wsim::cell_sort_noalloc(positions_xyz, count, origin, cells_per_axis, cell_length,
+                                sorted_keys, sorted_indices, cell_start, as_stream(stream));
+    });
+}
 const uint32_t* wsim_grid_cell_start(wsim_grid g) { return g ? g->view->cell_start() : nullptr; }
 const uint32_t* wsim_grid_sorted_indices(wsim_grid g) {
     return (g && g->view->capacity()) ? g->view->sorted_indices() : nullptr;

@@ -114,6 +114,14 @@ private:
 // Plain 16-byte streaming copy kernel (copy.hip).
 void device_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
 
+// CellGrid::build into caller arrays (sorted keys and indices: count
+// entries; cell_start: C + 1) with its scratch in a fixed per-device array:
+// allocates nothing.  At most 2^21 particles; calls that overlap on one
+// device race on the scratch (grid_noalloc.hip).
+void cell_sort_noalloc(const float* xyz, size_t count, const float origin[3],
+                       const int32_t dims[3], float cell_length, u32* sorted_keys,
+                       u32* sorted_indices, u32* cell_start, hipStream_t stream);
+
 // Fluid surface mesh from particle positions (surface.hip): splat, Gaussian
 // blur, iso level, marching cubes (tools/particles_to_mesh.py:77-138).
 struct MeshParams {
