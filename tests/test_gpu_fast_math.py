@@ -99,6 +99,16 @@ def test_fast_math_random_box(seed, pform):
     compare_substeps(x, v, xb, cfg, [1e-3] * 4, f"random box {seed} form {pform}")
 
 
+@pytest.mark.parametrize("pform", [0, 1])
+def test_fast_math_block_on_the_floor(pform):
+    # the block rests on the wall layers: every bottom particle has boundary
+    # neighbours, whose terms the tolerance mode sums once per substep
+    x, v, xb, cfg = random_box(n=3000, seed=7, pressure_form=pform)
+    x[:, 1] -= np.float32(0.14)
+    v[:, 1] -= np.float32(0.3)
+    compare_substeps(x, v, xb, cfg, [1e-3] * 4, f"block on the floor, form {pform}")
+
+
 def test_fast_math_no_boundary_and_divergence_solve():
     x, v, _, cfg = random_box(n=3000, seed=5, boundary=False, pressure_form=1,
                               enable_divergence_solve=True, max_divergence_iters=10)

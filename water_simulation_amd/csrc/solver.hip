@@ -229,7 +229,7 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
     bool unlisted = false;  // an in-support fluid pair with a zero gradient
     uint4 nc = make_uint4(0u, 0u, 0u, 0u);  // COUNT: candidates, neighbours (fluid, boundary)
     const StageRanges R = L.plan[i / kTile];
-    for_stencil_rows(P.gf, cs, shift, p.x, p.y, p.z, [&](u32 s) {
+    auto fluid_pair = [&](u32 s) {
         const float4 q = xs[s];
         const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
         const float r2 = PairMath<F>::r2(rx, ry, rz);
@@ -253,13 +253,16 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
         } else if (s != i) {
             unlisted = true;
         }
-    });
+    };
+    if (F) for_stencil_zrows(P.gf, cs, shift, p.x, p.y, p.z, fluid_pair);
+    else for_stencil_rows(P.gf, cs, shift, p.x, p.y, p.z, fluid_pair);
     // XSPH sums over every in-support pair (W != 0) and reads the fluid list;
     // a pair the list leaves out (coincident particles) sends this particle
     // to the stencil walk in every list kernel
     if (unlisted) cf = 0xFFFFu;
+    float gbx = 0.0f, gby = 0.0f, gbz = 0.0f;  // tolerance mode: sum_b m_b grad W_ib
     if (P.nb && boundary_near(P.gb, P.bnear, p.x, p.y, p.z)) {
-        for_stencil_rows(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
+        auto boundary_pair = [&](u32 s) {
             const float4 q = P.xb[s];
             const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
             const float r2 = PairMath<F>::r2(rx, ry, rz);
@@ -271,12 +274,20 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
             float wv, c;
             PairMath<F>::wc(r2, P.sp, wv, c);
             grad_terms<F>(rho, sx, sy, sz, sq, q.w, wv, c, rx, ry, rz);
-            if (c != 0.0f) {
+            if (F && L.gb) {
+                const float mc = q.w * c;
+                gbx = __builtin_fmaf(rx, mc, gbx);
+                gby = __builtin_fmaf(ry, mc, gby);
+                gbz = __builtin_fmaf(rz, mc, gbz);
+            } else if (c != 0.0f) {
                 if (cb < L.kb) L.b[pair_at(cb, i, L.kb)] = make_uint2(s, __float_as_uint(c));
                 ++cb;
             }
-        });
+        };
+        if (F) for_stencil_zrows(P.gb, P.csb, 0u, p.x, p.y, p.z, boundary_pair);
+        else for_stencil_rows(P.gb, P.csb, 0u, p.x, p.y, p.z, boundary_pair);
     }
+    if (F && L.gb) L.gb[i] = make_float4(gbx, gby, gbz, 0.0f);
     L.cnt[i] = pack_counts(cf, cb);
     if (COUNT) counts[i] = nc;
     rho_out[i] = rho;
@@ -573,6 +584,11 @@ __global__ __launch_bounds__(NT) void k_tension_force(Nbr P, PairList L,
 }
 
 // rho_dot terms (dfsph_kernels.cu:210, 222) for one pair with gradient scalar c.
+// v . Gb (tolerance mode's boundary term of the density change)
+__device__ __forceinline__ float dot_gb(const float4& v, const float4& g) {
+    return __builtin_fmaf(v.x, g.x, __builtin_fmaf(v.y, g.y, v.z * g.z));
+}
+
 template <bool F>
 __device__ __forceinline__ void rho_dot_fluid(float& rd, const float4& p, const float4& vi,
                                               const float4& q, const float4& vj, float c,
@@ -686,7 +702,7 @@ __global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* 
                     const float c = stencil_c<F>(r2, P.sp);
                     rho_dot_fluid<F>(rd, p, vi, q, vs[s], c, P.mass);
                 });
-                if (P.nb) {
+                if (P.nb && !(F && L.gb)) {
                     for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
                         const float4 q = P.xb[s];
                         const float r2 = PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
@@ -696,6 +712,7 @@ __global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* 
                     });
                 }
             }
+            if (F && L.gb) rd += dot_gb(vi, L.gb[i]);
             const float rho_i = rho[i];
             float kap, err;
             if (MODE == 0) {
@@ -846,7 +863,7 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
                 const float c = stencil_c<F>(r2, P.sp);
                 rho_dot_fluid<F>(rd, p, vi, q, vs[s], c, P.mass);
             });
-            if (P.nb) {
+            if (P.nb && !(F && L.gb)) {
                 for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
                     const float4 q = P.xb[s];
                     const float r2 = PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
@@ -856,6 +873,7 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
                 });
             }
         }
+        if (F && L.gb) rd += dot_gb(vi, L.gb[i]);
         const float rho_i = rho[i];
         float kap, err;
         if (MODE == 0) {
@@ -1003,7 +1021,7 @@ __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* _
             const float c = stencil_c<F>(r2, P.sp);
             kappa_term<F>(dx, dy, dz, p, q, c, P.mass * (ki + q.w), dt);
         });
-        if (P.nb) {
+        if (P.nb && !(F && L.gb)) {
             for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
                 const float4 q = P.xb[s];
                 const float r2 = PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
@@ -1012,6 +1030,13 @@ __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* _
                 kappa_term<F>(dx, dy, dz, p, q, c, q.w * (ki + bf), dt);
             });
         }
+    }
+    if (F && L.gb) {  // sum_b m_b (k_i + k_b) grad W_ib = (k_i + k_b) Gb_i
+        const float4 g = L.gb[i];
+        const float k = -(dt * (ki + bf));
+        dx = __builtin_fmaf(k, g.x, dx);
+        dy = __builtin_fmaf(k, g.y, dy);
+        dz = __builtin_fmaf(k, g.z, dz);
     }
     float4 v = vin[i];
     v.x += dx;
@@ -1124,7 +1149,7 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
             const float c = stencil_c<F>(r2, P.sp);
             kappa_term<F>(dx, dy, dz, p, q, c, P.mass * (ki + q.w), dt);
         });
-        if (P.nb) {
+        if (P.nb && !(F && L.gb)) {
             for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
                 const float4 q = P.xb[s];
                 const float r2 = PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
@@ -1133,6 +1158,13 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
                 kappa_term<F>(dx, dy, dz, p, q, c, q.w * (ki + bf), dt);
             });
         }
+    }
+    if (F && L.gb) {  // sum_b m_b (k_i + k_b) grad W_ib = (k_i + k_b) Gb_i
+        const float4 g = L.gb[i];
+        const float k = -(dt * (ki + bf));
+        dx = __builtin_fmaf(k, g.x, dx);
+        dy = __builtin_fmaf(k, g.y, dy);
+        dz = __builtin_fmaf(k, g.z, dz);
     }
     float4 v = vin[i];
     v.x += dx;
@@ -1826,6 +1858,7 @@ void DFSPHSolver::init(const std::vector<u32>& counts, const std::vector<float>&
             WSIM_HIP(hipStreamSynchronize(stream_));
         }
     }
+    if (cfg_.fast_math && nb_) gb_ = DeviceBuffer<float4>(slots_cap);
     if (batched_) layout(counts);
     WSIM_HIP(hipStreamSynchronize(stream_));
 }
@@ -1905,6 +1938,7 @@ PairList DFSPHSolver::pair_list() {
     L.plan = plan_.data();
     L.kf = kPairDepthFluid;
     L.kb = nb_ ? kPairDepthBoundary : 0u;
+    L.gb = (cfg_.fast_math && nb_) ? gb_.data() : nullptr;
     return L;
 }
 

@@ -97,6 +97,37 @@ __device__ __forceinline__ void for_stencil_rows(const GridGeom& g, const u32* _
     }
 }
 
+// The same 27 cells walked as nine z-contiguous rows (dx, dy in {-1, 0, 1},
+// cells cz - 1 .. cz + 1 of a row are consecutive keys, so each row is one
+// slot range): the same candidates in another order, for the tolerance
+// mode, whose sums need no fixed order.  A wave's trip count per row is the
+// maximum over its lanes of the row's candidates, which varies far less
+// than the per-cell maximum of the reference-order walk (a lattice at the
+// cell spacing leaves 0, 1, 2, 4 or 8 particles per cell).
+template <class F>
+__device__ __forceinline__ void for_stencil_zrows(const GridGeom& g, const u32* __restrict__ cs,
+                                                  u32 shift, float px, float py, float pz, F&& f) {
+    const int cx = g.cx(px), cy = g.cy(py), cz = g.cz(pz);
+    const int lo = cz - 1 < 0 ? 0 : cz - 1;
+    const int hi = cz + 1 > g.dz - 1 ? g.dz - 1 : cz + 1;
+    u32 b[9], e[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const int nx = cx + r / 3 - 1, ny = cy + r % 3 - 1;
+        const bool in = lo <= hi && nx >= 0 && nx < g.dx && ny >= 0 && ny < g.dy;
+        const u32 row = in ? (static_cast<u32>(nx) * static_cast<u32>(g.dy) + static_cast<u32>(ny)) *
+                                 static_cast<u32>(g.dz)
+                           : 0u;
+        const u32 bk = cs[in ? row + static_cast<u32>(lo) : 0u];
+        const u32 ek = cs[in ? row + static_cast<u32>(hi) + 1u : 0u];
+        b[r] = in ? bk : 0u;
+        e[r] = in ? ek : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+        for (u32 s = b[r]; s < e[r]; ++s) f(s + shift);
+}
+
 // Boundary proximity: bit k of the mask is set when the 27-cell stencil of
 // boundary-grid cell k holds a boundary particle.  A fluid particle whose
 // home cell has a clear bit has no boundary candidates, so the boundary walk

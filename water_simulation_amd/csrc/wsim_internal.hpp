@@ -361,6 +361,11 @@ struct PairList {
     uint16_t* fi;        // fluid pairs as LDS entries of the tile's staging plan (staged tiles)
     const StageRanges* plan;  // one per 256-slot tile
     u32 kf, kb;          // list depths
+    // Tolerance mode: per slot, sum_b m_b grad W_ib over the particle's
+    // boundary neighbours (positions are frozen through the solves, so the
+    // boundary terms of every pressure sweep are v_i . Gb_i and
+    // (k_i + k_b) Gb_i); no boundary pair lists are kept then.
+    float4* gb;
 };
 
 // Depth of the per-particle pair lists (pairs with a non-zero kernel
@@ -553,6 +558,7 @@ private:
     DeviceBuffer<uint2> pairs_f_, pairs_b_;
     DeviceBuffer<uint16_t> pairs_fi_;
     DeviceBuffer<float4> nrm_;  // surface-tension normals (when enabled)
+    DeviceBuffer<float4> gb_;   // boundary gradient sums (tolerance mode with a boundary)
     DeviceBuffer<u32> inv_;     // working slot of each original index (plain solver)
     DeviceBuffer<u32> bnear_;  // boundary proximity mask (see stencil.hpp)
     DeviceBuffer<StageRanges> plan_;
