@@ -10,11 +10,18 @@ damping 0.5, eta = 0.1 %, at most 100 density iterations, CFL time step
 (grid build, density/alpha, gravity, XSPH, density solve, advection, max |v|).
 
 The headline runs the pressure solve with kappa/rho scaling (Bender and
-Koschier), which meets eta in ~3 iterations; the reference's kappa/rho^2
-scaling never reaches eta (it stops at the 100-iteration cap with a ~5 %
-mean density error) and is timed beside it as "reference_pressure_form".
-Further legs: the 64-scene batched mix against sequential plain solvers, and
-the adjoint of the inverse-problem rollout against its forward pass.
+Koschier), which meets eta in ~3 iterations, with the tolerance mode of the
+neighbour kernels (fast_math: hardware reciprocal square root and
+reciprocal and fused multiply-adds in the pair terms, as the reference's own
+-use_fast_math build evaluates them; every field within 1e-5 relative of the
+binary64 oracle per substep, see tests/test_gpu_fast_math.py).  Beside it:
+the bit-exact IEEE path on the same substeps ("exact_mode"), and the
+reference's kappa/rho^2 scaling ("reference_pressure_form"), which never
+reaches eta (it stops at the 100-iteration cap with a ~2.5 % mean density
+error).  Further legs: the 64-scene batched mix against sequential plain
+solvers (kappa/rho, scenes leave the solve at different iterations; and the
+kappa/rho^2 form at its iteration cap), and the adjoint of the
+inverse-problem rollout against its forward pass.
 
 Prints one JSON line.  value = particle-substeps/s with all inputs resident
 in HBM; roofline = the dominant kernel's algorithmic bytes per launch over
@@ -116,7 +123,7 @@ def cpu_baseline(x, v, xb, cfg, dts, budget_s: float):
     }, done, iters
 
 
-def dam_break(ws, x, xb, cfg, steps: int, warmup: int, timing: bool) -> dict:
+def dam_break(ws, x, xb, cfg, steps: int, warmup: int, timing: bool, graphs: bool = True) -> dict:
     """The large dam break from rest: `warmup` substeps, then `steps` timed
     substeps (CFL dt, lambda = 0.4, velocity floor 0.5 m/s).  The throughput
     is timed with graph replay and the pipelined substep loop; with `timing`
@@ -131,7 +138,7 @@ def dam_break(ws, x, xb, cfg, steps: int, warmup: int, timing: bool) -> dict:
     bnd.resize(nb)
     bnd.set_positions(xb)
     solver = ws.DFSPHSolver(fluid, bnd, ws.DFSPHConfig(**cfg))
-    solver.set_graphs(True)
+    solver.set_graphs(graphs)
     ts = ws.TimeStepper()
     # sim_cli's substep loop (dt = next_dt(max(max|v|, 0.5), r, remaining))
     # with a time budget that never binds: `steps` CFL substeps
@@ -169,11 +176,15 @@ def dam_break(ws, x, xb, cfg, steps: int, warmup: int, timing: bool) -> dict:
     return out
 
 
-def batched_leg(steps: int, warmup: int, tune_reps: int):
+def batched_leg(steps: int, warmup: int, tune_reps: int, pressure_form: int = 1,
+                fast_math: bool = False, graphs: bool = True, eager_sequential: bool = True):
     """64 uneven box scenes (seeded mix, ~1.05M fluid particles in total): one
     BatchSolver against 64 plain DFSPHSolvers run one after another, both on
-    the same per-scene dt schedule; the sequential side is timed with and
-    without graph replay and the speedup is taken against the faster."""
+    the same per-scene dt schedule and the same eta; the sequential side is
+    timed with graph replay (and, with eager_sequential, without it) and the
+    speedup is taken against the faster.  With kappa/rho scaling every scene
+    leaves its density solve at its own iteration (the batch runs until the
+    slowest scene of each substep); the per-scene spread is reported."""
     import water_simulation_amd as ws
     from water_simulation_amd import scenes as SC
 
@@ -181,7 +192,8 @@ def batched_leg(steps: int, warmup: int, tune_reps: int):
     counts = [len(s.positions) for s in mix]
     total = int(sum(counts))
     xb = ws.sample_aabb_boundary((0, 0, 0), (SC.BOX,) * 3, SC.WALL_SPACING, SC.WALL_LAYERS)
-    cfg = ws.DFSPHConfig(**SC.box_config())
+    over = dict(pressure_form=pressure_form, fast_math=fast_math)
+    cfg = ws.DFSPHConfig(**SC.box_config(**over))
     ts = ws.TimeStepper()
 
     def walls():
@@ -199,9 +211,10 @@ def batched_leg(steps: int, warmup: int, tune_reps: int):
     fluid.set_velocities(np.concatenate([s.velocities for s in mix]))
     bw = walls()
     batch = ws.BatchSolver(fluid, bw, cfg, counts, [s.viscosity for s in mix])
+    batch.set_graphs(graphs)
     tune = batch.autotune(reps=tune_reps) if tune_reps > 0 else []
     vmax = [SC.initial_max_speed(s.velocities) for s in mix]
-    sched, iters = [], []
+    sched, per_scene = [], []
     ws.device_synchronize()
     t_b = 0.0
     for k in range(warmup + steps):
@@ -215,20 +228,22 @@ def batched_leg(steps: int, warmup: int, tune_reps: int):
         st = batch.all_stats()
         vmax = [s["max_velocity"] for s in st]
         if k >= warmup:
-            iters.append(np.mean([s["density_iters"] for s in st]))
+            per_scene.append([s["density_iters"] for s in st])
     xb_final = fluid.get_positions()
+    err_final = [s["mean_density_error"] for s in batch.all_stats()]
     sched = np.array(sched)
+    it = np.array(per_scene, np.float64)  # (substeps, scenes)
 
-    def sequential(graphs: bool):
+    def sequential(seq_graphs: bool):
         t, same = 0.0, True
         for s, scene in enumerate(mix):
             f = ws.ParticleStore(counts[s])
             f.resize(counts[s])
             f.set_positions(scene.positions)
             f.set_velocities(scene.velocities)
-            c = ws.DFSPHConfig(**SC.box_config(viscosity=scene.viscosity))
+            c = ws.DFSPHConfig(**SC.box_config(viscosity=scene.viscosity, **over))
             p = ws.DFSPHSolver(f, walls(), c)
-            p.set_graphs(graphs)
+            p.set_graphs(seq_graphs)
             for k in range(warmup + steps):
                 t0 = time.perf_counter()
                 p.step(float(sched[k, s]))
@@ -241,24 +256,32 @@ def batched_leg(steps: int, warmup: int, tune_reps: int):
             p.close()
         return t, same
 
-    t_eager, same_e = sequential(False)
-    t_graph, same_g = sequential(True)
-    t_seq = min(t_eager, t_graph)
+    t_graph, same_g = sequential(graphs)
+    t_eager, same_e = sequential(False) if (eager_sequential and graphs) else (None, True)
+    t_seq = min(t for t in (t_graph, t_eager) if t is not None)
+    eta = float(cfg.eta_density * cfg.rest_density)
     return {
         "workload": f"uneven batched mix: 64 box scenes (0.6 m, r=0.005 m, walls 0.01 m x 3 "
                     f"layers), {min(counts)}..{max(counts)} particles per scene, {total} total, "
                     f"v ~ N(0, 0.5 m/s), XSPH in [0.005, 0.05], eta=0.1%",
+        "pressure_form": "kappa/rho" if pressure_form == 1 else "reference kappa/rho^2",
+        "fast_math": bool(fast_math),
+        "graphs": bool(graphs),
         "scenes": 64,
         "particles": total,
         "substeps": steps,
         "warmup": warmup,
         "batched_s": t_b,
-        "sequential_eager_s": t_eager,
         "sequential_graphs_s": t_graph,
+        "sequential_eager_s": t_eager,
         "speedup": t_seq / t_b,
         "batched_particle_substeps_per_s": total * steps / t_b,
         "sequential_particle_substeps_per_s": total * steps / t_seq,
-        "mean_density_iters": float(np.mean(iters)),
+        "mean_density_iters": float(it.mean()),
+        "iters_per_substep_min_mean_max": [float(it.min(1).mean()), float(it.mean()),
+                                           float(it.max(1).mean())],
+        "substeps_with_uneven_exit": float(np.mean(it.min(1) != it.max(1))),
+        "scenes_meeting_eta_last_substep": int(sum(e < eta for e in err_final)),
         "bitwise_equal_to_sequential": bool(same_e and same_g),
         "shapes": batch.shapes(),
         "autotune_candidates": len(tune),
@@ -389,16 +412,29 @@ def main() -> int:
     ap.add_argument("--n-side", type=int, default=100)
     ap.add_argument("--pressure-form", type=int, default=1,
                     help="1: kappa/rho (meets eta; the headline), 0: the reference's kappa/rho^2")
-    ap.add_argument("--fast-math", type=int, default=0,
-                    help="1: tolerance-mode pair arithmetic of the neighbour kernels (hardware "
-                         "rsqrt/rcp and fma, as the reference's -use_fast_math build)")
+    ap.add_argument("--fast-math", type=int, default=1,
+                    help="1 (headline): tolerance-mode pair arithmetic of the neighbour kernels "
+                         "(hardware rsqrt/rcp and fma, as the reference's -use_fast_math build); "
+                         "0: the bit-exact IEEE path")
+    ap.add_argument("--exact-steps", type=int, default=-1,
+                    help="substeps timed on the bit-exact path beside a tolerance-mode headline "
+                         "(-1: as --steps, 0: skip)")
+    ap.add_argument("--mid-steps", type=int, default=20,
+                    help="substeps timed on the mid dam break (128,000 particles; 0: skip)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every kernel eagerly instead of replaying captured HIP graphs "
+                         "(for profilers whose graph-launch interception fails)")
     ap.add_argument("--reference-steps", type=int, default=20,
                     help="substeps timed with the reference kappa/rho^2 form (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--kernel-times", action="store_true", help="print per-kernel table to stderr")
-    ap.add_argument("--batch-steps", type=int, default=20,
-                    help="substeps per scene in the batched-vs-sequential leg (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=300,
+                    help="substeps per scene in the batched-vs-sequential leg with kappa/rho "
+                         "(scenes leave their solves at different iterations; 0: skip it)")
+    ap.add_argument("--batch-capped-steps", type=int, default=20,
+                    help="substeps per scene of the same comparison with the reference's "
+                         "kappa/rho^2 form, every scene at the iteration cap (0: skip it)")
     ap.add_argument("--batch-warmup", type=int, default=3)
     ap.add_argument("--tune-reps", type=int, default=3)
     ap.add_argument("--only-batch", action="store_true", help="run only the batched leg")
@@ -453,15 +489,17 @@ def main() -> int:
     if args.only_adjoint:
         print(json.dumps({"adjoint": adjoint_leg(args.adjoint_substeps, 20)}))
         return 0
+    graphs = not args.no_graphs
     if args.only_batch:
         print(json.dumps({"batched": batched_leg(args.batch_steps, args.batch_warmup,
-                                                 args.tune_reps)}))
+                                                 args.tune_reps, 1, bool(args.fast_math),
+                                                 graphs)}))
         return 0
 
     x, xb, cfg = make_workload(args.n_side)
     cfg = dict(cfg, fast_math=bool(args.fast_math))
     run = dam_break(ws, x, xb, dict(cfg, pressure_form=args.pressure_form), args.steps,
-                    args.warmup, timing=True)
+                    args.warmup, timing=True, graphs=graphs)
     n, nb = len(x), len(xb)
     value, elapsed, st, kt, iters, dts = (run[k] for k in ("value", "elapsed", "stats",
                                                             "kernel_times", "iters", "dts"))
@@ -473,15 +511,53 @@ def main() -> int:
     launches, total_ms = timed[dom]
     avg_s = total_ms / launches / 1e3
     achieved = ALG_BYTES[dom] * n / avg_s / 1e9
-    traffic = None
-    tfile = ROOT / "profiles" / f"traffic_{dom}.json"
+    # HBM bytes per launch come from rocprofv3 counter runs (FETCH_SIZE and
+    # WRITE_SIZE, calibrated on a copy kernel of known size), which cannot
+    # run inside this process: they are read from the committed summary of
+    # this kernel in this pair-arithmetic mode, named in traffic_source
+    traffic, traffic_source = None, None
+    tfile = ROOT / "profiles" / f"traffic_{dom}{'_fast' if args.fast_math else ''}.json"
     if tfile.exists():
         traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+        traffic_source = f"{tfile.relative_to(ROOT)} (rocprofv3 FETCH_SIZE/WRITE_SIZE runs, " \
+                         "not this process)"
+
+    exact_mode = None
+    exact_steps = args.steps if args.exact_steps < 0 else args.exact_steps
+    if args.fast_math and exact_steps > 0:
+        ex = dam_break(ws, x, xb, dict(cfg, pressure_form=args.pressure_form, fast_math=False),
+                       exact_steps, args.warmup, timing=False, graphs=graphs)
+        exact_mode = {
+            "fast_math": False,
+            "value": ex["value"],
+            "ms_per_step": 1e3 * ex["elapsed"] / exact_steps,
+            "steps": exact_steps,
+            "mean_density_iters": float(np.mean(ex["iters"])),
+            "final_mean_density_error": ex["stats"]["mean_density_error"],
+            "eta_met_fraction_of_substeps": ex["eta_met_fraction"],
+            "note": "bit-exact IEEE binary32 path (equal to the binary32 oracle), same workload",
+        }
+
+    mid = None
+    if args.mid_steps > 0:
+        xm, xbm, cfgm = make_mid_workload()
+        mrun = {}
+        for fm in ([True, False] if args.fast_math else [False]):
+            r = dam_break(ws, xm, xbm, dict(cfgm, pressure_form=args.pressure_form, fast_math=fm),
+                          args.mid_steps, args.warmup, timing=False, graphs=graphs)
+            mrun["tolerance_mode" if fm else "exact_mode"] = {
+                "value": r["value"], "ms_per_step": 1e3 * r["elapsed"] / args.mid_steps,
+                "mean_density_iters": float(np.mean(r["iters"])),
+                "eta_met_fraction_of_substeps": r["eta_met_fraction"]}
+        mid = {"workload": f"mid dam break: {len(xm)} fluid + {len(xbm)} boundary particles, "
+                           "r=0.005 m, 0.4x0.8x0.4 m column in a 1.6x0.9x0.8 m tank, CFL substeps",
+               "steps": args.mid_steps, "warmup": args.warmup,
+               "unit": "particle-substeps/s", **mrun}
 
     reference_form = None
     if args.reference_steps > 0 and args.pressure_form != 0:
         ref = dam_break(ws, x, xb, dict(cfg, pressure_form=0), args.reference_steps, 10,
-                        timing=False)
+                        timing=False, graphs=graphs)
         rs = ref["stats"]
         reference_form = {
             "pressure_form": "reference kappa/rho^2",
@@ -504,7 +580,12 @@ def main() -> int:
 
     batched = None
     if args.batch_steps > 0:
-        batched = batched_leg(args.batch_steps, args.batch_warmup, args.tune_reps)
+        batched = batched_leg(args.batch_steps, args.batch_warmup, args.tune_reps, 1,
+                              bool(args.fast_math), graphs, eager_sequential=False)
+    batched_capped = None
+    if args.batch_capped_steps > 0:
+        batched_capped = batched_leg(args.batch_capped_steps, args.batch_warmup, args.tune_reps,
+                                     0, bool(args.fast_math), graphs, eager_sequential=False)
 
     adjoint = None
     if args.adjoint_substeps > 0:
@@ -534,6 +615,10 @@ def main() -> int:
             "n_fluid": n,
             "n_boundary": nb,
             "pressure_form": "reference kappa/rho^2" if args.pressure_form == 0 else "kappa/rho",
+            "pair_arithmetic": ("tolerance mode (fast_math: hardware rsqrt/rcp, fma; per-substep "
+                                "fields within 1e-5 relative of the binary64 oracle)"
+                                if args.fast_math else "bit-exact IEEE binary32"),
+            "graph_replay": graphs,
             "max_density_iters": cfg["max_density_iters"],
             "eta_density": cfg["eta_density"],
             "mean_density_iters": float(np.mean(iters)),
@@ -552,6 +637,10 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS,
             "traffic": traffic,
+            "traffic_source": traffic_source,
+            "limiter": "VALU issue and gather/LDS latency of the pair loops, not HBM bandwidth "
+                       "(profiles/large_dam_break_kernel_counters.md); algorithmic bytes count "
+                       "each particle's own arrays once and no neighbour gathers",
             "alg_bytes_per_launch": ALG_BYTES[dom] * n,
             "mean_launch_us": avg_s * 1e6,
             "launches": launches,
@@ -559,8 +648,11 @@ def main() -> int:
                       "over the same timed substeps from the same state",
         },
         "cpu_baseline": cpu,
+        "exact_mode": exact_mode,
+        "mid_dam_break": mid,
         "reference_pressure_form": reference_form,
         "batched": batched,
+        "batched_reference_form_capped": batched_capped,
         "adjoint": adjoint,
         "lbvh": lbvh_leg(x, args.lbvh_reps) if args.lbvh_reps > 0 else None,
         "mesh": (mesh_leg(run["x_end"], cfg["particle_radius"], args.mesh_reps,
