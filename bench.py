@@ -639,7 +639,7 @@ def main() -> int:
             "traffic": traffic,
             "traffic_source": traffic_source,
             "limiter": "VALU issue and gather/LDS latency of the pair loops, not HBM bandwidth "
-                       "(profiles/large_dam_break_kernel_counters.md); algorithmic bytes count "
+                       "(profiles/large_dam_break_kernel_traffic.md); algorithmic bytes count "
                        "each particle's own arrays once and no neighbour gathers",
             "alg_bytes_per_launch": ALG_BYTES[dom] * n,
             "mean_launch_us": avg_s * 1e6,
