@@ -7,13 +7,16 @@ cpu_baseline leg of bench.py -- never by the product package.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 from typing import Optional
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-ORACLE_LIB = HERE / "liboracle.so"
+# WSIM_ORACLE_LIB selects another build of the oracle (the mutation suite's
+# planted-mistake builds); the default is the one build() makes.
+ORACLE_LIB = Path(os.environ.get("WSIM_ORACLE_LIB", str(HERE / "liboracle.so")))
 REF_LIB = HERE / "_ref" / "libref_host.so"
 
 F32, F64, DUAL = 32, 64, 33

@@ -24,6 +24,14 @@
 //    dfsph.cu:288-300).
 #include "wsim_oracle.h"
 
+// Mutation hooks for tests/test_oracle_mutations.py: a build with
+// -DORC_MUTANT=k plants mistake k (a plausible slip in one formula) so that
+// the suite can show that the pinning tests reject it.  0 (the default) is
+// the restatement itself.
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -140,6 +148,7 @@ template <class R> inline R W(R r, R l) {
     const R sigma = R(1) / (kpi<R>() * l * l * l);
     if (q < R(1)) return sigma * (R(1) - R(1.5) * q * q + R(0.75) * q * q * q);
     const R t = R(2) - q;
+    if (ORC_MUTANT == 1) return sigma * R(0.25) * t * t;  // mutant: (2 - q)^2
     return sigma * R(0.25) * t * t * t;
 }
 
@@ -158,6 +167,7 @@ template <class R> inline V3<R> gradW(const V3<R>& rij, R l) {
         const R t = R(2) - q;
         dWdq = sigma * R(-0.75) * t * t;
     }
+    if (ORC_MUTANT == 2 && q < R(1)) dWdq = sigma * (R(-3) * q + R(2.5) * q * q);  // mutant: 2.5 q^2
     const R c = dWdq / (l * r);
     return {rij.x * c, rij.y * c, rij.z * c};
 }
@@ -248,7 +258,7 @@ template <class R> struct Grid {
         const int cz = hc ? hc[2] : cell_coord(p.z, oz, inv, D[2]);
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
+                for (int dx = -1; dx <= (ORC_MUTANT == 3 ? 0 : 1); ++dx) {  // mutant 3: no +x column
                     const int nx = cx + dx, ny = cy + dy, nz = cz + dz;
                     if (nx < 0 || nx >= D[0] || ny < 0 || ny >= D[1] || nz < 0 || nz >= D[2])
                         continue;
@@ -422,7 +432,7 @@ template <class R> struct Solver {
                     sq += mg.x * mg.x + mg.y * mg.y + mg.z * mg.z;
                 }, hc(i));
             }
-            const R denom = sg.x * sg.x + sg.y * sg.y + sg.z * sg.z + sq;
+            const R denom = (ORC_MUTANT == 4 ? R(0) : sg.x * sg.x + sg.y * sg.y + sg.z * sg.z) + sq;
             alpha[i] = (denom > k1em9<R>()) ? (rho[i] / denom) : R(0);
         }
     }
@@ -462,6 +472,7 @@ template <class R> struct Solver {
                 const R w = W(d, h);
                 const R rho_avg = R(0.5) * (rho_i + rho[j]);
                 const R c = eps * (mass / o_fmax(rho_avg, k1em9<R>())) * w;
+                if (ORC_MUTANT == 8) { dl.x += c * vj.x; dl.y += c * vj.y; dl.z += c * vj.z; return; }
                 dl.x += c * (vj.x - vi.x);
                 dl.y += c * (vj.y - vi.y);
                 dl.z += c * (vj.z - vi.z);
@@ -551,7 +562,8 @@ template <class R> struct Solver {
             const V3<R> pj = x[j];
             const V3<R> vj = v[j];
             const V3<R> g = gradW(V3<R>{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z}, h);
-            const V3<R> dv{vi.x - vj.x, vi.y - vj.y, vi.z - vj.z};
+            const V3<R> dv = ORC_MUTANT == 5 ? V3<R>{vj.x - vi.x, vj.y - vi.y, vj.z - vi.z}
+                                             : V3<R>{vi.x - vj.x, vi.y - vj.y, vi.z - vj.z};
             rd += mass * (dv.x * g.x + dv.y * g.y + dv.z * g.z);
         }, hc(i));
         if (nb > 0) {
@@ -603,8 +615,8 @@ template <class R> struct Solver {
             const V3<R> pi = x[i];
             const R rho_i = rho[i];
             const R k_i = kap[i];
-            const R ki = bk ? k_i / o_fmax(rho_i, k1em9<R>())
-                            : k_i / o_fmax(rho_i * rho_i, k1em9<R>());
+            const R ki = (bk || ORC_MUTANT == 6) ? k_i / o_fmax(rho_i, k1em9<R>())
+                                                 : k_i / o_fmax(rho_i * rho_i, k1em9<R>());
             V3<R> dl{0, 0, 0};
             fg.for_neighbors(pi, [&](u32 j) {
                 if (j == u32(i)) return;
@@ -620,8 +632,9 @@ template <class R> struct Solver {
                 dl.z -= dtr * c2 * g.z;
             }, hc(i));
             if (nb > 0) {
-                const R bf = bk ? k_i / o_fmax(rho0, k1em9<R>())
-                                : k_i / o_fmax(rho0 * rho0, k1em9<R>());
+                const R bf = ORC_MUTANT == 7 ? R(0)
+                             : bk ? k_i / o_fmax(rho0, k1em9<R>())
+                                  : k_i / o_fmax(rho0 * rho0, k1em9<R>());
                 bg.for_neighbors(pi, [&](u32 j) {
                     const V3<R> pj = xb[j];
                     const V3<R> g = gradW(V3<R>{pi.x - pj.x, pi.y - pj.y, pi.z - pj.z}, h);

@@ -320,3 +320,84 @@ def test_regression_diagnostics_closed_forms():
     assert st["kinetic_energy"] == pytest.approx(0.5 * m * (vs ** 2).sum(), rel=1e-14)
     out = s.get(O.FIELD_X)
     assert out[3, 0] == float(np.float32(1e-3))  # domain_min + 1e-3f
+
+
+def dwdq(q, sig):
+    """dW/dq of the cubic spline (sph_kernels.cuh:25-41), binary64."""
+    return sig * (-3.0 * q + 2.25 * q * q) if q < 1.0 else -0.75 * sig * (2.0 - q) ** 2
+
+
+def test_density_change_closed_form_for_a_linear_velocity_field():
+    """rho-dot (dfsph_kernels.cu:196-224) of an interior particle of an ideal
+    lattice (spacing = cell = l) under v(x) = A x + u: with grad W = r c(|r|),
+    sum_j m (A r_j) . r_j c_j = m tr(A) S, S = l^2 (2 c(l) + 8 c(sqrt2 l) +
+    8 c(sqrt3 l)) by the lattice's cubic symmetry, whatever the constant u.
+    One density iteration (max_density_iters = 1) leaves rho* = rho +
+    dt rho-dot in the predicted density.  A flipped (v_j - v_i), a dropped
+    mass, W in place of grad W or a missing neighbour shell fails it."""
+    h = 1.0 / 64
+    x = lattice(9, h)
+    A = np.array([[0.3, 0.1, -0.2], [0.05, -0.7, 0.4], [0.2, 0.15, 0.25]])
+    u = np.array([0.4, -1.1, 0.25])
+    v = x.astype(np.float64) @ A.T + u
+    dt = 1e-3
+    s = O.Solver(cfg_box(h, 0.5, max_density_iters=1), x, precision=O.F64)
+    s.set_state_f64(x.astype(np.float64), v)
+    s.step(dt)
+    centre = (4 * 9 + 4) * 9 + 4
+    rho, rho_adv = s.get(O.FIELD_RHO)[centre], s.get(O.FIELD_RHO_ADV)[centre]
+    m = 1000.0 * h ** 3
+    sig = 1.0 / (PI_F * h ** 3)
+    c = lambda q: dwdq(q, sig) / (h * h * q)  # dW/dq / (l d), d = q l
+    S = h * h * (2 * c(1.0) + 8 * c(S2) + 8 * c(S3))
+    rho_dot = m * np.trace(A) * S
+    assert (rho_adv - rho) / float(np.float32(dt)) == pytest.approx(rho_dot, rel=1e-9)
+    assert abs(rho_dot) > 1.0  # a non-trivial rate
+
+
+@pytest.mark.parametrize("pform", [0, 1])
+def test_boundary_pressure_mirror_term_closed_form(pform):
+    """Akinci boundary terms of density, alpha, kappa and the pressure
+    velocity update (dfsph_kernels.cu:95-107, 145-163, 226-231, 281-296) for a
+    lone fluid particle at height l above a one-layer wall lattice (spacing =
+    cell = l), directly over a wall particle: its nine wall neighbours lie at
+    l (1), sqrt2 l (4) and sqrt3 l (4), all at height l.  Boundary masses are
+    rho0 / (sigma (2 + (2 - sqrt2)^3)) (interior of the layer); with v = 0 and
+    g = 0 the first sweep applies
+        dv = -dt m_b (k_i / rho^x + k_i / rho0^x) sum_b grad W_ib,   x = 2 (form 0), 1 (form 1),
+    the second term being the mirror pressure of the wall.  A dropped mirror
+    term, a wrong power of rho or rho0, or a sign fails it."""
+    h = 1.0 / 64
+    i = np.arange(11, dtype=np.float64)
+    g = np.stack(np.meshgrid(i, [0.0], i, indexing="ij"), -1).reshape(-1, 3)
+    xb = ((g + 0.5) * h + np.array([5 * h, 5 * h, 5 * h])).astype(np.float32)  # plane at cell y = 5
+    xp = np.array([[10.5 * h, 6.5 * h, 10.5 * h]], np.float32)                 # over wall point (5, 5)
+    dt = 1e-3
+    # spacing 2r = 1.4 l: the particle's own mass puts rho above rho0
+    cfg = cfg_box(h, 0.5, particle_radius=0.7 * h, max_density_iters=1, pressure_form=pform)
+    s = O.Solver(cfg, xp, xb=xb, precision=O.F64)
+    s.step(dt)
+    sig = 1.0 / (PI_F * h ** 3)
+    rho0 = 1000.0
+    r32 = float(np.float32(0.7 * h))
+    m = rho0 * (2 * r32) ** 3
+    Wq = lambda q: sig * (1 - 1.5 * q * q + 0.75 * q ** 3) if q < 1 else 0.25 * sig * (2 - q) ** 3
+    mb = rho0 / (Wq(0) + 4 * Wq(1.0) + 4 * Wq(S2))
+    assert s.get(O.FIELD_BMASS)[5 * 11 + 5] == pytest.approx(mb, rel=1e-12)
+    rho = m * Wq(0) + mb * (Wq(1.0) + 4 * Wq(S2) + 4 * Wq(S3))
+    c = lambda q: dwdq(q, sig) / (h * h * q)
+    G = mb * h * (c(1.0) + 4 * c(S2) + 4 * c(S3))  # y component; x and z cancel
+    Q = mb * mb * (c(1.0) ** 2 * h * h + 4 * c(S2) ** 2 * 2 * h * h + 4 * c(S3) ** 2 * 3 * h * h)
+    alpha = rho / (G * G + Q)
+    dtf = float(np.float32(dt))  # the solver's time step is binary32
+    kappa = max(0.0, rho - rho0) * alpha / (dtf * dtf)
+    p = 2 if pform == 0 else 1
+    dv = -dtf * (kappa / rho ** p + kappa / rho0 ** p) * G
+    assert rho > rho0 * 1.05
+    assert s.get(O.FIELD_RHO)[0] == pytest.approx(rho, rel=1e-12)
+    assert s.get(O.FIELD_ALPHA)[0] == pytest.approx(alpha, rel=1e-10)
+    assert s.get(O.FIELD_KAPPA)[0] == pytest.approx(kappa, rel=1e-10)
+    vout = s.get(O.FIELD_V)[0]
+    assert vout[1] == pytest.approx(dv, rel=1e-10)
+    assert dv > 0  # the wall pushes the compressed particle up
+    assert abs(vout[0]) < 1e-12 * abs(dv) and abs(vout[2]) < 1e-12 * abs(dv)
