@@ -123,13 +123,14 @@ def cpu_baseline(x, v, xb, cfg, dts, budget_s: float):
     }, done, iters
 
 
-def dam_break(ws, x, xb, cfg, steps: int, warmup: int, timing: bool, graphs: bool = True) -> dict:
+def dam_break(ws, x, xb, cfg, steps: int, warmup: int, timing: bool, graphs: bool = False) -> dict:
     """The large dam break from rest: `warmup` substeps, then `steps` timed
     substeps (CFL dt, lambda = 0.4, velocity floor 0.5 m/s).  The throughput
-    is timed with graph replay and the pipelined substep loop; with `timing`
+    is timed with the library's substep loop (the CFL step on the device, the
+    next substep's grid phase behind each state read); with `timing`
     the same substeps then run again from the same state with a HIP event
-    pair around every kernel launch, for the per-kernel durations (events
-    between launches rule out graph replay and the pipelining)."""
+    pair around every kernel launch, for the per-kernel durations (the events
+    between launches serialise the host with the device)."""
     n, nb = len(x), len(xb)
     fluid = ws.ParticleStore(n)
     fluid.resize(n)
@@ -421,9 +422,13 @@ def main() -> int:
                          "(-1: as --steps, 0: skip)")
     ap.add_argument("--mid-steps", type=int, default=20,
                     help="substeps timed on the mid dam break (128,000 particles; 0: skip)")
-    ap.add_argument("--no-graphs", action="store_true",
-                    help="launch every kernel eagerly instead of replaying captured HIP graphs "
-                         "(for profilers whose graph-launch interception fails)")
+    ap.add_argument("--graphs", action="store_true",
+                    help="replay captured HIP graphs in the plain-solver legs instead of "
+                         "launching every kernel eagerly (on ROCm 7.2 a graph launch costs the "
+                         "host ~15-20 us per kernel node, more than the eager launches it "
+                         "replaces, so eager launches keep the device busier)")
+    ap.add_argument("--batch-no-graphs", action="store_true",
+                    help="launch the batched leg's kernels eagerly instead of replaying graphs")
     ap.add_argument("--reference-steps", type=int, default=20,
                     help="substeps timed with the reference kappa/rho^2 form (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -489,11 +494,12 @@ def main() -> int:
     if args.only_adjoint:
         print(json.dumps({"adjoint": adjoint_leg(args.adjoint_substeps, 20)}))
         return 0
-    graphs = not args.no_graphs
+    graphs = bool(args.graphs)
+    batch_graphs = not args.batch_no_graphs
     if args.only_batch:
         print(json.dumps({"batched": batched_leg(args.batch_steps, args.batch_warmup,
                                                  args.tune_reps, 1, bool(args.fast_math),
-                                                 graphs)}))
+                                                 batch_graphs)}))
         return 0
 
     x, xb, cfg = make_workload(args.n_side)
@@ -581,11 +587,11 @@ def main() -> int:
     batched = None
     if args.batch_steps > 0:
         batched = batched_leg(args.batch_steps, args.batch_warmup, args.tune_reps, 1,
-                              bool(args.fast_math), graphs, eager_sequential=False)
+                              bool(args.fast_math), batch_graphs, eager_sequential=False)
     batched_capped = None
     if args.batch_capped_steps > 0:
         batched_capped = batched_leg(args.batch_capped_steps, args.batch_warmup, args.tune_reps,
-                                     0, bool(args.fast_math), graphs, eager_sequential=False)
+                                     0, bool(args.fast_math), batch_graphs, eager_sequential=False)
 
     adjoint = None
     if args.adjoint_substeps > 0:
@@ -619,6 +625,8 @@ def main() -> int:
                                 "fields within 1e-5 relative of the binary64 oracle)"
                                 if args.fast_math else "bit-exact IEEE binary32"),
             "graph_replay": graphs,
+            "substep_loop": "library advance(): CFL step on the device, one state read per "
+                            "substep overlapped by the next substep's grid phase",
             "max_density_iters": cfg["max_density_iters"],
             "eta_density": cfg["eta_density"],
             "mean_density_iters": float(np.mean(iters)),

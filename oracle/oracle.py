@@ -88,6 +88,7 @@ _SIGS = {
     "orc_lbvh": (None, [vp, vp, vp, sz, vp, vp, vp, vp]),
     "orc_sample_aabb_boundary": (sz, [F3, F3, C.c_float, C.c_int, vp, sz]),
     "orc_lattice_block": (sz, [F3, F3, C.c_float, vp, sz]),
+    "orc_damping_decay": (C.c_float, [C.c_float, C.c_float]),
     "orc_next_dt": (C.c_float, [C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_float,
                                 C.c_float, C.c_float]),
     "orc_config_default": (None, [C.POINTER(OrcConfig)]),
@@ -238,6 +239,10 @@ def lattice_block(lo, hi, spacing: float) -> np.ndarray:
     out = np.empty((n, 3), np.float32)
     lib().orc_lattice_block(_f3(lo), _f3(hi), spacing, out.ctypes.data, n)
     return out
+
+
+def damping_decay(damping, dt) -> float:
+    return lib().orc_damping_decay(damping, dt)
 
 
 def next_dt(frame_dt, dt_max, cfl_lambda, max_substeps, max_velocity, particle_radius,

@@ -437,10 +437,10 @@ template <class R> struct Solver {
         }
     }
 
-    // dfsph_kernels.cu:347-355 with exp(-damping*dt) evaluated on the host
+    // dfsph_kernels.cu:347-355 with exp(-damping*dt) evaluated once per substep
     void gravity(float dt) {
         R decay;
-        if constexpr (F32Values<R>::value) decay = R(std::exp(-cfg.damping * dt));
+        if constexpr (F32Values<R>::value) decay = R(orc_damping_decay(cfg.damping, dt));
         else decay = std::exp(-R(cfg.damping) * R(dt));
         const R gx = R(cfg.gravity[0]), gy = R(cfg.gravity[1]), gz = R(cfg.gravity[2]);
         const R dtr = R(dt);
@@ -1077,6 +1077,27 @@ size_t orc_lattice_block(const float lo[3], const float hi[3], float spacing, fl
                 ++count;
             }
     return count;
+}
+
+// exp(-damping * dt) rounded to binary32 (dfsph_kernels.cu:351; the
+// reference takes fast-math expf per particle): the product in binary32,
+// then exp in binary64 by k = round(x / ln2), r = x - k ln2 (ln2 in a 32-bit
+// head and a tail), a degree-13 Taylor polynomial of e^r in Horner form,
+// 2^k applied exactly, one rounding to binary32.  The library evaluates
+// the same sequence of IEEE operations on the host and on the device.
+float orc_damping_decay(float damping, float dt) {
+    const float xf = -damping * dt;
+    const double x = static_cast<double>(xf);
+    if (!(x > -745.0)) return std::isnan(x) ? xf : 0.0f;
+    if (x > 709.0) return std::numeric_limits<float>::infinity();
+    const double kd = x * 1.4426950408889634;
+    const int k = static_cast<int>(kd + (kd >= 0.0 ? 0.5 : -0.5));
+    const double r = (x - static_cast<double>(k) * 6.93147180369123816490e-01) -
+                     static_cast<double>(k) * 1.90821492927058770002e-10;
+    double p = 1.0;
+    for (int j = 13; j >= 1; --j) p = 1.0 + p * (r / static_cast<double>(j));
+    const int k1 = k / 2, k2 = k - k1;
+    return static_cast<float>(std::ldexp(std::ldexp(p, k1), k2));
 }
 
 // timestep.h:23-28

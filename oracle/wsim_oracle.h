@@ -123,6 +123,8 @@ size_t orc_sample_aabb_boundary(const float lo[3], const float hi[3], float spac
 /* apps/sim_cli/main.cpp:65-74 and tests/regression/test_settled_puddle.cu:24-32 */
 size_t orc_lattice_block(const float lo[3], const float hi[3], float spacing,
                          float* out, size_t cap);
+/* exp(-damping*dt) in binary32 (dfsph_kernels.cu:351), the library's rule */
+float orc_damping_decay(float damping, float dt);
 /* core/include/water/core/timestep.h:23-28 */
 float orc_next_dt(float frame_dt, float dt_max, float cfl_lambda, uint32_t max_substeps,
                   float max_velocity, float particle_radius, float remaining);

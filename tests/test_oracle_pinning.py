@@ -232,6 +232,26 @@ def test_next_dt_closed_form():
     assert O.next_dt(1 / 24, 1 / 24, 0.4, 16, 0.5, 0.01, 0.001) == pytest.approx(0.001, rel=1e-6)
 
 
+def test_damping_decay_is_exp_rounded_to_binary32():
+    # exp(x) for x = RN32(-damping * dt): against numpy's binary64 exp rounded
+    # once to binary32, which is the correctly rounded value except within
+    # 1e-16 relative of a rounding boundary; the oracle may differ there by
+    # one unit in the last place, and nowhere else
+    rng = np.random.default_rng(3)
+    damping = np.concatenate([rng.uniform(0, 5, 20000), [0.0, 0.5, 3.0, 60.0, 1e4]]).astype(np.float32)
+    dt = np.concatenate([rng.uniform(0, 0.05, 20000), [1e-3, 1 / 24, 4e-3, 2.0, 1.0]]).astype(np.float32)
+    got = np.array([O.damping_decay(float(a), float(b)) for a, b in zip(damping, dt)], np.float32)
+    x = (-damping * dt).astype(np.float32)
+    want = np.exp(x.astype(np.float64)).astype(np.float32)
+    ulps = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
+    assert ulps.max() <= 1
+    assert np.count_nonzero(ulps) <= 2
+    assert O.damping_decay(0.0, 0.01) == 1.0
+    assert O.damping_decay(1e4, 1.0) == 0.0        # underflow
+    assert O.damping_decay(-100.0, 1.0) == np.inf  # exp(100) overflows binary32
+    assert O.damping_decay(-50.0, 1.0) == np.float32(np.exp(50.0))
+
+
 def test_lattice_counts_match_reference():
     # README.md:75-88 (single_drop: 180 particles); settled puddle 17^3
     assert len(O.lattice_block((0.45, 0.30, 0.45), (0.55, 0.40, 0.55), 0.02)) == 180

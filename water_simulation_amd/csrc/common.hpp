@@ -103,8 +103,9 @@ private:
 template <class T> class PinnedBuffer {
 public:
     PinnedBuffer() = default;
-    explicit PinnedBuffer(size_t count) : size_(count) {
-        if (count) WSIM_HIP(hipHostMalloc(reinterpret_cast<void**>(&ptr_), count * sizeof(T), 0));
+    explicit PinnedBuffer(size_t count, unsigned flags = 0) : size_(count) {
+        if (count)
+            WSIM_HIP(hipHostMalloc(reinterpret_cast<void**>(&ptr_), count * sizeof(T), flags));
     }
     ~PinnedBuffer() {
         if (ptr_) (void)hipHostFree(ptr_);

@@ -54,6 +54,32 @@ struct GridGeom {
     }
 };
 
+// exp(-damping * dt) rounded to binary32, the velocity decay of a substep
+// (dfsph_kernels.cu:351 evaluates it per particle with fast-math expf).  The
+// same operations run on the host and on the device: the product in
+// binary32, then exp in binary64 by reduction x = k ln2 + r (ln2 split in a
+// 32-bit head and a tail, |r| <= ln2/2), a degree-13 Taylor polynomial in
+// Horner form (truncation error below 1e-16 relative), scaling by 2^k and
+// one rounding to binary32.  Every step is an IEEE operation without
+// contraction, so a time step computed on the device gives the decay the
+// host computes for the same step, bit for bit.
+__host__ __device__ inline float damping_decay(float damping, float dt) {
+    const float xf = -damping * dt;
+    const double x = static_cast<double>(xf);
+    if (!(x > -745.0)) return (x != x) ? xf : 0.0f;  // NaN stays NaN; underflow
+    if (x > 709.0) return static_cast<float>(1e300 * 1e300);  // +inf
+    const double kd = x * 1.4426950408889634;  // x / ln2
+    const int k = static_cast<int>(kd + (kd >= 0.0 ? 0.5 : -0.5));
+    const double ln2_hi = 6.93147180369123816490e-01;  // 32 significant bits
+    const double ln2_lo = 1.90821492927058770002e-10;
+    const double r = (x - static_cast<double>(k) * ln2_hi) - static_cast<double>(k) * ln2_lo;
+    double p = 1.0;
+    for (int j = 13; j >= 1; --j) p = 1.0 + p * (r / static_cast<double>(j));
+    // times 2^k (k in [-1075, 1024]) in two exact scalings
+    const int k1 = k / 2, k2 = k - k1;
+    return static_cast<float>(ldexp(ldexp(p, k1), k2));
+}
+
 // Constants of the cubic spline for one smoothing length l, hoisted out of
 // the pair loop.  sigma = 1/(pi*l*l*l) is the same binary32 value the
 // reference recomputes per call (sph_kernels.cuh:15,31).

@@ -42,7 +42,9 @@
 // FP contraction and with IEEE divide/sqrt: each field equals the binary32
 // oracle bit for bit, for every launch shape and in every batch.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstring>
 #include <initializer_list>
 #include <limits>
 
@@ -1448,6 +1450,59 @@ __global__ __launch_bounds__(NS) void k_reset_solve(const SceneSeg* __restrict__
     }
 }
 
+// advance(): the frame's time-step rule and budget into device memory.
+__global__ void k_frame_begin(FrameParams fp, FrameParams* __restrict__ dst,
+                              DeviceState* __restrict__ st) {
+    if (threadIdx.x != 0) return;
+    *dst = fp;
+    st->t_rem = fp.time;
+}
+
+// The CFL step of sim_cli's substep loop on the device (main.cpp:176-177,
+// timestep.h:23-28): dt = next_dt(max(max|v|, v_floor), r, remaining), with
+// the comparisons of std::max and std::min over {dt_max, dt_cfl, remaining,
+// max(dt_cfl, dt_floor)} in their order, then the substep parameters the
+// host otherwise uploads (dt, dt^2, the velocity decay, gravity * dt).  The
+// host evaluates the same rule on the published state and checks that both
+// agree bit for bit.
+__global__ void k_cfl(const FrameParams* __restrict__ fp, SceneSeg* __restrict__ seg,
+                      DeviceState* __restrict__ st) {
+    if (threadIdx.x != 0) return;
+    const FrameParams f = *fp;
+    const float mv = st->max_v;
+    const float a = (mv < f.v_floor) ? f.v_floor : mv;  // std::max(max|v|, v_floor)
+    const float v = (a < 1e-6f) ? 1e-6f : a;             // std::max(v, 1e-6f)
+    const float dt_cfl = f.cfl_lambda * f.particle_radius / v;
+    const float dt_floor = f.frame_dt / static_cast<float>(f.max_substeps);
+    const float rem = st->t_rem;
+    float d = f.dt_max;
+    if (dt_cfl < d) d = dt_cfl;
+    if (rem < d) d = rem;
+    const float hi = (dt_cfl < dt_floor) ? dt_floor : dt_cfl;
+    if (hi < d) d = hi;
+    SceneSeg& g = seg[0];
+    g.dt = d;
+    g.dt2 = d * d;
+    g.decay = damping_decay(f.damping, d);
+    g.gdx = f.gravity[0] * d;
+    g.gdy = f.gravity[1] * d;
+    g.gdz = f.gravity[2] * d;
+    st->dt = d;
+    st->t_rem = rem - d;
+}
+
+// The state of scene 0 into mapped host memory, then a new sequence number
+// with release ordering at system scope: the host that sees the number sees
+// the state.
+__global__ void k_publish(const DeviceState* __restrict__ st, u32* __restrict__ seq,
+                          StatePub* pub) {
+    if (threadIdx.x != 0) return;
+    pub->st = st[0];
+    const u32 k = *seq + 1u;
+    *seq = k;
+    __hip_atomic_store(&pub->seq, k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // ---- launch-shape dispatch ------------------------------------------------
 u32 grid_for(u32 nslots, u32 nt) { return div_up(nslots, nt); }
 
@@ -1659,6 +1714,8 @@ enum PhaseKey : u64 {
     kPhDivIters = 6,
     kPhPrefixGrid = 7,
     kPhPrefixSolve = 8,
+    kPhSubstep = 9,
+    kPhContinue = 10,
 };
 u64 phase_key(u64 kind, u64 b = 0, u64 e = 0) { return (kind << 48) | (b << 24) | e; }
 
@@ -1754,7 +1811,6 @@ DFSPHSolver::DFSPHSolver(ParticleStore& fluid, ParticleStore* boundary, const DF
 DFSPHSolver::~DFSPHSolver() {
     (void)hipStreamSynchronize(stream_);
     drop_graphs();
-    if (state_ev_) (void)hipEventDestroy(state_ev_);
     if (own_stream_) (void)hipStreamDestroy(own_stream_);
 }
 
@@ -1822,6 +1878,17 @@ void DFSPHSolver::init(const std::vector<u32>& counts, const std::vector<float>&
     dseg_ = DeviceBuffer<SceneSeg>(scenes);
     dstate_ = DeviceBuffer<DeviceState>(scenes);
     dstate_.zero(stream_);
+    if (!pub_.data()) {
+        dframe_ = DeviceBuffer<FrameParams>(1);
+        pubseq_ = DeviceBuffer<u32>(1);
+        pubseq_.zero(stream_);
+        pub_ = PinnedBuffer<StatePub>(1, hipHostMallocMapped | hipHostMallocCoherent);
+        std::memset(pub_.data(), 0, sizeof(StatePub));
+        void* d = nullptr;
+        WSIM_HIP(hipHostGetDevicePointer(&d, pub_.data(), 0));
+        pub_dev_ = static_cast<StatePub*>(d);
+        pub_expect_ = 0;
+    }
     hstate_ = PinnedBuffer<DeviceState>(scenes);
     stats_.assign(scenes, StepStats{});
 
@@ -1951,7 +2018,7 @@ void DFSPHSolver::set_substep(const float* dt) {
         const float d = dt[s];
         g.dt = d;
         g.dt2 = d * d;
-        g.decay = std::exp(-cfg_.damping * d);
+        g.decay = damping_decay(cfg_.damping, d);
         g.gdx = cfg_.gravity[0] * d;
         g.gdy = cfg_.gravity[1] * d;
         g.gdz = cfg_.gravity[2] * d;
@@ -1966,11 +2033,16 @@ void DFSPHSolver::step(float dt) {
     step_scenes(d.data());
 }
 
-// Only the velocity update and what follows it depend on a substep's dt, so
-// with the divergence solve off the grid, density, alpha and pair lists of
-// substep k + 1 are enqueued before the host waits for substep k's state:
-// the wait, the time-step computation and the launches of the host overlap
-// the device's work on the next grid.
+// sim_cli's CFL substep loop (main.cpp:171-178) in the library.  With the
+// divergence solve off, each substep is one launch sequence (graph): the
+// device computes the substep's time step from the last max |v| and the time
+// left (k_cfl), runs gravity, XSPH, the first chunk of density iterations
+// and advection, publishes the state to mapped host memory and goes on with
+// the next substep's grid, density, alpha and pair lists, which depend on
+// the positions only.  The host waits for the published state (its read
+// overlaps that grid phase), evaluates the same time-step rule on it for its
+// own record and checks it against the device's, and only continues the
+// density solve itself when the first chunk did not finish it.
 u32 DFSPHSolver::advance(const TimeStepConfig& ts, float v_floor, float time, u32 max_substeps,
                          float* dts, StepStats* stats) {
     if (batched_) fail(kInvalidArgument, "advance: a batched solver steps with step_scenes");
@@ -1979,17 +2051,37 @@ u32 DFSPHSolver::advance(const TimeStepConfig& ts, float v_floor, float time, u3
     if (pipe && (counts_.empty() || counts_[0] != nsz)) layout({static_cast<u32>(nsz)});
     float rem = time;
     u32 sub = 0;
+    if (pipe && rem > 1e-6f && max_substeps > 0) {
+        hseg_[0].eps = viscosity_[0];
+        WSIM_HIP(hipMemcpyAsync(dseg_.data(), hseg_.data(), sizeof(SceneSeg),
+                                hipMemcpyHostToDevice, stream_));
+        FrameParams fp{};
+        fp.frame_dt = ts.frame_dt;
+        fp.dt_max = ts.dt_max;
+        fp.cfl_lambda = ts.cfl_lambda;
+        fp.max_substeps = ts.max_substeps;
+        fp.v_floor = v_floor;
+        fp.particle_radius = cfg_.particle_radius;
+        fp.damping = cfg_.damping;
+        for (int a = 0; a < 3; ++a) fp.gravity[a] = cfg_.gravity[a];
+        fp.time = time;
+        k_frame_begin<<<1, 64, 0, stream_>>>(fp, dframe_.data(), dstate_.data());
+        WSIM_LAUNCH_CHECK();
+    }
     bool grid_ready = false;  // the next substep's grid phase is enqueued
     while (rem > 1e-6f && sub < max_substeps) {
         const float dt = next_dt(ts, std::max(max_velocity(), v_floor), cfg_.particle_radius, rem);
         if (!pipe) {
             step(dt);
         } else {
-            set_substep(&dt);
-            if (!grid_ready) run_phase(phase_key(kPhPrefixGrid), [&] { phase_prefix_grid(); });
-            run_phase(phase_key(kPhPrefixSolve), [&] { phase_prefix_solve(true); });
             const bool next = rem - dt > 1e-6f && sub + 1 < max_substeps;
-            grid_ready = substep_pipelined(next);
+            if (!grid_ready) run_phase(phase_key(kPhPrefixGrid), [&] { phase_prefix_grid(); });
+            grid_ready = substep_device(next);
+            const float dd = hstate_[0].dt;
+            if (std::memcmp(&dd, &dt, sizeof(float)) != 0)
+                fail(kRuntimeError, "advance: the device time step differs from the host's");
+            hseg_[0].dt = dt;
+            hseg_[0].dt2 = dt * dt;
         }
         if (dts) dts[sub] = dt;
         if (stats) stats[sub] = stats_[0];
@@ -1999,54 +2091,76 @@ u32 DFSPHSolver::advance(const TimeStepConfig& ts, float v_floor, float time, u3
     return sub;
 }
 
-// The density solve and advection of a plain substep whose prefix is
-// enqueued.  With `next`, the grid phase of the following substep is
-// enqueued behind the state read; returns whether it stands.  When the first
-// iteration chunk does not finish the solve, that grid phase ran on the
-// store's unchanged positions and rebuilt exactly this substep's grid, lists,
-// density and alpha (functions of the positions only), so the solve goes on
-// and the next substep's grid is built after advection.
-bool DFSPHSolver::substep_pipelined(bool next) {
+// Waits for the state k_publish writes to mapped host memory and copies it
+// to hstate_.  A stream that fails, or finishes without publishing, ends the
+// wait with an error.
+// The wait polls only the published word for its first 20 ms (a substep
+// takes well under that; querying the stream meanwhile competes with the
+// runtime's own submission of the queued launches), then also the stream.
+void DFSPHSolver::wait_published() {
+    const u32 want = ++pub_expect_;
+    StatePub* p = pub_.data();
+    const auto t0 = std::chrono::steady_clock::now();
+    for (u64 spin = 1;; ++spin) {
+        if (__atomic_load_n(&p->seq, __ATOMIC_ACQUIRE) == want) break;
+        __builtin_ia32_pause();
+        if ((spin & 4095u) != 0) continue;
+        const auto waited = std::chrono::steady_clock::now() - t0;
+        if (waited < std::chrono::milliseconds(20)) continue;
+        const hipError_t e = hipStreamQuery(stream_);
+        if (e != hipSuccess && e != hipErrorNotReady) WSIM_HIP(e);
+        if (e == hipSuccess && __atomic_load_n(&p->seq, __ATOMIC_ACQUIRE) != want)
+            fail(kDeviceError, "advance: the substep finished without publishing its state");
+        if (waited > std::chrono::seconds(120))
+            fail(kDeviceError, "advance: timed out waiting for the substep state");
+    }
+    std::memcpy(hstate_.data(), &p->st, sizeof(DeviceState));
+}
+
+bool DFSPHSolver::substep_device(bool next) {
     const u32 maxit = cfg_.max_density_iters;
-    u32 chunk = 2;
-    while (chunk < last_iters_[0] && chunk < maxit) chunk <<= 1;
-    const u32 end = std::min(chunk, maxit);
-    if (end > 0) run_phase(phase_key(kPhDensityIters, 0, end), [&] { phase_iterations(0, 0, end); });
-    const bool last = end >= maxit;
-    run_phase(phase_key(kPhAdvect, last ? 1 : 0), [&] { phase_advect(last); });
-    const u32 scenes = static_cast<u32>(counts_.size());
-    WSIM_HIP(hipMemcpyAsync(hstate_.data(), dstate_.data(), scenes * sizeof(DeviceState),
-                            hipMemcpyDeviceToHost, stream_));
-    if (!state_ev_) WSIM_HIP(hipEventCreateWithFlags(&state_ev_, hipEventDisableTiming));
-    WSIM_HIP(hipEventRecord(state_ev_, stream_));
-    if (next) run_phase(phase_key(kPhPrefixGrid), [&] { phase_prefix_grid(); });
-    WSIM_HIP(hipEventSynchronize(state_ev_));
-    bool all = true;
-    for (u32 s = 0; s < scenes; ++s) all = all && hstate_[s].advected;
-    if (all) {
-        take_state();
-        return next;
-    }
-    u32 it = end;
-    chunk = chunk * 2 > 32 ? 32 : chunk * 2;
-    while (it < maxit) {
-        const u32 e = (maxit - it < chunk) ? maxit : it + chunk;
-        run_phase(phase_key(kPhDensityIters, it, e), [&] { phase_iterations(0, it, e); });
-        it = e;
-        if (it >= maxit) break;
-        run_phase(phase_key(kPhAdvect, 0), [&] { phase_advect(false); });
-        fetch_state();
-        bool done = true;
-        for (u32 s = 0; s < scenes; ++s) done = done && hstate_[s].advected;
-        if (done) {
+    // first chunk: one iteration more than the previous substep needed, so
+    // only a jump of two or more iterations needs a second chunk
+    u32 chunk = std::max(1u, std::min(last_iters_[0] + 1u, 64u));
+    u32 begin = 0, end = std::min(chunk, maxit);
+    // two instances of each graph, used in turn: a graph is not launched
+    // again while its previous launch is still running its tail
+    parity_ ^= 1u;
+    for (bool first = true;; first = false) {
+        const bool last = end >= maxit;
+        // a continuation chunk leaves the next grid phase to the host: its
+        // tail would rebuild this substep's grid again whenever the solve
+        // goes on past it
+        const bool tail = next && first;
+        const u32 flags = (tail ? 1u : 0u) | (last ? 2u : 0u) | (parity_ << 2);
+        const u64 key = first ? phase_key(kPhSubstep, end, flags)
+                              : phase_key(kPhContinue, begin, end | (flags << 16));
+        run_phase(key, [&] {
+            if (first) {
+                k_cfl<<<1, 64, 0, stream_>>>(dframe_.data(), dseg_.data(), dstate_.data());
+                WSIM_LAUNCH_CHECK();
+                phase_prefix_solve(true);
+            }
+            if (end > begin) phase_iterations(0, begin, end);
+            phase_advect(last);
+            k_publish<<<1, 64, 0, stream_>>>(dstate_.data(), pubseq_.data(), pub_dev_);
+            WSIM_LAUNCH_CHECK();
+        });
+        // behind an unfinished solve this rebuilds exactly this substep's
+        // grid, lists, density and alpha from the unchanged store positions
+        // (functions of the positions only).  A graph of its own: launching
+        // the next substep while a large graph still runs its tail stalls
+        // that tail's dispatch.
+        if (tail) run_phase(phase_key(kPhPrefixGrid), [&] { phase_prefix_grid(); });
+        wait_published();
+        if (hstate_[0].advected) {
             take_state();
-            return false;
+            return tail;
         }
-        chunk = chunk * 2 > 32 ? 32 : chunk * 2;
+        chunk = std::min(chunk * 2u, 32u);
+        begin = end;
+        end = std::min(end + chunk, maxit);
     }
-    run_phase(phase_key(kPhAdvect, 1), [&] { phase_advect(true); });
-    read_state();
-    return false;
 }
 
 void DFSPHSolver::step_scenes(const float* dt) {

@@ -298,6 +298,26 @@ struct DeviceState {
     double mass;
     double mom[3];
     double ke;
+    float t_rem;        // advance(): time left in the frame after this substep
+    float dt;           // advance(): this substep's CFL time step
+};
+
+// The time-step rule of an advance() call, read by the device CFL step
+// (timestep.h:8-34 and sim_cli's velocity floor, main.cpp:176-177).
+struct FrameParams {
+    float frame_dt, dt_max, cfl_lambda;
+    u32 max_substeps;
+    float v_floor, particle_radius, damping;
+    float gravity[3];
+    float time;  // the frame's time budget
+};
+
+// A substep's state as the device publishes it to mapped host memory: the
+// state, then (released after it) a sequence number the host waits for.
+struct StatePub {
+    DeviceState st;
+    u32 seq;
+    u32 pad[15];
 };
 
 enum KernelKind : int {
@@ -504,7 +524,13 @@ private:
     void layout(const std::vector<u32>& counts);
     void set_substep(const float* dt);
     void run_substep();
-    bool substep_pipelined(bool next);
+    // One substep of advance() as one launch sequence: the device CFL step,
+    // gravity/XSPH, the first chunk of density iterations, advection, the
+    // state published to host memory and, with `next`, the following
+    // substep's grid phase; an unfinished solve continues in chunks of the
+    // same shape.  Returns whether the grid phase of the next substep stands.
+    bool substep_device(bool next);
+    void wait_published();
     void phase_prefix(bool gravity);
     void phase_prefix_grid();
     void phase_prefix_solve(bool gravity);
@@ -574,7 +600,12 @@ private:
     KernelShapes shapes_;
     std::vector<TuneRecord> tune_log_;
     u32 last_iters_[2] = {0, 0};  // iterations of the previous substep's solves
-    hipEvent_t state_ev_ = nullptr;  // state read of a pipelined substep
+    DeviceBuffer<FrameParams> dframe_;  // advance(): the time-step rule on the device
+    DeviceBuffer<u32> pubseq_;          // sequence number of the last published state
+    PinnedBuffer<StatePub> pub_;        // published state (mapped, coherent host memory)
+    StatePub* pub_dev_ = nullptr;       // device address of pub_
+    u32 pub_expect_ = 0;
+    u32 parity_ = 0;  // which instance of the substep graph runs next
     bool use_graphs_ = false, capturing_ = false;
     std::map<u64, hipGraphExec_t> graphs_;
 
