@@ -94,8 +94,28 @@ def make_mid_workload():
     return x, xb, cfg
 
 
+def host_cpu() -> dict:
+    """The host CPU as lscpu reports it: model, sockets, cores per socket,
+    threads per core, and the CPUs this process may run on."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, val = line.partition(":")
+            info[k.strip()] = val.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"model": info.get("Model name"), "sockets": info.get("Socket(s)"),
+            "cores_per_socket": info.get("Core(s) per socket"),
+            "threads_per_core": info.get("Thread(s) per core"), "cpus_online": info.get("CPU(s)"),
+            "cpus_available_to_process": len(os.sched_getaffinity(0))}
+
+
 def cpu_baseline(x, v, xb, cfg, dts, budget_s: float):
-    """Binary32 oracle on the host cores from the given state (bounded sample)."""
+    """Binary32 oracle on the host cores from the given state (bounded sample).
+    Its grid build is a parallel stable radix sort and a parallel cell-table
+    fill; every other stage is OpenMP-parallel over particles."""
     from oracle import oracle as O
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
@@ -120,6 +140,7 @@ def cpu_baseline(x, v, xb, cfg, dts, budget_s: float):
                   f"{el:.1f} s, binary32 oracle, OpenMP)",
         "ms_per_substep": el / max(done, 1) * 1e3,
         "mean_density_iters": iters / max(done, 1),
+        "host": host_cpu(),
     }, done, iters
 
 

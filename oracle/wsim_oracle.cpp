@@ -229,19 +229,60 @@ template <class R> struct Grid {
         for (long long i = 0; i < (long long)n; ++i)
             keys[i] = home ? cell_key(home[3 * i], home[3 * i + 1], home[3 * i + 2], D)
                            : key_of(x[i]);
-        // Stable counting sort on (key, id): identical order to a stable LSD
-        // radix sort of (key, iota) pairs (cell_grid.cu:98-110).
-        std::vector<u32> start(size_t(C) + 2, 0);
-        for (size_t i = 0; i < n; ++i) start[size_t(keys[i]) + 1]++;
-        for (size_t c = 0; c <= C; ++c) start[c + 1] += start[c];
-        // cell_start[c] = number of particles with key < c, for c in [0, C]
-        // (fill + scatter + right-to-left min, cell_grid.cu:119-137)
-        for (size_t c = 0; c <= C; ++c) cs[c] = start[c];
-        for (size_t i = 0; i < n; ++i) {
-            const u32 slot = start[keys[i]]++;
-            perm[slot] = u32(i);
-            sorted_keys[slot] = keys[i];
+        // Stable LSD radix sort of (key, id) pairs, 8-bit digits, as the
+        // reference's SortPairs over (key, iota) (cell_grid.cu:98-110).  Each
+        // pass is parallel and stable: thread t counts and then scatters its
+        // own contiguous chunk, after the chunks of lower threads.
+        int bits = 0;
+        while (bits < 32 && (C >> bits) != 0) ++bits;
+        std::vector<u32> k2(n), p2(n);
+        sorted_keys = keys;
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)n; ++i) perm[i] = u32(i);
+        const int T = omp_get_max_threads();
+        std::vector<size_t> off(size_t(T) * 256);
+        for (int shift = 0; shift < bits; shift += 8) {
+            std::fill(off.begin(), off.end(), size_t(0));
+#pragma omp parallel num_threads(T)
+            {
+                const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+                const size_t lo = n * size_t(t) / size_t(nt), hi = n * size_t(t + 1) / size_t(nt);
+                size_t* h = off.data() + size_t(t) * 256;
+                for (size_t i = lo; i < hi; ++i) ++h[(sorted_keys[i] >> shift) & 255u];
+#pragma omp barrier
+#pragma omp single
+                {
+                    size_t run = 0;
+                    for (int d = 0; d < 256; ++d)
+                        for (int u = 0; u < nt; ++u) {
+                            const size_t c = off[size_t(u) * 256 + d];
+                            off[size_t(u) * 256 + d] = run;
+                            run += c;
+                        }
+                }
+                for (size_t i = lo; i < hi; ++i) {
+                    const size_t o = h[(sorted_keys[i] >> shift) & 255u]++;
+                    k2[o] = sorted_keys[i];
+                    p2[o] = perm[i];
+                }
+            }
+            sorted_keys.swap(k2);
+            perm.swap(p2);
         }
+        // cell_start[c] = number of particles with key < c, for c in [0, C]
+        // (fill + scatter + right-to-left min, cell_grid.cu:119-137): slot i
+        // that starts a run of key k fills (previous key, k]; the cells after
+        // the last key take n.  Every entry is written once.
+#pragma omp parallel for schedule(dynamic, 4096)
+        for (long long i = 0; i < (long long)n; ++i) {
+            const u32 k = sorted_keys[i];
+            if (i > 0 && sorted_keys[i - 1] == k) continue;
+            const size_t c0 = i > 0 ? size_t(sorted_keys[i - 1]) + 1 : 0;
+            for (size_t c = c0; c <= k; ++c) cs[c] = u32(i);
+        }
+        const size_t tail0 = n ? size_t(sorted_keys[n - 1]) + 1 : 0;
+#pragma omp parallel for schedule(static)
+        for (long long c = (long long)tail0; c <= (long long)C; ++c) cs[c] = u32(n);
     }
 
     void home_of(const V3<R>& p, int16_t* out) const {
