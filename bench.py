@@ -199,12 +199,14 @@ def dam_break(ws, x, xb, cfg, steps: int, warmup: int, timing: bool, graphs: boo
 
 
 def batched_leg(steps: int, warmup: int, tune_reps: int, pressure_form: int = 1,
-                fast_math: bool = False, graphs: bool = True, eager_sequential: bool = True):
+                fast_math: bool = False, graphs: bool = True, both_launch_modes: bool = True,
+                eta: float = 1e-3):
     """64 uneven box scenes (seeded mix, ~1.05M fluid particles in total): one
     BatchSolver against 64 plain DFSPHSolvers run one after another, both on
     the same per-scene dt schedule and the same eta; the sequential side is
-    timed with graph replay (and, with eager_sequential, without it) and the
-    speedup is taken against the faster.  With kappa/rho scaling every scene
+    timed in the batch's launch mode (graph replay or eager launches) and,
+    with both_launch_modes, in the other one too, and the speedup is taken
+    against the faster.  With kappa/rho scaling every scene
     leaves its density solve at its own iteration (the batch runs until the
     slowest scene of each substep); the per-scene spread is reported."""
     import water_simulation_amd as ws
@@ -214,7 +216,7 @@ def batched_leg(steps: int, warmup: int, tune_reps: int, pressure_form: int = 1,
     counts = [len(s.positions) for s in mix]
     total = int(sum(counts))
     xb = ws.sample_aabb_boundary((0, 0, 0), (SC.BOX,) * 3, SC.WALL_SPACING, SC.WALL_LAYERS)
-    over = dict(pressure_form=pressure_form, fast_math=fast_math)
+    over = dict(pressure_form=pressure_form, fast_math=fast_math, eta_density=eta)
     cfg = ws.DFSPHConfig(**SC.box_config(**over))
     ts = ws.TimeStepper()
 
@@ -278,14 +280,14 @@ def batched_leg(steps: int, warmup: int, tune_reps: int, pressure_form: int = 1,
             p.close()
         return t, same
 
-    t_graph, same_g = sequential(graphs)
-    t_eager, same_e = sequential(False) if (eager_sequential and graphs) else (None, True)
-    t_seq = min(t for t in (t_graph, t_eager) if t is not None)
+    t_same, same_a = sequential(graphs)  # the batch's own launch mode
+    t_other, same_b = sequential(not graphs) if both_launch_modes else (None, True)
+    t_seq = min(t for t in (t_same, t_other) if t is not None)
     eta = float(cfg.eta_density * cfg.rest_density)
     return {
         "workload": f"uneven batched mix: 64 box scenes (0.6 m, r=0.005 m, walls 0.01 m x 3 "
                     f"layers), {min(counts)}..{max(counts)} particles per scene, {total} total, "
-                    f"v ~ N(0, 0.5 m/s), XSPH in [0.005, 0.05], eta=0.1%",
+                    f"v ~ N(0, 0.5 m/s), XSPH in [0.005, 0.05], eta={100 * eta:g}%",
         "pressure_form": "kappa/rho" if pressure_form == 1 else "reference kappa/rho^2",
         "fast_math": bool(fast_math),
         "graphs": bool(graphs),
@@ -294,8 +296,8 @@ def batched_leg(steps: int, warmup: int, tune_reps: int, pressure_form: int = 1,
         "substeps": steps,
         "warmup": warmup,
         "batched_s": t_b,
-        "sequential_graphs_s": t_graph,
-        "sequential_eager_s": t_eager,
+        "sequential_s": t_same,
+        "sequential_other_launch_mode_s": t_other,
         "speedup": t_seq / t_b,
         "batched_particle_substeps_per_s": total * steps / t_b,
         "sequential_particle_substeps_per_s": total * steps / t_seq,
@@ -303,8 +305,9 @@ def batched_leg(steps: int, warmup: int, tune_reps: int, pressure_form: int = 1,
         "iters_per_substep_min_mean_max": [float(it.min(1).mean()), float(it.mean()),
                                            float(it.max(1).mean())],
         "substeps_with_uneven_exit": float(np.mean(it.min(1) != it.max(1))),
+        "per_scene_mean_iters_min_max": [float(it.mean(0).min()), float(it.mean(0).max())],
         "scenes_meeting_eta_last_substep": int(sum(e < eta for e in err_final)),
-        "bitwise_equal_to_sequential": bool(same_e and same_g),
+        "bitwise_equal_to_sequential": bool(same_a and same_b),
         "shapes": batch.shapes(),
         "autotune_candidates": len(tune),
     }
@@ -444,12 +447,10 @@ def main() -> int:
     ap.add_argument("--mid-steps", type=int, default=20,
                     help="substeps timed on the mid dam break (128,000 particles; 0: skip)")
     ap.add_argument("--graphs", action="store_true",
-                    help="replay captured HIP graphs in the plain-solver legs instead of "
-                         "launching every kernel eagerly (on ROCm 7.2 a graph launch costs the "
-                         "host ~15-20 us per kernel node, more than the eager launches it "
-                         "replaces, so eager launches keep the device busier)")
-    ap.add_argument("--batch-no-graphs", action="store_true",
-                    help="launch the batched leg's kernels eagerly instead of replaying graphs")
+                    help="replay captured HIP graphs instead of launching every kernel eagerly "
+                         "(on ROCm 7.2 a graph launch costs the host ~15-20 us per kernel node, "
+                         "more than the eager launches it replaces, so eager launches keep the "
+                         "device busier)")
     ap.add_argument("--reference-steps", type=int, default=20,
                     help="substeps timed with the reference kappa/rho^2 form (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -461,9 +462,14 @@ def main() -> int:
     ap.add_argument("--batch-capped-steps", type=int, default=20,
                     help="substeps per scene of the same comparison with the reference's "
                          "kappa/rho^2 form, every scene at the iteration cap (0: skip it)")
+    ap.add_argument("--batch-tight-steps", type=int, default=100,
+                    help="substeps of the batched leg at eta = 0.01 %% (kappa/rho), where the "
+                         "scenes leave the density solve at different iterations (0: skip)")
     ap.add_argument("--batch-warmup", type=int, default=3)
     ap.add_argument("--tune-reps", type=int, default=3)
     ap.add_argument("--only-batch", action="store_true", help="run only the batched leg")
+    ap.add_argument("--batch-eta", type=float, default=1e-3,
+                    help="eta of the batched leg run by --only-batch")
     ap.add_argument("--adjoint-substeps", type=int, default=200,
                     help="substeps of the adjoint leg (0: skip it)")
     ap.add_argument("--only-adjoint", action="store_true", help="run only the adjoint leg")
@@ -516,11 +522,11 @@ def main() -> int:
         print(json.dumps({"adjoint": adjoint_leg(args.adjoint_substeps, 20)}))
         return 0
     graphs = bool(args.graphs)
-    batch_graphs = not args.batch_no_graphs
+    batch_graphs = graphs
     if args.only_batch:
         print(json.dumps({"batched": batched_leg(args.batch_steps, args.batch_warmup,
                                                  args.tune_reps, 1, bool(args.fast_math),
-                                                 batch_graphs)}))
+                                                 batch_graphs, eta=args.batch_eta)}))
         return 0
 
     x, xb, cfg = make_workload(args.n_side)
@@ -608,11 +614,16 @@ def main() -> int:
     batched = None
     if args.batch_steps > 0:
         batched = batched_leg(args.batch_steps, args.batch_warmup, args.tune_reps, 1,
-                              bool(args.fast_math), batch_graphs, eager_sequential=False)
+                              bool(args.fast_math), batch_graphs, both_launch_modes=False)
+    batched_tight = None
+    if args.batch_tight_steps > 0:
+        batched_tight = batched_leg(args.batch_tight_steps, args.batch_warmup, args.tune_reps, 1,
+                                    bool(args.fast_math), batch_graphs, both_launch_modes=False,
+                                    eta=1e-4)
     batched_capped = None
     if args.batch_capped_steps > 0:
         batched_capped = batched_leg(args.batch_capped_steps, args.batch_warmup, args.tune_reps,
-                                     0, bool(args.fast_math), batch_graphs, eager_sequential=False)
+                                     0, bool(args.fast_math), batch_graphs, both_launch_modes=False)
 
     adjoint = None
     if args.adjoint_substeps > 0:
@@ -681,6 +692,7 @@ def main() -> int:
         "mid_dam_break": mid,
         "reference_pressure_form": reference_form,
         "batched": batched,
+        "batched_tight_eta": batched_tight,
         "batched_reference_form_capped": batched_capped,
         "adjoint": adjoint,
         "lbvh": lbvh_leg(x, args.lbvh_reps) if args.lbvh_reps > 0 else None,
