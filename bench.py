@@ -283,7 +283,7 @@ def batched_leg(steps: int, warmup: int, tune_reps: int, pressure_form: int = 1,
     t_same, same_a = sequential(graphs)  # the batch's own launch mode
     t_other, same_b = sequential(not graphs) if both_launch_modes else (None, True)
     t_seq = min(t for t in (t_same, t_other) if t is not None)
-    eta = float(cfg.eta_density * cfg.rest_density)
+    eta_abs = float(cfg.eta_density * cfg.rest_density)
     return {
         "workload": f"uneven batched mix: 64 box scenes (0.6 m, r=0.005 m, walls 0.01 m x 3 "
                     f"layers), {min(counts)}..{max(counts)} particles per scene, {total} total, "
@@ -306,7 +306,7 @@ def batched_leg(steps: int, warmup: int, tune_reps: int, pressure_form: int = 1,
                                            float(it.max(1).mean())],
         "substeps_with_uneven_exit": float(np.mean(it.min(1) != it.max(1))),
         "per_scene_mean_iters_min_max": [float(it.mean(0).min()), float(it.mean(0).max())],
-        "scenes_meeting_eta_last_substep": int(sum(e < eta for e in err_final)),
+        "scenes_meeting_eta_last_substep": int(sum(e < eta_abs for e in err_final)),
         "bitwise_equal_to_sequential": bool(same_a and same_b),
         "shapes": batch.shapes(),
         "autotune_candidates": len(tune),
