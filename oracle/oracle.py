@@ -75,6 +75,7 @@ _SIGS = {
     "orc_tape_builds": (sz, [vp]),
     "orc_solver_set_tape": (None, [vp, vp, C.c_int]),
     "orc_set_xsph_in_place": (None, [C.c_int]),
+    "orc_set_analysis": (None, [C.c_int, C.c_int]),
     "orc_get_threads": (C.c_int, []),
     "orc_W_f32": (C.c_float, [C.c_float, C.c_float]),
     "orc_W_f64": (C.c_double, [C.c_double, C.c_double]),
@@ -239,6 +240,13 @@ def lattice_block(lo, hi, spacing: float) -> np.ndarray:
     out = np.empty((n, 3), np.float32)
     lib().orc_lattice_block(_f3(lo), _f3(hi), spacing, out.ctypes.data, n)
     return out
+
+
+def set_analysis(freeze_geometry: bool = False, free_advect: int = 0) -> None:
+    """Analysis-only switches (0 everywhere else): dual-number kernel terms
+    without position tangents; advection without the |v| clamp (bit 0) or the
+    domain clamp (bit 1)."""
+    lib().orc_set_analysis(int(freeze_geometry), int(free_advect))
 
 
 def damping_decay(damping, dt) -> float:

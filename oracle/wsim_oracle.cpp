@@ -135,6 +135,18 @@ enum : uint16_t {
 // produce, dfsph_kernels.cu:317,338-340) instead of the Jacobi sweep.
 int g_xsph_in_place = 0;
 
+// Analysis-only switches for the study of the adjoint's sensitivity growth
+// (tests/test_oracle_adjoint_growth.py); both 0 in every other use.
+//   g_freeze_geometry: for R = DualF, the kernel values and gradients carry
+//     no tangent, so positions reach the next state only through advection
+//     (the pressure, XSPH and boundary terms act on the velocity tangents
+//     with the geometry held at its values);
+//   g_free_advect: bit 0 drops the |v| clamp, bit 1 the domain clamp.
+int g_freeze_geometry = 0;
+int g_free_advect = 0;
+template <class R> inline R geom(R a) { return a; }
+template <> inline DualF geom(DualF a) { return g_freeze_geometry ? DualF(a.v) : a; }
+
 // Constants are the reference's binary32 literals, widened for R = double.
 template <class R> inline R kpi() { return R(3.14159265358979323846f); }
 template <class R> inline R k1em9() { return R(1e-9f); }
@@ -142,7 +154,8 @@ template <class R> inline R k1em12() { return R(1e-12f); }
 
 // ---------------------------------------------------------------------------
 // sph_kernels.cuh:12-22  cubic spline W(r, l), support [0, 2l)
-template <class R> inline R W(R r, R l) {
+template <class R> inline R W(R r_in, R l) {
+    const R r = geom(r_in);
     const R q = r / l;
     if (q >= R(2)) return R(0);
     const R sigma = R(1) / (kpi<R>() * l * l * l);
@@ -153,7 +166,8 @@ template <class R> inline R W(R r, R l) {
 }
 
 // sph_kernels.cuh:25-41  grad_i W(x_i - x_j, l)
-template <class R> inline V3<R> gradW(const V3<R>& rij, R l) {
+template <class R> inline V3<R> gradW(const V3<R>& rij_in, R l) {
+    const V3<R> rij{geom(rij_in.x), geom(rij_in.y), geom(rij_in.z)};
     const R r2 = rij.x * rij.x + rij.y * rij.y + rij.z * rij.z;
     const R r = o_sqrt(r2);
     if (r < k1em12<R>()) return {R(0), R(0), R(0)};
@@ -764,7 +778,8 @@ template <class R> struct Solver {
             R* vc[3] = {&vv.x, &vv.y, &vv.z};
             uint16_t f = 0;
             const R sp = o_sqrt(vv.x * vv.x + vv.y * vv.y + vv.z * vv.z);
-            if (cin ? (cin[i] & kClampSpeed) : (sp > vmax)) {
+            const bool speed_on = !(g_free_advect & 1), box_on = !(g_free_advect & 2);
+            if (speed_on && (cin ? (cin[i] & kClampSpeed) : (sp > vmax))) {
                 const R s = vmax / sp;
                 vv.x *= s; vv.y *= s; vv.z *= s;
                 f |= kClampSpeed;
@@ -772,7 +787,7 @@ template <class R> struct Solver {
             V3<R> p{x[i].x + vv.x * dtr, x[i].y + vv.y * dtr, x[i].z + vv.z * dtr};
             R* pc[3] = {&p.x, &p.y, &p.z};
             for (int a = 0; a < 3; ++a) {
-                if (cin ? (cin[i] & (kClampLo << a)) : (*pc[a] < lo[a])) {
+                if (box_on && (cin ? (cin[i] & (kClampLo << a)) : (*pc[a] < lo[a]))) {
                     *pc[a] = lo[a];
                     f |= kClampLo << a;
                     if (cin ? (cin[i] & (kZeroLo << a)) : (*vc[a] < R(0))) {
@@ -782,7 +797,7 @@ template <class R> struct Solver {
                 }
             }
             for (int a = 0; a < 3; ++a) {
-                if (cin ? (cin[i] & (kClampHi << a)) : (*pc[a] > hi[a])) {
+                if (box_on && (cin ? (cin[i] & (kClampHi << a)) : (*pc[a] > hi[a]))) {
                     *pc[a] = hi[a];
                     f |= kClampHi << a;
                     if (cin ? (cin[i] & (kZeroHi << a)) : (*vc[a] > R(0))) {
@@ -917,6 +932,10 @@ u32 spread3(u32 v) {
 extern "C" {
 
 void orc_set_xsph_in_place(int on) { g_xsph_in_place = on; }
+void orc_set_analysis(int freeze_geometry, int free_advect) {
+    g_freeze_geometry = freeze_geometry;
+    g_free_advect = free_advect;
+}
 
 void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 int orc_get_threads(void) { return omp_get_max_threads(); }

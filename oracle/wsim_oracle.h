@@ -86,6 +86,9 @@ void orc_set_threads(int n);
 /* Test-only: XSPH as an in-place sequential sweep (one schedule of the
  * reference's racy in-place kernel) instead of the Jacobi sweep. */
 void orc_set_xsph_in_place(int on);
+/* analysis-only: DualF kernel terms without position tangents; advection
+   without the |v| clamp (bit 0) or the domain clamp (bit 1) */
+void orc_set_analysis(int freeze_geometry, int free_advect);
 int  orc_get_threads(void);
 
 /* --- SPH kernel (core/include/water/core/sph_kernels.cuh:12-41) --- */
