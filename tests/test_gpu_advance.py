@@ -1,10 +1,13 @@
 """The CFL substep loop (sim_cli's frame loop, apps/sim_cli/main.cpp:171-178)
 run by the library: DFSPHSolver.advance against the same loop written on the
-host around step(dt).  The time steps, positions, velocities and statistics
-must agree bit for bit, with the next substep's grid phase pipelined behind
-the state read, across first iteration chunks that do and do not finish the
-solve, with and without graph replay, and on the serial paths (divergence
-solve, kernel timing, an empty store).
+host around step(dt).  The time steps (computed on the device by advance,
+on the host by the loop), positions, velocities, the attributes advance()
+writes back when it returns (it keeps the state in working-slot order
+between substeps) and the statistics must agree bit for bit, with the next
+substep's grid phase pipelined behind the state read, across first
+iteration chunks that do and do not finish the solve, with and without graph
+replay, and on the serial paths (divergence solve, kernel timing, an empty
+store).
 """
 from __future__ import annotations
 
@@ -43,10 +46,9 @@ def _host_loop(solver, ts, v_floor, time, max_sub, r, iters=None):
 
 
 def _first_chunk(last):
-    c = 2
-    while c < last:
-        c *= 2
-    return c
+    # the library's first chunk of density iterations: one more than the
+    # previous substep needed
+    return max(1, min(last + 1, 64))
 
 
 @pytest.mark.parametrize("graphs", [False, True])
@@ -70,6 +72,9 @@ def test_advance_matches_host_loop(graphs, over):
         assert [q["density_iters"] for q in recs] == iters[n0:]
         assert_bitexact(fb.get_positions(), fa.get_positions(), f"frame {frame} x")
         assert_bitexact(fb.get_velocities(), fa.get_velocities(), f"frame {frame} v")
+        # the attributes advance() writes back when it returns
+        for name in ("dfsph_density", "dfsph_alpha", "dfsph_kappa", "dfsph_density_adv"):
+            assert_bitexact(fb.get_attribute(name), fa.get_attribute(name), f"frame {frame} {name}")
         sa, sb = a.stats(), b.stats()
         for k in ("density_iters", "divergence_iters", "nan_count"):
             assert sa[k] == sb[k], (frame, k)

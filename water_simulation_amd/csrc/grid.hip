@@ -213,6 +213,16 @@ void CellGrid::build(const float* positions_xyz, size_t count) {
     build_cell_start(k_[cur_].data(), n, g_.C + 1, cs_.data(), stream_, cpt_);
 }
 
+void CellGrid::build_staged(size_t count) {
+    if (count > capacity_) fail(kInvalidArgument, "CellGrid: staged keys exceed the capacity");
+    count_ = count;
+    if (count == 0) return;
+    const u32 n = static_cast<u32>(count);
+    cur_ = radix_sort_pairs(k_[0].data(), v_[0].data(), k_[1].data(), v_[1].data(), count,
+                            key_bits(g_.C), hist_.data(), stream_);
+    build_cell_start(k_[cur_].data(), n, g_.C + 1, cs_.data(), stream_, cpt_);
+}
+
 void CellGrid::reserve_segments(u32 scenes) {
     if (scenes == 0) fail(kInvalidArgument, "CellGrid: at least one scene");
     if (scenes == segments_) return;

@@ -1399,6 +1399,132 @@ __global__ __launch_bounds__(NS) void k_advect_gather(
     vstats_partial(q, V);
 }
 
+// ---- advance(): state kept in working-slot order between substeps ----------
+//
+// Inside advance() a substep's advection writes the advected state in its
+// own working-slot order (xa, va) instead of scattering it to the caller's
+// store; the next substep gathers from there and the store is written once,
+// when advance() returns.  The grid, the permutation and every value are
+// the ones the store round trip gives.
+
+// k_advect on one scene's working slots, writing xa/va in slot order.  The
+// statistics partials cover the same 256-slot blocks as k_advect_gather's
+// 256-index blocks (binary64 momentum and energy sums then add in another
+// order; max |v|, NaN and escapee counts are order-free).
+__global__ __launch_bounds__(NS) void k_advect_sorted(
+    Nbr P, const float4* __restrict__ xs, const float4* __restrict__ vs, float vmax, float lx,
+    float ly, float lz, float hx, float hy, float hz, float4* __restrict__ xa,
+    float4* __restrict__ va, const DeviceState* __restrict__ st, int force, VStats V) {
+    const u32 i = blockIdx.x * NS + threadIdx.x;
+    PStat q;
+    if (!advect_due(st[0], force)) return;  // grid-uniform
+    const SceneSeg& G = P.seg[0];
+    if (i < G.n) {
+        const float dt = G.dt;
+        const float4 x4 = xs[i];
+        float4 v = vs[i];
+        const float sp = norm3(v.x, v.y, v.z);
+        if (sp > vmax) {
+            const float f = vmax / sp;
+            v.x *= f;
+            v.y *= f;
+            v.z *= f;
+        }
+        float px = x4.x + v.x * dt, py = x4.y + v.y * dt, pz = x4.z + v.z * dt;
+        if (px < lx) { px = lx; if (v.x < 0) v.x = 0; }
+        if (py < ly) { py = ly; if (v.y < 0) v.y = 0; }
+        if (pz < lz) { pz = lz; if (v.z < 0) v.z = 0; }
+        if (px > hx) { px = hx; if (v.x > 0) v.x = 0; }
+        if (py > hy) { py = hy; if (v.y > 0) v.y = 0; }
+        if (pz > hz) { pz = hz; if (v.z > 0) v.z = 0; }
+        xa[i] = make_float4(px, py, pz, 0.0f);
+        va[i] = make_float4(v.x, v.y, v.z, 0.0f);
+        const float lo[3] = {lx, ly, lz}, hi[3] = {hx, hy, hz};
+        q = particle_stat(P.mass, px, py, pz, v.x, v.y, v.z, lo, hi);
+    }
+    vstats_partial(q, V);
+}
+
+// Cell keys (cell_grid.cu:24-30) in original index order, each index reading
+// its particle through inv (its slot in the substep just advected): the
+// advected position, or, when that substep's solve has not finished
+// (nothing advected), the position it started from, which rebuilds that
+// substep's own grid.
+__global__ __launch_bounds__(NS) void k_keys_sorted(u32 n, GridGeom g,
+                                                    const u32* __restrict__ inv,
+                                                    const float4* __restrict__ xa,
+                                                    const float4* __restrict__ xs,
+                                                    const DeviceState* __restrict__ st,
+                                                    u32* __restrict__ keys,
+                                                    u32* __restrict__ vals) {
+    const u32 id = blockIdx.x * NS + threadIdx.x;
+    if (id >= n) return;
+    const u32 o = inv[id];
+    const float4 p = st[0].advected ? xa[o] : xs[o];
+    keys[id] = g.key_of(p.x, p.y, p.z);
+    vals[id] = id;
+}
+
+// Sorted positions of the new grid gathered from the advected slots; records
+// each slot's previous slot (for the velocities, gathered once dt is known)
+// and moves inv to the new slots.  Without an advection the permutation is
+// the previous one and nothing moves.
+__global__ __launch_bounds__(NS) void k_reorder_sorted(Nbr P, const u32* __restrict__ perm,
+                                                       const float4* __restrict__ xa,
+                                                       const DeviceState* __restrict__ st,
+                                                       u32* __restrict__ inv,
+                                                       u32* __restrict__ src,
+                                                       float4* __restrict__ xs) {
+    const u32 s = blockIdx.x * NS + threadIdx.x;
+    if (!st[0].advected) return;  // grid-uniform
+    if (s >= P.seg[0].n) return;
+    const u32 id = perm[s];
+    const u32 o = inv[id];
+    xs[s] = xa[o];
+    src[s] = o;
+    inv[id] = s;
+}
+
+// Sorted velocities with gravity and damping (dfsph_kernels.cu:347-355), the
+// expression of k_reorder, gathered from the advected slots.
+__global__ __launch_bounds__(NS) void k_reorder_v_sorted(Nbr P, const u32* __restrict__ src,
+                                                         const float4* __restrict__ va,
+                                                         float4* __restrict__ vs, int gravity) {
+    const u32 s = blockIdx.x * NS + threadIdx.x;
+    const SceneSeg& G = P.seg[0];
+    if (s >= G.n) return;
+    const float4 w = va[src[s]];
+    float a = w.x, b = w.y, c = w.z;
+    if (gravity) {
+        a = a * G.decay + G.gdx;
+        b = b * G.decay + G.gdy;
+        c = c * G.decay + G.gdz;
+    }
+    vs[s] = make_float4(a, b, c, 0.0f);
+}
+
+// The state of advance()'s last substep to the caller's store, in original
+// order: positions and velocities from xa/va, and the substep's attributes.
+__global__ __launch_bounds__(NS) void k_write_back_sorted(
+    Nbr P, const u32* __restrict__ perm, const float4* __restrict__ xa,
+    const float4* __restrict__ va, StoreOut out, const float* __restrict__ rho,
+    const float* __restrict__ alpha, const float* __restrict__ kappa,
+    const float* __restrict__ rho_adv, int wb_solve) {
+    const u32 s = blockIdx.x * NS + threadIdx.x;
+    if (s >= P.seg[0].n) return;
+    const u32 id = perm[s];
+    const u64 o = 3ull * id;
+    const float4 x = xa[s], v = va[s];
+    *reinterpret_cast<float3*>(out.x + o) = make_float3(x.x, x.y, x.z);
+    *reinterpret_cast<float3*>(out.v + o) = make_float3(v.x, v.y, v.z);
+    out.rho[id] = rho[s];
+    out.alpha[id] = alpha[s];
+    if (wb_solve) {
+        out.kappa[id] = kappa[s];
+        out.rho_adv[id] = rho_adv[s];
+    }
+}
+
 // End of the divergence solve: velocities and the recomputed attributes back
 // to original order, then max |v|.
 __global__ __launch_bounds__(NS) void k_div_writeback(Nbr P, const float4* __restrict__ xs,
@@ -1868,6 +1994,12 @@ void DFSPHSolver::init(const std::vector<u32>& counts, const std::vector<float>&
     pairs_fi_ = DeviceBuffer<uint16_t>(static_cast<size_t>(kPairDepthFluid) * slots_cap);
     if (cfg_.enable_surface_tension) nrm_ = DeviceBuffer<float4>(slots_cap);
     inv_ = DeviceBuffer<u32>(slots_cap);
+    if (!batched_) {
+        xa_ = DeviceBuffer<float4>(slots_cap);
+        va_ = DeviceBuffer<float4>(slots_cap);
+        src_ = DeviceBuffer<u32>(slots_cap);
+    }
+    state_sorted_ = grid_sorted_ = defer_store_ = false;
     plan_ = DeviceBuffer<StageRanges>(slots_cap / kTile);
     part_d_ = DeviceBuffer<double>(slots_cap / kTile);
     part_f_ = DeviceBuffer<float>(slots_cap / 64);
@@ -2069,25 +2201,44 @@ u32 DFSPHSolver::advance(const TimeStepConfig& ts, float v_floor, float time, u3
         WSIM_LAUNCH_CHECK();
     }
     bool grid_ready = false;  // the next substep's grid phase is enqueued
-    while (rem > 1e-6f && sub < max_substeps) {
-        const float dt = next_dt(ts, std::max(max_velocity(), v_floor), cfg_.particle_radius, rem);
-        if (!pipe) {
-            step(dt);
-        } else {
-            const bool next = rem - dt > 1e-6f && sub + 1 < max_substeps;
-            if (!grid_ready) run_phase(phase_key(kPhPrefixGrid), [&] { phase_prefix_grid(); });
-            grid_ready = substep_device(next);
-            const float dd = hstate_[0].dt;
-            if (std::memcmp(&dd, &dt, sizeof(float)) != 0)
-                fail(kRuntimeError, "advance: the device time step differs from the host's");
-            hseg_[0].dt = dt;
-            hseg_[0].dt2 = dt * dt;
+    // between the substeps of this call the state stays in working-slot
+    // order; the store is written when the call returns (or throws)
+    defer_store_ = pipe;
+    try {
+        while (rem > 1e-6f && sub < max_substeps) {
+            const float dt =
+                next_dt(ts, std::max(max_velocity(), v_floor), cfg_.particle_radius, rem);
+            if (!pipe) {
+                step(dt);
+            } else {
+                const bool next = rem - dt > 1e-6f && sub + 1 < max_substeps;
+                if (!grid_ready) {
+                    grid_sorted_ = state_sorted_;
+                    run_phase(phase_key(kPhPrefixGrid, state_sorted_ ? 1 : 0),
+                              [&] { phase_prefix_grid(); });
+                }
+                grid_ready = substep_device(next);
+                const float dd = hstate_[0].dt;
+                if (std::memcmp(&dd, &dt, sizeof(float)) != 0)
+                    fail(kRuntimeError, "advance: the device time step differs from the host's");
+                hseg_[0].dt = dt;
+                hseg_[0].dt2 = dt * dt;
+            }
+            if (dts) dts[sub] = dt;
+            if (stats) stats[sub] = stats_[0];
+            rem -= dt;
+            ++sub;
         }
-        if (dts) dts[sub] = dt;
-        if (stats) stats[sub] = stats_[0];
-        rem -= dt;
-        ++sub;
+    } catch (...) {
+        defer_store_ = false;
+        try {
+            write_back_sorted();
+        } catch (...) {
+        }
+        throw;
     }
+    defer_store_ = false;
+    write_back_sorted();
     return sub;
 }
 
@@ -2132,7 +2283,8 @@ bool DFSPHSolver::substep_device(bool next) {
         // tail would rebuild this substep's grid again whenever the solve
         // goes on past it
         const bool tail = next && first;
-        const u32 flags = (tail ? 1u : 0u) | (last ? 2u : 0u) | (parity_ << 2);
+        const u32 flags =
+            (tail ? 1u : 0u) | (last ? 2u : 0u) | (parity_ << 2) | (grid_sorted_ ? 8u : 0u);
         const u64 key = first ? phase_key(kPhSubstep, end, flags)
                               : phase_key(kPhContinue, begin, end | (flags << 16));
         run_phase(key, [&] {
@@ -2146,12 +2298,18 @@ bool DFSPHSolver::substep_device(bool next) {
             k_publish<<<1, 64, 0, stream_>>>(dstate_.data(), pubseq_.data(), pub_dev_);
             WSIM_LAUNCH_CHECK();
         });
+        // the advection above writes xa_/va_: later grids gather from there
+        state_sorted_ = true;
         // behind an unfinished solve this rebuilds exactly this substep's
         // grid, lists, density and alpha from the unchanged store positions
         // (functions of the positions only).  A graph of its own: launching
         // the next substep while a large graph still runs its tail stalls
         // that tail's dispatch.
-        if (tail) run_phase(phase_key(kPhPrefixGrid), [&] { phase_prefix_grid(); });
+        if (tail) {
+            grid_sorted_ = state_sorted_;
+            run_phase(phase_key(kPhPrefixGrid, state_sorted_ ? 1 : 0),
+                      [&] { phase_prefix_grid(); });
+        }
         wait_published();
         if (hstate_[0].advected) {
             take_state();
@@ -2213,19 +2371,45 @@ void DFSPHSolver::phase_prefix(bool gravity) {
 
 // The part of a substep's prefix that does not depend on its time step:
 // grid, sorted positions, staging plans, density, alpha and pair lists.
+// The grid phase gathers from xa_/va_ when state_sorted_ (advance()); the
+// caller sets grid_sorted_ = state_sorted_ before enqueuing it (outside any
+// captured body, so graph replays see the same), which selects the matching
+// velocity gather in phase_prefix_solve.
 void DFSPHSolver::phase_prefix_grid() {
     const u32 scenes = static_cast<u32>(counts_.size());
-    tbegin(kKGrid);
-    if (batched_) fluid_grid_.build_segmented(fluid_.positions(), total_, dstarts_.data(), scenes);
-    else fluid_grid_.build(fluid_.positions(), total_);
-    tend();
+    if (state_sorted_) {
+        // advance(): keys and positions from the advected slots (xa_, inv_)
+        const u32 n = total_;
+        tbegin(kKGrid);
+        k_keys_sorted<<<div_up(n, NS), NS, 0, stream_>>>(n, fluid_grid_.geom(), inv_.data(),
+                                                         xa_.data(), xs_.data(), dstate_.data(),
+                                                         fluid_grid_.staged_keys(),
+                                                         fluid_grid_.staged_values());
+        WSIM_LAUNCH_CHECK();
+        fluid_grid_.build_staged(n);
+        tend();
+        const Nbr P = nbr();
+        tbegin(kKReorder);
+        k_reorder_sorted<<<nslots_ / NS, NS, 0, stream_>>>(P, fluid_grid_.sorted_indices(),
+                                                           xa_.data(), dstate_.data(), inv_.data(),
+                                                           src_.data(), xs_.data());
+        WSIM_LAUNCH_CHECK();
+        tend();
+    } else {
+        tbegin(kKGrid);
+        if (batched_) fluid_grid_.build_segmented(fluid_.positions(), total_, dstarts_.data(), scenes);
+        else fluid_grid_.build(fluid_.positions(), total_);
+        tend();
+        const Nbr P = nbr();
+        tbegin(kKReorder);
+        k_reorder<<<nslots_ / NS, NS, 0, stream_>>>(P, fluid_grid_.sorted_indices(),
+                                                     fluid_.positions(), fluid_.velocities(),
+                                                     xs_.data(), nullptr, 0, 1,
+                                                     batched_ ? nullptr : inv_.data());
+        WSIM_LAUNCH_CHECK();
+        tend();
+    }
     const Nbr P = nbr();
-    tbegin(kKReorder);
-    k_reorder<<<nslots_ / NS, NS, 0, stream_>>>(P, fluid_grid_.sorted_indices(), fluid_.positions(),
-                                                 fluid_.velocities(), xs_.data(), nullptr, 0, 1,
-                                                 batched_ ? nullptr : inv_.data());
-    WSIM_LAUNCH_CHECK();
-    tend();
     k_stage_plan<<<div_up(nslots_ / kTile, NS), NS, 0, stream_>>>(P, plan_.data());
     WSIM_LAUNCH_CHECK();
     tbegin(kKDensityAlpha);
@@ -2242,9 +2426,14 @@ void DFSPHSolver::phase_prefix_solve(bool gravity) {
     const bool xsph = gravity && any_xsph_;
     float4* vtarget = xsph ? vpre_.data() : vs_.data();
     tbegin(kKReorder);
-    k_reorder<<<nslots_ / NS, NS, 0, stream_>>>(P, fluid_grid_.sorted_indices(), fluid_.positions(),
-                                                 fluid_.velocities(), xs_.data(), vtarget,
-                                                 gravity ? 1 : 0, 2, nullptr);
+    if (grid_sorted_)
+        k_reorder_v_sorted<<<nslots_ / NS, NS, 0, stream_>>>(P, src_.data(), va_.data(), vtarget,
+                                                             gravity ? 1 : 0);
+    else
+        k_reorder<<<nslots_ / NS, NS, 0, stream_>>>(P, fluid_grid_.sorted_indices(),
+                                                     fluid_.positions(), fluid_.velocities(),
+                                                     xs_.data(), vtarget, gravity ? 1 : 0, 2,
+                                                     nullptr);
     WSIM_LAUNCH_CHECK();
     tend();
     if (xsph) {
@@ -2360,7 +2549,13 @@ void DFSPHSolver::phase_advect(bool force) {
     const StoreOut out = store_out(fluid_, h_density_, h_alpha_, h_kappa_, h_density_adv_);
     const float eps = 1e-3f;  // dfsph_kernels.cu:377
     tbegin(kKAdvect);
-    if (!batched_) {
+    if (defer_store_) {
+        k_advect_sorted<<<nslots_ / NS, NS, 0, stream_>>>(
+            P, xs_.data(), vs_.data(), 5.0f, cfg_.domain_min[0] + eps, cfg_.domain_min[1] + eps,
+            cfg_.domain_min[2] + eps, cfg_.domain_max[0] - eps, cfg_.domain_max[1] - eps,
+            cfg_.domain_max[2] - eps, xa_.data(), va_.data(), dstate_.data(), force ? 1 : 0,
+            vstats());
+    } else if (!batched_) {
         k_advect_gather<<<nslots_ / NS, NS, 0, stream_>>>(
             P, xs_.data(), vs_.data(), inv_.data(), 5.0f, cfg_.domain_min[0] + eps,
             cfg_.domain_min[1] + eps, cfg_.domain_min[2] + eps, cfg_.domain_max[0] - eps,
@@ -2382,6 +2577,17 @@ void DFSPHSolver::phase_advect(bool force) {
     k_finish_vstats<1024><<<static_cast<u32>(counts_.size()), 1024, 0, stream_>>>(
         dseg_.data(), vstats(), dstate_.data(), 1, force ? 1 : 0);
     WSIM_LAUNCH_CHECK();
+}
+
+void DFSPHSolver::write_back_sorted() {
+    if (!state_sorted_) return;
+    const Nbr P = nbr();
+    const StoreOut out = store_out(fluid_, h_density_, h_alpha_, h_kappa_, h_density_adv_);
+    k_write_back_sorted<<<nslots_ / NS, NS, 0, stream_>>>(
+        P, fluid_grid_.sorted_indices(), xa_.data(), va_.data(), out, rho_.data(), alpha_.data(),
+        kappa_.data(), rho_adv_.data(), cfg_.max_density_iters > 0 ? 1 : 0);
+    WSIM_LAUNCH_CHECK();
+    state_sorted_ = false;
 }
 
 void DFSPHSolver::phase_div_tail() {
@@ -2435,6 +2641,7 @@ void DFSPHSolver::solve(int mode) {
 }
 
 void DFSPHSolver::run_substep() {
+    grid_sorted_ = false;  // step() paths gather from the store
     run_phase(phase_key(kPhPrefix), [&] { phase_prefix(true); });
     if (cfg_.enable_divergence_solve) {
         solve(0);
@@ -2590,6 +2797,7 @@ void DFSPHSolver::step_taped(float dt, SubstepTape& t) {
     WSIM_HIP(hipMemcpyAsync(t.v.data(), fluid_.velocities(), 3 * nsz * sizeof(float),
                             hipMemcpyDeviceToDevice, stream_));
     set_substep(&dt);
+    grid_sorted_ = false;
     phase_prefix(true);
     WSIM_HIP(hipMemcpyAsync(t.w.data(), vs_.data(), ns * sizeof(float4), hipMemcpyDeviceToDevice,
                             stream_));
@@ -2642,6 +2850,7 @@ void DFSPHSolver::autotune(int reps) {
     std::vector<float> dt(scenes);
     for (u32 s = 0; s < scenes; ++s) dt[s] = hseg_[s].dt > 0.0f ? hseg_[s].dt : 1e-3f;
     set_substep(dt.data());
+    grid_sorted_ = false;
     phase_prefix(true);
 
     hipEvent_t ev[2];

@@ -90,6 +90,13 @@ public:
     void set_cells_per_thread(u32 cpt);
     u32 cells_per_thread() const { return cpt_; }
 
+    // Keys and original indices written by the caller into staged_keys() and
+    // staged_values() (count <= capacity() entries each), then sorted and
+    // tabled as build() does with the keys it computes itself.
+    u32* staged_keys() { return k_[0].data(); }
+    u32* staged_values() { return v_[0].data(); }
+    void build_staged(size_t count);
+
     const u32* cell_start() const { return cs_.data(); }
     const u32* sorted_indices() const { return v_[cur_].data(); }
     const u32* sorted_keys() const { return k_[cur_].data(); }
@@ -537,6 +544,7 @@ private:
     dev::Nbr nbr() const;
     void phase_iterations(int mode, u32 begin, u32 end);
     void phase_advect(bool force);
+    void write_back_sorted();  // the state in xa_/va_ (and rho, alpha, ...) to the store
     void fetch_state();
     void take_state();
     void phase_div_tail();
@@ -586,6 +594,14 @@ private:
     DeviceBuffer<float4> nrm_;  // surface-tension normals (when enabled)
     DeviceBuffer<float4> gb_;   // boundary gradient sums (tolerance mode with a boundary)
     DeviceBuffer<u32> inv_;     // working slot of each original index (plain solver)
+    // advance(): the advected state in the working-slot order of the substep
+    // that advected it, and for each slot of a grid built from it, the slot
+    // its particle came from
+    DeviceBuffer<float4> xa_, va_;
+    DeviceBuffer<u32> src_;
+    bool defer_store_ = false;   // advection writes xa_/va_, not the store
+    bool state_sorted_ = false;  // the current state is in xa_/va_ (the store is stale)
+    bool grid_sorted_ = false;   // the current grid phase gathered from xa_/va_
     DeviceBuffer<u32> bnear_;  // boundary proximity mask (see stencil.hpp)
     DeviceBuffer<StageRanges> plan_;
     DeviceBuffer<u32> pair_cnt_;
