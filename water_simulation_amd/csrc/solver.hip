@@ -1214,43 +1214,124 @@ __device__ __forceinline__ PStat particle_stat(float mass, float x, float y, flo
     return r;
 }
 
+// The seven reductions of a workgroup's statistics in one pass: each wave
+// reduces all of them with shuffles, one barrier, then thread 0 combines the
+// waves in index order (the same trees as block_max / block_sum, one barrier
+// instead of fourteen).
+template <int NT> struct StatReduce {
+    float f[NT / 64];
+    u32 c[NT / 64], e[NT / 64];
+    double d[kMomParts][NT / 64];
+};
+template <int NT>
+__device__ __forceinline__ bool stat_reduce(float& m, u32& c, u32& e, double (&d)[kMomParts],
+                                            StatReduce<NT>& R) {
+    m = wave_max(m);
+    c = wave_sum_u32(c);
+    e = wave_sum_u32(e);
+#pragma unroll
+    for (int k = 0; k < kMomParts; ++k) d[k] = wave_sum(d[k]);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        R.f[w] = m;
+        R.c[w] = c;
+        R.e[w] = e;
+#pragma unroll
+        for (int k = 0; k < kMomParts; ++k) R.d[k][w] = d[k];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return false;
+    m = 0.0f;
+    c = e = 0u;
+#pragma unroll
+    for (int k = 0; k < kMomParts; ++k) d[k] = 0.0;
+    for (int i = 0; i < NT / 64; ++i) {
+        m = fmaxf(m, R.f[i]);
+        c += R.c[i];
+        e += R.e[i];
+#pragma unroll
+        for (int k = 0; k < kMomParts; ++k) d[k] += R.d[k][i];
+    }
+    return true;
+}
+
 __device__ __forceinline__ void vstats_partial(const PStat& q, const VStats& V) {
-    __shared__ float s_f[NS / 64];
-    __shared__ u32 s_u[NS / 64];
-    __shared__ double s_d[NS / 64];
-    const float m = block_max<NS>(q.mag, s_f);
-    const u32 c = block_sum_u32<NS>(q.bad, s_u);
-    const u32 e = block_sum_u32<NS>(q.esc, s_u);
-    const double d0 = block_sum<NS>(q.m, s_d);
-    const double d1 = block_sum<NS>(q.px, s_d);
-    const double d2 = block_sum<NS>(q.py, s_d);
-    const double d3 = block_sum<NS>(q.pz, s_d);
-    const double d4 = block_sum<NS>(q.ke, s_d);
-    if (threadIdx.x == 0) {
+    __shared__ StatReduce<NS> R;
+    float m = q.mag;
+    u32 c = q.bad, e = q.esc;
+    double d[kMomParts] = {q.m, q.px, q.py, q.pz, q.ke};
+    if (stat_reduce<NS>(m, c, e, d, R)) {
         V.pmax[blockIdx.x] = m;
         V.pnan[blockIdx.x] = c;
         V.pesc[blockIdx.x] = e;
         double* o = V.pmom + kMomParts * static_cast<u64>(blockIdx.x);
-        o[0] = d0;
-        o[1] = d1;
-        o[2] = d2;
-        o[3] = d3;
-        o[4] = d4;
+#pragma unroll
+        for (int k = 0; k < kMomParts; ++k) o[k] = d[k];
     }
 }
 
+// The CFL step of sim_cli's substep loop on the device (main.cpp:176-177,
+// timestep.h:23-28): dt = next_dt(max(max|v|, v_floor), r, remaining), with
+// the comparisons of std::max and std::min over {dt_max, dt_cfl, remaining,
+// max(dt_cfl, dt_floor)} in their order, then the substep parameters the
+// host otherwise uploads (dt, dt^2, the velocity decay, gravity * dt).  The
+// host evaluates the same rule on the published state and checks that both
+// agree bit for bit.
+__device__ inline void cfl_step(const FrameParams& f, SceneSeg& g, DeviceState& st) {
+    const float mv = st.max_v;
+    const float a = (mv < f.v_floor) ? f.v_floor : mv;  // std::max(max|v|, v_floor)
+    const float v = (a < 1e-6f) ? 1e-6f : a;             // std::max(v, 1e-6f)
+    const float dt_cfl = f.cfl_lambda * f.particle_radius / v;
+    const float dt_floor = f.frame_dt / static_cast<float>(f.max_substeps);
+    const float rem = st.t_rem;
+    float d = f.dt_max;
+    if (dt_cfl < d) d = dt_cfl;
+    if (rem < d) d = rem;
+    const float hi = (dt_cfl < dt_floor) ? dt_floor : dt_cfl;
+    if (hi < d) d = hi;
+    g.dt = d;
+    g.dt2 = d * d;
+    g.decay = damping_decay(f.damping, d);
+    g.gdx = f.gravity[0] * d;
+    g.gdy = f.gravity[1] * d;
+    g.gdz = f.gravity[2] * d;
+    st.dt = d;
+}
+
+// A scene's state into mapped host memory, then a new sequence number with
+// release ordering at system scope: the host that sees the number sees the
+// state.
+__device__ inline void publish_state(const DeviceState& S, u32* seq, StatePub* pub) {
+    pub->st = S;
+    const u32 k = *seq + 1u;
+    *seq = k;
+    __hip_atomic_store(&pub->seq, k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// advance()'s per-substep device step, folded into k_finish_vstats: the
+// time-step rule, the substep's published state and its sequence word.
+struct FrameStep {
+    const FrameParams* fp;  // null outside advance()
+    StatePub* pub;
+    u32* seq;
+};
+
 // One workgroup per scene.  guarded = 1: only scenes the advection just
 // processed (the advect_due test, evaluated before this kernel marks them).
+// With F.fp (advance(), one scene): publishes the state whether or not the
+// solve finished, and after an advection first takes the substep's dt off
+// the time left and then computes the next substep's dt.
 template <int NT>
-__global__ __launch_bounds__(NT) void k_finish_vstats(const SceneSeg* __restrict__ seg,
-                                                      VStats V, DeviceState* st, int guarded,
-                                                      int force) {
-    __shared__ float s_f[NT / 64];
-    __shared__ u32 s_u[NT / 64];
-    __shared__ double s_d[NT / 64];
+__global__ __launch_bounds__(NT) void k_finish_vstats(SceneSeg* __restrict__ seg, VStats V,
+                                                      DeviceState* st, int guarded, int force,
+                                                      FrameStep F) {
+    __shared__ StatReduce<NT> R;
     const SceneSeg& G = seg[blockIdx.x];
     DeviceState& S = st[blockIdx.x];
-    if (guarded && !advect_due(S, force)) return;  // block-uniform
+    if (guarded && !advect_due(S, force)) {  // block-uniform
+        if (F.fp && threadIdx.x == 0) publish_state(S, F.seq, F.pub);
+        return;
+    }
     float m = 0.0f;
     u32 c = 0, e = 0;
     double d[kMomParts] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1262,12 +1343,7 @@ __global__ __launch_bounds__(NT) void k_finish_vstats(const SceneSeg* __restrict
 #pragma unroll
         for (int k = 0; k < kMomParts; ++k) d[k] += V.pmom[kMomParts * static_cast<u64>(b) + k];
     }
-    m = block_max<NT>(m, s_f);
-    c = block_sum_u32<NT>(c, s_u);
-    e = block_sum_u32<NT>(e, s_u);
-#pragma unroll
-    for (int k = 0; k < kMomParts; ++k) d[k] = block_sum<NT>(d[k], s_d);
-    if (threadIdx.x == 0) {
+    if (stat_reduce<NT>(m, c, e, d, R)) {
         S.max_v = m;
         S.nan_count = c;
         S.escapees = e;
@@ -1277,6 +1353,11 @@ __global__ __launch_bounds__(NT) void k_finish_vstats(const SceneSeg* __restrict
         S.mom[2] = d[3];
         S.ke = d[4];
         if (guarded) S.advected = 1u;
+        if (F.fp) {
+            S.t_rem = S.t_rem - S.dt;
+            publish_state(S, F.seq, F.pub);
+            cfl_step(*F.fp, seg[blockIdx.x], S);
+        }
     }
 }
 
@@ -1576,57 +1657,14 @@ __global__ __launch_bounds__(NS) void k_reset_solve(const SceneSeg* __restrict__
     }
 }
 
-// advance(): the frame's time-step rule and budget into device memory.
+// advance(): the frame's time-step rule and budget into device memory, and
+// the first substep's time step.
 __global__ void k_frame_begin(FrameParams fp, FrameParams* __restrict__ dst,
-                              DeviceState* __restrict__ st) {
+                              SceneSeg* __restrict__ seg, DeviceState* __restrict__ st) {
     if (threadIdx.x != 0) return;
     *dst = fp;
     st->t_rem = fp.time;
-}
-
-// The CFL step of sim_cli's substep loop on the device (main.cpp:176-177,
-// timestep.h:23-28): dt = next_dt(max(max|v|, v_floor), r, remaining), with
-// the comparisons of std::max and std::min over {dt_max, dt_cfl, remaining,
-// max(dt_cfl, dt_floor)} in their order, then the substep parameters the
-// host otherwise uploads (dt, dt^2, the velocity decay, gravity * dt).  The
-// host evaluates the same rule on the published state and checks that both
-// agree bit for bit.
-__global__ void k_cfl(const FrameParams* __restrict__ fp, SceneSeg* __restrict__ seg,
-                      DeviceState* __restrict__ st) {
-    if (threadIdx.x != 0) return;
-    const FrameParams f = *fp;
-    const float mv = st->max_v;
-    const float a = (mv < f.v_floor) ? f.v_floor : mv;  // std::max(max|v|, v_floor)
-    const float v = (a < 1e-6f) ? 1e-6f : a;             // std::max(v, 1e-6f)
-    const float dt_cfl = f.cfl_lambda * f.particle_radius / v;
-    const float dt_floor = f.frame_dt / static_cast<float>(f.max_substeps);
-    const float rem = st->t_rem;
-    float d = f.dt_max;
-    if (dt_cfl < d) d = dt_cfl;
-    if (rem < d) d = rem;
-    const float hi = (dt_cfl < dt_floor) ? dt_floor : dt_cfl;
-    if (hi < d) d = hi;
-    SceneSeg& g = seg[0];
-    g.dt = d;
-    g.dt2 = d * d;
-    g.decay = damping_decay(f.damping, d);
-    g.gdx = f.gravity[0] * d;
-    g.gdy = f.gravity[1] * d;
-    g.gdz = f.gravity[2] * d;
-    st->dt = d;
-    st->t_rem = rem - d;
-}
-
-// The state of scene 0 into mapped host memory, then a new sequence number
-// with release ordering at system scope: the host that sees the number sees
-// the state.
-__global__ void k_publish(const DeviceState* __restrict__ st, u32* __restrict__ seq,
-                          StatePub* pub) {
-    if (threadIdx.x != 0) return;
-    pub->st = st[0];
-    const u32 k = *seq + 1u;
-    *seq = k;
-    __hip_atomic_store(&pub->seq, k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    cfl_step(fp, seg[0], *st);
 }
 
 // ---- launch-shape dispatch ------------------------------------------------
@@ -2166,15 +2204,17 @@ void DFSPHSolver::step(float dt) {
 }
 
 // sim_cli's CFL substep loop (main.cpp:171-178) in the library.  With the
-// divergence solve off, each substep is one launch sequence (graph): the
-// device computes the substep's time step from the last max |v| and the time
-// left (k_cfl), runs gravity, XSPH, the first chunk of density iterations
-// and advection, publishes the state to mapped host memory and goes on with
-// the next substep's grid, density, alpha and pair lists, which depend on
-// the positions only.  The host waits for the published state (its read
-// overlaps that grid phase), evaluates the same time-step rule on it for its
-// own record and checks it against the device's, and only continues the
-// density solve itself when the first chunk did not finish it.
+// divergence solve off, each substep is one launch sequence: gravity, XSPH,
+// the first chunk of density iterations and advection, whose statistics
+// kernel publishes the state to mapped host memory and computes the next
+// substep's time step from the new max |v| and the time left (the first one
+// comes from k_frame_begin); then the next substep's grid, density, alpha
+// and pair lists, which depend on the positions only.  The host waits for
+// the published state (its read overlaps that grid phase), evaluates the
+// same time-step rule for its own record and checks it against the
+// device's, and only continues the density solve when the first chunk did
+// not finish it.  Between substeps the state stays in working-slot order
+// (xa_, va_); the store is written when the call returns.
 u32 DFSPHSolver::advance(const TimeStepConfig& ts, float v_floor, float time, u32 max_substeps,
                          float* dts, StepStats* stats) {
     if (batched_) fail(kInvalidArgument, "advance: a batched solver steps with step_scenes");
@@ -2197,7 +2237,7 @@ u32 DFSPHSolver::advance(const TimeStepConfig& ts, float v_floor, float time, u3
         fp.damping = cfg_.damping;
         for (int a = 0; a < 3; ++a) fp.gravity[a] = cfg_.gravity[a];
         fp.time = time;
-        k_frame_begin<<<1, 64, 0, stream_>>>(fp, dframe_.data(), dstate_.data());
+        k_frame_begin<<<1, 64, 0, stream_>>>(fp, dframe_.data(), dseg_.data(), dstate_.data());
         WSIM_LAUNCH_CHECK();
     }
     bool grid_ready = false;  // the next substep's grid phase is enqueued
@@ -2242,8 +2282,8 @@ u32 DFSPHSolver::advance(const TimeStepConfig& ts, float v_floor, float time, u3
     return sub;
 }
 
-// Waits for the state k_publish writes to mapped host memory and copies it
-// to hstate_.  A stream that fails, or finishes without publishing, ends the
+// Waits for the state k_finish_vstats publishes to mapped host memory and
+// copies it to hstate_.  A stream that fails, or finishes without publishing, ends the
 // wait with an error.
 // The wait polls only the published word for its first 20 ms (a substep
 // takes well under that; querying the stream meanwhile competes with the
@@ -2288,15 +2328,11 @@ bool DFSPHSolver::substep_device(bool next) {
         const u64 key = first ? phase_key(kPhSubstep, end, flags)
                               : phase_key(kPhContinue, begin, end | (flags << 16));
         run_phase(key, [&] {
-            if (first) {
-                k_cfl<<<1, 64, 0, stream_>>>(dframe_.data(), dseg_.data(), dstate_.data());
-                WSIM_LAUNCH_CHECK();
-                phase_prefix_solve(true);
-            }
+            if (first) phase_prefix_solve(true);
             if (end > begin) phase_iterations(0, begin, end);
+            // advection; its statistics kernel publishes the state and, once
+            // the substep is done, computes the next substep's time step
             phase_advect(last);
-            k_publish<<<1, 64, 0, stream_>>>(dstate_.data(), pubseq_.data(), pub_dev_);
-            WSIM_LAUNCH_CHECK();
         });
         // the advection above writes xa_/va_: later grids gather from there
         state_sorted_ = true;
@@ -2574,8 +2610,10 @@ void DFSPHSolver::phase_advect(bool force) {
     tend();
     // also marks the scenes advected (after the divergence solve, the
     // writeback reports max |v| instead)
+    const FrameStep F = defer_store_ ? FrameStep{dframe_.data(), pub_dev_, pubseq_.data()}
+                                     : FrameStep{nullptr, nullptr, nullptr};
     k_finish_vstats<1024><<<static_cast<u32>(counts_.size()), 1024, 0, stream_>>>(
-        dseg_.data(), vstats(), dstate_.data(), 1, force ? 1 : 0);
+        dseg_.data(), vstats(), dstate_.data(), 1, force ? 1 : 0, F);
     WSIM_LAUNCH_CHECK();
 }
 
@@ -2604,7 +2642,7 @@ void DFSPHSolver::phase_div_tail() {
     WSIM_LAUNCH_CHECK();
     tend();
     k_finish_vstats<1024><<<static_cast<u32>(counts_.size()), 1024, 0, stream_>>>(
-        dseg_.data(), vstats(), dstate_.data(), 0, 1);
+        dseg_.data(), vstats(), dstate_.data(), 0, 1, FrameStep{nullptr, nullptr, nullptr});
     WSIM_LAUNCH_CHECK();
 }
 
