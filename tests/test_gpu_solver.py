@@ -177,3 +177,23 @@ def test_fluid_grid_view_matches_oracle_grid():
     bkeys, bperm, bcs = O.grid_build(xb, origin, dims, cfg["smoothing_length"])
     assert_bitexact(bg.read_sorted_indices(), bperm, "boundary grid perm")
     assert_bitexact(bg.read_cell_start(), bcs, "boundary grid cell_start")
+
+
+@pytest.mark.parametrize("seed,scale", [(11, 1.0), (12, 20.0)])
+def test_fluid_grid_partial_table_rebuild(seed, scale):
+    # the solver's grid rewrites only the cells between the previous and the
+    # current key span; over substeps whose spans grow, shrink and (scale 20,
+    # particles thrown at the walls) reach the domain clamp, every table
+    # equals the oracle's full table of the positions it was built from
+    x, v, xb, cfg = random_box(n=1500, seed=seed)
+    v = (v * scale).astype(np.float32)
+    pair = Pair(x, v, xb, cfg)
+    origin, dims = O.make_grid(cfg["domain_min"], cfg["domain_max"], cfg["smoothing_length"])
+    g = pair.gpu.fluid_grid()
+    for k in range(12):
+        x0 = pair.fluid.get_positions()
+        pair.step(0.004)
+        keys, perm, cs = O.grid_build(x0, origin, dims, cfg["smoothing_length"])
+        assert_bitexact(g.read_sorted_indices(), perm, f"substep {k} perm")
+        assert_bitexact(g.read_cell_start(), cs, f"substep {k} cell_start")
+

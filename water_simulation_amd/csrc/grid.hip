@@ -53,7 +53,9 @@ __device__ inline u32 wave_lower_bound(const u32* __restrict__ keys, u32 n, u32 
 }
 
 __global__ __launch_bounds__(256) void k_cell_keys(const float* __restrict__ xyz, u32 n, GridGeom g,
-                                                   u32* __restrict__ keys, u32* __restrict__ vals) {
+                                                   u32* __restrict__ keys, u32* __restrict__ vals,
+                                                   u32* span) {
+    advance_span(span);
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const u64 o = 3ull * i;
@@ -87,7 +89,8 @@ __global__ __launch_bounds__(256) void k_seg_keys(const float* __restrict__ xyz,
 // Each workgroup owns CPT * 256 consecutive cells.
 template <int CPT>
 __global__ __launch_bounds__(CS_THREADS) void k_cell_start(const u32* __restrict__ keys, u32 n,
-                                                           u32 ncs, u32* __restrict__ cs) {
+                                                           u32 ncs, u32* __restrict__ cs,
+                                                           u32* span) {
     constexpr int CS_PER_THREAD = CPT;
     constexpr u32 CS_TILE = CS_THREADS * CPT;
     __shared__ u32 s_mark[CS_TILE + CS_TILE / 16];
@@ -96,6 +99,25 @@ __global__ __launch_bounds__(CS_THREADS) void k_cell_start(const u32* __restrict
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const u32 c0 = blockIdx.x * CS_TILE;
     const u32 c1 = (ncs - c0 < CS_TILE) ? ncs : c0 + CS_TILE;
+    if (span && n > 0) {
+        // this build's slot and the previous build's (written by the
+        // previous launch, read-only here); all workgroups record the same
+        const u32 par = span[kSpanCounter] & 1u;
+        u32* cur = span + 4 * par;
+        const u32* prev = span + 4 * (par ^ 1u);
+        const u32 kmin = keys[0], kmax = keys[n - 1];
+        if (tid == 0) {
+            cur[0] = kmin;
+            cur[1] = kmax;
+            cur[2] = n;
+            cur[3] = 1u;
+        }
+        if (prev[3] == 1u && prev[2] == n) {
+            const u32 lo = min(kmin, prev[0]), hi = max(kmax, prev[1]);
+            // cells c <= lo hold 0 and cells c > hi hold n in both tables
+            if (c1 - 1u <= lo || c0 > hi) return;  // block-uniform
+        }
+    }
     if (w < 2) {
         const u32 r = wave_lower_bound(keys, n, w == 0 ? c0 : c1);
         if (lane == 0) s_k[w] = r;
@@ -143,19 +165,19 @@ __global__ __launch_bounds__(CS_THREADS) void k_cell_start(const u32* __restrict
 }  // namespace
 
 void build_cell_start(const u32* sorted_keys, u32 n, u32 ncs, u32* cell_start, hipStream_t stream,
-                      u32 cpt) {
+                      u32 cpt, u32* span) {
     switch (cpt) {
     case 4:
         k_cell_start<4><<<div_up(ncs, CS_THREADS * 4), CS_THREADS, 0, stream>>>(sorted_keys, n, ncs,
-                                                                                cell_start);
+                                                                                cell_start, span);
         break;
     case 8:
         k_cell_start<8><<<div_up(ncs, CS_THREADS * 8), CS_THREADS, 0, stream>>>(sorted_keys, n, ncs,
-                                                                                cell_start);
+                                                                                cell_start, span);
         break;
     case 16:
-        k_cell_start<16><<<div_up(ncs, CS_THREADS * 16), CS_THREADS, 0, stream>>>(sorted_keys, n,
-                                                                                  ncs, cell_start);
+        k_cell_start<16><<<div_up(ncs, CS_THREADS * 16), CS_THREADS, 0, stream>>>(
+            sorted_keys, n, ncs, cell_start, span);
         break;
     default:
         fail(kInvalidArgument, "build_cell_start: cells per thread must be 4, 8 or 16");
@@ -181,6 +203,17 @@ CellGrid::CellGrid(const float origin[3], const int32_t dims[3], float cell_leng
     g_.C = static_cast<u32>(C);
     cs_ = DeviceBuffer<u32>(C + 1);
     cs_.zero(stream_);  // cell_grid.cu:63
+    span_ = DeviceBuffer<u32>(kSpanWords);
+    reset_span();
+}
+
+// Both slots invalid: the next build writes the whole table.
+void CellGrid::reset_span() { span_.zero(stream_); }
+
+void CellGrid::set_track_span(bool on) {
+    if (on && segments_ != 1) fail(kInvalidArgument, "CellGrid: span tracking needs one scene");
+    track_span_ = on;
+    reset_span();
 }
 
 void CellGrid::reserve(size_t capacity) {
@@ -196,6 +229,7 @@ void CellGrid::reserve(size_t capacity) {
     }
     hist_ = DeviceBuffer<u32>(radix_hist_elems(capacity));
     cur_ = 0;
+    reset_span();
 }
 
 void CellGrid::build(const float* positions_xyz, size_t count) {
@@ -206,11 +240,12 @@ void CellGrid::build(const float* positions_xyz, size_t count) {
     if (count > capacity_) reserve(count);
     count_ = count;
     const u32 n = static_cast<u32>(count);
-    k_cell_keys<<<div_up(n, 256), 256, 0, stream_>>>(positions_xyz, n, g_, k_[0].data(), v_[0].data());
+    k_cell_keys<<<div_up(n, 256), 256, 0, stream_>>>(positions_xyz, n, g_, k_[0].data(), v_[0].data(),
+                                                     span());
     WSIM_LAUNCH_CHECK();
     cur_ = radix_sort_pairs(k_[0].data(), v_[0].data(), k_[1].data(), v_[1].data(), count,
                             key_bits(g_.C), hist_.data(), stream_);
-    build_cell_start(k_[cur_].data(), n, g_.C + 1, cs_.data(), stream_, cpt_);
+    build_cell_start(k_[cur_].data(), n, g_.C + 1, cs_.data(), stream_, cpt_, span());
 }
 
 void CellGrid::build_staged(size_t count) {
@@ -220,7 +255,7 @@ void CellGrid::build_staged(size_t count) {
     const u32 n = static_cast<u32>(count);
     cur_ = radix_sort_pairs(k_[0].data(), v_[0].data(), k_[1].data(), v_[1].data(), count,
                             key_bits(g_.C), hist_.data(), stream_);
-    build_cell_start(k_[cur_].data(), n, g_.C + 1, cs_.data(), stream_, cpt_);
+    build_cell_start(k_[cur_].data(), n, g_.C + 1, cs_.data(), stream_, cpt_, span());
 }
 
 void CellGrid::reserve_segments(u32 scenes) {
@@ -231,6 +266,8 @@ void CellGrid::reserve_segments(u32 scenes) {
     cs_ = DeviceBuffer<u32>(ncs);
     cs_.zero(stream_);
     segments_ = scenes;
+    if (scenes != 1) track_span_ = false;
+    reset_span();
 }
 
 void CellGrid::build_segmented(const float* positions_xyz, size_t count, const u32* starts,

@@ -1537,7 +1537,8 @@ __global__ __launch_bounds__(NS) void k_keys_sorted(u32 n, GridGeom g,
                                                     const float4* __restrict__ xs,
                                                     const DeviceState* __restrict__ st,
                                                     u32* __restrict__ keys,
-                                                    u32* __restrict__ vals) {
+                                                    u32* __restrict__ vals, u32* span) {
+    advance_span(span);
     const u32 id = blockIdx.x * NS + threadIdx.x;
     if (id >= n) return;
     const u32 o = inv[id];
@@ -2018,6 +2019,7 @@ void DFSPHSolver::init(const std::vector<u32>& counts, const std::vector<float>&
 
     fluid_grid_.reserve(batched_ ? fluid_.count() : fluid_.capacity());
     fluid_grid_.reserve_segments(scenes);
+    fluid_grid_.set_track_span(!batched_);
     xs_ = DeviceBuffer<float4>(slots_cap);
     vs_ = DeviceBuffer<float4>(slots_cap);
     vpre_ = DeviceBuffer<float4>(slots_cap);
@@ -2420,7 +2422,8 @@ void DFSPHSolver::phase_prefix_grid() {
         k_keys_sorted<<<div_up(n, NS), NS, 0, stream_>>>(n, fluid_grid_.geom(), inv_.data(),
                                                          xa_.data(), xs_.data(), dstate_.data(),
                                                          fluid_grid_.staged_keys(),
-                                                         fluid_grid_.staged_values());
+                                                         fluid_grid_.staged_values(),
+                                                         fluid_grid_.span());
         WSIM_LAUNCH_CHECK();
         fluid_grid_.build_staged(n);
         tend();

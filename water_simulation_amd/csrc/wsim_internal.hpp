@@ -95,7 +95,17 @@ public:
     // tabled as build() does with the keys it computes itself.
     u32* staged_keys() { return k_[0].data(); }
     u32* staged_values() { return v_[0].data(); }
+    // the staged keys' kernel calls advance_span(span()) (one grid only)
     void build_staged(size_t count);
+
+    // Rewrite only the cells of the cell_start table that can change: for
+    // c <= min(smallest key) every table holds 0 and for c > max(largest
+    // key) every table holds n, so with the same n only the cells between
+    // the previous and current key spans are written.  For a grid rebuilt
+    // every substep from few-cell moves (one scene; the span record lives on
+    // the device, so graph replays keep it).
+    void set_track_span(bool on);
+    u32* span() { return track_span_ ? span_.data() : nullptr; }
 
     const u32* cell_start() const { return cs_.data(); }
     const u32* sorted_indices() const { return v_[cur_].data(); }
@@ -112,7 +122,9 @@ private:
     float cell_;
     hipStream_t stream_;
     size_t capacity_ = 0, count_ = 0;
-    DeviceBuffer<u32> k_[2], v_[2], hist_, cs_;
+    DeviceBuffer<u32> k_[2], v_[2], hist_, cs_, span_;
+    bool track_span_ = false;
+    void reset_span();
     int cur_ = 0;
     u32 segments_ = 1;
     u32 cpt_ = 16;
@@ -187,9 +199,21 @@ struct PairOpsReport {
 };
 PairOpsReport pair_ops_check(float l);
 
-// Cell-start table from sorted keys (one pass over the C+1 cells).
+// Cell-start table from sorted keys (one pass over the C+1 cells).  With
+// `span` (a CellGrid's key-span record, see CellGrid::set_track_span) only
+// the cells that can differ from the previous table of the same record are
+// written.
 void build_cell_start(const u32* sorted_keys, u32 n, u32 num_cells_plus_one, u32* cell_start,
-                      hipStream_t stream, u32 cells_per_thread = 16);
+                      hipStream_t stream, u32 cells_per_thread = 16, u32* span = nullptr);
+
+// Key-span record of a cell table: two slots of (smallest key, largest key,
+// particle count, valid), the slot of the current build chosen by the parity
+// of a build counter that the build's key kernel advances (kSpanCounter).
+constexpr u32 kSpanWords = 9;
+constexpr u32 kSpanCounter = 8;
+__device__ inline void advance_span(u32* span) {
+    if (span && blockIdx.x == 0 && threadIdx.x == 0) span[kSpanCounter] += 1u;
+}
 
 // ---------------------------------------------------------------------------
 // Morton path of SpatialAccel (spatial_accel.h:19-53, spatial_accel.cu:11-40,
