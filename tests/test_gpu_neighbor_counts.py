@@ -67,6 +67,19 @@ def check(x, v, xb, cfg, tag):
     return got
 
 
+def test_counts_tolerance_mode_solver():
+    # a tolerance-mode solver keeps no boundary pair lists; its counting walk
+    # still counts both stores exactly, and its next substep is unaffected
+    x, v, xb, cfg = random_box(n=3000, seed=5)
+    check(x, v, xb, dict(cfg, fast_math=True), "tolerance mode")
+    s, f, _ = solver_for(x, v, xb, dict(cfg, fast_math=True))
+    s.neighbor_counts()
+    s.step(0.002)
+    t, g, _ = solver_for(x, v, xb, dict(cfg, fast_math=True))
+    t.step(0.002)
+    assert np.array_equal(f.get_positions().view(np.uint32), g.get_positions().view(np.uint32))
+
+
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_counts_random_box(seed):
     x, v, xb, cfg = random_box(n=4000, seed=seed)

@@ -2097,7 +2097,12 @@ void DFSPHSolver::init(const std::vector<u32>& counts, const std::vector<float>&
             WSIM_HIP(hipStreamSynchronize(stream_));
         }
     }
-    if (cfg_.fast_math && nb_) gb_ = DeviceBuffer<float4>(slots_cap);
+    if (cfg_.fast_math && nb_) {
+        // tolerance mode keeps per-particle boundary gradient sums instead of
+        // boundary pair lists
+        gb_ = DeviceBuffer<float4>(slots_cap);
+        pairs_b_ = DeviceBuffer<uint2>();
+    }
     if (batched_) layout(counts);
     WSIM_HIP(hipStreamSynchronize(stream_));
 }
@@ -2176,7 +2181,7 @@ PairList DFSPHSolver::pair_list() {
     L.fi = pairs_fi_.data();
     L.plan = plan_.data();
     L.kf = kPairDepthFluid;
-    L.kb = nb_ ? kPairDepthBoundary : 0u;
+    L.kb = (nb_ && pairs_b_.data()) ? kPairDepthBoundary : 0u;
     L.gb = (cfg_.fast_math && nb_) ? gb_.data() : nullptr;
     return L;
 }
