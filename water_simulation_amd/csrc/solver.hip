@@ -213,13 +213,56 @@ __device__ __forceinline__ void grad_terms(float& rho, float& sx, float& sy, flo
 // (dfsph_kernels.cu:13-35) for the fluid and the boundary store: candidates
 // (every slot of the in-grid stencil cells, self included) and neighbours
 // (candidates with sqrtf(r^2) / l < 2, i.e. r^2 < far2).
+// Stable partition of a workgroup's slots, tile by tile (groups of at most
+// 256 threads, so a wave never mixes tiles): slots whose particle is near a
+// wall first, in slot order, then the others.  Returns the slot this thread
+// processes.  Only which lane computes which particle changes.
+template <int NT>
+__device__ __forceinline__ u32 near_slots_first(u32 slot, bool near) {
+    constexpr int GS = NT < kTile ? NT : kTile;
+    constexpr int GW = GS / kWave;
+    __shared__ u32 s_cnt[NT / kWave];
+    __shared__ u32 s_slot[NT];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int g0 = (t / GS) * GS;  // first thread of this thread's group
+    const int w0 = g0 / kWave;
+    const u64 bal = __ballot(near);
+    if (lane == 0) s_cnt[w] = static_cast<u32>(__popcll(bal));
+    __syncthreads();
+    u32 before = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < GW; ++k) {
+        const u32 c = s_cnt[w0 + k];
+        if (w0 + k < w) before += c;
+        total += c;
+    }
+    const u32 below = before + static_cast<u32>(__popcll(bal & ((1ull << lane) - 1ull)));
+    const u32 r = static_cast<u32>(t - g0);
+    s_slot[g0 + (near ? below : total + (r - below))] = slot;
+    __syncthreads();
+    return s_slot[t];
+}
+
 template <int NT, bool F, bool COUNT = false>
 __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
                                                       const float4* __restrict__ xs,
                                                       float* __restrict__ rho_out,
                                                       float* __restrict__ alpha_out,
                                                       uint4* __restrict__ counts = nullptr) {
-    const u32 i = slot_index<NT>(P.remap);
+    // Particles near a wall also walk the boundary store.  Cell order puts
+    // the start of every z-row next to a wall, so most waves would hold a few
+    // such lanes and all wait through their walk; grouping them first keeps
+    // the boundary walk to the waves that need it.
+    const u32 i0 = slot_index<NT>(P.remap);
+    bool near0 = false;
+    if (P.nb && i0 < P.nslots) {
+        const SceneSeg& G0 = P.seg[scene_of_slot(P, i0)];
+        if (i0 - G0.off < G0.n) {
+            const float4 q = xs[i0];
+            near0 = boundary_near(P.gb, P.bnear, q.x, q.y, q.z);
+        }
+    }
+    const u32 i = near_slots_first<NT>(i0, near0);
     if (i >= P.nslots) return;
     const SceneSeg& G = P.seg[scene_of_slot(P, i)];
     if (i - G.off >= G.n) return;
