@@ -524,9 +524,12 @@ def main() -> int:
     graphs = bool(args.graphs)
     batch_graphs = graphs
     if args.only_batch:
+        # the sequential side in the batch's own launch mode only (profilers
+        # whose interception of graph launches fails see no graph replay)
         print(json.dumps({"batched": batched_leg(args.batch_steps, args.batch_warmup,
                                                  args.tune_reps, 1, bool(args.fast_math),
-                                                 batch_graphs, eta=args.batch_eta)}))
+                                                 batch_graphs, both_launch_modes=False,
+                                                 eta=args.batch_eta)}))
         return 0
 
     x, xb, cfg = make_workload(args.n_side)
