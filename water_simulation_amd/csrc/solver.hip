@@ -1592,16 +1592,21 @@ __global__ __launch_bounds__(NS) void k_keys_sorted(u32 n, GridGeom g,
 
 // Sorted positions of the new grid gathered from the advected slots; records
 // each slot's previous slot (for the velocities, gathered once dt is known)
-// and moves inv to the new slots.  Without an advection the permutation is
-// the previous one and nothing moves.
+// and moves inv to the new slots.  Thread 0 of each workgroup (one 256-slot
+// tile) also writes the tile's staging plan, as k_stage_plan does.  Without
+// an advection the grid is the previous one: nothing moves and the plans
+// stand.
 __global__ __launch_bounds__(NS) void k_reorder_sorted(Nbr P, const u32* __restrict__ perm,
                                                        const float4* __restrict__ xa,
                                                        const DeviceState* __restrict__ st,
                                                        u32* __restrict__ inv,
                                                        u32* __restrict__ src,
-                                                       float4* __restrict__ xs) {
+                                                       float4* __restrict__ xs,
+                                                       StageRanges* __restrict__ plan) {
+    static_assert(NS == kTile, "one workgroup per tile");
     const u32 s = blockIdx.x * NS + threadIdx.x;
     if (!st[0].advected) return;  // grid-uniform
+    if (threadIdx.x == 0) plan[blockIdx.x] = stage_ranges(P, P.seg[0], blockIdx.x * kTile);
     if (s >= P.seg[0].n) return;
     const u32 id = perm[s];
     const u32 o = inv[id];
@@ -1612,11 +1617,18 @@ __global__ __launch_bounds__(NS) void k_reorder_sorted(Nbr P, const u32* __restr
 
 // Sorted velocities with gravity and damping (dfsph_kernels.cu:347-355), the
 // expression of k_reorder, gathered from the advected slots.
+// With `st`, thread 0 also opens the density solve (k_reset_solve's work for
+// the one scene).
 __global__ __launch_bounds__(NS) void k_reorder_v_sorted(Nbr P, const u32* __restrict__ src,
                                                          const float4* __restrict__ va,
-                                                         float4* __restrict__ vs, int gravity) {
+                                                         float4* __restrict__ vs, int gravity,
+                                                         DeviceState* __restrict__ st) {
     const u32 s = blockIdx.x * NS + threadIdx.x;
     const SceneSeg& G = P.seg[0];
+    if (st && s == 0) {
+        st[0].conv_iter = G.n ? kNone : 0u;
+        st[0].advected = 0u;
+    }
     if (s >= G.n) return;
     const float4 w = va[src[s]];
     float a = w.x, b = w.y, c = w.z;
@@ -2479,7 +2491,7 @@ void DFSPHSolver::phase_prefix_grid() {
         tbegin(kKReorder);
         k_reorder_sorted<<<nslots_ / NS, NS, 0, stream_>>>(P, fluid_grid_.sorted_indices(),
                                                            xa_.data(), dstate_.data(), inv_.data(),
-                                                           src_.data(), xs_.data());
+                                                           src_.data(), xs_.data(), plan_.data());
         WSIM_LAUNCH_CHECK();
         tend();
     } else {
@@ -2497,8 +2509,10 @@ void DFSPHSolver::phase_prefix_grid() {
         tend();
     }
     const Nbr P = nbr();
-    k_stage_plan<<<div_up(nslots_ / kTile, NS), NS, 0, stream_>>>(P, plan_.data());
-    WSIM_LAUNCH_CHECK();
+    if (!state_sorted_) {  // (the sorted reorder writes the plans itself)
+        k_stage_plan<<<div_up(nslots_ / kTile, NS), NS, 0, stream_>>>(P, plan_.data());
+        WSIM_LAUNCH_CHECK();
+    }
     tbegin(kKDensityAlpha);
     launch_density_alpha(shapes_.density_alpha_nt, stream_, nslots_, P, pair_list(), xs_.data(),
                          rho_.data(), alpha_.data(), cfg_.fast_math != 0);
@@ -2513,9 +2527,12 @@ void DFSPHSolver::phase_prefix_solve(bool gravity) {
     const bool xsph = gravity && any_xsph_;
     float4* vtarget = xsph ? vpre_.data() : vs_.data();
     tbegin(kKReorder);
+    // the sorted gather also opens the density solve when it follows gravity
+    const bool fold_reset = grid_sorted_ && gravity;
     if (grid_sorted_)
         k_reorder_v_sorted<<<nslots_ / NS, NS, 0, stream_>>>(P, src_.data(), va_.data(), vtarget,
-                                                             gravity ? 1 : 0);
+                                                             gravity ? 1 : 0,
+                                                             fold_reset ? dstate_.data() : nullptr);
     else
         k_reorder<<<nslots_ / NS, NS, 0, stream_>>>(P, fluid_grid_.sorted_indices(),
                                                      fluid_.positions(), fluid_.velocities(),
@@ -2543,9 +2560,11 @@ void DFSPHSolver::phase_prefix_solve(bool gravity) {
         WSIM_LAUNCH_CHECK();
         tend();
     }
-    k_reset_solve<<<div_up(scenes, NS), NS, 0, stream_>>>(dseg_.data(), dstate_.data(), scenes,
-                                                          gravity ? 0 : 1);
-    WSIM_LAUNCH_CHECK();
+    if (!fold_reset) {
+        k_reset_solve<<<div_up(scenes, NS), NS, 0, stream_>>>(dseg_.data(), dstate_.data(), scenes,
+                                                              gravity ? 0 : 1);
+        WSIM_LAUNCH_CHECK();
+    }
 }
 
 void DFSPHSolver::neighbor_counts(u32* cand_f, u32* nbr_f, u32* cand_b, u32* nbr_b) {
