@@ -500,6 +500,24 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
                 xsph_term<F>(dx, dy, dz, p, vi, rho_i, q3, w3, r3, eps, P, m > 3u);
             });
         } else {
+            // four pairs per trip: their gathers are issued together, the
+            // terms added in list order
+            for (; k + 4u <= cf; k += 4u) {
+                u32 j[4];
+                float4 q[4], w[4];
+                float rj[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) j[u] = ld_pair(L.f + pair_at(k + u, i, L.kf)).x;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    q[u] = xs[j[u]];
+                    w[u] = vin[j[u]];
+                    rj[u] = rho[j[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    xsph_term<F>(dx, dy, dz, p, vi, rho_i, q[u], w[u], rj[u], eps, P);
+            }
             for (; k < cf; ++k) {
                 const u32 j = ld_pair(L.f + pair_at(k, i, L.kf)).x;
                 xsph_term<F>(dx, dy, dz, p, vi, rho_i, xs[j], vin[j], rho[j], eps, P);
