@@ -7,27 +7,32 @@
 // the permutation a stable radix sort produces, bit for bit.
 //
 // Per digit pass (reduce-then-scan):
-//   rs_upsweep   -- per tile digit histogram; same-digit lanes of a wave are
-//                   grouped with ballots and counted by one LDS atomic;
+//   rs_upsweep   -- per tile digit histogram, one LDS atomic per key (counts
+//                   do not depend on the order the atomics land in);
 //   rs_scan_rows -- one workgroup per digit scans that digit's per-tile counts
 //                   and records the digit total;
 //   rs_downsweep -- each wave ranks its keys stably (ballot match + per-wave
 //                   LDS counters), the tile is reordered in LDS, and written
 //                   out digit-run by digit-run so consecutive lanes store to
 //                   consecutive addresses.
+// The sweeps run 1024-thread workgroups over 2048-key tiles: at a million
+// keys a pass is bound by launch and latency, not bytes, and the wide tiles
+// halve the per-tile histogram matrix and its strided accesses (against
+// 1024-key tiles) while keeping each lane's serial ranking work short.
 #include "device_math.hpp"
 #include "radix_sort.hpp"
 
 namespace wsim {
 namespace {
 
-constexpr int RS_THREADS = 256;
+constexpr int RS_THREADS = 1024;                    // sweep workgroup
 constexpr int RS_WAVES = RS_THREADS / kWave;
-constexpr int RS_ITEMS = 4;
-constexpr int RS_WAVE_TILE = kWave * RS_ITEMS;      // 256 keys per wave
-constexpr int RS_TILE = RS_THREADS * RS_ITEMS;      // 1024 keys per workgroup
+constexpr int RS_ITEMS = 2;
+constexpr int RS_WAVE_TILE = kWave * RS_ITEMS;      // 128 keys per wave
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;      // 2048 keys per workgroup
+constexpr int RS_DIGITS = 256;                      // threads of the digit scans
 constexpr int RS_ROW_PER = 4;  // tiles per thread and round of the row scan
-static_assert(RS_THREADS == 256, "one digit per thread in the digit scans");
+static_assert(RS_THREADS >= RS_DIGITS, "threads [0, 256) own one digit each");
 
 // Lanes of the wave whose digit equals this lane's digit (nbits <= 8).
 __device__ inline u64 match_digit(u32 d, int nbits) {
@@ -40,19 +45,23 @@ __device__ inline u64 match_digit(u32 d, int nbits) {
     return m;
 }
 
-// Exclusive scan of one u32 per thread over a 256-thread workgroup.
+// Exclusive scan of one u32 per thread over threads [0, 256) of the
+// workgroup.  Every thread of the workgroup calls it (it holds two
+// barriers); threads from 256 on pass anything and ignore the result.
+// s_tmp holds 4 entries.
 __device__ inline u32 block_exclusive_scan256(u32 v, u32* s_tmp) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    constexpr int kW = RS_DIGITS / kWave;
     u32 x = v;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const u32 y = __shfl_up(x, o);
         if (lane >= o) x += y;
     }
-    if (lane == 63) s_tmp[w] = x;
+    if (lane == 63 && w < kW) s_tmp[w] = x;
     __syncthreads();
     u32 off = 0;
-    for (int i = 0; i < w; ++i) off += s_tmp[i];
+    for (int i = 0; i < w && i < kW; ++i) off += s_tmp[i];
     __syncthreads();
     return off + x - v;
 }
@@ -60,19 +69,16 @@ __device__ inline u32 block_exclusive_scan256(u32 v, u32* s_tmp) {
 __global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const u32* __restrict__ keys, u32 n,
                                                          int shift, int nbits,
                                                          u32* __restrict__ hist, u32 nblocks) {
-    __shared__ u32 cnt[256];
+    __shared__ u32 cnt[RS_DIGITS];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const u32 dmask = (1u << nbits) - 1u;
-    cnt[tid] = 0;
+    if (tid < RS_DIGITS) cnt[tid] = 0;
     __syncthreads();
     const u32 base = blockIdx.x * RS_TILE + w * RS_WAVE_TILE;
-#pragma unroll 4
+#pragma unroll
     for (int r = 0; r < RS_ITEMS; ++r) {
         const u32 idx = base + r * kWave + lane;
-        const bool valid = idx < n;
-        const u32 d = valid ? (keys[idx] >> shift) & dmask : 0u;
-        const u64 m = match_digit(d, nbits) & __ballot(valid);
-        if (valid && lane == __builtin_ctzll(m)) atomicAdd(&cnt[d], static_cast<u32>(__popcll(m)));
+        if (idx < n) atomicAdd(&cnt[(keys[idx] >> shift) & dmask], 1u);
     }
     __syncthreads();
     if (static_cast<u32>(tid) <= dmask) hist[static_cast<u64>(tid) * nblocks + blockIdx.x] = cnt[tid];
@@ -83,15 +89,15 @@ __global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const u32* __restrict__
 // writes the digit's total.  Each thread owns up to RS_ROW_PER consecutive
 // tiles of the row; rows longer than RS_THREADS * RS_ROW_PER loop with a
 // carry.  The downsweep adds the exclusive scan of the digit totals.
-__global__ __launch_bounds__(RS_THREADS) void rs_scan_rows(u32* __restrict__ hist, u32 nblocks,
-                                                           u32* __restrict__ totals) {
-    __shared__ u32 s_tmp[RS_WAVES];
+__global__ __launch_bounds__(RS_DIGITS) void rs_scan_rows(u32* __restrict__ hist, u32 nblocks,
+                                                          u32* __restrict__ totals) {
+    __shared__ u32 s_tmp[RS_DIGITS / kWave];
     __shared__ u32 s_carry;
     u32* row = hist + static_cast<u64>(blockIdx.x) * nblocks;
     const int tid = threadIdx.x;
     if (tid == 0) s_carry = 0;
     __syncthreads();
-    for (u32 base = 0; base < nblocks; base += RS_THREADS * RS_ROW_PER) {
+    for (u32 base = 0; base < nblocks; base += RS_DIGITS * RS_ROW_PER) {
         const u32 first = base + static_cast<u32>(tid) * RS_ROW_PER;
         u32 v[RS_ROW_PER];
         u32 t = 0;
@@ -107,7 +113,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scan_rows(u32* __restrict__ his
             run += v[k];
         }
         __syncthreads();
-        if (tid == RS_THREADS - 1) s_carry = run;
+        if (tid == RS_DIGITS - 1) s_carry = run;
         __syncthreads();
     }
     if (tid == 0) totals[blockIdx.x] = s_carry;
@@ -117,16 +123,16 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
     const u32* __restrict__ kin, const u32* __restrict__ vin, u32* __restrict__ kout,
     u32* __restrict__ vout, u32 n, int shift, int nbits, const u32* __restrict__ offs,
     u32 nblocks, const u32* __restrict__ totals) {
-    __shared__ u32 s_wc[RS_WAVES][256];
-    __shared__ u32 s_start[256];
-    __shared__ u32 s_goff[256];
-    __shared__ u32 s_tmp[RS_WAVES];
+    __shared__ u32 s_wc[RS_WAVES][RS_DIGITS];
+    __shared__ u32 s_start[RS_DIGITS];
+    __shared__ u32 s_goff[RS_DIGITS];
+    __shared__ u32 s_tmp[RS_DIGITS / kWave];
     __shared__ u32 s_k[RS_TILE];
     __shared__ u32 s_v[RS_TILE];
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const u32 dmask = (1u << nbits) - 1u;
-    for (int i = tid; i < RS_WAVES * 256; i += RS_THREADS) (&s_wc[0][0])[i] = 0;
+    for (int i = tid; i < RS_WAVES * RS_DIGITS; i += RS_THREADS) (&s_wc[0][0])[i] = 0;
     __syncthreads();
 
     const u32 tile0 = blockIdx.x * RS_TILE;
@@ -160,9 +166,10 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
     }
     __syncthreads();
 
-    // Per digit (one per thread): exclusive prefix over waves, tile total.
-    u32 total = 0;
-    {
+    // Per digit (threads [0, 256), one digit each): exclusive prefix over
+    // waves, tile total, this tile's offset in the digit's run.
+    u32 total = 0, dtot = 0;
+    if (tid < RS_DIGITS) {
         const int d = tid;
 #pragma unroll
         for (int ww = 0; ww < RS_WAVES; ++ww) {
@@ -170,14 +177,16 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
             s_wc[ww][d] = total;
             total += c;
         }
-        s_goff[d] = (static_cast<u32>(d) <= dmask) ? offs[static_cast<u64>(d) * nblocks + blockIdx.x] : 0u;
+        const bool used = static_cast<u32>(d) <= dmask;
+        s_goff[d] = used ? offs[static_cast<u64>(d) * nblocks + blockIdx.x] : 0u;
+        dtot = used ? totals[d] : 0u;  // digit base: scan of the digit totals
     }
-    // digit base: exclusive scan of the digit totals over the whole input
-    const u32 dtot = (static_cast<u32>(tid) <= dmask) ? totals[tid] : 0u;
     const u32 dbase = block_exclusive_scan256(dtot, s_tmp);
-    s_goff[tid] += dbase;
     const u32 start = block_exclusive_scan256(total, s_tmp);
-    s_start[tid] = start;
+    if (tid < RS_DIGITS) {
+        s_goff[tid] += dbase;
+        s_start[tid] = start;
+    }
     __syncthreads();
 
 #pragma unroll
@@ -206,7 +215,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
 
 // Per-tile digit counts plus the 256 digit totals.
 size_t radix_hist_elems(size_t n) {
-    return static_cast<size_t>(256) * div_up(n ? n : 1, RS_TILE) + 256;
+    return static_cast<size_t>(RS_DIGITS) * div_up(n ? n : 1, RS_TILE) + RS_DIGITS;
 }
 
 int radix_sort_pairs(u32* k0, u32* v0, u32* k1, u32* v1, size_t n, int bits, u32* hist,
@@ -224,8 +233,8 @@ int radix_sort_pairs(u32* k0, u32* v0, u32* k1, u32* v1, size_t n, int bits, u32
         const int nbits = (bits - shift) < 8 ? (bits - shift) : 8;
         rs_upsweep<<<nblocks, RS_THREADS, 0, stream>>>(kin, nn, shift, nbits, hist, nblocks);
         WSIM_LAUNCH_CHECK();
-        u32* totals = hist + static_cast<size_t>(256) * nblocks;
-        rs_scan_rows<<<1u << nbits, RS_THREADS, 0, stream>>>(hist, nblocks, totals);
+        u32* totals = hist + static_cast<size_t>(RS_DIGITS) * nblocks;
+        rs_scan_rows<<<1u << nbits, RS_DIGITS, 0, stream>>>(hist, nblocks, totals);
         WSIM_LAUNCH_CHECK();
         rs_downsweep<<<nblocks, RS_THREADS, 0, stream>>>(kin, vin, kout, vout, nn, shift, nbits,
                                                          hist, nblocks, totals);
