@@ -248,13 +248,15 @@ void CellGrid::build(const float* positions_xyz, size_t count) {
     build_cell_start(k_[cur_].data(), n, g_.C + 1, cs_.data(), stream_, cpt_, span());
 }
 
-void CellGrid::build_staged(size_t count) {
+u32 CellGrid::first_digit_mask() const { return radix_first_digit_mask(key_bits(g_.C)); }
+
+void CellGrid::build_staged(size_t count, bool first_counted) {
     if (count > capacity_) fail(kInvalidArgument, "CellGrid: staged keys exceed the capacity");
     count_ = count;
     if (count == 0) return;
     const u32 n = static_cast<u32>(count);
     cur_ = radix_sort_pairs(k_[0].data(), v_[0].data(), k_[1].data(), v_[1].data(), count,
-                            key_bits(g_.C), hist_.data(), stream_);
+                            key_bits(g_.C), hist_.data(), stream_, first_counted);
     build_cell_start(k_[cur_].data(), n, g_.C + 1, cs_.data(), stream_, cpt_, span());
 }
 

@@ -25,11 +25,12 @@
 namespace wsim {
 namespace {
 
-constexpr int RS_THREADS = 1024;                    // sweep workgroup
+constexpr int RS_THREADS = kRadixThreads;           // sweep workgroup (1024)
 constexpr int RS_WAVES = RS_THREADS / kWave;
 constexpr int RS_ITEMS = 2;
 constexpr int RS_WAVE_TILE = kWave * RS_ITEMS;      // 128 keys per wave
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;      // 2048 keys per workgroup
+static_assert(RS_TILE == kRadixTile, "tile of the header's counting helpers");
 constexpr int RS_DIGITS = 256;                      // threads of the digit scans
 constexpr int RS_ROW_PER = 4;  // tiles per thread and round of the row scan
 static_assert(RS_THREADS >= RS_DIGITS, "threads [0, 256) own one digit each");
@@ -72,16 +73,14 @@ __global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const u32* __restrict__
     __shared__ u32 cnt[RS_DIGITS];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const u32 dmask = (1u << nbits) - 1u;
-    if (tid < RS_DIGITS) cnt[tid] = 0;
-    __syncthreads();
+    radix_count_begin(cnt);
     const u32 base = blockIdx.x * RS_TILE + w * RS_WAVE_TILE;
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; ++r) {
         const u32 idx = base + r * kWave + lane;
-        if (idx < n) atomicAdd(&cnt[(keys[idx] >> shift) & dmask], 1u);
+        if (idx < n) radix_count_key(cnt, (keys[idx] >> shift) & dmask);
     }
-    __syncthreads();
-    if (static_cast<u32>(tid) <= dmask) hist[static_cast<u64>(tid) * nblocks + blockIdx.x] = cnt[tid];
+    radix_count_end(cnt, dmask, hist, nblocks);
 }
 
 // Row scan of the digit-major histogram matrix: workgroup d turns row d (the
@@ -219,7 +218,7 @@ size_t radix_hist_elems(size_t n) {
 }
 
 int radix_sort_pairs(u32* k0, u32* v0, u32* k1, u32* v1, size_t n, int bits, u32* hist,
-                     hipStream_t stream) {
+                     hipStream_t stream, bool first_counted) {
     if (n <= 1 || bits <= 0) return 0;
     if (n > 0xFFFFF000ull) fail(kInvalidArgument, "radix_sort_pairs: too many elements");
     const u32 nn = static_cast<u32>(n);
@@ -231,8 +230,10 @@ int radix_sort_pairs(u32* k0, u32* v0, u32* k1, u32* v1, size_t n, int bits, u32
     int where = 0;
     for (int shift = 0; shift < bits; shift += 8) {
         const int nbits = (bits - shift) < 8 ? (bits - shift) : 8;
-        rs_upsweep<<<nblocks, RS_THREADS, 0, stream>>>(kin, nn, shift, nbits, hist, nblocks);
-        WSIM_LAUNCH_CHECK();
+        if (shift > 0 || !first_counted) {
+            rs_upsweep<<<nblocks, RS_THREADS, 0, stream>>>(kin, nn, shift, nbits, hist, nblocks);
+            WSIM_LAUNCH_CHECK();
+        }
         u32* totals = hist + static_cast<size_t>(RS_DIGITS) * nblocks;
         rs_scan_rows<<<1u << nbits, RS_DIGITS, 0, stream>>>(hist, nblocks, totals);
         WSIM_LAUNCH_CHECK();

@@ -1591,21 +1591,29 @@ __global__ __launch_bounds__(NS) void k_advect_sorted(
 // its particle through inv (its slot in the substep just advected): the
 // advected position, or, when that substep's solve has not finished
 // (nothing advected), the position it started from, which rebuilds that
-// substep's own grid.
-__global__ __launch_bounds__(NS) void k_keys_sorted(u32 n, GridGeom g,
-                                                    const u32* __restrict__ inv,
-                                                    const float4* __restrict__ xa,
-                                                    const float4* __restrict__ xs,
-                                                    const DeviceState* __restrict__ st,
-                                                    u32* __restrict__ keys,
-                                                    u32* __restrict__ vals, u32* span) {
+// substep's own grid.  Workgroup t writes the sort's tile t and counts the
+// first digit of its keys for the sort (radix_count_*).
+__global__ __launch_bounds__(kRadixThreads) void k_keys_sorted(
+    u32 n, GridGeom g, const u32* __restrict__ inv, const float4* __restrict__ xa,
+    const float4* __restrict__ xs, const DeviceState* __restrict__ st, u32* __restrict__ keys,
+    u32* __restrict__ vals, u32* span, u32 dmask, u32* __restrict__ hist, u32 tiles) {
+    __shared__ u32 cnt[256];
     advance_span(span);
-    const u32 id = blockIdx.x * NS + threadIdx.x;
-    if (id >= n) return;
-    const u32 o = inv[id];
-    const float4 p = st[0].advected ? xa[o] : xs[o];
-    keys[id] = g.key_of(p.x, p.y, p.z);
-    vals[id] = id;
+    radix_count_begin(cnt);
+    const bool adv = st[0].advected != 0;
+#pragma unroll
+    for (int r = 0; r < kRadixTile / kRadixThreads; ++r) {
+        const u32 id = blockIdx.x * kRadixTile + r * kRadixThreads + threadIdx.x;
+        if (id < n) {
+            const u32 o = inv[id];
+            const float4 p = adv ? xa[o] : xs[o];
+            const u32 key = g.key_of(p.x, p.y, p.z);
+            keys[id] = key;
+            vals[id] = id;
+            radix_count_key(cnt, key & dmask);
+        }
+    }
+    radix_count_end(cnt, dmask, hist, tiles);
 }
 
 // Sorted positions of the new grid gathered from the advected slots; records
@@ -2497,13 +2505,13 @@ void DFSPHSolver::phase_prefix_grid() {
         // advance(): keys and positions from the advected slots (xa_, inv_)
         const u32 n = total_;
         tbegin(kKGrid);
-        k_keys_sorted<<<div_up(n, NS), NS, 0, stream_>>>(n, fluid_grid_.geom(), inv_.data(),
-                                                         xa_.data(), xs_.data(), dstate_.data(),
-                                                         fluid_grid_.staged_keys(),
-                                                         fluid_grid_.staged_values(),
-                                                         fluid_grid_.span());
+        const u32 tiles = radix_tiles(n);
+        k_keys_sorted<<<tiles, kRadixThreads, 0, stream_>>>(
+            n, fluid_grid_.geom(), inv_.data(), xa_.data(), xs_.data(), dstate_.data(),
+            fluid_grid_.staged_keys(), fluid_grid_.staged_values(), fluid_grid_.span(),
+            fluid_grid_.first_digit_mask(), fluid_grid_.sort_hist(), tiles);
         WSIM_LAUNCH_CHECK();
-        fluid_grid_.build_staged(n);
+        fluid_grid_.build_staged(n, true);
         tend();
         const Nbr P = nbr();
         tbegin(kKReorder);

@@ -95,8 +95,13 @@ public:
     // tabled as build() does with the keys it computes itself.
     u32* staged_keys() { return k_[0].data(); }
     u32* staged_values() { return v_[0].data(); }
-    // the staged keys' kernel calls advance_span(span()) (one grid only)
-    void build_staged(size_t count);
+    // The staged keys' kernel calls advance_span(span()) (one grid only).
+    // With first_counted it has also counted the sort's first digit into
+    // sort_hist() (radix_count_*, tile t of the keys by workgroup t, digit
+    // mask first_digit_mask()), and the sort starts at its scan.
+    u32* sort_hist() { return hist_.data(); }
+    u32 first_digit_mask() const;
+    void build_staged(size_t count, bool first_counted = false);
 
     // Rewrite only the cells of the cell_start table that can change: for
     // c <= min(smallest key) every table holds 0 and for c > max(largest
