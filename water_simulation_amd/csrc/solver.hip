@@ -243,12 +243,20 @@ __device__ __forceinline__ u32 near_slots_first(u32 slot, bool near) {
     return s_slot[t];
 }
 
+// With `open`, after an advection (advance(), one scene) thread 0 also opens
+// the next density solve (k_reset_solve's work): the reorder before this
+// kernel was the last to test the advected flag.
 template <int NT, bool F, bool COUNT = false>
 __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
                                                       const float4* __restrict__ xs,
                                                       float* __restrict__ rho_out,
                                                       float* __restrict__ alpha_out,
-                                                      uint4* __restrict__ counts = nullptr) {
+                                                      uint4* __restrict__ counts = nullptr,
+                                                      DeviceState* open = nullptr) {
+    if (open && blockIdx.x == 0 && threadIdx.x == 0 && open->advected) {
+        open->conv_iter = P.seg[0].n ? kNone : 0u;
+        open->advected = 0u;
+    }
     // Particles near a wall also walk the boundary store.  Cell order puts
     // the start of every z-row next to a wall, so most waves would hold a few
     // such lanes and all wait through their walk; grouping them first keeps
@@ -1545,9 +1553,12 @@ __global__ __launch_bounds__(NS) void k_advect_gather(
 //
 // Inside advance() a substep's advection writes the advected state in its
 // own working-slot order (xa, va) instead of scattering it to the caller's
-// store; the next substep gathers from there and the store is written once,
-// when advance() returns.  The grid, the permutation and every value are
-// the ones the store round trip gives.
+// store; the next substep gathers from there (positions, and velocities with
+// gravity, in one reorder) and the store is written once, when advance()
+// returns.  The grid, the permutation and every value are the ones the store
+// round trip gives.  Slots move little between substeps, so the reorder's
+// gathers from the previous slots stay nearly sequential; only the cell keys,
+// which the stable sort needs in original index order, gather at random.
 
 // k_advect on one scene's working slots, writing xa/va in slot order.  The
 // statistics partials cover the same 256-slot blocks as k_advect_gather's
@@ -1616,54 +1627,32 @@ __global__ __launch_bounds__(kRadixThreads) void k_keys_sorted(
     radix_count_end(cnt, dmask, hist, tiles);
 }
 
-// Sorted positions of the new grid gathered from the advected slots; records
-// each slot's previous slot (for the velocities, gathered once dt is known)
-// and moves inv to the new slots.  Thread 0 of each workgroup (one 256-slot
-// tile) also writes the tile's staging plan, as k_stage_plan does.  Without
-// an advection the grid is the previous one: nothing moves and the plans
-// stand.
+// The new grid's working slots gathered from the advected slots: positions,
+// and velocities with gravity and damping (dfsph_kernels.cu:347-355, the
+// expression of k_reorder) for the next substep, whose time step the
+// statistics kernel has already computed; inv moves to the new slots.  Thread 0 of each workgroup (one 256-slot tile)
+// also writes the tile's staging plan, as k_stage_plan does.  Without an
+// advection the grid is the previous one: nothing moves and the plans stand.
 __global__ __launch_bounds__(NS) void k_reorder_sorted(Nbr P, const u32* __restrict__ perm,
                                                        const float4* __restrict__ xa,
+                                                       const float4* __restrict__ va,
                                                        const DeviceState* __restrict__ st,
                                                        u32* __restrict__ inv,
-                                                       u32* __restrict__ src,
                                                        float4* __restrict__ xs,
+                                                       float4* __restrict__ vs,
                                                        StageRanges* __restrict__ plan) {
     static_assert(NS == kTile, "one workgroup per tile");
     const u32 s = blockIdx.x * NS + threadIdx.x;
     if (!st[0].advected) return;  // grid-uniform
     if (threadIdx.x == 0) plan[blockIdx.x] = stage_ranges(P, P.seg[0], blockIdx.x * kTile);
-    if (s >= P.seg[0].n) return;
+    const SceneSeg& G = P.seg[0];
+    if (s >= G.n) return;
     const u32 id = perm[s];
     const u32 o = inv[id];
+    const float4 w = va[o];
     xs[s] = xa[o];
-    src[s] = o;
+    vs[s] = make_float4(w.x * G.decay + G.gdx, w.y * G.decay + G.gdy, w.z * G.decay + G.gdz, 0.0f);
     inv[id] = s;
-}
-
-// Sorted velocities with gravity and damping (dfsph_kernels.cu:347-355), the
-// expression of k_reorder, gathered from the advected slots.
-// With `st`, thread 0 also opens the density solve (k_reset_solve's work for
-// the one scene).
-__global__ __launch_bounds__(NS) void k_reorder_v_sorted(Nbr P, const u32* __restrict__ src,
-                                                         const float4* __restrict__ va,
-                                                         float4* __restrict__ vs, int gravity,
-                                                         DeviceState* __restrict__ st) {
-    const u32 s = blockIdx.x * NS + threadIdx.x;
-    const SceneSeg& G = P.seg[0];
-    if (st && s == 0) {
-        st[0].conv_iter = G.n ? kNone : 0u;
-        st[0].advected = 0u;
-    }
-    if (s >= G.n) return;
-    const float4 w = va[src[s]];
-    float a = w.x, b = w.y, c = w.z;
-    if (gravity) {
-        a = a * G.decay + G.gdx;
-        b = b * G.decay + G.gdy;
-        c = c * G.decay + G.gdz;
-    }
-    vs[s] = make_float4(a, b, c, 0.0f);
 }
 
 // The state of advance()'s last substep to the caller's store, in original
@@ -1754,21 +1743,23 @@ u32 grid_for(u32 nslots, u32 nt) { return div_up(nslots, nt); }
 
 template <bool F>
 void density_alpha_mode(u32 nt, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
-                        const float4* xs, float* rho, float* alpha) {
+                        const float4* xs, float* rho, float* alpha, DeviceState* open) {
     switch (nt) {
-    case 64: k_density_alpha<64, F><<<grid_for(nslots, 64), 64, 0, s>>>(P, L, xs, rho, alpha); break;
-    case 128: k_density_alpha<128, F><<<grid_for(nslots, 128), 128, 0, s>>>(P, L, xs, rho, alpha); break;
-    case 256: k_density_alpha<256, F><<<grid_for(nslots, 256), 256, 0, s>>>(P, L, xs, rho, alpha); break;
-    case 512: k_density_alpha<512, F><<<grid_for(nslots, 512), 512, 0, s>>>(P, L, xs, rho, alpha); break;
+    case 64: k_density_alpha<64, F><<<grid_for(nslots, 64), 64, 0, s>>>(P, L, xs, rho, alpha, nullptr, open); break;
+    case 128: k_density_alpha<128, F><<<grid_for(nslots, 128), 128, 0, s>>>(P, L, xs, rho, alpha, nullptr, open); break;
+    case 256: k_density_alpha<256, F><<<grid_for(nslots, 256), 256, 0, s>>>(P, L, xs, rho, alpha, nullptr, open); break;
+    case 512: k_density_alpha<512, F><<<grid_for(nslots, 512), 512, 0, s>>>(P, L, xs, rho, alpha, nullptr, open); break;
     default: fail(kInvalidArgument, "density_alpha workgroup size");
     }
     WSIM_LAUNCH_CHECK();
 }
-// fast: the tolerance-mode pair arithmetic (PairMath<true>)
+// fast: the tolerance-mode pair arithmetic (PairMath<true>); open: see
+// k_density_alpha
 void launch_density_alpha(u32 nt, hipStream_t s, u32 nslots, const Nbr& P, const PairList& L,
-                          const float4* xs, float* rho, float* alpha, bool fast) {
-    if (fast) density_alpha_mode<true>(nt, s, nslots, P, L, xs, rho, alpha);
-    else density_alpha_mode<false>(nt, s, nslots, P, L, xs, rho, alpha);
+                          const float4* xs, float* rho, float* alpha, bool fast,
+                          DeviceState* open = nullptr) {
+    if (fast) density_alpha_mode<true>(nt, s, nslots, P, L, xs, rho, alpha, open);
+    else density_alpha_mode<false>(nt, s, nslots, P, L, xs, rho, alpha, open);
 }
 
 template <bool F>
@@ -2118,7 +2109,6 @@ void DFSPHSolver::init(const std::vector<u32>& counts, const std::vector<float>&
     if (!batched_) {
         xa_ = DeviceBuffer<float4>(slots_cap);
         va_ = DeviceBuffer<float4>(slots_cap);
-        src_ = DeviceBuffer<u32>(slots_cap);
     }
     state_sorted_ = grid_sorted_ = defer_store_ = false;
     plan_ = DeviceBuffer<StageRanges>(slots_cap / kTile);
@@ -2502,7 +2492,8 @@ void DFSPHSolver::phase_prefix(bool gravity) {
 void DFSPHSolver::phase_prefix_grid() {
     const u32 scenes = static_cast<u32>(counts_.size());
     if (state_sorted_) {
-        // advance(): keys and positions from the advected slots (xa_, inv_)
+        // advance(): keys, positions and velocities from the advected slots
+        // (xa_, va_, inv_) or, behind an unfinished solve, this substep's
         const u32 n = total_;
         tbegin(kKGrid);
         const u32 tiles = radix_tiles(n);
@@ -2515,9 +2506,11 @@ void DFSPHSolver::phase_prefix_grid() {
         tend();
         const Nbr P = nbr();
         tbegin(kKReorder);
-        k_reorder_sorted<<<nslots_ / NS, NS, 0, stream_>>>(P, fluid_grid_.sorted_indices(),
-                                                           xa_.data(), dstate_.data(), inv_.data(),
-                                                           src_.data(), xs_.data(), plan_.data());
+        // the velocity target of the next prefix (the pre-XSPH velocities
+        // when any scene has XSPH)
+        k_reorder_sorted<<<nslots_ / NS, NS, 0, stream_>>>(
+            P, fluid_grid_.sorted_indices(), xa_.data(), va_.data(), dstate_.data(), inv_.data(),
+            xs_.data(), any_xsph_ ? vpre_.data() : vs_.data(), plan_.data());
         WSIM_LAUNCH_CHECK();
         tend();
     } else {
@@ -2541,7 +2534,8 @@ void DFSPHSolver::phase_prefix_grid() {
     }
     tbegin(kKDensityAlpha);
     launch_density_alpha(shapes_.density_alpha_nt, stream_, nslots_, P, pair_list(), xs_.data(),
-                         rho_.data(), alpha_.data(), cfg_.fast_math != 0);
+                         rho_.data(), alpha_.data(), cfg_.fast_math != 0,
+                         state_sorted_ ? dstate_.data() : nullptr);
     tend();
 }
 
@@ -2552,20 +2546,20 @@ void DFSPHSolver::phase_prefix_solve(bool gravity) {
     const Nbr P = nbr();
     const bool xsph = gravity && any_xsph_;
     float4* vtarget = xsph ? vpre_.data() : vs_.data();
-    tbegin(kKReorder);
-    // the sorted gather also opens the density solve when it follows gravity
-    const bool fold_reset = grid_sorted_ && gravity;
-    if (grid_sorted_)
-        k_reorder_v_sorted<<<nslots_ / NS, NS, 0, stream_>>>(P, src_.data(), va_.data(), vtarget,
-                                                             gravity ? 1 : 0,
-                                                             fold_reset ? dstate_.data() : nullptr);
-    else
+    // A grid built from the advected state (advance()) has gathered the
+    // velocities with gravity into vtarget already, and its density kernel
+    // opened the solve.
+    if (grid_sorted_ && !gravity)
+        fail(kRuntimeError, "DFSPHSolver: a slot-order grid is only followed by a gravity prefix");
+    if (!grid_sorted_) {
+        tbegin(kKReorder);
         k_reorder<<<nslots_ / NS, NS, 0, stream_>>>(P, fluid_grid_.sorted_indices(),
                                                      fluid_.positions(), fluid_.velocities(),
                                                      xs_.data(), vtarget, gravity ? 1 : 0, 2,
                                                      nullptr);
-    WSIM_LAUNCH_CHECK();
-    tend();
+        WSIM_LAUNCH_CHECK();
+        tend();
+    }
     if (xsph) {
         tbegin(kKXsph);
         launch_xsph(shapes_.xsph_nt, stream_, nslots_, P, pair_list(), xs_.data(), vpre_.data(),
@@ -2586,7 +2580,7 @@ void DFSPHSolver::phase_prefix_solve(bool gravity) {
         WSIM_LAUNCH_CHECK();
         tend();
     }
-    if (!fold_reset) {
+    if (!grid_sorted_) {
         k_reset_solve<<<div_up(scenes, NS), NS, 0, stream_>>>(dseg_.data(), dstate_.data(), scenes,
                                                               gravity ? 0 : 1);
         WSIM_LAUNCH_CHECK();
