@@ -281,8 +281,15 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
     u32 cf = 0, cb = 0;
     bool unlisted = false;  // an in-support fluid pair with a zero gradient
     uint4 nc = make_uint4(0u, 0u, 0u, 0u);  // COUNT: candidates, neighbours (fluid, boundary)
+    // the tile's plan is wave-uniform: a wave's particles share one tile
     const StageRanges R = L.plan[i / kTile];
-    auto fluid_pair = [&](u32 s) {
+    const bool staged = __builtin_amdgcn_readfirstlane(R.total) != 0u;
+    const StageOffsets O(R);
+    // entry cf of this particle's list: stage_entry_at(cf, i, kf) and
+    // pair_at(cf, i, kf) from their values at cf = 0
+    uint16_t* const fi = L.fi + stage_entry_at(0u, i, L.kf);
+    uint2* const f8 = L.f + pair_at(0u, i, L.kf);
+    auto fluid_pair = [&](u32 s, int xk) {
         const float4 q = xs[s];
         const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
         const float r2 = PairMath<F>::r2(rx, ry, rz);
@@ -297,10 +304,10 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
         grad_terms<F>(rho, sx, sy, sz, sq, P.mass, wv, c, rx, ry, rz);
         if (c != 0.0f) {
             if (cf < L.kf) {
-                if (R.total)
-                    L.fi[stage_entry_at(cf, i, L.kf)] = static_cast<uint16_t>(stage_index(R, s));
+                if (staged)
+                    fi[((cf >> 2) << 8) | (cf & 3u)] = static_cast<uint16_t>(O.entry(s, xk));
                 else
-                    L.f[pair_at(cf, i, L.kf)] = make_uint2(s, __float_as_uint(c));
+                    f8[cf << 6] = make_uint2(s, __float_as_uint(c));
             }
             ++cf;
         } else if (s != i) {

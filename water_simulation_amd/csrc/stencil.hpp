@@ -4,6 +4,8 @@
 // the scene lookup of a slot, and the per-particle pair lists.
 #pragma once
 
+#include <type_traits>
+
 #include "wsim_internal.hpp"
 
 namespace wsim {
@@ -62,6 +64,15 @@ __device__ __forceinline__ void for_stencil(const GridGeom& g, const u32* __rest
     }
 }
 
+// A walk's callback takes the slot, or the slot and the candidate's x
+// offset in the stencil as 0, 1, 2 (dx + 1): the staged range of its tile
+// that holds it (see stage_ranges).
+template <class F>
+__device__ __forceinline__ void visit_slot(F& f, u32 s, int xk) {
+    if constexpr (std::is_invocable_v<F&, u32, int>) f(s, xk);
+    else f(s);
+}
+
 // for_stencil with the cell-table loads of each (dz, dy) row issued
 // together: the three cells' [start, end) pairs are loaded before the first
 // slot of the row is visited, so a row waits on one table latency instead of
@@ -92,7 +103,7 @@ __device__ __forceinline__ void for_stencil_rows(const GridGeom& g, const u32* _
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k)
-                for (u32 s = b[k]; s < e[k]; ++s) f(s + shift);
+                for (u32 s = b[k]; s < e[k]; ++s) visit_slot(f, s + shift, k);
         }
     }
 }
@@ -125,7 +136,7 @@ __device__ __forceinline__ void for_stencil_zrows(const GridGeom& g, const u32* 
     }
 #pragma unroll
     for (int r = 0; r < 9; ++r)
-        for (u32 s = b[r]; s < e[r]; ++s) f(s + shift);
+        for (u32 s = b[r]; s < e[r]; ++s) visit_slot(f, s + shift, r / 3);
 }
 
 // Boundary proximity: bit k of the mask is set when the 27-cell stencil of
@@ -242,6 +253,20 @@ __device__ inline StageRanges stage_ranges(const Nbr& P, const SceneSeg& G, u32 
 __device__ __forceinline__ u32 stage_index(const StageRanges& R, u32 j) {
     return j >= R.s2 ? R.b2 + (j - R.s2) : (j >= R.s1 ? R.b1 + (j - R.s1) : j - R.s0);
 }
+// The same for a stencil candidate of x offset dx = xk - 1: every stencil
+// cell at that offset of a particle whose key k lies in [kmin, kmax] has a
+// key in [k + dx Dy Dz - Dz - 1, k + dx Dy Dz + Dz + 1], inside range xk,
+// so the entry is j plus that range's offset.  The three offsets are
+// wave-uniform (a wave's particles share one tile).
+struct StageOffsets {
+    u32 off[3];
+    __device__ explicit StageOffsets(const StageRanges& R)
+        : off{uniform(0u - R.s0), uniform(R.b1 - R.s1), uniform(R.b2 - R.s2)} {}
+    __device__ static u32 uniform(u32 v) {
+        return static_cast<u32>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+    }
+    __device__ __forceinline__ u32 entry(u32 j, int xk) const { return j + off[xk]; }
+};
 // Working slot of staged entry e.
 __device__ __forceinline__ u32 stage_source(const StageRanges& R, u32 e) {
     return e >= R.b2 ? R.s2 + (e - R.b2) : (e >= R.b1 ? R.s1 + (e - R.b1) : R.s0 + e);
