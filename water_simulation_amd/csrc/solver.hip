@@ -257,6 +257,27 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
         open->conv_iter = P.seg[0].n ? kNone : 0u;
         open->advected = 0u;
     }
+    // With one tile per workgroup, a tile whose neighbourhood fits the
+    // staging plan reads its candidates' positions from LDS.
+    constexpr bool kStage = NT == kTile;
+    __shared__ float4 s_x[kStage ? kStageCap : 1];
+    bool lds = false;
+    if constexpr (kStage) {
+        const u32 tile = P.remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+        const StageRanges Rt = L.plan[tile];
+        if (Rt.total) {
+            // every load of a thread is issued before its LDS stores
+            const StageLanes E(Rt);
+            f4v x[kStagePer];
+#pragma unroll
+            for (int u = 0; u < kStagePer; ++u) x[u] = ld4v(xs, E.src[u]);
+#pragma unroll
+            for (int u = 0; u < kStagePer; ++u)
+                if (E.has(u)) reinterpret_cast<f4v*>(s_x)[E.entry(u)] = x[u];
+        }
+        lds = Rt.total != 0u;
+        __syncthreads();
+    }
     // Particles near a wall also walk the boundary store.  Cell order puts
     // the start of every z-row next to a wall, so most waves would hold a few
     // such lanes and all wait through their walk; grouping them first keeps
@@ -289,8 +310,7 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
     // pair_at(cf, i, kf) from their values at cf = 0
     uint16_t* const fi = L.fi + stage_entry_at(0u, i, L.kf);
     uint2* const f8 = L.f + pair_at(0u, i, L.kf);
-    auto fluid_pair = [&](u32 s, int xk) {
-        const float4 q = xs[s];
+    auto fluid_pair = [&](u32 s, int xk, const float4& q) {
         const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
         const float r2 = PairMath<F>::r2(rx, ry, rz);
         if (COUNT) {
@@ -314,8 +334,15 @@ __global__ __launch_bounds__(NT) void k_density_alpha(Nbr P, PairList L,
             unlisted = true;
         }
     };
-    if (F) for_stencil_zrows(P.gf, cs, shift, p.x, p.y, p.z, fluid_pair);
-    else for_stencil_rows(P.gf, cs, shift, p.x, p.y, p.z, fluid_pair);
+    auto from_lds = [&](u32 s, int xk) { fluid_pair(s, xk, s_x[O.entry(s, xk)]); };
+    auto from_global = [&](u32 s, int xk) { fluid_pair(s, xk, xs[s]); };
+    if (lds) {
+        if (F) for_stencil_zrows(P.gf, cs, shift, p.x, p.y, p.z, from_lds);
+        else for_stencil_rows(P.gf, cs, shift, p.x, p.y, p.z, from_lds);
+    } else {
+        if (F) for_stencil_zrows(P.gf, cs, shift, p.x, p.y, p.z, from_global);
+        else for_stencil_rows(P.gf, cs, shift, p.x, p.y, p.z, from_global);
+    }
     // XSPH sums over every in-support pair (W != 0) and reads the fluid list;
     // a pair the list leaves out (coincident particles) sends this particle
     // to the stencil walk in every list kernel
