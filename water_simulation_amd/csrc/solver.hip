@@ -1438,13 +1438,34 @@ __global__ __launch_bounds__(NT) void k_finish_vstats(SceneSeg* __restrict__ seg
     float m = 0.0f;
     u32 c = 0, e = 0;
     double d[kMomParts] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (u32 t = threadIdx.x; t < G.ntiles; t += NT) {
-        const u32 b = G.tile0 + t;
-        m = fmaxf(m, V.pmax[b]);
-        c += V.pnan[b];
-        e += V.pesc[b];
+    // each thread adds its tiles t, t + NT, t + 2 NT, ... in that order; the
+    // loads of four of them are issued before the first is added
+    constexpr int kRound = 4;
+    for (u32 t0 = threadIdx.x; t0 < G.ntiles; t0 += kRound * NT) {
+        float pm[kRound];
+        u32 pc[kRound], pe[kRound];
+        double pd[kRound][kMomParts];
 #pragma unroll
-        for (int k = 0; k < kMomParts; ++k) d[k] += V.pmom[kMomParts * static_cast<u64>(b) + k];
+        for (int r = 0; r < kRound; ++r) {
+            const u32 t = t0 + static_cast<u32>(r) * NT;
+            if (t < G.ntiles) {
+                const u32 b = G.tile0 + t;
+                pm[r] = V.pmax[b];
+                pc[r] = V.pnan[b];
+                pe[r] = V.pesc[b];
+#pragma unroll
+                for (int k = 0; k < kMomParts; ++k) pd[r][k] = V.pmom[kMomParts * static_cast<u64>(b) + k];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kRound; ++r) {
+            if (t0 + static_cast<u32>(r) * NT >= G.ntiles) break;
+            m = fmaxf(m, pm[r]);
+            c += pc[r];
+            e += pe[r];
+#pragma unroll
+            for (int k = 0; k < kMomParts; ++k) d[k] += pd[r][k];
+        }
     }
     if (stat_reduce<NT>(m, c, e, d, R)) {
         S.max_v = m;
