@@ -867,10 +867,11 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
     const u32 t0 = tile * kTile;
     const u32 i = t0 + threadIdx.x;
     const u32 sc = scene_of_slot(P, t0);
+    // the plan's load is issued with the convergence test's
+    const StageRanges R = L.plan[tile];
     const u32 conv = MODE == 0 ? st[sc].conv_iter : st[sc].div_conv_iter;
     if (conv < iter) return;  // the whole tile belongs to one scene
     const SceneSeg& G = P.seg[sc];
-    const StageRanges R = L.plan[tile];
     if (R.total) {
         // every load of a thread is issued before its LDS stores
         const StageLanes E(R);
@@ -1165,10 +1166,11 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
     const u32 i = t0 + threadIdx.x;
     const u32 sc = scene_of_slot(P, t0);
     DeviceState* sst = st + sc;
+    // the plan's load is issued with the convergence test's
+    const StageRanges R = L.plan[tile];
     const u32 conv = mode == 0 ? sst->conv_iter : sst->div_conv_iter;
     if (conv < iter) return;  // tile-uniform
     const SceneSeg& G = P.seg[sc];
-    const StageRanges R = L.plan[tile];
     if (R.total) {
         // every load of a thread is issued before its LDS stores
         const StageLanes E(R);
@@ -1625,12 +1627,14 @@ __global__ __launch_bounds__(NS) void k_advect_sorted(
     float4* __restrict__ va, const DeviceState* __restrict__ st, int force, VStats V) {
     const u32 i = blockIdx.x * NS + threadIdx.x;
     PStat q;
+    // the slot's loads are issued with the state's (slots [0, nslots) are
+    // allocated)
+    const float4 x4 = xs[i];
+    float4 v = vs[i];
     if (!advect_due(st[0], force)) return;  // grid-uniform
     const SceneSeg& G = P.seg[0];
     if (i < G.n) {
         const float dt = G.dt;
-        const float4 x4 = xs[i];
-        float4 v = vs[i];
         const float sp = norm3(v.x, v.y, v.z);
         if (sp > vmax) {
             const float f = vmax / sp;
@@ -1695,15 +1699,17 @@ __global__ __launch_bounds__(NS) void k_reorder_sorted(Nbr P, const u32* __restr
                                                        u32* __restrict__ inv,
                                                        float4* __restrict__ xs,
                                                        float4* __restrict__ vs,
-                                                       StageRanges* __restrict__ plan) {
+                                                       StageRanges* __restrict__ plan, u32 n) {
     static_assert(NS == kTile, "one workgroup per tile");
     const u32 s = blockIdx.x * NS + threadIdx.x;
+    // the slot's index loads are issued with the state's (n: the scene's
+    // particle count)
+    const u32 id = s < n ? perm[s] : 0u;
+    const u32 o = s < n ? inv[id] : 0u;
     if (!st[0].advected) return;  // grid-uniform
     if (threadIdx.x == 0) plan[blockIdx.x] = stage_ranges(P, P.seg[0], blockIdx.x * kTile);
     const SceneSeg& G = P.seg[0];
-    if (s >= G.n) return;
-    const u32 id = perm[s];
-    const u32 o = inv[id];
+    if (s >= n) return;
     const float4 w = va[o];
     xs[s] = xa[o];
     vs[s] = make_float4(w.x * G.decay + G.gdx, w.y * G.decay + G.gdy, w.z * G.decay + G.gdz, 0.0f);
@@ -2565,7 +2571,7 @@ void DFSPHSolver::phase_prefix_grid() {
         // when any scene has XSPH)
         k_reorder_sorted<<<nslots_ / NS, NS, 0, stream_>>>(
             P, fluid_grid_.sorted_indices(), xa_.data(), va_.data(), dstate_.data(), inv_.data(),
-            xs_.data(), any_xsph_ ? vpre_.data() : vs_.data(), plan_.data());
+            xs_.data(), any_xsph_ ? vpre_.data() : vs_.data(), plan_.data(), total_);
         WSIM_LAUNCH_CHECK();
         tend();
     } else {
