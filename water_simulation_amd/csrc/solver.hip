@@ -1621,10 +1621,14 @@ __global__ __launch_bounds__(NS) void k_advect_gather(
 // statistics partials cover the same 256-slot blocks as k_advect_gather's
 // 256-index blocks (binary64 momentum and energy sums then add in another
 // order; max |v|, NaN and escapee counts are order-free).
+// Also writes each slot's cell key of the advected position (ka), which the
+// next grid's key kernel gathers in original index order: 4 bytes a particle
+// from a 4 MB array at a million particles instead of its 16-byte position.
 __global__ __launch_bounds__(NS) void k_advect_sorted(
     Nbr P, const float4* __restrict__ xs, const float4* __restrict__ vs, float vmax, float lx,
     float ly, float lz, float hx, float hy, float hz, float4* __restrict__ xa,
-    float4* __restrict__ va, const DeviceState* __restrict__ st, int force, VStats V) {
+    float4* __restrict__ va, u32* __restrict__ ka, const DeviceState* __restrict__ st, int force,
+    VStats V) {
     const u32 i = blockIdx.x * NS + threadIdx.x;
     PStat q;
     // the slot's loads are issued with the state's (slots [0, nslots) are
@@ -1651,6 +1655,7 @@ __global__ __launch_bounds__(NS) void k_advect_sorted(
         if (pz > hz) { pz = hz; if (v.z > 0) v.z = 0; }
         xa[i] = make_float4(px, py, pz, 0.0f);
         va[i] = make_float4(v.x, v.y, v.z, 0.0f);
+        ka[i] = P.gf.key_of(px, py, pz);
         const float lo[3] = {lx, ly, lz}, hi[3] = {hx, hy, hz};
         q = particle_stat(P.mass, px, py, pz, v.x, v.y, v.z, lo, hi);
     }
@@ -1658,13 +1663,14 @@ __global__ __launch_bounds__(NS) void k_advect_sorted(
 }
 
 // Cell keys (cell_grid.cu:24-30) in original index order, each index reading
-// its particle through inv (its slot in the substep just advected): the
-// advected position, or, when that substep's solve has not finished
-// (nothing advected), the position it started from, which rebuilds that
-// substep's own grid.  Workgroup t writes the sort's tile t and counts the
-// first digit of its keys for the sort (radix_count_*).
+// its particle through inv (its slot in the substep just advected): the key
+// the advection computed (ka), or, when that substep's solve has not
+// finished (nothing advected), the key of the position it started from,
+// which rebuilds that substep's own grid.  Workgroup t writes the sort's
+// tile t and counts the first digit of its keys for the sort
+// (radix_count_*).
 __global__ __launch_bounds__(kRadixThreads) void k_keys_sorted(
-    u32 n, GridGeom g, const u32* __restrict__ inv, const float4* __restrict__ xa,
+    u32 n, GridGeom g, const u32* __restrict__ inv, const u32* __restrict__ ka,
     const float4* __restrict__ xs, const DeviceState* __restrict__ st, u32* __restrict__ keys,
     u32* __restrict__ vals, u32* span, u32 dmask, u32* __restrict__ hist, u32 tiles) {
     __shared__ u32 cnt[256];
@@ -1676,8 +1682,13 @@ __global__ __launch_bounds__(kRadixThreads) void k_keys_sorted(
         const u32 id = blockIdx.x * kRadixTile + r * kRadixThreads + threadIdx.x;
         if (id < n) {
             const u32 o = inv[id];
-            const float4 p = adv ? xa[o] : xs[o];
-            const u32 key = g.key_of(p.x, p.y, p.z);
+            u32 key;
+            if (adv) {
+                key = ka[o];
+            } else {
+                const float4 p = xs[o];
+                key = g.key_of(p.x, p.y, p.z);
+            }
             keys[id] = key;
             vals[id] = id;
             radix_count_key(cnt, key & dmask);
@@ -2170,6 +2181,7 @@ void DFSPHSolver::init(const std::vector<u32>& counts, const std::vector<float>&
     if (!batched_) {
         xa_ = DeviceBuffer<float4>(slots_cap);
         va_ = DeviceBuffer<float4>(slots_cap);
+        ka_ = DeviceBuffer<u32>(slots_cap);
     }
     state_sorted_ = grid_sorted_ = defer_store_ = false;
     plan_ = DeviceBuffer<StageRanges>(slots_cap / kTile);
@@ -2559,7 +2571,7 @@ void DFSPHSolver::phase_prefix_grid() {
         tbegin(kKGrid);
         const u32 tiles = radix_tiles(n);
         k_keys_sorted<<<tiles, kRadixThreads, 0, stream_>>>(
-            n, fluid_grid_.geom(), inv_.data(), xa_.data(), xs_.data(), dstate_.data(),
+            n, fluid_grid_.geom(), inv_.data(), ka_.data(), xs_.data(), dstate_.data(),
             fluid_grid_.staged_keys(), fluid_grid_.staged_values(), fluid_grid_.span(),
             fluid_grid_.first_digit_mask(), fluid_grid_.sort_hist(), tiles);
         WSIM_LAUNCH_CHECK();
@@ -2740,7 +2752,7 @@ void DFSPHSolver::phase_advect(bool force) {
         k_advect_sorted<<<nslots_ / NS, NS, 0, stream_>>>(
             P, xs_.data(), vs_.data(), 5.0f, cfg_.domain_min[0] + eps, cfg_.domain_min[1] + eps,
             cfg_.domain_min[2] + eps, cfg_.domain_max[0] - eps, cfg_.domain_max[1] - eps,
-            cfg_.domain_max[2] - eps, xa_.data(), va_.data(), dstate_.data(), force ? 1 : 0,
+            cfg_.domain_max[2] - eps, xa_.data(), va_.data(), ka_.data(), dstate_.data(), force ? 1 : 0,
             vstats());
     } else if (!batched_) {
         k_advect_gather<<<nslots_ / NS, NS, 0, stream_>>>(

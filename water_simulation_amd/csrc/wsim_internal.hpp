@@ -624,8 +624,9 @@ private:
     DeviceBuffer<float4> gb_;   // boundary gradient sums (tolerance mode with a boundary)
     DeviceBuffer<u32> inv_;     // working slot of each original index (plain solver)
     // advance(): the advected state in the working-slot order of the substep
-    // that advected it
+    // that advected it, and the cell key of each advected position
     DeviceBuffer<float4> xa_, va_;
+    DeviceBuffer<u32> ka_;
     bool defer_store_ = false;   // advection writes xa_/va_, not the store
     bool state_sorted_ = false;  // the current state is in xa_/va_ (the store is stale)
     bool grid_sorted_ = false;   // the current grid phase gathered from xa_/va_
