@@ -235,6 +235,15 @@ __device__ inline float spline_gradc_near(float d, const Spline& sp) {
     const float c = pair_div(q < 1.0f ? inner : outer, sp.l * d, sp);
     return d < 1e-12f ? 0.0f : c;
 }
+// spline_gradc_near for a listed pair: its factor was not 0, so d >= 1e-12
+// and the test is left out (same bits).
+__device__ inline float spline_gradc_listed(float d, const Spline& sp) {
+    const float q = pair_q(d, sp);
+    const float inner = sp.sigma * (-3.0f * q + 2.25f * q * q);
+    const float t = 2.0f - q;
+    const float outer = sp.sigma_m075 * t * t;
+    return pair_div(q < 1.0f ? inner : outer, sp.l * d, sp);
+}
 
 __device__ inline float norm3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
 // The radicand of norm3, in the same operation order.
@@ -263,6 +272,10 @@ template <> struct PairMath<false> {
     }
     __device__ static inline float c(float r2, const Spline& sp) {
         return spline_gradc_near(pair_sqrt(r2, sp), sp);
+    }
+    // c of a listed pair (c != 0 when the list was built)
+    __device__ static inline float c_listed(float r2, const Spline& sp) {
+        return spline_gradc_listed(pair_sqrt(r2, sp), sp);
     }
     __device__ static inline void wc(float r2, const Spline& sp, float& wv, float& cv) {
         const float d = pair_sqrt(r2, sp);
@@ -310,6 +323,17 @@ template <> struct PairMath<true> {
         float d, inv_d;
         dist(r2, d, inv_d);
         return c_of_q(d * sp.inv_l, d, inv_d, sp);
+    }
+    // c of a listed pair: its factor was not 0, so r2 > 0 and d >= 1e-12,
+    // and the tests for both are left out (same bits)
+    __device__ static inline float c_listed(float r2, const Spline& sp) {
+        const float inv_d = __builtin_amdgcn_rsqf(r2);
+        const float d = r2 * inv_d;
+        const float q = d * sp.inv_l;
+        const float inner = sp.ck_in * __builtin_fmaf(2.25f, q, -3.0f);
+        const float t = 2.0f - q;
+        const float outer = sp.ck_out * (t * t) * inv_d;
+        return q < 1.0f ? inner : outer;
     }
     __device__ static inline void wc(float r2, const Spline& sp, float& wv, float& cv) {
         float d, inv_d;

@@ -753,6 +753,10 @@ __global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* 
             const float4 p = xs[i];
             const float4 vi = vs[i];
             float rd = 0.0f;
+            // tolerance mode: the fluid terms sum c (u . r) in rf, times m once
+            float rf = 0.0f;
+            float& racc = F ? rf : rd;
+            const float mf = F ? 1.0f : P.mass;
             const u32 cw = L.cnt[i];
             const u32 cf = cw & 0xFFFFu, cb = cw >> 16;
             if (cf <= L.kf && cb <= L.kb) {
@@ -772,15 +776,15 @@ __global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* 
                     }
 #pragma unroll
                     for (int u = 0; u < PU; ++u)
-                        rho_dot_fluid<F>(rd, p, vi, q[u], w[u], R.total ? pair_c_f<F>(p, q[u], P.sp) : c[u],
-                                      P.mass);
+                        rho_dot_fluid<F>(racc, p, vi, q[u], w[u], R.total ? pair_c_listed<F>(p, q[u], P.sp) : c[u],
+                                      mf);
                 }
                 for (; k < cf; ++k) {
                     u32 j;
                     float c;
                     fluid_entry(L, R, k, i, j, c);
                     const float4 q = xs[j];
-                    rho_dot_fluid<F>(rd, p, vi, q, vs[j], R.total ? pair_c_f<F>(p, q, P.sp) : c, P.mass);
+                    rho_dot_fluid<F>(racc, p, vi, q, vs[j], R.total ? pair_c_listed<F>(p, q, P.sp) : c, mf);
                 }
                 k = 0;
                 for (; k + PU <= cb; k += PU) {
@@ -805,7 +809,7 @@ __global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* 
                     const float r2 = PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
                     if (r2 >= P.sp.far2) return;
                     const float c = stencil_c<F>(r2, P.sp);
-                    rho_dot_fluid<F>(rd, p, vi, q, vs[s], c, P.mass);
+                    rho_dot_fluid<F>(racc, p, vi, q, vs[s], c, mf);
                 });
                 if (P.nb && !(F && L.gb)) {
                     for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
@@ -817,6 +821,7 @@ __global__ __launch_bounds__(NT) void k_change(Nbr P, PairList L, const float4* 
                     });
                 }
             }
+            if (F) rd = __builtin_fmaf(P.mass, rf, rd);
             if (F && L.gb) rd += dot_gb(vi, L.gb[i]);
             const float rho_i = rho[i];
             float kap, err;
@@ -897,6 +902,10 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
         const float4 p = xs[i];
         const float4 vi = vs[i];
         float rd = 0.0f;
+        // tolerance mode: the fluid terms sum c (u . r) in rf, times m once
+        float rf = 0.0f;
+        float& racc = F ? rf : rd;
+        const float mf = F ? 1.0f : P.mass;
         const u32 cw = L.cnt[i];
         const u32 cf = cw & 0xFFFFu, cb = cw >> 16;
         if (cf <= L.kf && cb <= L.kb) {
@@ -905,7 +914,7 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
                 // compact list: LDS entries only; c is recomputed from the staged
                 // positions with the expression that produced it
                 auto term = [&](const float4& q, const float4& w, bool keep) {
-                    rho_dot_fluid<F>(rd, p, vi, q, w, pair_c_f<F>(p, q, P.sp), P.mass, keep);
+                    rho_dot_fluid<F>(racc, p, vi, q, w, pair_c_listed<F>(p, q, P.sp), mf, keep);
                 };
                 auto at = [&](u32 a, float4& q, float4& w) {
                     const float4 x = s_a[a];
@@ -937,11 +946,11 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
                     }
 #pragma unroll
                     for (int u = 0; u < PU; ++u)
-                        rho_dot_fluid<F>(rd, p, vi, q[u], w[u], __uint_as_float(e[u].y), P.mass);
+                        rho_dot_fluid<F>(racc, p, vi, q[u], w[u], __uint_as_float(e[u].y), mf);
                 }
                 for (; k < cf; ++k) {
                     const uint2 e = ld_pair(L.f + pair_at(k, i, L.kf));
-                    rho_dot_fluid<F>(rd, p, vi, xs[e.x], vs[e.x], __uint_as_float(e.y), P.mass);
+                    rho_dot_fluid<F>(racc, p, vi, xs[e.x], vs[e.x], __uint_as_float(e.y), mf);
                 }
             }
             k = 0;
@@ -967,7 +976,7 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
                 const float r2 = PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
                 if (r2 >= P.sp.far2) return;
                 const float c = stencil_c<F>(r2, P.sp);
-                rho_dot_fluid<F>(rd, p, vi, q, vs[s], c, P.mass);
+                rho_dot_fluid<F>(racc, p, vi, q, vs[s], c, mf);
             });
             if (P.nb && !(F && L.gb)) {
                 for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
@@ -979,6 +988,7 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
                 });
             }
         }
+        if (F) rd = __builtin_fmaf(P.mass, rf, rd);
         if (F && L.gb) rd += dot_gb(vi, L.gb[i]);
         const float rho_i = rho[i];
         float kap, err;
@@ -1071,6 +1081,13 @@ __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* _
     const float4 p = xk[i];
     const float ki = p.w;
     float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+    // tolerance mode: the fluid terms sum -(k_i + k_j) c r in a, times dt m once
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    float& fx = F ? ax : dx;
+    float& fy = F ? ay : dy;
+    float& fz = F ? az : dz;
+    const float fdt = F ? 1.0f : dt;
+    auto mk = [&](float kj) { return F ? ki + kj : P.mass * (ki + kj); };
     float bf = 0.0f;
     if (P.nb) {
         const float k_i = kappa[i];
@@ -1091,15 +1108,15 @@ __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* _
             for (int u = 0; u < PU; ++u) q[u] = xk[j[u]];
 #pragma unroll
             for (int u = 0; u < PU; ++u)
-                kappa_term<F>(dx, dy, dz, p, q[u], R.total ? pair_c_f<F>(p, q[u], P.sp) : c[u],
-                           P.mass * (ki + q[u].w), dt);
+                kappa_term<F>(fx, fy, fz, p, q[u], R.total ? pair_c_listed<F>(p, q[u], P.sp) : c[u],
+                           mk(q[u].w), fdt);
         }
         for (; k < cf; ++k) {
             u32 j;
             float c;
             fluid_entry(L, R, k, i, j, c);
             const float4 q = xk[j];
-            kappa_term<F>(dx, dy, dz, p, q, R.total ? pair_c_f<F>(p, q, P.sp) : c, P.mass * (ki + q.w), dt);
+            kappa_term<F>(fx, fy, fz, p, q, R.total ? pair_c_listed<F>(p, q, P.sp) : c, mk(q.w), fdt);
         }
         k = 0;
         for (; k + PU <= cb; k += PU) {
@@ -1125,7 +1142,7 @@ __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* _
             const float r2 = PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
             if (r2 >= P.sp.far2) return;
             const float c = stencil_c<F>(r2, P.sp);
-            kappa_term<F>(dx, dy, dz, p, q, c, P.mass * (ki + q.w), dt);
+            kappa_term<F>(fx, fy, fz, p, q, c, mk(q.w), fdt);
         });
         if (P.nb && !(F && L.gb)) {
             for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
@@ -1136,6 +1153,12 @@ __global__ __launch_bounds__(NT) void k_apply(Nbr P, PairList L, const float4* _
                 kappa_term<F>(dx, dy, dz, p, q, c, q.w * (ki + bf), dt);
             });
         }
+    }
+    if (F) {
+        const float sc = dt * P.mass;
+        dx = __builtin_fmaf(sc, ax, dx);
+        dy = __builtin_fmaf(sc, ay, dy);
+        dz = __builtin_fmaf(sc, az, dz);
     }
     if (F && L.gb) {  // sum_b m_b (k_i + k_b) grad W_ib = (k_i + k_b) Gb_i
         const float4 g = L.gb[i];
@@ -1192,6 +1215,13 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
     const float4 p = xk[i];
     const float ki = p.w;
     float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+    // tolerance mode: the fluid terms sum -(k_i + k_j) c r in a, times dt m once
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    float& fx = F ? ax : dx;
+    float& fy = F ? ay : dy;
+    float& fz = F ? az : dz;
+    const float fdt = F ? 1.0f : dt;
+    auto mk = [&](float kj) { return F ? ki + kj : P.mass * (ki + kj); };
     float bf = 0.0f;
     if (P.nb) {
         const float k_i = kappa[i];
@@ -1203,7 +1233,7 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
         u32 k = 0;
         if (staged) {
             auto term = [&](const float4& q, bool keep) {
-                kappa_term<F>(dx, dy, dz, p, q, pair_c_f<F>(p, q, P.sp), P.mass * (ki + q.w), dt, keep);
+                kappa_term<F>(fx, fy, fz, p, q, pair_c_listed<F>(p, q, P.sp), mk(q.w), fdt, keep);
             };
             for_stage_quads(L, i, cf, [&](u32 a0, u32 a1, u32 a2, u32 a3, u32 m) {
                 const float4 q0 = s_x[a0], q1 = s_x[a1], q2 = s_x[a2], q3 = s_x[a3];
@@ -1222,12 +1252,12 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
                 for (int u = 0; u < PU; ++u) q[u] = xk[e[u].x];
 #pragma unroll
                 for (int u = 0; u < PU; ++u)
-                    kappa_term<F>(dx, dy, dz, p, q[u], __uint_as_float(e[u].y), P.mass * (ki + q[u].w), dt);
+                    kappa_term<F>(fx, fy, fz, p, q[u], __uint_as_float(e[u].y), mk(q[u].w), fdt);
             }
             for (; k < cf; ++k) {
                 const uint2 e = ld_pair(L.f + pair_at(k, i, L.kf));
                 const float4 q = xk[e.x];
-                kappa_term<F>(dx, dy, dz, p, q, __uint_as_float(e.y), P.mass * (ki + q.w), dt);
+                kappa_term<F>(fx, fy, fz, p, q, __uint_as_float(e.y), mk(q.w), fdt);
             }
         }
         k = 0;
@@ -1254,7 +1284,7 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
             const float r2 = PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
             if (r2 >= P.sp.far2) return;
             const float c = stencil_c<F>(r2, P.sp);
-            kappa_term<F>(dx, dy, dz, p, q, c, P.mass * (ki + q.w), dt);
+            kappa_term<F>(fx, fy, fz, p, q, c, mk(q.w), fdt);
         });
         if (P.nb && !(F && L.gb)) {
             for_stencil(P.gb, P.csb, 0u, p.x, p.y, p.z, [&](u32 s) {
@@ -1265,6 +1295,12 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
                 kappa_term<F>(dx, dy, dz, p, q, c, q.w * (ki + bf), dt);
             });
         }
+    }
+    if (F) {
+        const float sc = dt * P.mass;
+        dx = __builtin_fmaf(sc, ax, dx);
+        dy = __builtin_fmaf(sc, ay, dy);
+        dz = __builtin_fmaf(sc, az, dz);
     }
     if (F && L.gb) {  // sum_b m_b (k_i + k_b) grad W_ib = (k_i + k_b) Gb_i
         const float4 g = L.gb[i];

@@ -338,6 +338,12 @@ __device__ __forceinline__ float pair_c_f(const float4& p, const float4& q, cons
     if (F) return PairMath<true>::c(PairMath<true>::r2(p.x - q.x, p.y - q.y, p.z - q.z), sp);
     return pair_c(p, q, sp);
 }
+// pair_c_f for a pair of the fluid list (its c was not 0 when the list was
+// built): the same bits without the tests for coincident particles.
+template <bool F>
+__device__ __forceinline__ float pair_c_listed(const float4& p, const float4& q, const Spline& sp) {
+    return PairMath<F>::c_listed(PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z), sp);
+}
 // Gradient factor of a stencil candidate inside the support (r2 < far2).
 template <bool F>
 __device__ __forceinline__ float stencil_c(float r2, const Spline& sp) {
