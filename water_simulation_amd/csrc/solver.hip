@@ -414,6 +414,41 @@ __device__ __forceinline__ void xsph_term(float& dx, float& dy, float& dz, const
                     rho_j, eps, P, keep);
 }
 
+// One XSPH term of a pair from the fluid list (r2 > 0).  Binary32 path:
+// xsph_term into (dx, dy, dz).  Tolerance mode: the term without its
+// pair-independent factor 2 eps m sigma, summed in (tx, ty, tz), which
+// xsph_listed_scale applies once per particle:
+//   (W / sigma) / max(rho_i + rho_j, 2e-9) (v_j - v_i).
+template <bool F>
+__device__ __forceinline__ void xsph_listed(float& dx, float& dy, float& dz, float& tx, float& ty,
+                                            float& tz, const float4& p, const float4& vi,
+                                            float rho_i, const float4& q, const float4& vj,
+                                            float rho_j, float eps, const Nbr& P,
+                                            bool keep = true) {
+    if (!F) {
+        xsph_term<F>(dx, dy, dz, p, vi, rho_i, q, vj, rho_j, eps, P, keep);
+        return;
+    }
+    const float r2 = PairMath<true>::r2(p.x - q.x, p.y - q.y, p.z - q.z);
+    const float s = r2 * __builtin_amdgcn_rsqf(r2) * P.sp.inv_l;
+    const float inner = __builtin_fmaf(s * s, __builtin_fmaf(0.75f, s, -1.5f), 1.0f);
+    const float t = 2.0f - s;
+    const float w = s < 1.0f ? inner : 0.25f * (t * t * t);
+    const float k = keep ? w * __builtin_amdgcn_rcpf(fmaxf(rho_i + rho_j, 2.0f * 1e-9f)) : 0.0f;
+    tx = __builtin_fmaf(k, vj.x - vi.x, tx);
+    ty = __builtin_fmaf(k, vj.y - vi.y, ty);
+    tz = __builtin_fmaf(k, vj.z - vi.z, tz);
+}
+template <bool F>
+__device__ __forceinline__ void xsph_listed_scale(float& dx, float& dy, float& dz, float tx,
+                                                  float ty, float tz, float eps, const Nbr& P) {
+    if (!F) return;
+    const float K = 2.0f * eps * P.mass * P.sp.sigma;
+    dx = K * tx;
+    dy = K * ty;
+    dz = K * tz;
+}
+
 // XSPH as a Jacobi sweep: every neighbour velocity is the pre-XSPH one.  A
 // scene without viscosity copies its velocities (dfsph.cu:198 skips XSPH).
 // The fluid pair list holds every in-support pair in stencil order (pairs
@@ -437,6 +472,7 @@ __global__ __launch_bounds__(NT) void k_xsph(Nbr P, PairList L, const float4* __
     const float4 p = xs[i];
     const float rho_i = rho[i];
     float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+    float tx = 0.0f, ty = 0.0f, tz = 0.0f;  // tolerance mode: see xsph_listed
     const u32 cf = L.cnt[i] & 0xFFFFu;
     if (cf <= L.kf) {
         const StageRanges R = L.plan[i / kTile];
@@ -456,13 +492,14 @@ __global__ __launch_bounds__(NT) void k_xsph(Nbr P, PairList L, const float4* __
             }
 #pragma unroll
             for (int u = 0; u < PU; ++u)
-                xsph_term<F>(dx, dy, dz, p, vi, rho_i, q[u], vj[u], rj[u], eps, P);
+                xsph_listed<F>(dx, dy, dz, tx, ty, tz, p, vi, rho_i, q[u], vj[u], rj[u], eps, P);
         }
         for (; k < cf; ++k) {
             u32 j;
             fluid_entry(L, R, k, i, j, c_unused);
-            xsph_term<F>(dx, dy, dz, p, vi, rho_i, xs[j], vin[j], rho[j], eps, P);
+            xsph_listed<F>(dx, dy, dz, tx, ty, tz, p, vi, rho_i, xs[j], vin[j], rho[j], eps, P);
         }
+        xsph_listed_scale<F>(dx, dy, dz, tx, ty, tz, eps, P);
         vout[i] = make_float4(vi.x + dx, vi.y + dy, vi.z + dz, 0.0f);
         return;
     }
@@ -523,6 +560,7 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
     const float4 p = xs[i];
     const float rho_i = rho[i];
     float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+    float tx = 0.0f, ty = 0.0f, tz = 0.0f;  // tolerance mode: see xsph_listed
     const u32 cf = L.cnt[i] & 0xFFFFu;
     if (cf <= L.kf) {
         u32 k = 0;
@@ -536,10 +574,10 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
                 const float4 w1 = make_float4(q1.w, b1.x, b1.y, 0.0f);
                 const float4 w2 = make_float4(q2.w, b2.x, b2.y, 0.0f);
                 const float4 w3 = make_float4(q3.w, b3.x, b3.y, 0.0f);
-                xsph_term<F>(dx, dy, dz, p, vi, rho_i, q0, w0, r0, eps, P);
-                xsph_term<F>(dx, dy, dz, p, vi, rho_i, q1, w1, r1, eps, P, m > 1u);
-                xsph_term<F>(dx, dy, dz, p, vi, rho_i, q2, w2, r2, eps, P, m > 2u);
-                xsph_term<F>(dx, dy, dz, p, vi, rho_i, q3, w3, r3, eps, P, m > 3u);
+                xsph_listed<F>(dx, dy, dz, tx, ty, tz, p, vi, rho_i, q0, w0, r0, eps, P);
+                xsph_listed<F>(dx, dy, dz, tx, ty, tz, p, vi, rho_i, q1, w1, r1, eps, P, m > 1u);
+                xsph_listed<F>(dx, dy, dz, tx, ty, tz, p, vi, rho_i, q2, w2, r2, eps, P, m > 2u);
+                xsph_listed<F>(dx, dy, dz, tx, ty, tz, p, vi, rho_i, q3, w3, r3, eps, P, m > 3u);
             });
         } else {
             // four pairs per trip: their gathers are issued together, the
@@ -558,13 +596,14 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
-                    xsph_term<F>(dx, dy, dz, p, vi, rho_i, q[u], w[u], rj[u], eps, P);
+                    xsph_listed<F>(dx, dy, dz, tx, ty, tz, p, vi, rho_i, q[u], w[u], rj[u], eps, P);
             }
             for (; k < cf; ++k) {
                 const u32 j = ld_pair(L.f + pair_at(k, i, L.kf)).x;
-                xsph_term<F>(dx, dy, dz, p, vi, rho_i, xs[j], vin[j], rho[j], eps, P);
+                xsph_listed<F>(dx, dy, dz, tx, ty, tz, p, vi, rho_i, xs[j], vin[j], rho[j], eps, P);
             }
         }
+        xsph_listed_scale<F>(dx, dy, dz, tx, ty, tz, eps, P);
         vout[i] = make_float4(vi.x + dx, vi.y + dy, vi.z + dz, 0.0f);
         return;
     }
