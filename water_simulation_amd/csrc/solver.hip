@@ -1501,13 +1501,15 @@ struct FrameStep {
 // With F.fp (advance(), one scene): publishes the state whether or not the
 // solve finished, and after an advection first takes the substep's dt off
 // the time left and then computes the next substep's dt.
+// The statistics step of scene sc by one workgroup of NT threads (every
+// thread calls it).
 template <int NT>
-__global__ __launch_bounds__(NT) void k_finish_vstats(SceneSeg* __restrict__ seg, VStats V,
-                                                      DeviceState* st, int guarded, int force,
-                                                      FrameStep F) {
+__device__ __forceinline__ void finish_vstats(SceneSeg* __restrict__ seg, VStats V,
+                                              DeviceState* st, int guarded, int force,
+                                              FrameStep F, u32 sc) {
     __shared__ StatReduce<NT> R;
-    const SceneSeg& G = seg[blockIdx.x];
-    DeviceState& S = st[blockIdx.x];
+    const SceneSeg& G = seg[sc];
+    DeviceState& S = st[sc];
     if (guarded && !advect_due(S, force)) {  // block-uniform
         if (F.fp && threadIdx.x == 0) publish_state(S, F.seq, F.pub);
         return;
@@ -1557,9 +1559,15 @@ __global__ __launch_bounds__(NT) void k_finish_vstats(SceneSeg* __restrict__ seg
         if (F.fp) {
             S.t_rem = S.t_rem - S.dt;
             publish_state(S, F.seq, F.pub);
-            cfl_step(*F.fp, seg[blockIdx.x], S);
+            cfl_step(*F.fp, seg[sc], S);
         }
     }
+}
+template <int NT>
+__global__ __launch_bounds__(NT) void k_finish_vstats(SceneSeg* __restrict__ seg, VStats V,
+                                                      DeviceState* st, int guarded, int force,
+                                                      FrameStep F) {
+    finish_vstats<NT>(seg, V, st, guarded, force, F, blockIdx.x);
 }
 
 struct StoreOut {
@@ -1743,15 +1751,23 @@ __global__ __launch_bounds__(NS) void k_advect_sorted(
 // finished (nothing advected), the key of the position it started from,
 // which rebuilds that substep's own grid.  Workgroup t writes the sort's
 // tile t and counts the first digit of its keys for the sort
-// (radix_count_*).
+// (radix_count_*).  With `fold` (a grid enqueued right behind the advection)
+// workgroup 0 first runs the advection's statistics step, which marks the
+// scene advected; every workgroup takes the same decision from the flags it
+// reads, before or after that mark.
 __global__ __launch_bounds__(kRadixThreads) void k_keys_sorted(
     u32 n, GridGeom g, const u32* __restrict__ inv, const u32* __restrict__ ka,
-    const float4* __restrict__ xs, const DeviceState* __restrict__ st, u32* __restrict__ keys,
-    u32* __restrict__ vals, u32* span, u32 dmask, u32* __restrict__ hist, u32 tiles) {
+    const float4* __restrict__ xs, DeviceState* st, u32* __restrict__ keys,
+    u32* __restrict__ vals, u32* span, u32 dmask, u32* __restrict__ hist, u32 tiles, int fold,
+    int force, SceneSeg* __restrict__ seg, VStats V, FrameStep F) {
     __shared__ u32 cnt[256];
+    const bool adv = st[0].advected != 0u || (fold && advect_due(st[0], force));
+    if (fold && blockIdx.x == 0) {
+        __syncthreads();  // every thread has read the flags
+        finish_vstats<kRadixThreads>(seg, V, st, 1, force, F, 0u);
+    }
     advance_span(span);
     radix_count_begin(cnt);
-    const bool adv = st[0].advected != 0;
 #pragma unroll
     for (int r = 0; r < kRadixTile / kRadixThreads; ++r) {
         const u32 id = blockIdx.x * kRadixTile + r * kRadixThreads + threadIdx.x;
@@ -2467,6 +2483,7 @@ u32 DFSPHSolver::advance(const TimeStepConfig& ts, float v_floor, float time, u3
         WSIM_LAUNCH_CHECK();
     }
     bool grid_ready = false;  // the next substep's grid phase is enqueued
+    grid_fold_ = false;
     // between the substeps of this call the state stays in working-slot
     // order; the store is written when the call returns (or throws)
     defer_store_ = pipe;
@@ -2556,9 +2573,10 @@ bool DFSPHSolver::substep_device(bool next) {
         run_phase(key, [&] {
             if (first) phase_prefix_solve(true);
             if (end > begin) phase_iterations(0, begin, end);
-            // advection; its statistics kernel publishes the state and, once
-            // the substep is done, computes the next substep's time step
-            phase_advect(last);
+            // advection; its statistics step publishes the state and, once
+            // the substep is done, computes the next substep's time step (in
+            // the grid phase's key kernel when that phase follows here)
+            phase_advect(last, tail);
         });
         // the advection above writes xa_/va_: later grids gather from there
         state_sorted_ = true;
@@ -2569,8 +2587,11 @@ bool DFSPHSolver::substep_device(bool next) {
         // that tail's dispatch.
         if (tail) {
             grid_sorted_ = state_sorted_;
-            run_phase(phase_key(kPhPrefixGrid, state_sorted_ ? 1 : 0),
+            grid_fold_ = true;
+            grid_fold_force_ = last;
+            run_phase(phase_key(kPhPrefixGrid, (state_sorted_ ? 1u : 0u) | 2u | (last ? 4u : 0u)),
                       [&] { phase_prefix_grid(); });
+            grid_fold_ = false;
         }
         wait_published();
         if (hstate_[0].advected) {
@@ -2645,10 +2666,14 @@ void DFSPHSolver::phase_prefix_grid() {
         const u32 n = total_;
         tbegin(kKGrid);
         const u32 tiles = radix_tiles(n);
+        // behind the first chunk of a substep the key kernel also runs the
+        // advection's statistics step (phase_advect left it out)
+        const FrameStep F{dframe_.data(), pub_dev_, pubseq_.data()};
         k_keys_sorted<<<tiles, kRadixThreads, 0, stream_>>>(
             n, fluid_grid_.geom(), inv_.data(), ka_.data(), xs_.data(), dstate_.data(),
             fluid_grid_.staged_keys(), fluid_grid_.staged_values(), fluid_grid_.span(),
-            fluid_grid_.first_digit_mask(), fluid_grid_.sort_hist(), tiles);
+            fluid_grid_.first_digit_mask(), fluid_grid_.sort_hist(), tiles,
+            grid_fold_ ? 1 : 0, grid_fold_force_ ? 1 : 0, dseg_.data(), vstats(), F);
         WSIM_LAUNCH_CHECK();
         fluid_grid_.build_staged(n, true);
         tend();
@@ -2817,7 +2842,7 @@ static StoreOut store_out(ParticleStore& f, size_t hd, size_t ha, size_t hk, siz
     return out;
 }
 
-void DFSPHSolver::phase_advect(bool force) {
+void DFSPHSolver::phase_advect(bool force, bool fold_finish) {
     const Nbr P = make_nbr(fluid_grid_, boundary_grid_.get(), xb_.data(), bnear_.data(), nb_, cfg_, mass_, nslots_,
                            dseg_.data(), batched_ ? tile_scene_.data() : nullptr, shapes_.remap);
     const StoreOut out = store_out(fluid_, h_density_, h_alpha_, h_kappa_, h_density_adv_);
@@ -2847,7 +2872,9 @@ void DFSPHSolver::phase_advect(bool force) {
     WSIM_LAUNCH_CHECK();
     tend();
     // also marks the scenes advected (after the divergence solve, the
-    // writeback reports max |v| instead)
+    // writeback reports max |v| instead); with fold_finish the next grid's
+    // key kernel, enqueued right behind, runs it
+    if (fold_finish) return;
     const FrameStep F = defer_store_ ? FrameStep{dframe_.data(), pub_dev_, pubseq_.data()}
                                      : FrameStep{nullptr, nullptr, nullptr};
     k_finish_vstats<1024><<<static_cast<u32>(counts_.size()), 1024, 0, stream_>>>(

@@ -572,7 +572,7 @@ private:
     void phase_prefix_solve(bool gravity);
     dev::Nbr nbr() const;
     void phase_iterations(int mode, u32 begin, u32 end);
-    void phase_advect(bool force);
+    void phase_advect(bool force, bool fold_finish = false);
     void write_back_sorted();  // the state in xa_/va_ (and rho, alpha, ...) to the store
     void fetch_state();
     void take_state();
@@ -630,6 +630,9 @@ private:
     bool defer_store_ = false;   // advection writes xa_/va_, not the store
     bool state_sorted_ = false;  // the current state is in xa_/va_ (the store is stale)
     bool grid_sorted_ = false;   // the current grid phase gathered from xa_/va_
+    // the grid phase being enqueued follows the first chunk of a substep
+    // directly and runs its statistics step (with the chunk's force flag)
+    bool grid_fold_ = false, grid_fold_force_ = false;
     DeviceBuffer<u32> bnear_;  // boundary proximity mask (see stencil.hpp)
     DeviceBuffer<StageRanges> plan_;
     DeviceBuffer<u32> pair_cnt_;
