@@ -88,6 +88,41 @@ def test_advance_matches_host_loop(graphs, over):
         assert 0 < len(misses) < len(iters), iters
 
 
+@pytest.mark.parametrize("fast_math", [False, True])
+def test_advance_tiny_grid_matches_host_loop(fast_math):
+    """A 5 x 5 x 5 cell grid (domain 3 smoothing lengths wide, plus the
+    one-cell margin on each side; every length a power of two, so the cell
+    count is exact): 7-bit cell keys, so the stable sort behind advance()'s
+    key kernel, which counts the first digit itself, is a single pass of
+    fewer than 8 bits."""
+    rng = np.random.default_rng(3)
+    h = 2.0 ** -6
+    r = h / 2
+    g = np.stack(np.meshgrid(*[np.arange(3)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    x = (r + 2 * r * g + rng.uniform(-0.3 * r, 0.3 * r, g.shape)).astype(np.float32)
+    v = rng.normal(0.0, 0.3, x.shape).astype(np.float32)
+    cfg = dict(rest_density=1000.0, particle_radius=r, smoothing_length=h, viscosity=0.01,
+               damping=0.5, domain_min=(0, 0, 0), domain_max=(3 * h,) * 3,
+               max_density_iters=20, eta_density=1e-3, fast_math=fast_math)
+    stores, solvers = [], []
+    for _ in range(2):
+        f = ws.ParticleStore(len(x))
+        f.resize(len(x))
+        f.set_positions(x)
+        f.set_velocities(v)
+        stores.append(f)
+        solvers.append(ws.DFSPHSolver(f, None, ws.DFSPHConfig(**cfg)))
+    a, b = solvers
+    ts = ws.TimeStepper(ws.TimeStepperConfig(frame_dt=1 / 60, max_substeps=16))
+    for frame in range(2):
+        want = _host_loop(a, ts, 0.5, 1 / 60, 12, r)
+        got = b.advance(1 / 60, 12, ts, v_floor=0.5)
+        assert len(got) > 1
+        assert_bitexact(got, want, f"frame {frame} dts")
+        assert_bitexact(stores[1].get_positions(), stores[0].get_positions(), f"frame {frame} x")
+        assert_bitexact(stores[1].get_velocities(), stores[0].get_velocities(), f"frame {frame} v")
+
+
 def test_advance_stops_on_time_and_count():
     x, v, xb, cfg = random_box(n=1000, seed=4)
     _, _, s = _solver(x, v, xb, cfg, graphs=True)
