@@ -249,15 +249,12 @@ __device__ inline StageRanges stage_ranges(const Nbr& P, const SceneSeg& G, u32 
     return R;
 }
 
-// LDS entry of neighbour slot j (which lies in one of the staged ranges).
-__device__ __forceinline__ u32 stage_index(const StageRanges& R, u32 j) {
-    return j >= R.s2 ? R.b2 + (j - R.s2) : (j >= R.s1 ? R.b1 + (j - R.s1) : j - R.s0);
-}
-// The same for a stencil candidate of x offset dx = xk - 1: every stencil
-// cell at that offset of a particle whose key k lies in [kmin, kmax] has a
-// key in [k + dx Dy Dz - Dz - 1, k + dx Dy Dz + Dz + 1], inside range xk,
-// so the entry is j plus that range's offset.  The three offsets are
-// wave-uniform (a wave's particles share one tile).
+// LDS entry of neighbour slot j, a stencil candidate of x offset
+// dx = xk - 1 of a particle of a staged tile: every stencil cell at that
+// offset of a particle whose key k lies in [kmin, kmax] has a key in
+// [k + dx Dy Dz - Dz - 1, k + dx Dy Dz + Dz + 1], inside range xk, so the
+// entry is j plus that range's offset.  The three offsets are wave-uniform
+// (a wave's particles share one tile).
 struct StageOffsets {
     u32 off[3];
     __device__ explicit StageOffsets(const StageRanges& R)
@@ -331,15 +328,10 @@ __device__ __forceinline__ void for_stage_quads(const PairList& L, u32 i, u32 cf
 __device__ __forceinline__ float pair_c(const float4& p, const float4& q, const Spline& sp) {
     return spline_gradc_near(pair_sqrt(dist2(p.x - q.x, p.y - q.y, p.z - q.z), sp), sp);
 }
-// pair_c in either pair-math mode (the list build and the list readers of a
-// mode use the same expression, so a recomputed c equals the recorded one)
-template <bool F>
-__device__ __forceinline__ float pair_c_f(const float4& p, const float4& q, const Spline& sp) {
-    if (F) return PairMath<true>::c(PairMath<true>::r2(p.x - q.x, p.y - q.y, p.z - q.z), sp);
-    return pair_c(p, q, sp);
-}
-// pair_c_f for a pair of the fluid list (its c was not 0 when the list was
-// built): the same bits without the tests for coincident particles.
+// c of a pair of the fluid list in either pair-math mode, with the
+// expression of the list build, so a recomputed c equals the recorded one
+// (its c was not 0 when the list was built, so the tests for coincident
+// particles are left out: the same bits).
 template <bool F>
 __device__ __forceinline__ float pair_c_listed(const float4& p, const float4& q, const Spline& sp) {
     return PairMath<F>::c_listed(PairMath<F>::r2(p.x - q.x, p.y - q.y, p.z - q.z), sp);
