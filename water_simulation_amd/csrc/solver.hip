@@ -1791,9 +1791,10 @@ __global__ __launch_bounds__(kRadixThreads) void k_keys_sorted(
 // The new grid's working slots gathered from the advected slots: positions,
 // and velocities with gravity and damping (dfsph_kernels.cu:347-355, the
 // expression of k_reorder) for the next substep, whose time step the
-// statistics kernel has already computed; inv moves to the new slots.  Thread 0 of each workgroup (one 256-slot tile)
-// also writes the tile's staging plan, as k_stage_plan does.  Without an
-// advection the grid is the previous one: nothing moves and the plans stand.
+// statistics step has already computed; inv moves to the new slots.  Thread
+// 0 of each workgroup (one 256-slot tile) also writes the tile's staging
+// plan, as k_stage_plan does.  Without an advection the grid is the previous
+// one: nothing moves and the plans stand.
 __global__ __launch_bounds__(NS) void k_reorder_sorted(Nbr P, const u32* __restrict__ perm,
                                                        const float4* __restrict__ xa,
                                                        const float4* __restrict__ va,
@@ -2662,7 +2663,8 @@ void DFSPHSolver::phase_prefix_grid() {
     const u32 scenes = static_cast<u32>(counts_.size());
     if (state_sorted_) {
         // advance(): keys, positions and velocities from the advected slots
-        // (xa_, va_, inv_) or, behind an unfinished solve, this substep's
+        // (ka_, xa_, va_ through inv_) or, behind an unfinished solve, the
+        // keys of this substep's own positions (and nothing moves)
         const u32 n = total_;
         tbegin(kKGrid);
         const u32 tiles = radix_tiles(n);
