@@ -1502,11 +1502,12 @@ struct FrameStep {
 // solve finished, and after an advection first takes the substep's dt off
 // the time left and then computes the next substep's dt.
 // The statistics step of scene sc by one workgroup of NT threads (every
-// thread calls it).
+// thread calls it), over `parts` partials from the scene's first tile (0:
+// one per tile of the scene).
 template <int NT>
 __device__ __forceinline__ void finish_vstats(SceneSeg* __restrict__ seg, VStats V,
                                               DeviceState* st, int guarded, int force,
-                                              FrameStep F, u32 sc) {
+                                              FrameStep F, u32 sc, u32 parts) {
     __shared__ StatReduce<NT> R;
     const SceneSeg& G = seg[sc];
     DeviceState& S = st[sc];
@@ -1520,14 +1521,15 @@ __device__ __forceinline__ void finish_vstats(SceneSeg* __restrict__ seg, VStats
     // each thread adds its tiles t, t + NT, t + 2 NT, ... in that order; the
     // loads of four of them are issued before the first is added
     constexpr int kRound = 4;
-    for (u32 t0 = threadIdx.x; t0 < G.ntiles; t0 += kRound * NT) {
+    const u32 np = parts ? parts : G.ntiles;
+    for (u32 t0 = threadIdx.x; t0 < np; t0 += kRound * NT) {
         float pm[kRound];
         u32 pc[kRound], pe[kRound];
         double pd[kRound][kMomParts];
 #pragma unroll
         for (int r = 0; r < kRound; ++r) {
             const u32 t = t0 + static_cast<u32>(r) * NT;
-            if (t < G.ntiles) {
+            if (t < np) {
                 const u32 b = G.tile0 + t;
                 pm[r] = V.pmax[b];
                 pc[r] = V.pnan[b];
@@ -1538,7 +1540,7 @@ __device__ __forceinline__ void finish_vstats(SceneSeg* __restrict__ seg, VStats
         }
 #pragma unroll
         for (int r = 0; r < kRound; ++r) {
-            if (t0 + static_cast<u32>(r) * NT >= G.ntiles) break;
+            if (t0 + static_cast<u32>(r) * NT >= np) break;
             m = fmaxf(m, pm[r]);
             c += pc[r];
             e += pe[r];
@@ -1566,8 +1568,8 @@ __device__ __forceinline__ void finish_vstats(SceneSeg* __restrict__ seg, VStats
 template <int NT>
 __global__ __launch_bounds__(NT) void k_finish_vstats(SceneSeg* __restrict__ seg, VStats V,
                                                       DeviceState* st, int guarded, int force,
-                                                      FrameStep F) {
-    finish_vstats<NT>(seg, V, st, guarded, force, F, blockIdx.x);
+                                                      FrameStep F, u32 parts = 0) {
+    finish_vstats<NT>(seg, V, st, guarded, force, F, blockIdx.x, parts);
 }
 
 struct StoreOut {
@@ -1707,20 +1709,48 @@ __global__ __launch_bounds__(NS) void k_advect_gather(
 // Also writes each slot's cell key of the advected position (ka), which the
 // next grid's key kernel gathers in original index order: 4 bytes a particle
 // from a 4 MB array at a million particles instead of its 16-byte position.
+// Each thread advects two slots (i and i + NS of a 2 NS-slot block) and
+// adds their statistics before the workgroup's reduction, so the partials
+// cover 2 NS slots (kAdvectSlots).
+constexpr u32 kAdvectSlots = 2 * NS;
+__device__ __forceinline__ PStat stat_add(const PStat& a, const PStat& b) {
+    PStat r;
+    r.mag = fmaxf(a.mag, b.mag);
+    r.bad = a.bad + b.bad;
+    r.esc = a.esc + b.esc;
+    r.m = a.m + b.m;
+    r.px = a.px + b.px;
+    r.py = a.py + b.py;
+    r.pz = a.pz + b.pz;
+    r.ke = a.ke + b.ke;
+    return r;
+}
 __global__ __launch_bounds__(NS) void k_advect_sorted(
     Nbr P, const float4* __restrict__ xs, const float4* __restrict__ vs, float vmax, float lx,
     float ly, float lz, float hx, float hy, float hz, float4* __restrict__ xa,
     float4* __restrict__ va, u32* __restrict__ ka, const DeviceState* __restrict__ st, int force,
     VStats V) {
-    const u32 i = blockIdx.x * NS + threadIdx.x;
-    PStat q;
-    // the slot's loads are issued with the state's (slots [0, nslots) are
+    const u32 i0 = blockIdx.x * kAdvectSlots + threadIdx.x;
+    PStat q[2];
+    // the slots' loads are issued with the state's (slots [0, nslots) are
     // allocated)
-    const float4 x4 = xs[i];
-    float4 v = vs[i];
+    float4 x4[2], v4[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const u32 i = i0 + u * NS;
+        if (i < P.nslots) {
+            x4[u] = xs[i];
+            v4[u] = vs[i];
+        }
+    }
     if (!advect_due(st[0], force)) return;  // grid-uniform
     const SceneSeg& G = P.seg[0];
-    if (i < G.n) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const u32 i = i0 + u * NS;
+        if (i >= G.n) continue;
+        float4 v = v4[u];
+        const float4 xx = x4[u];
         const float dt = G.dt;
         const float sp = norm3(v.x, v.y, v.z);
         if (sp > vmax) {
@@ -1729,7 +1759,7 @@ __global__ __launch_bounds__(NS) void k_advect_sorted(
             v.y *= f;
             v.z *= f;
         }
-        float px = x4.x + v.x * dt, py = x4.y + v.y * dt, pz = x4.z + v.z * dt;
+        float px = xx.x + v.x * dt, py = xx.y + v.y * dt, pz = xx.z + v.z * dt;
         if (px < lx) { px = lx; if (v.x < 0) v.x = 0; }
         if (py < ly) { py = ly; if (v.y < 0) v.y = 0; }
         if (pz < lz) { pz = lz; if (v.z < 0) v.z = 0; }
@@ -1740,9 +1770,9 @@ __global__ __launch_bounds__(NS) void k_advect_sorted(
         va[i] = make_float4(v.x, v.y, v.z, 0.0f);
         ka[i] = P.gf.key_of(px, py, pz);
         const float lo[3] = {lx, ly, lz}, hi[3] = {hx, hy, hz};
-        q = particle_stat(P.mass, px, py, pz, v.x, v.y, v.z, lo, hi);
+        q[u] = particle_stat(P.mass, px, py, pz, v.x, v.y, v.z, lo, hi);
     }
-    vstats_partial(q, V);
+    vstats_partial(stat_add(q[0], q[1]), V);
 }
 
 // Cell keys (cell_grid.cu:24-30) in original index order, each index reading
@@ -1759,12 +1789,12 @@ __global__ __launch_bounds__(kRadixThreads) void k_keys_sorted(
     u32 n, GridGeom g, const u32* __restrict__ inv, const u32* __restrict__ ka,
     const float4* __restrict__ xs, DeviceState* st, u32* __restrict__ keys,
     u32* __restrict__ vals, u32* span, u32 dmask, u32* __restrict__ hist, u32 tiles, int fold,
-    int force, SceneSeg* __restrict__ seg, VStats V, FrameStep F) {
+    int force, SceneSeg* __restrict__ seg, VStats V, FrameStep F, u32 parts) {
     __shared__ u32 cnt[256];
     const bool adv = st[0].advected != 0u || (fold && advect_due(st[0], force));
     if (fold && blockIdx.x == 0) {
         __syncthreads();  // every thread has read the flags
-        finish_vstats<kRadixThreads>(seg, V, st, 1, force, F, 0u);
+        finish_vstats<kRadixThreads>(seg, V, st, 1, force, F, 0u, parts);
     }
     advance_span(span);
     radix_count_begin(cnt);
@@ -2675,7 +2705,8 @@ void DFSPHSolver::phase_prefix_grid() {
             n, fluid_grid_.geom(), inv_.data(), ka_.data(), xs_.data(), dstate_.data(),
             fluid_grid_.staged_keys(), fluid_grid_.staged_values(), fluid_grid_.span(),
             fluid_grid_.first_digit_mask(), fluid_grid_.sort_hist(), tiles,
-            grid_fold_ ? 1 : 0, grid_fold_force_ ? 1 : 0, dseg_.data(), vstats(), F);
+            grid_fold_ ? 1 : 0, grid_fold_force_ ? 1 : 0, dseg_.data(), vstats(), F,
+            div_up(nslots_, kAdvectSlots));
         WSIM_LAUNCH_CHECK();
         fluid_grid_.build_staged(n, true);
         tend();
@@ -2851,7 +2882,7 @@ void DFSPHSolver::phase_advect(bool force, bool fold_finish) {
     const float eps = 1e-3f;  // dfsph_kernels.cu:377
     tbegin(kKAdvect);
     if (defer_store_) {
-        k_advect_sorted<<<nslots_ / NS, NS, 0, stream_>>>(
+        k_advect_sorted<<<div_up(nslots_, kAdvectSlots), NS, 0, stream_>>>(
             P, xs_.data(), vs_.data(), 5.0f, cfg_.domain_min[0] + eps, cfg_.domain_min[1] + eps,
             cfg_.domain_min[2] + eps, cfg_.domain_max[0] - eps, cfg_.domain_max[1] - eps,
             cfg_.domain_max[2] - eps, xa_.data(), va_.data(), ka_.data(), dstate_.data(), force ? 1 : 0,
@@ -2879,8 +2910,10 @@ void DFSPHSolver::phase_advect(bool force, bool fold_finish) {
     if (fold_finish) return;
     const FrameStep F = defer_store_ ? FrameStep{dframe_.data(), pub_dev_, pubseq_.data()}
                                      : FrameStep{nullptr, nullptr, nullptr};
+    // (the slot-order advection writes one partial per kAdvectSlots slots)
     k_finish_vstats<1024><<<static_cast<u32>(counts_.size()), 1024, 0, stream_>>>(
-        dseg_.data(), vstats(), dstate_.data(), 1, force ? 1 : 0, F);
+        dseg_.data(), vstats(), dstate_.data(), 1, force ? 1 : 0, F,
+        defer_store_ ? div_up(nslots_, kAdvectSlots) : 0u);
     WSIM_LAUNCH_CHECK();
 }
 
