@@ -538,6 +538,12 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
         return;
     }
     const StageRanges R = L.plan[tile];
+    // this slot's own inputs, loaded with the staging loads (slots
+    // [0, nslots) are allocated)
+    const float4 vi = vin[i];
+    const float4 p = xs[i];
+    const float rho_i = rho[i];
+    const u32 cf = L.cnt[i] & 0xFFFFu;
     if (R.total) {
         // every load of a thread is issued before its LDS stores
         const StageLanes E(R);
@@ -556,12 +562,8 @@ __global__ __launch_bounds__(kTile) void k_xsph_st(Nbr P, PairList L, const floa
     }
     __syncthreads();
     if (l >= G.n) return;
-    const float4 vi = vin[i];
-    const float4 p = xs[i];
-    const float rho_i = rho[i];
     float dx = 0.0f, dy = 0.0f, dz = 0.0f;
     float tx = 0.0f, ty = 0.0f, tz = 0.0f;  // tolerance mode: see xsph_listed
-    const u32 cf = L.cnt[i] & 0xFFFFu;
     if (cf <= L.kf) {
         u32 k = 0;
         if (R.total) {
@@ -916,6 +918,11 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
     const u32 conv = MODE == 0 ? st[sc].conv_iter : st[sc].div_conv_iter;
     if (conv < iter) return;  // the whole tile belongs to one scene
     const SceneSeg& G = P.seg[sc];
+    // this slot's own inputs, loaded with the staging loads (slots
+    // [0, nslots) are allocated)
+    const float4 p = xs[i];
+    const float4 vi = vs[i];
+    const u32 cw = L.cnt[i];
     if (R.total) {
         // every load of a thread is issued before its LDS stores
         const StageLanes E(R);
@@ -938,14 +945,11 @@ __global__ __launch_bounds__(kTile) void k_change_st(Nbr P, PairList L,
     if (i - G.off < G.n) {
         const u32* cs = P.csf + G.key_base;
         const u32 shift = G.off - G.start;
-        const float4 p = xs[i];
-        const float4 vi = vs[i];
         float rd = 0.0f;
         // tolerance mode: the fluid terms sum c (u . r) in rf, times m once
         float rf = 0.0f;
         float& racc = F ? rf : rd;
         const float mf = F ? 1.0f : P.mass;
-        const u32 cw = L.cnt[i];
         const u32 cf = cw & 0xFFFFu, cb = cw >> 16;
         if (cf <= L.kf && cb <= L.kb) {
             u32 k = 0;
@@ -1233,6 +1237,10 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
     const u32 conv = mode == 0 ? sst->conv_iter : sst->div_conv_iter;
     if (conv < iter) return;  // tile-uniform
     const SceneSeg& G = P.seg[sc];
+    // this slot's own inputs, loaded with the staging loads (slots
+    // [0, nslots) are allocated)
+    const float4 p = xk[i];
+    const u32 cw = L.cnt[i];
     if (R.total) {
         // every load of a thread is issued before its LDS stores
         const StageLanes E(R);
@@ -1251,7 +1259,6 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
     const u32* cs = P.csf + G.key_base;
     const u32 shift = G.off - G.start;
     const float dt = G.dt;
-    const float4 p = xk[i];
     const float ki = p.w;
     float dx = 0.0f, dy = 0.0f, dz = 0.0f;
     // tolerance mode: the fluid terms sum -(k_i + k_j) c r in a, times dt m once
@@ -1266,7 +1273,6 @@ __global__ __launch_bounds__(kTile) void k_apply_st(Nbr P, PairList L,
         const float k_i = kappa[i];
         bf = pform ? k_i / fmaxf(rho0, 1e-9f) : k_i / fmaxf(rho0 * rho0, 1e-9f);
     }
-    const u32 cw = L.cnt[i];
     const u32 cf = cw & 0xFFFFu, cb = cw >> 16;
     if (cf <= L.kf && cb <= L.kb) {
         u32 k = 0;
