@@ -131,6 +131,11 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const u32 dmask = (1u << nbits) - 1u;
+    // this tile's offset in each digit's run and the digit totals (threads
+    // [0, 256), one digit each), loaded with the keys
+    const bool digit_thread = static_cast<u32>(tid) <= dmask;
+    const u32 goff = digit_thread ? offs[static_cast<u64>(tid) * nblocks + blockIdx.x] : 0u;
+    const u32 dtot = digit_thread ? totals[tid] : 0u;
     for (int i = tid; i < RS_WAVES * RS_DIGITS; i += RS_THREADS) (&s_wc[0][0])[i] = 0;
     __syncthreads();
 
@@ -167,7 +172,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
 
     // Per digit (threads [0, 256), one digit each): exclusive prefix over
     // waves, tile total, this tile's offset in the digit's run.
-    u32 total = 0, dtot = 0;
+    u32 total = 0;
     if (tid < RS_DIGITS) {
         const int d = tid;
 #pragma unroll
@@ -176,10 +181,9 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
             s_wc[ww][d] = total;
             total += c;
         }
-        const bool used = static_cast<u32>(d) <= dmask;
-        s_goff[d] = used ? offs[static_cast<u64>(d) * nblocks + blockIdx.x] : 0u;
-        dtot = used ? totals[d] : 0u;  // digit base: scan of the digit totals
+        s_goff[d] = goff;
     }
+    // digit base: scan of the digit totals
     const u32 dbase = block_exclusive_scan256(dtot, s_tmp);
     const u32 start = block_exclusive_scan256(total, s_tmp);
     if (tid < RS_DIGITS) {

@@ -1797,6 +1797,14 @@ __global__ __launch_bounds__(kRadixThreads) void k_keys_sorted(
     u32* __restrict__ vals, u32* span, u32 dmask, u32* __restrict__ hist, u32 tiles, int fold,
     int force, SceneSeg* __restrict__ seg, VStats V, FrameStep F, u32 parts) {
     __shared__ u32 cnt[256];
+    constexpr int kItems = kRadixTile / kRadixThreads;
+    // the indices' slots, loaded with the state's
+    u32 slot[kItems];
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+        const u32 id = blockIdx.x * kRadixTile + r * kRadixThreads + threadIdx.x;
+        slot[r] = id < n ? inv[id] : 0u;
+    }
     const bool adv = st[0].advected != 0u || (fold && advect_due(st[0], force));
     if (fold && blockIdx.x == 0) {
         __syncthreads();  // every thread has read the flags
@@ -1805,10 +1813,10 @@ __global__ __launch_bounds__(kRadixThreads) void k_keys_sorted(
     advance_span(span);
     radix_count_begin(cnt);
 #pragma unroll
-    for (int r = 0; r < kRadixTile / kRadixThreads; ++r) {
+    for (int r = 0; r < kItems; ++r) {
         const u32 id = blockIdx.x * kRadixTile + r * kRadixThreads + threadIdx.x;
         if (id < n) {
-            const u32 o = inv[id];
+            const u32 o = slot[r];
             u32 key;
             if (adv) {
                 key = ka[o];
