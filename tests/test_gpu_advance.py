@@ -123,6 +123,34 @@ def test_advance_tiny_grid_matches_host_loop(fast_math):
         assert_bitexact(stores[1].get_velocities(), stores[0].get_velocities(), f"frame {frame} v")
 
 
+def test_advance_single_particle_matches_host_loop():
+    """One particle: the sort behind advance()'s key kernel has nothing to
+    order (its first-digit counts go unused) and the cell table holds one run."""
+    h = 2.0 ** -6
+    r = h / 2
+    x = np.array([[1.5 * h, 1.5 * h, 1.5 * h]], np.float32)
+    v = np.array([[0.2, -0.1, 0.05]], np.float32)
+    cfg = dict(rest_density=1000.0, particle_radius=r, smoothing_length=h, viscosity=0.01,
+               damping=0.5, domain_min=(0, 0, 0), domain_max=(3 * h,) * 3,
+               max_density_iters=20, eta_density=1e-3)
+    stores, solvers = [], []
+    for _ in range(2):
+        f = ws.ParticleStore(1)
+        f.resize(1)
+        f.set_positions(x)
+        f.set_velocities(v)
+        stores.append(f)
+        solvers.append(ws.DFSPHSolver(f, None, ws.DFSPHConfig(**cfg)))
+    a, b = solvers
+    ts = ws.TimeStepper(ws.TimeStepperConfig(frame_dt=1 / 60, max_substeps=16))
+    want = _host_loop(a, ts, 0.5, 1 / 60, 12, r)
+    got = b.advance(1 / 60, 12, ts, v_floor=0.5)
+    assert len(got) > 1
+    assert_bitexact(got, want, "dts")
+    assert_bitexact(stores[1].get_positions(), stores[0].get_positions(), "x")
+    assert_bitexact(stores[1].get_velocities(), stores[0].get_velocities(), "v")
+
+
 def test_advance_stops_on_time_and_count():
     x, v, xb, cfg = random_box(n=1000, seed=4)
     _, _, s = _solver(x, v, xb, cfg, graphs=True)
