@@ -178,6 +178,7 @@ def dam_break(ws, x, xb, cfg, steps: int, warmup: int, timing: bool, graphs: boo
     stats = solver.stats()
     dts = [float(d) for d in dts]
     iters = [q["density_iters"] for q in recs]
+    div_iters = [q["divergence_iters"] for q in recs]
     errs = [q["mean_density_error"] for q in recs]
     kernel_times = {}
     if timing:
@@ -192,7 +193,8 @@ def dam_break(ws, x, xb, cfg, steps: int, warmup: int, timing: bool, graphs: boo
     eta = cfg.get("eta_density", 1e-3) * cfg.get("rest_density", 1000.0)
     out = {"value": n * steps / elapsed, "elapsed": elapsed, "stats": stats,
            "eta_met_fraction": float(np.mean(np.array(errs) < eta)),
-           "kernel_times": kernel_times, "iters": iters, "dts": dts, "x0": x0, "v0": v0,
+           "kernel_times": kernel_times, "iters": iters, "div_iters": div_iters, "dts": dts,
+           "x0": x0, "v0": v0,
            "cfg": cfg, "x_end": fluid.get_positions()}
     solver.close()
     return out
@@ -665,6 +667,8 @@ def main() -> int:
             "max_density_iters": cfg["max_density_iters"],
             "eta_density": cfg["eta_density"],
             "mean_density_iters": float(np.mean(iters)),
+            "divergence_solve": bool(cfg.get("enable_divergence_solve", False)),
+            "mean_divergence_iters": float(np.mean(run["div_iters"])),
             "final_mean_density_error": st["mean_density_error"],
             "eta_reached_last_substep": bool(st["mean_density_error"] <
                                              cfg["eta_density"] * cfg["rest_density"]),
