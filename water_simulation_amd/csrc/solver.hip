@@ -59,11 +59,14 @@ using namespace dev;
 
 // parts: 1 positions, 2 velocities, 3 both.
 // inv (optional, with parts & 1): inv[original index] = working slot.
+// tag_ids (with parts & 1): the position's w holds the original index's bits
+// (advance() carries them through its slot-order state); otherwise 0.
 __global__ __launch_bounds__(NS) void k_reorder(Nbr P, const u32* __restrict__ perm,
                                                 const float* __restrict__ x,
                                                 const float* __restrict__ v,
                                                 float4* __restrict__ xs, float4* __restrict__ vs,
-                                                int gravity, int parts, u32* __restrict__ inv) {
+                                                int gravity, int parts, u32* __restrict__ inv,
+                                                int tag_ids = 0) {
     const u32 i = blockIdx.x * NS + threadIdx.x;
     if (i >= P.nslots) return;
     const SceneSeg& G = P.seg[scene_of_slot(P, i)];
@@ -73,7 +76,7 @@ __global__ __launch_bounds__(NS) void k_reorder(Nbr P, const u32* __restrict__ p
     const u64 o = 3ull * id;
     if (parts & 1) {
         const float3 p = *reinterpret_cast<const float3*>(x + o);
-        xs[i] = make_float4(p.x, p.y, p.z, 0.0f);
+        xs[i] = make_float4(p.x, p.y, p.z, tag_ids ? __uint_as_float(id) : 0.0f);
         if (inv) inv[id] = i;
     }
     if (!(parts & 2)) return;
@@ -1704,11 +1707,15 @@ __global__ __launch_bounds__(NS) void k_advect_gather(
 // store; the next substep gathers from there (positions, and velocities with
 // gravity, in one reorder) and the store is written once, when advance()
 // returns.  The grid, the permutation and every value are the ones the store
-// round trip gives.  Slots move little between substeps, so the reorder's
-// gathers from the previous slots stay nearly sequential; only the cell keys,
-// which the stable sort needs in original index order, gather at random.
+// round trip gives.  The stable sort needs the cell keys in original index
+// order, and they gather at random (through inv); its values are the
+// particles' previous slots, and each position's w carries the particle's
+// original index, so the reorder gathers from the previous slots, which
+// stay nearly sequential because slots move little between substeps, and
+// only inv's update lands at random.
 
-// k_advect on one scene's working slots, writing xa/va in slot order.  The
+// k_advect on one scene's working slots, writing xa/va in slot order (xa's w
+// keeps the original index that xs carries).  The
 // statistics partials cover the same 256-slot blocks as k_advect_gather's
 // 256-index blocks (binary64 momentum and energy sums then add in another
 // order; max |v|, NaN and escapee counts are order-free).
@@ -1772,7 +1779,7 @@ __global__ __launch_bounds__(NS) void k_advect_sorted(
         if (px > hx) { px = hx; if (v.x > 0) v.x = 0; }
         if (py > hy) { py = hy; if (v.y > 0) v.y = 0; }
         if (pz > hz) { pz = hz; if (v.z > 0) v.z = 0; }
-        xa[i] = make_float4(px, py, pz, 0.0f);
+        xa[i] = make_float4(px, py, pz, xx.w);  // w: the original index
         va[i] = make_float4(v.x, v.y, v.z, 0.0f);
         ka[i] = P.gf.key_of(px, py, pz);
         const float lo[3] = {lx, ly, lz}, hi[3] = {hx, hy, hz};
@@ -1785,9 +1792,9 @@ __global__ __launch_bounds__(NS) void k_advect_sorted(
 // its particle through inv (its slot in the substep just advected): the key
 // the advection computed (ka), or, when that substep's solve has not
 // finished (nothing advected), the key of the position it started from,
-// which rebuilds that substep's own grid.  Workgroup t writes the sort's
-// tile t and counts the first digit of its keys for the sort
-// (radix_count_*).  With `fold` (a grid enqueued right behind the advection)
+// which rebuilds that substep's own grid.  The sort's value of each index
+// is that slot.  Workgroup t writes the sort's tile t and counts the first
+// digit of its keys for the sort (radix_count_*).  With `fold` (a grid enqueued right behind the advection)
 // workgroup 0 first runs the advection's statistics step, which marks the
 // scene advected; every workgroup takes the same decision from the flags it
 // reads, before or after that mark.
@@ -1825,21 +1832,23 @@ __global__ __launch_bounds__(kRadixThreads) void k_keys_sorted(
                 key = g.key_of(p.x, p.y, p.z);
             }
             keys[id] = key;
-            vals[id] = id;
+            vals[id] = o;
             radix_count_key(cnt, key & dmask);
         }
     }
     radix_count_end(cnt, dmask, hist, tiles);
 }
 
-// The new grid's working slots gathered from the advected slots: positions,
-// and velocities with gravity and damping (dfsph_kernels.cu:347-355, the
-// expression of k_reorder) for the next substep, whose time step the
-// statistics step has already computed; inv moves to the new slots.  Thread
+// The new grid's working slots gathered from the advected slots (prev: the
+// sorted values, each new slot's previous slot): positions with the
+// original index in w, and velocities with gravity and damping
+// (dfsph_kernels.cu:347-355, the expression of k_reorder) for the next
+// substep, whose time step the statistics step has already computed; inv
+// moves to the new slots.  Thread
 // 0 of each workgroup (one 256-slot tile) also writes the tile's staging
 // plan, as k_stage_plan does.  Without an advection the grid is the previous
 // one: nothing moves and the plans stand.
-__global__ __launch_bounds__(NS) void k_reorder_sorted(Nbr P, const u32* __restrict__ perm,
+__global__ __launch_bounds__(NS) void k_reorder_sorted(Nbr P, const u32* __restrict__ prev,
                                                        const float4* __restrict__ xa,
                                                        const float4* __restrict__ va,
                                                        const DeviceState* __restrict__ st,
@@ -1849,32 +1858,31 @@ __global__ __launch_bounds__(NS) void k_reorder_sorted(Nbr P, const u32* __restr
                                                        StageRanges* __restrict__ plan, u32 n) {
     static_assert(NS == kTile, "one workgroup per tile");
     const u32 s = blockIdx.x * NS + threadIdx.x;
-    // the slot's index loads are issued with the state's (n: the scene's
-    // particle count)
-    const u32 id = s < n ? perm[s] : 0u;
-    const u32 o = s < n ? inv[id] : 0u;
+    // the slot's source is loaded with the state (n: the scene's particle
+    // count)
+    const u32 o = s < n ? prev[s] : 0u;
     if (!st[0].advected) return;  // grid-uniform
     if (threadIdx.x == 0) plan[blockIdx.x] = stage_ranges(P, P.seg[0], blockIdx.x * kTile);
     const SceneSeg& G = P.seg[0];
     if (s >= n) return;
+    const float4 x = xa[o];
     const float4 w = va[o];
-    xs[s] = xa[o];
+    xs[s] = x;
     vs[s] = make_float4(w.x * G.decay + G.gdx, w.y * G.decay + G.gdy, w.z * G.decay + G.gdz, 0.0f);
-    inv[id] = s;
+    inv[__float_as_uint(x.w)] = s;
 }
 
 // The state of advance()'s last substep to the caller's store, in original
 // order: positions and velocities from xa/va, and the substep's attributes.
 __global__ __launch_bounds__(NS) void k_write_back_sorted(
-    Nbr P, const u32* __restrict__ perm, const float4* __restrict__ xa,
-    const float4* __restrict__ va, StoreOut out, const float* __restrict__ rho,
-    const float* __restrict__ alpha, const float* __restrict__ kappa,
-    const float* __restrict__ rho_adv, int wb_solve) {
+    Nbr P, const float4* __restrict__ xa, const float4* __restrict__ va, StoreOut out,
+    const float* __restrict__ rho, const float* __restrict__ alpha,
+    const float* __restrict__ kappa, const float* __restrict__ rho_adv, int wb_solve) {
     const u32 s = blockIdx.x * NS + threadIdx.x;
     if (s >= P.seg[0].n) return;
-    const u32 id = perm[s];
-    const u64 o = 3ull * id;
     const float4 x = xa[s], v = va[s];
+    const u32 id = __float_as_uint(x.w);
+    const u64 o = 3ull * id;
     *reinterpret_cast<float3*>(out.x + o) = make_float3(x.x, x.y, x.z);
     *reinterpret_cast<float3*>(out.v + o) = make_float3(v.x, v.y, v.z);
     out.rho[id] = rho[s];
@@ -2743,7 +2751,8 @@ void DFSPHSolver::phase_prefix_grid() {
         k_reorder<<<nslots_ / NS, NS, 0, stream_>>>(P, fluid_grid_.sorted_indices(),
                                                      fluid_.positions(), fluid_.velocities(),
                                                      xs_.data(), nullptr, 0, 1,
-                                                     batched_ ? nullptr : inv_.data());
+                                                     batched_ ? nullptr : inv_.data(),
+                                                     defer_store_ ? 1 : 0);
         WSIM_LAUNCH_CHECK();
         tend();
     }
@@ -2936,8 +2945,8 @@ void DFSPHSolver::write_back_sorted() {
     const Nbr P = nbr();
     const StoreOut out = store_out(fluid_, h_density_, h_alpha_, h_kappa_, h_density_adv_);
     k_write_back_sorted<<<nslots_ / NS, NS, 0, stream_>>>(
-        P, fluid_grid_.sorted_indices(), xa_.data(), va_.data(), out, rho_.data(), alpha_.data(),
-        kappa_.data(), rho_adv_.data(), cfg_.max_density_iters > 0 ? 1 : 0);
+        P, xa_.data(), va_.data(), out, rho_.data(), alpha_.data(), kappa_.data(),
+        rho_adv_.data(), cfg_.max_density_iters > 0 ? 1 : 0);
     WSIM_LAUNCH_CHECK();
     state_sorted_ = false;
 }
