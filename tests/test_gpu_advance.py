@@ -6,8 +6,8 @@ writes back when it returns (it keeps the state in working-slot order
 between substeps) and the statistics must agree bit for bit, with the next
 substep's grid phase pipelined behind the state read, across first
 iteration chunks that do and do not finish the solve, with and without graph
-replay, and on the serial paths (divergence solve, kernel timing, an empty
-store).
+replay, with surface tension, and on the serial paths (divergence solve,
+kernel timing, an empty store).
 """
 from __future__ import annotations
 
@@ -56,6 +56,7 @@ def _first_chunk(last):
     dict(),                                   # iteration counts vary, chunks miss and hit
     dict(max_density_iters=2),                # the first chunk covers every iteration
     dict(enable_divergence_solve=True, max_divergence_iters=10),  # serial path
+    dict(enable_surface_tension=True, surface_tension=0.5),      # tension kernels in the loop
 ])
 def test_advance_matches_host_loop(graphs, over):
     x, v, xb, cfg = random_box(n=8000, seed=21, **over)
