@@ -1722,10 +1722,11 @@ __global__ __launch_bounds__(NS) void k_advect_gather(
 // Also writes each slot's cell key of the advected position (ka), which the
 // next grid's key kernel gathers in original index order: 4 bytes a particle
 // from a 4 MB array at a million particles instead of its 16-byte position.
-// Each thread advects two slots (i and i + NS of a 2 NS-slot block) and
-// adds their statistics before the workgroup's reduction, so the partials
-// cover 2 NS slots (kAdvectSlots).
-constexpr u32 kAdvectSlots = 2 * NS;
+// Each thread advects kAdvectPer slots (i, i + NS, ... of a kAdvectSlots
+// block) and adds their statistics before the workgroup's reduction, so the
+// partials cover kAdvectSlots slots.
+constexpr int kAdvectPer = 4;
+constexpr u32 kAdvectSlots = kAdvectPer * NS;
 __device__ __forceinline__ PStat stat_add(const PStat& a, const PStat& b) {
     PStat r;
     r.mag = fmaxf(a.mag, b.mag);
@@ -1744,12 +1745,12 @@ __global__ __launch_bounds__(NS) void k_advect_sorted(
     float4* __restrict__ va, u32* __restrict__ ka, const DeviceState* __restrict__ st, int force,
     VStats V) {
     const u32 i0 = blockIdx.x * kAdvectSlots + threadIdx.x;
-    PStat q[2];
+    PStat q[kAdvectPer];
     // the slots' loads are issued with the state's (slots [0, nslots) are
     // allocated)
-    float4 x4[2], v4[2];
+    float4 x4[kAdvectPer], v4[kAdvectPer];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < kAdvectPer; ++u) {
         const u32 i = i0 + u * NS;
         if (i < P.nslots) {
             x4[u] = xs[i];
@@ -1759,7 +1760,7 @@ __global__ __launch_bounds__(NS) void k_advect_sorted(
     if (!advect_due(st[0], force)) return;  // grid-uniform
     const SceneSeg& G = P.seg[0];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < kAdvectPer; ++u) {
         const u32 i = i0 + u * NS;
         if (i >= G.n) continue;
         float4 v = v4[u];
@@ -1785,7 +1786,10 @@ __global__ __launch_bounds__(NS) void k_advect_sorted(
         const float lo[3] = {lx, ly, lz}, hi[3] = {hx, hy, hz};
         q[u] = particle_stat(P.mass, px, py, pz, v.x, v.y, v.z, lo, hi);
     }
-    vstats_partial(stat_add(q[0], q[1]), V);
+    PStat t = q[0];
+#pragma unroll
+    for (int u = 1; u < kAdvectPer; ++u) t = stat_add(t, q[u]);
+    vstats_partial(t, V);
 }
 
 // Cell keys (cell_grid.cu:24-30) in original index order, each index reading
