@@ -33,22 +33,31 @@ def _solver(x, v, xb, cfg, graphs):
     return fluid, bnd, s
 
 
-def _host_loop(solver, ts, v_floor, time, max_sub, r, iters=None):
+def _host_loop(solver, ts, v_floor, time, max_sub, r, iters=None, errs=None):
     dts, rem = [], np.float32(time)
     while rem > np.float32(1e-6) and len(dts) < max_sub:
         dt = np.float32(ts.next_dt(max(solver.max_velocity(), v_floor), r, float(rem)))
         solver.step(float(dt))
         if iters is not None:
             iters.append(solver.stats()["density_iters"])
+        if errs is not None:
+            errs.append(solver.stats()["mean_density_error"])
         dts.append(dt)
         rem = np.float32(rem - dt)
     return np.array(dts, np.float32)
 
 
-def _first_chunk(last):
-    # the library's first chunk of density iterations: one more than the
-    # previous substep needed
-    return max(1, min(last + 1, 64))
+def _first_chunk(last, ratio):
+    # the library's first chunk of density iterations: the previous substep's
+    # count, one more when its solve ended above half the tolerance (ratio =
+    # its final mean density error over eta rho0), at least 2
+    return max(2, min(last + (1 if ratio > 0.5 else 0), 64))
+
+
+def _misses(iters, errs, tol):
+    # substeps whose solve needed more iterations than their first chunk held
+    return [k for k in range(len(iters))
+            if iters[k] > _first_chunk(iters[k - 1] if k else 0, errs[k - 1] / tol if k else 1.0)]
 
 
 @pytest.mark.parametrize("graphs", [False, True])
@@ -64,10 +73,10 @@ def test_advance_matches_host_loop(graphs, over):
     r = cfg["particle_radius"]
     fa, ba, a = _solver(x, v, xb, cfg, graphs=False)
     fb, bb, b = _solver(x, v, xb, cfg, graphs=graphs)
-    iters = []
+    iters, errs = [], []
     for frame in range(2):
         n0 = len(iters)
-        want = _host_loop(a, ts, 0.5, 1 / 24, 12, r, iters)
+        want = _host_loop(a, ts, 0.5, 1 / 24, 12, r, iters, errs)
         got, recs = b.advance(1 / 24, 12, ts, v_floor=0.5, with_stats=True)
         assert_bitexact(got, want, f"frame {frame} dts")
         assert [q["density_iters"] for q in recs] == iters[n0:]
@@ -84,9 +93,8 @@ def test_advance_matches_host_loop(graphs, over):
     if not over:
         # some substeps needed more iterations than their first chunk holds
         # (the pipelined grid phase was then superseded), others did not
-        misses = [k for k in range(len(iters))
-                  if iters[k] > _first_chunk(iters[k - 1] if k else 0)]
-        assert 0 < len(misses) < len(iters), iters
+        misses = _misses(iters, errs, cfg["eta_density"] * cfg["rest_density"])
+        assert 0 < len(misses) < len(iters), (iters, errs)
 
 
 @pytest.mark.parametrize("fast_math", [False, True])
@@ -173,8 +181,8 @@ def test_advance_full_size_dam_break_matches_host_loop(fast_math):
     r = cfg["particle_radius"]
     fa, ba, a = _solver(x, v, xb, cfg, graphs=False)
     fb, bb, b = _solver(x, v, xb, cfg, graphs=False)
-    iters = []
-    want = _host_loop(a, ts, 0.5, 1.0, 24, r, iters)
+    iters, errs = [], []
+    want = _host_loop(a, ts, 0.5, 1.0, 24, r, iters, errs)
     got, recs = b.advance(1.0, 24, ts, v_floor=0.5, with_stats=True)
     assert len(got) == 24
     assert_bitexact(got, want, "1M dts")
@@ -189,7 +197,7 @@ def test_advance_full_size_dam_break_matches_host_loop(fast_math):
     # binary64 sums over blocks of slots (advance) or of indices (step)
     for k in ("total_mass", "kinetic_energy"):
         assert abs(sa[k] - sb[k]) <= 1e-12 * abs(sa[k]), k
-    misses = [k for k in range(len(iters)) if iters[k] > _first_chunk(iters[k - 1] if k else 0)]
+    misses = _misses(iters, errs, cfg["eta_density"] * cfg["rest_density"])
     print(f"\n1M advance (fast_math={fast_math}): iterations {iters}, first-chunk misses {misses}")
     for s in (a, b):
         s.close()

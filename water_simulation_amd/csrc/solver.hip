@@ -2610,9 +2610,13 @@ void DFSPHSolver::wait_published() {
 
 bool DFSPHSolver::substep_device(bool next) {
     const u32 maxit = cfg_.max_density_iters;
-    // first chunk: one iteration more than the previous substep needed, so
-    // only a jump of two or more iterations needs a second chunk
-    u32 chunk = std::max(1u, std::min(last_iters_[0] + 1u, 64u));
+    // first chunk: the previous substep's iteration count, plus one when its
+    // solve ended above half the tolerance (the count then tends to grow),
+    // and at least the two iterations every density solve runs.  A chunk
+    // that falls short costs a continuation chunk behind a superseded grid
+    // phase; one iteration too many costs two early-exit launches.
+    const u32 extra = last_err_ratio_ > 0.5 ? 1u : 0u;
+    u32 chunk = std::max(2u, std::min(last_iters_[0] + extra, 64u));
     u32 begin = 0, end = std::min(chunk, maxit);
     // two instances of each graph, used in turn: a graph is not launched
     // again while its previous launch is still running its tail
@@ -3059,6 +3063,8 @@ void DFSPHSolver::fetch_state() {
 void DFSPHSolver::take_state() {
     const u32 scenes = static_cast<u32>(counts_.size());
     u32 max_d = 0, max_v = 0;
+    double ratio = 0.0;
+    const double tol = static_cast<double>(cfg_.eta_density) * cfg_.rest_density;
     for (u32 s = 0; s < scenes; ++s) {
         const DeviceState& h = hstate_[s];
         StepStats& st = stats_[s];
@@ -3082,9 +3088,11 @@ void DFSPHSolver::take_state() {
         st.substeps += 1;
         max_d = std::max(max_d, st.density_iters);
         max_v = std::max(max_v, st.divergence_iters);
+        ratio = std::max(ratio, tol > 0.0 ? st.mean_density_error / tol : 1.0);
     }
     last_iters_[0] = max_d;
     last_iters_[1] = max_v;
+    last_err_ratio_ = ratio;
     if (timing_) {
         int valid[kKNumKinds];
         for (int k = 0; k < kKNumKinds; ++k) valid[k] = std::numeric_limits<int>::max();

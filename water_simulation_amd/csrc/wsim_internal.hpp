@@ -647,6 +647,7 @@ private:
     KernelShapes shapes_;
     std::vector<TuneRecord> tune_log_;
     u32 last_iters_[2] = {0, 0};  // iterations of the previous substep's solves
+    double last_err_ratio_ = 1.0;  // its density solve's final error over eta rho0 (scene max)
     DeviceBuffer<FrameParams> dframe_;  // advance(): the time-step rule on the device
     DeviceBuffer<u32> pubseq_;          // sequence number of the last published state
     PinnedBuffer<StatePub> pub_;        // published state (mapped, coherent host memory)
