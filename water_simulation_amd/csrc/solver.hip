@@ -2610,13 +2610,7 @@ void DFSPHSolver::wait_published() {
 
 bool DFSPHSolver::substep_device(bool next) {
     const u32 maxit = cfg_.max_density_iters;
-    // first chunk: the previous substep's iteration count, plus one when its
-    // solve ended above half the tolerance (the count then tends to grow),
-    // and at least the two iterations every density solve runs.  A chunk
-    // that falls short costs a continuation chunk behind a superseded grid
-    // phase; one iteration too many costs two early-exit launches.
-    const u32 extra = last_err_ratio_ > 0.5 ? 1u : 0u;
-    u32 chunk = std::max(2u, std::min(last_iters_[0] + extra, 64u));
+    u32 chunk = first_density_chunk();
     u32 begin = 0, end = std::min(chunk, maxit);
     // two instances of each graph, used in turn: a graph is not launched
     // again while its previous launch is still running its tail
@@ -2989,15 +2983,27 @@ bool DFSPHSolver::all_converged(int mode) {
     return true;
 }
 
+// First chunk of density iterations: the previous substep's iteration
+// count, plus one when its solve ended above half the tolerance (the count
+// then tends to grow), and at least the two iterations every density solve
+// runs (it tests from its second iteration on).  A chunk that falls short
+// costs a continuation chunk (in advance(), behind a superseded grid phase);
+// one iteration too many costs two early-exit launches.
+u32 DFSPHSolver::first_density_chunk() const {
+    const u32 extra = last_err_ratio_ > 0.5 ? 1u : 0u;
+    return std::max(2u, std::min(last_iters_[0] + extra, 64u));
+}
+
 // Iterations run in chunks between host checks; inside a chunk, scenes that
-// converged turn the remaining launches into no-ops.  The first chunk is the
-// previous substep's iteration count rounded up to a power of two (at least
-// 2 for the density solve, which tests from its second iteration on), then
-// chunks double up to 32.
+// converged turn the remaining launches into no-ops.  The density solve's
+// first chunk is first_density_chunk(); the divergence solve's is the
+// previous substep's count rounded up to a power of two.  Chunks then double
+// up to 32.
 void DFSPHSolver::solve(int mode) {
     const u32 maxit = mode == 0 ? cfg_.max_density_iters : cfg_.max_divergence_iters;
-    u32 it = 0, chunk = mode == 0 ? 2 : 1;
-    while (chunk < last_iters_[mode] && chunk < maxit) chunk <<= 1;
+    u32 it = 0, chunk = 1;
+    if (mode == 0) chunk = first_density_chunk();
+    else while (chunk < last_iters_[1] && chunk < maxit) chunk <<= 1;
     while (it < maxit) {
         const u32 end = (maxit - it < chunk) ? maxit : it + chunk;
         run_phase(phase_key(mode == 0 ? kPhDensityIters : kPhDivIters, it, end),
@@ -3027,8 +3033,7 @@ void DFSPHSolver::run_substep() {
     // delivers the substep's statistics: one host round trip per substep
     // when the first chunk suffices.
     const u32 maxit = cfg_.max_density_iters;
-    u32 it = 0, chunk = 2;
-    while (chunk < last_iters_[0] && chunk < maxit) chunk <<= 1;
+    u32 it = 0, chunk = first_density_chunk();
     while (it < maxit) {
         const u32 end = (maxit - it < chunk) ? maxit : it + chunk;
         run_phase(phase_key(kPhDensityIters, it, end), [&] { phase_iterations(0, it, end); });

@@ -578,6 +578,7 @@ private:
     void take_state();
     void phase_div_tail();
     void solve(int mode);
+    u32 first_density_chunk() const;
     bool all_converged(int mode);
     void read_state();
     void run_phase(u64 key, const std::function<void()>& body);
